@@ -1,0 +1,248 @@
+"""ctypes binding of the C ABI in include/crt_hip.h (libcrt_hip.so).
+
+This is the thin Python view of the HIP core used by the tests and the
+benchmark.  There is no fall-back: if libcrt_hip.so is missing or no GPU is
+present, calls raise instead of computing anything on the CPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG, "libcrt_hip.so")
+
+CRT_OK = 0
+CRT_ERR_INVALID = 1
+CRT_ERR_TYPE = 2
+CRT_ERR_OOM = 3
+CRT_ERR_HIP = 4
+CRT_ERR_STATE = 5
+CRT_ERR_ADMISSION = 6
+
+CRT_OUT_DEVICE = 1
+CRT_ANY_HIT = 2
+
+LIGHT_POINT, LIGHT_AREA, LIGHT_DIRECTIONAL = 0, 1, 2
+
+
+class CrtError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("crt error %d: %s" % (code, msg))
+        self.code = code
+
+
+class Camera(C.Structure):
+    _fields_ = [("position", C.c_double * 3), ("rotation", C.c_double * 9), ("focal_length", C.c_double),
+                ("resolution", C.c_double * 2), ("sensor_size", C.c_double * 2), ("z_positive", C.c_int32),
+                ("pad_", C.c_int32)]
+
+
+class Light(C.Structure):
+    _fields_ = [("type", C.c_int32), ("pad_", C.c_int32), ("intensity", C.c_double), ("position", C.c_double * 3),
+                ("rotation", C.c_double * 9), ("size", C.c_double * 2), ("direction", C.c_double * 3)]
+
+
+class BuildOpts(C.Structure):
+    _fields_ = [("max_leaf_size", C.c_uint32), ("flags", C.c_uint32)]
+
+
+class BvhInfo(C.Structure):
+    _fields_ = [("n_tris", C.c_uint32), ("n_wide_nodes", C.c_uint32), ("n_leaves", C.c_uint32),
+                ("max_depth", C.c_uint32), ("stack_capacity", C.c_uint32), ("pad_", C.c_uint32),
+                ("build_ms", C.c_double), ("sah_cost", C.c_double), ("bounds", C.c_double * 6),
+                ("sphere", C.c_double * 4), ("device_bytes", C.c_uint64)]
+
+
+class Timing(C.Structure):
+    _fields_ = [("total_ms", C.c_double), ("trace_ms", C.c_double), ("shadow_ms", C.c_double),
+                ("rays_traced", C.c_uint64), ("shadow_rays", C.c_uint64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class RenderOpts(C.Structure):
+    _fields_ = [("min_samples", C.c_int32), ("max_samples", C.c_int32), ("noise_threshold", C.c_double),
+                ("num_bounces", C.c_int32), ("pad_", C.c_uint32), ("seed", C.c_uint64)]
+
+
+_p = C.c_void_p
+_lib = None
+
+# name -> (restype, argtypes); the exported-symbol test checks this list
+# against include/crt_hip.h.
+SIGNATURES = {
+    "crt_device_init": (C.c_int, [C.c_int]),
+    "crt_last_error": (C.c_char_p, []),
+    "crt_version": (C.c_int, []),
+    "crt_scene_create": (C.c_int, [_p, C.c_uint32, _p, C.c_uint32, _p, _p, C.c_uint32, _p, _p, _p, _p]),
+    "crt_scene_info": (C.c_int, [_p, _p]),
+    "crt_scene_destroy": (None, [_p]),
+    "crt_pass": (C.c_int, [_p, _p, _p, _p, _p, _p, _p, C.c_uint32, _p, _p]),
+    "crt_trace_rays": (C.c_int, [_p, C.c_uint64, _p, _p, _p, _p, _p, _p, _p, _p, C.c_uint32, _p, _p]),
+    "crt_sun_pass": (C.c_int, [_p, _p, _p, _p, _p, _p, C.c_uint32, _p, _p]),
+}
+
+
+def load(path: str = LIB_PATH):
+    """Load libcrt_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(path):
+            raise RuntimeError("libcrt_hip.so not found at %s: run __graft_entry__.build()" % path)
+        lib = C.CDLL(path)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def _check(code):
+    if code != CRT_OK:
+        msg = load().crt_last_error().decode(errors="replace")
+        if code == CRT_ERR_INVALID:
+            raise ValueError(msg)
+        if code == CRT_ERR_TYPE:
+            raise TypeError(msg)
+        if code == CRT_ERR_OOM:
+            raise MemoryError(msg)
+        raise CrtError(code, msg)
+
+
+_initialised = False
+
+
+def init(device: int = 0):
+    global _initialised
+    if not _initialised:
+        _check(load().crt_device_init(device))
+        _initialised = True
+
+
+def ptr(a):
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        assert a.flags["C_CONTIGUOUS"]
+        return a.ctypes.data_as(C.c_void_p)
+    return C.c_void_p(int(a))   # raw device address (e.g. torch tensor.data_ptr())
+
+
+def make_camera(cam: dict) -> Camera:
+    c = Camera()
+    c.position[:] = [float(x) for x in np.asarray(cam["position"], dtype=np.float64).reshape(3)]
+    c.rotation[:] = [float(x) for x in np.asarray(cam["rotation"], dtype=np.float64).reshape(9)]
+    c.focal_length = float(cam["focal_length"])
+    c.resolution[:] = [float(x) for x in cam["resolution"]]
+    c.sensor_size[:] = [float(x) for x in cam["sensor_size"]]
+    c.z_positive = int(bool(cam.get("z_positive", False)))
+    return c
+
+
+def make_light(light: dict) -> Light:
+    l = Light()
+    l.type = {"point": LIGHT_POINT, "area": LIGHT_AREA, "directional": LIGHT_DIRECTIONAL}[light.get("type", "area")]
+    l.intensity = float(light["intensity"])
+    l.position[:] = [float(x) for x in np.asarray(light.get("position", np.zeros(3)), dtype=np.float64)]
+    l.rotation[:] = [float(x) for x in np.asarray(light.get("rotation", np.eye(3)), dtype=np.float64).reshape(9)]
+    l.size[:] = [float(x) for x in light.get("size", [0.0, 0.0])]
+    l.direction[:] = [float(x) for x in np.asarray(light.get("direction", np.zeros(3)), dtype=np.float64)]
+    return l
+
+
+class Scene:
+    """A mesh + GPU-built BVH resident in HBM (crt_scene_create)."""
+
+    def __init__(self, vertices, indices, tri_instance=None, vertex_normals=None, colors=None, smooth=None,
+                 max_leaf_size=8):
+        init()
+        v = np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1, 3)
+        i = np.ascontiguousarray(indices, dtype=np.uint32).reshape(-1, 3)
+        inst = None if tri_instance is None else np.ascontiguousarray(tri_instance, dtype=np.uint32)
+        vn = None if vertex_normals is None else np.ascontiguousarray(vertex_normals, dtype=np.float32)
+        ni = 1 if inst is None else int(inst.max())
+        col = None if colors is None else np.ascontiguousarray(colors, dtype=np.float32).reshape(-1, 3)
+        if col is not None:
+            ni = max(ni, col.shape[0])
+        sm = None if smooth is None else np.ascontiguousarray(smooth, dtype=np.uint8)
+        opts = BuildOpts(max_leaf_size, 0)
+        h = C.c_void_p()
+        _check(load().crt_scene_create(ptr(v), v.shape[0], ptr(i), i.shape[0], ptr(inst), ptr(vn), ni, ptr(col),
+                                       ptr(sm), C.byref(opts), C.byref(h)))
+        self.h = h
+        self.n_tris = i.shape[0]
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            load().crt_scene_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self) -> dict:
+        inf = BvhInfo()
+        _check(load().crt_scene_info(self.h, C.byref(inf)))
+        d = {k: getattr(inf, k) for k, _ in BvhInfo._fields_ if k != "pad_"}
+        d["bounds"] = list(inf.bounds)
+        d["sphere"] = list(inf.sphere)
+        return d
+
+    def camera_pass(self, cam: dict, want=("xyz", "normal", "instance", "depth", "tri")):
+        c = make_camera(cam)
+        w, h = int(np.floor(cam["resolution"][0])), int(np.floor(cam["resolution"][1]))
+        out = {}
+        if "xyz" in want: out["xyz"] = np.zeros((h, w, 3))
+        if "normal" in want: out["normal"] = np.zeros((h, w, 3))
+        if "instance" in want: out["instance"] = np.zeros((h, w), dtype=np.uint32)
+        if "depth" in want: out["depth"] = np.zeros((h, w))
+        if "tri" in want: out["tri"] = np.zeros((h, w), dtype=np.int32)
+        tm = Timing()
+        _check(load().crt_pass(self.h, C.byref(c), ptr(out.get("xyz")), ptr(out.get("normal")),
+                               ptr(out.get("instance")), ptr(out.get("depth")), ptr(out.get("tri")), 0, None,
+                               C.byref(tm)))
+        out["timing"] = tm.as_dict()
+        return out
+
+    def trace(self, o, d, tmin=None, tmax=None, any_hit=False):
+        o = np.ascontiguousarray(o, dtype=np.float64).reshape(-1, 3)
+        d = np.ascontiguousarray(d, dtype=np.float64).reshape(-1, 3)
+        n = o.shape[0]
+        t = np.empty(n)
+        tri = np.empty(n, dtype=np.int32)
+        u = np.empty(n)
+        v = np.empty(n)
+        tmin = None if tmin is None else np.ascontiguousarray(tmin, dtype=np.float64)
+        tmax = None if tmax is None else np.ascontiguousarray(tmax, dtype=np.float64)
+        tm = Timing()
+        flags = CRT_ANY_HIT if any_hit else 0
+        _check(load().crt_trace_rays(self.h, n, ptr(o), ptr(d), ptr(tmin), ptr(tmax), ptr(t), ptr(tri), ptr(u),
+                                     ptr(v), flags, None, C.byref(tm)))
+        return t, tri, u, v
+
+    def sun_pass(self, cam: dict, sun_dir, outputs=None, device=False, stream=None):
+        """Primary closest hit + one sun shadow ray per hit.  With device=True,
+        `outputs` holds device addresses {depth, tri, lit} (ints) and nothing is
+        copied back."""
+        c = make_camera(cam)
+        s = (C.c_double * 3)(*[float(x) for x in sun_dir])
+        tm = Timing()
+        if device:
+            o = outputs or {}
+            _check(load().crt_sun_pass(self.h, C.byref(c), s, ptr(o.get("depth")), ptr(o.get("tri")),
+                                       ptr(o.get("lit")), CRT_OUT_DEVICE, stream, C.byref(tm)))
+            return tm.as_dict()
+        w, h = int(np.floor(cam["resolution"][0])), int(np.floor(cam["resolution"][1]))
+        depth = np.zeros((h, w))
+        tri = np.zeros((h, w), dtype=np.int32)
+        lit = np.zeros((h, w), dtype=np.uint8)
+        _check(load().crt_sun_pass(self.h, C.byref(c), s, ptr(depth), ptr(tri), ptr(lit), 0, stream, C.byref(tm)))
+        return {"depth": depth, "tri": tri, "lit": lit, "timing": tm.as_dict()}

@@ -1,0 +1,547 @@
+// bvh_build.hip -- GPU construction of the 8-wide BVH.
+//
+// Replaces the reference's CPU build pipeline (SweepSahBuilder ->
+// ParallelReinsertionOptimizer -> NodeLayoutOptimizer,
+// src/crt/rendering_body_fixed/body_fixed_group.hpp:66-80) with:
+//   1. k_prep     : per-triangle AABB + centroid, block-reduced centroid and
+//                   scene bounds (utilities.hpp:159-189 counterpart)
+//   2. k_morton   : 63-bit Morton code of the centroid (21 bits per axis)
+//   3. radix sort : 8 passes of an 8-bit LSD sort -- per-tile LDS digit
+//                   histograms, one device-wide exclusive scan, and a stable
+//                   scatter whose ranks come from 64-lane ballot matching
+//   4. k_karras   : binary radix-tree emission (Karras 2012); every internal
+//                   node covers a contiguous range of sorted triangles
+//   5. k_refit    : bottom-up AABB refit with one atomic arrival flag per node
+//   6. k_collapse : top-down, level-synchronous collapse into 8-wide nodes;
+//                   each wide node greedily opens its largest-area inner
+//                   children until 8 slots are used; subtrees with at most
+//                   `max_leaf` triangles become leaves (contiguous ranges)
+//   7. k_emit     : triangles written in leaf order (48 B each)
+#include "crt_internal.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+namespace crt {
+namespace {
+
+constexpr int BLK = 256;
+
+__device__ inline uint32_t f2o(float f) {
+    const uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__host__ __device__ inline float o2f(uint32_t u) {
+    const uint32_t b = (u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u;
+    float f;
+    memcpy(&f, &b, 4);
+    return f;
+}
+
+// bounds[0..5]  : centroid min xyz, max xyz (ordered uint)
+// bounds[6..11] : triangle AABB min xyz, max xyz (ordered uint)
+__global__ __launch_bounds__(BLK) void k_prep(const float* __restrict__ V, const uint32_t* __restrict__ I,
+                                              uint32_t n, float4* __restrict__ lo, float4* __restrict__ hi,
+                                              float4* __restrict__ cen, uint32_t* __restrict__ bounds) {
+    __shared__ float red[12][BLK];
+    const uint32_t i = blockIdx.x * BLK + threadIdx.x;
+    float v[12];
+    if (i < n) {
+        const uint32_t a = I[3 * i], b = I[3 * i + 1], c = I[3 * i + 2];
+        const float ax = V[3 * a], ay = V[3 * a + 1], az = V[3 * a + 2];
+        const float bx = V[3 * b], by = V[3 * b + 1], bz = V[3 * b + 2];
+        const float cx = V[3 * c], cy = V[3 * c + 1], cz = V[3 * c + 2];
+        const float lx = fminf(ax, fminf(bx, cx)), ly = fminf(ay, fminf(by, cy)), lz = fminf(az, fminf(bz, cz));
+        const float hx = fmaxf(ax, fmaxf(bx, cx)), hy = fmaxf(ay, fmaxf(by, cy)), hz = fmaxf(az, fmaxf(bz, cz));
+        const float mx = (ax + bx + cx) * (1.0f / 3.0f), my = (ay + by + cy) * (1.0f / 3.0f),
+                    mz = (az + bz + cz) * (1.0f / 3.0f);
+        lo[i] = make_float4(lx, ly, lz, 0.f);
+        hi[i] = make_float4(hx, hy, hz, 0.f);
+        cen[i] = make_float4(mx, my, mz, 0.f);
+        v[0] = mx; v[1] = my; v[2] = mz; v[3] = mx; v[4] = my; v[5] = mz;
+        v[6] = lx; v[7] = ly; v[8] = lz; v[9] = hx; v[10] = hy; v[11] = hz;
+    } else {
+        const float inf = __int_as_float(0x7f800000);
+        for (int k = 0; k < 3; ++k) { v[k] = inf; v[3 + k] = -inf; v[6 + k] = inf; v[9 + k] = -inf; }
+    }
+    for (int k = 0; k < 12; ++k) red[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int s = BLK / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            for (int k = 0; k < 12; ++k) {
+                const bool mn = (k < 3) || (k >= 6 && k < 9);
+                const float x = red[k][threadIdx.x], y = red[k][threadIdx.x + s];
+                red[k][threadIdx.x] = mn ? fminf(x, y) : fmaxf(x, y);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 12) {
+        const int k = threadIdx.x;
+        const bool mn = (k < 3) || (k >= 6 && k < 9);
+        const uint32_t o = f2o(red[k][0]);
+        if (mn) atomicMin(&bounds[k], o); else atomicMax(&bounds[k], o);
+    }
+}
+
+__device__ inline uint64_t split3_21(uint32_t a) {
+    uint64_t x = a & 0x1FFFFFull;
+    x = (x | x << 32) & 0x1F00000000FFFFull;
+    x = (x | x << 16) & 0x1F0000FF0000FFull;
+    x = (x | x << 8) & 0x100F00F00F00F00Full;
+    x = (x | x << 4) & 0x10C30C30C30C30C3ull;
+    x = (x | x << 2) & 0x1249249249249249ull;
+    return x;
+}
+
+__global__ __launch_bounds__(BLK) void k_morton(const float4* __restrict__ cen, uint32_t n,
+                                                const uint32_t* __restrict__ bounds,
+                                                uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    const uint32_t i = blockIdx.x * BLK + threadIdx.x;
+    if (i >= n) return;
+    const float4 c = cen[i];
+    uint32_t q[3];
+    const float cc[3] = {c.x, c.y, c.z};
+    for (int k = 0; k < 3; ++k) {
+        const float lo = o2f(bounds[k]), hi = o2f(bounds[3 + k]);
+        const float ext = hi - lo;
+        float f = ext > 0.f ? (cc[k] - lo) / ext : 0.f;
+        f = fminf(fmaxf(f, 0.f), 1.f);
+        uint32_t qi = (uint32_t)(f * 2097152.0f);
+        q[k] = qi > 2097151u ? 2097151u : qi;
+    }
+    keys[i] = (split3_21(q[0]) << 2) | (split3_21(q[1]) << 1) | split3_21(q[2]);
+    vals[i] = i;
+}
+
+// ---------------- LSD radix sort (8-bit digits) ----------------
+constexpr int RS_ITEMS = 16;
+constexpr int RS_TILE = BLK * RS_ITEMS;
+
+__global__ __launch_bounds__(BLK) void k_rs_hist(const uint64_t* __restrict__ keys, uint32_t n, int shift,
+                                                 uint32_t nblocks, uint32_t* __restrict__ hist) {
+    __shared__ uint32_t cnt[256];
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * RS_TILE;
+    for (int it = 0; it < RS_ITEMS; ++it) {
+        const uint32_t i = base + it * BLK + threadIdx.x;
+        if (i < n) atomicAdd(&cnt[(uint32_t)(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[threadIdx.x * nblocks + blockIdx.x] = cnt[threadIdx.x];
+}
+
+// Exclusive scan of `total` counters in place, one block of 1024 threads.
+__global__ __launch_bounds__(1024) void k_rs_scan(uint32_t* __restrict__ data, uint32_t total) {
+    __shared__ uint32_t part[1024];
+    const uint32_t per = (total + 1023) / 1024;
+    const uint32_t beg = threadIdx.x * per;
+    const uint32_t end = min(beg + per, total);
+    uint32_t s = 0;
+    for (uint32_t i = beg; i < end; ++i) s += data[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    // Hillis-Steele inclusive scan over 1024 partial sums
+    for (int off = 1; off < 1024; off <<= 1) {
+        const uint32_t x = threadIdx.x >= (uint32_t)off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += x;
+        __syncthreads();
+    }
+    uint32_t run = threadIdx.x ? part[threadIdx.x - 1] : 0u;
+    for (uint32_t i = beg; i < end; ++i) {
+        const uint32_t x = data[i];
+        data[i] = run;
+        run += x;
+    }
+}
+
+__global__ __launch_bounds__(BLK) void k_rs_scatter(const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                    uint64_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                    uint32_t n, int shift, uint32_t nblocks,
+                                                    const uint32_t* __restrict__ hist) {
+    __shared__ uint32_t run[256];
+    __shared__ uint32_t wcnt[BLK / 64][256];
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6;
+    const int lane = tid & 63;
+    run[tid] = hist[tid * nblocks + blockIdx.x];
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const uint32_t base = blockIdx.x * RS_TILE;
+    for (int it = 0; it < RS_ITEMS; ++it) {
+        for (int w = 0; w < BLK / 64; ++w) wcnt[w][tid] = 0;
+        __syncthreads();
+        const uint32_t i = base + it * BLK + tid;
+        const bool valid = i < n;
+        const uint64_t key = valid ? kin[i] : 0ull;
+        const uint32_t val = valid ? vin[i] : 0u;
+        const uint32_t dig = (uint32_t)(key >> shift) & 255u;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (dig >> b) & 1u;
+            const uint64_t m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        const uint32_t rank = (uint32_t)__popcll(peers & lt);
+        if (valid && rank == 0) wcnt[wave][dig] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t off = run[dig] + rank;
+            for (int w = 0; w < wave; ++w) off += wcnt[w][dig];
+            kout[off] = key;
+            vout[off] = val;
+        }
+        __syncthreads();
+        uint32_t add = 0;
+        for (int w = 0; w < BLK / 64; ++w) add += wcnt[w][tid];
+        run[tid] += add;
+        __syncthreads();
+    }
+}
+
+// ---------------- Karras radix tree ----------------
+__device__ inline int delta(const uint64_t* __restrict__ k, int n, int i, int j) {
+    if (j < 0 || j >= n) return -1;
+    const uint64_t a = k[i], b = k[j];
+    if (a == b) return 64 + __clz((uint32_t)(i ^ j));
+    return __clzll(a ^ b);
+}
+
+// Unified node ids: internal 0..n-2, leaf k -> n-1+k.
+__global__ __launch_bounds__(BLK) void k_karras(const uint64_t* __restrict__ keys, int n,
+                                                int* __restrict__ left, int* __restrict__ right,
+                                                int* __restrict__ parent, int* __restrict__ rfirst,
+                                                int* __restrict__ rlast) {
+    const int i = blockIdx.x * BLK + threadIdx.x;
+    if (i >= n - 1) return;
+    const int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
+    const int dmin = delta(keys, n, i, i - d);
+    int lmax = 2;
+    while (delta(keys, n, i, i + lmax * d) > dmin) lmax <<= 1;
+    int l = 0;
+    for (int t = lmax >> 1; t >= 1; t >>= 1)
+        if (delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
+    const int j = i + l * d;
+    const int dnode = delta(keys, n, i, j);
+    int s = 0;
+    int t = l;
+    do {
+        t = (t + 1) >> 1;
+        if (delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
+    } while (t > 1);
+    const int g = i + s * d + min(d, 0);
+    const int lo = min(i, j), hi = max(i, j);
+    const int lc = (lo == g) ? (n - 1 + g) : g;
+    const int rc = (hi == g + 1) ? (n - 1 + g + 1) : g + 1;
+    left[i] = lc;
+    right[i] = rc;
+    parent[lc] = i;
+    parent[rc] = i;
+    rfirst[i] = lo;
+    rlast[i] = hi;
+}
+
+// Bottom-up refit.  blo/bhi hold boxes of all 2n-1 unified nodes (leaves
+// pre-filled from sorted triangle boxes).
+__global__ __launch_bounds__(BLK) void k_refit(int n, const int* __restrict__ left, const int* __restrict__ right,
+                                               const int* __restrict__ parent, float4* blo, float4* bhi,
+                                               uint32_t* __restrict__ flags) {
+    const int k = blockIdx.x * BLK + threadIdx.x;
+    if (k >= n) return;
+    int node = n - 1 + k;
+    int p = parent[node];
+    while (p >= 0) {
+        __threadfence();
+        const uint32_t old = atomicAdd(&flags[p], 1u);
+        if (old == 0u) return;          // first arrival: sibling not ready yet
+        __threadfence();
+        const int a = left[p], b = right[p];
+        // coherent reads of boxes written by other threads
+        float4 la, ha, lb, hb;
+        const float* pla = reinterpret_cast<const float*>(&blo[a]);
+        const float* pha = reinterpret_cast<const float*>(&bhi[a]);
+        const float* plb = reinterpret_cast<const float*>(&blo[b]);
+        const float* phb = reinterpret_cast<const float*>(&bhi[b]);
+        la.x = __hip_atomic_load(pla + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        la.y = __hip_atomic_load(pla + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        la.z = __hip_atomic_load(pla + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ha.x = __hip_atomic_load(pha + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ha.y = __hip_atomic_load(pha + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ha.z = __hip_atomic_load(pha + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        lb.x = __hip_atomic_load(plb + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        lb.y = __hip_atomic_load(plb + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        lb.z = __hip_atomic_load(plb + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        hb.x = __hip_atomic_load(phb + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        hb.y = __hip_atomic_load(phb + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        hb.z = __hip_atomic_load(phb + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        float* plp = reinterpret_cast<float*>(&blo[p]);
+        float* php = reinterpret_cast<float*>(&bhi[p]);
+        __hip_atomic_store(plp + 0, fminf(la.x, lb.x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(plp + 1, fminf(la.y, lb.y), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(plp + 2, fminf(la.z, lb.z), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(php + 0, fmaxf(ha.x, hb.x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(php + 1, fmaxf(ha.y, hb.y), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(php + 2, fmaxf(ha.z, hb.z), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        node = p;
+        p = parent[node];
+    }
+}
+
+// Leaf boxes in sorted order: blo[n-1+k] = box of triangle vals[k].
+__global__ __launch_bounds__(BLK) void k_leaf_boxes(int n, const uint32_t* __restrict__ vals,
+                                                    const float4* __restrict__ tlo, const float4* __restrict__ thi,
+                                                    float4* __restrict__ blo, float4* __restrict__ bhi) {
+    const int k = blockIdx.x * BLK + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t t = vals[k];
+    blo[n - 1 + k] = tlo[t];
+    bhi[n - 1 + k] = thi[t];
+}
+
+__device__ inline float half_area(float4 lo, float4 hi) {
+    const float dx = fmaxf(hi.x - lo.x, 0.f), dy = fmaxf(hi.y - lo.y, 0.f), dz = fmaxf(hi.z - lo.z, 0.f);
+    return dx * dy + dy * dz + dz * dx;
+}
+
+__device__ inline float pad_down(float x) { return x - fmaxf(fabsf(x) * 4.8e-7f, 1e-30f); }
+__device__ inline float pad_up(float x) { return x + fmaxf(fabsf(x) * 4.8e-7f, 1e-30f); }
+
+struct Item { int bnode; int wnode; };
+
+__global__ __launch_bounds__(BLK) void k_collapse(int n, const Item* __restrict__ in, uint32_t n_in,
+                                                  Item* __restrict__ out, uint32_t* __restrict__ counters,
+                                                  const int* __restrict__ left, const int* __restrict__ right,
+                                                  const int* __restrict__ rfirst, const int* __restrict__ rlast,
+                                                  const float4* __restrict__ blo, const float4* __restrict__ bhi,
+                                                  uint32_t max_leaf, float4* __restrict__ nodes) {
+    // counters[0] = wide node count, counters[1] = next frontier size, counters[2] = leaves
+    const uint32_t w = blockIdx.x * BLK + threadIdx.x;
+    if (w >= n_in) return;
+    const Item it = in[w];
+    auto count_of = [&](int b) -> uint32_t {
+        return b >= n - 1 ? 1u : (uint32_t)(rlast[b] - rfirst[b] + 1);
+    };
+    int slot[8];
+    int cnt = 0;
+    if (it.bnode >= n - 1) {
+        slot[cnt++] = it.bnode;           // single-leaf root
+    } else {
+        slot[cnt++] = left[it.bnode];
+        slot[cnt++] = right[it.bnode];
+    }
+    while (cnt < 8) {
+        int best = -1;
+        float barea = -1.f;
+        for (int k = 0; k < cnt; ++k) {
+            const int b = slot[k];
+            if (b >= n - 1) continue;                 // binary leaf
+            if (count_of(b) <= max_leaf) continue;    // will be a leaf
+            const float a = half_area(blo[b], bhi[b]);
+            if (a > barea) { barea = a; best = k; }
+        }
+        if (best < 0) break;
+        const int b = slot[best];
+        slot[best] = left[b];
+        slot[cnt++] = right[b];
+    }
+    float4* nd = nodes + (size_t)it.wnode * 16u;
+    uint32_t nleaf = 0;
+    for (int k = 0; k < 8; ++k) {
+        if (k < cnt) {
+            const int b = slot[k];
+            const float4 lo = blo[b], hi = bhi[b];
+            uint32_t ref;
+            const uint32_t c = count_of(b);
+            if (b >= n - 1 || c <= max_leaf) {
+                const uint32_t first = b >= n - 1 ? (uint32_t)(b - (n - 1)) : (uint32_t)rfirst[b];
+                ref = make_leaf_ref(first, c);
+                ++nleaf;
+            } else {
+                const uint32_t wn = atomicAdd(&counters[0], 1u);
+                const uint32_t q = atomicAdd(&counters[1], 1u);
+                out[q] = Item{b, (int)wn};
+                ref = wn;
+            }
+            nd[2 * k] = make_float4(pad_down(lo.x), pad_down(lo.y), pad_down(lo.z), __uint_as_float(ref));
+            nd[2 * k + 1] = make_float4(pad_up(hi.x), pad_up(hi.y), pad_up(hi.z), __uint_as_float(1u));
+        } else {
+            const float inf = __int_as_float(0x7f800000);
+            nd[2 * k] = make_float4(inf, inf, inf, __uint_as_float(REF_EMPTY));
+            nd[2 * k + 1] = make_float4(-inf, -inf, -inf, __uint_as_float(0u));
+        }
+    }
+    atomicAdd(&counters[2], nleaf);
+}
+
+__global__ __launch_bounds__(BLK) void k_emit(const float* __restrict__ V, const uint32_t* __restrict__ I, uint32_t n,
+                                              const uint32_t* __restrict__ vals, float4* __restrict__ tris) {
+    const uint32_t k = blockIdx.x * BLK + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t t = vals[k];
+    const uint32_t a = I[3 * t], b = I[3 * t + 1], c = I[3 * t + 2];
+    tris[3 * k + 0] = make_float4(V[3 * a], V[3 * a + 1], V[3 * a + 2], __uint_as_float(t));
+    tris[3 * k + 1] = make_float4(V[3 * b], V[3 * b + 1], V[3 * b + 2], 0.f);
+    tris[3 * k + 2] = make_float4(V[3 * c], V[3 * c + 1], V[3 * c + 2], 0.f);
+}
+
+inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+}  // namespace
+
+void build_bvh(const float* d_vertices, const uint32_t* d_indices, uint32_t n_tris, uint32_t max_leaf,
+               hipStream_t st, BuildResult& out) {
+    CRT_REQUIRE(n_tris > 0, CRT_ERR_INVALID, "scene has no triangles");
+    CRT_REQUIRE(n_tris < (1u << 27), CRT_ERR_INVALID, "scene exceeds 2^27 triangles");
+    CRT_REQUIRE(max_leaf >= 1 && max_leaf <= 16, CRT_ERR_INVALID, "max_leaf_size must be in 1..16");
+    const uint32_t n = n_tris;
+    hipEvent_t e0, e1;
+    CRT_HIP_CHECK(hipEventCreate(&e0));
+    CRT_HIP_CHECK(hipEventCreate(&e1));
+    CRT_HIP_CHECK(hipEventRecord(e0, st));
+
+    DevBuf tlo, thi, cen, bounds, keys[2], vals[2], hist;
+    tlo.alloc((size_t)n * 16);
+    thi.alloc((size_t)n * 16);
+    cen.alloc((size_t)n * 16);
+    bounds.alloc(12 * 4);
+    {
+        uint32_t init[12];
+        for (int k = 0; k < 12; ++k) {
+            const bool mn = (k < 3) || (k >= 6 && k < 9);
+            init[k] = mn ? 0xFFFFFFFFu : 0u;
+        }
+        CRT_HIP_CHECK(hipMemcpyAsync(bounds.p, init, sizeof(init), hipMemcpyHostToDevice, st));
+    }
+    hipLaunchKernelGGL(k_prep, dim3(cdiv(n, BLK)), dim3(BLK), 0, st, d_vertices, d_indices, n,
+                       tlo.as<float4>(), thi.as<float4>(), cen.as<float4>(), bounds.as<uint32_t>());
+    keys[0].alloc((size_t)n * 8);
+    keys[1].alloc((size_t)n * 8);
+    vals[0].alloc((size_t)n * 4);
+    vals[1].alloc((size_t)n * 4);
+    hipLaunchKernelGGL(k_morton, dim3(cdiv(n, BLK)), dim3(BLK), 0, st, cen.as<float4>(), n,
+                       bounds.as<uint32_t>(), keys[0].as<uint64_t>(), vals[0].as<uint32_t>());
+    const uint32_t nb = cdiv(n, RS_TILE);
+    hist.alloc((size_t)256 * nb * 4);
+    int cur = 0;
+    for (int pass = 0; pass < 8; ++pass) {
+        const int shift = pass * 8;
+        hipLaunchKernelGGL(k_rs_hist, dim3(nb), dim3(BLK), 0, st, keys[cur].as<uint64_t>(), n, shift, nb,
+                           hist.as<uint32_t>());
+        hipLaunchKernelGGL(k_rs_scan, dim3(1), dim3(1024), 0, st, hist.as<uint32_t>(), 256u * nb);
+        hipLaunchKernelGGL(k_rs_scatter, dim3(nb), dim3(BLK), 0, st, keys[cur].as<uint64_t>(),
+                           vals[cur].as<uint32_t>(), keys[cur ^ 1].as<uint64_t>(), vals[cur ^ 1].as<uint32_t>(),
+                           n, shift, nb, hist.as<uint32_t>());
+        cur ^= 1;
+    }
+    const uint64_t* sk = keys[cur].as<uint64_t>();
+    const uint32_t* sv = vals[cur].as<uint32_t>();
+
+    const uint32_t nn = 2 * n - 1;
+    DevBuf left, right, parent, rfirst, rlast, blo, bhi, flags;
+    left.alloc((size_t)std::max(n - 1, 1u) * 4);
+    right.alloc((size_t)std::max(n - 1, 1u) * 4);
+    rfirst.alloc((size_t)std::max(n - 1, 1u) * 4);
+    rlast.alloc((size_t)std::max(n - 1, 1u) * 4);
+    parent.alloc((size_t)nn * 4);
+    blo.alloc((size_t)nn * 16);
+    bhi.alloc((size_t)nn * 16);
+    flags.alloc((size_t)std::max(n - 1, 1u) * 4);
+    CRT_HIP_CHECK(hipMemsetAsync(parent.p, 0xFF, (size_t)nn * 4, st));   // root parent = -1
+    CRT_HIP_CHECK(hipMemsetAsync(flags.p, 0, flags.bytes, st));
+    if (n > 1)
+        hipLaunchKernelGGL(k_karras, dim3(cdiv(n - 1, BLK)), dim3(BLK), 0, st, sk, (int)n, left.as<int>(),
+                           right.as<int>(), parent.as<int>(), rfirst.as<int>(), rlast.as<int>());
+    hipLaunchKernelGGL(k_leaf_boxes, dim3(cdiv(n, BLK)), dim3(BLK), 0, st, (int)n, sv, tlo.as<float4>(),
+                       thi.as<float4>(), blo.as<float4>(), bhi.as<float4>());
+    if (n > 1)
+        hipLaunchKernelGGL(k_refit, dim3(cdiv(n, BLK)), dim3(BLK), 0, st, (int)n, left.as<int>(), right.as<int>(),
+                           parent.as<int>(), blo.as<float4>(), bhi.as<float4>(), flags.as<uint32_t>());
+
+    // ---- collapse to 8-wide, level by level ----
+    const uint32_t node_cap = std::max(n, 2u);
+    DevBuf nodes_tmp, items[2], counters;
+    nodes_tmp.alloc((size_t)node_cap * 256);
+    items[0].alloc((size_t)node_cap * sizeof(Item));
+    items[1].alloc((size_t)node_cap * sizeof(Item));
+    counters.alloc(4 * 4);
+    {
+        const uint32_t c0[4] = {1u, 0u, 0u, 0u};
+        CRT_HIP_CHECK(hipMemcpyAsync(counters.p, c0, sizeof(c0), hipMemcpyHostToDevice, st));
+        const Item root{n > 1 ? 0 : (int)(n - 1), 0};
+        CRT_HIP_CHECK(hipMemcpyAsync(items[0].p, &root, sizeof(Item), hipMemcpyHostToDevice, st));
+    }
+    uint32_t n_in = 1, depth = 0, fr = 0;
+    uint32_t host_counters[4] = {0, 0, 0, 0};
+    while (n_in > 0) {
+        ++depth;
+        CRT_HIP_CHECK(hipMemsetAsync(counters.as<uint32_t>() + 1, 0, 4, st));
+        hipLaunchKernelGGL(k_collapse, dim3(cdiv(n_in, BLK)), dim3(BLK), 0, st, (int)n, items[fr].as<Item>(), n_in,
+                           items[fr ^ 1].as<Item>(), counters.as<uint32_t>(), left.as<int>(), right.as<int>(),
+                           rfirst.as<int>(), rlast.as<int>(), blo.as<float4>(), bhi.as<float4>(), max_leaf,
+                           nodes_tmp.as<float4>());
+        CRT_HIP_CHECK(hipMemcpyAsync(host_counters, counters.p, 16, hipMemcpyDeviceToHost, st));
+        CRT_HIP_CHECK(hipStreamSynchronize(st));
+        n_in = host_counters[1];
+        fr ^= 1;
+        CRT_REQUIRE(host_counters[0] <= node_cap, CRT_ERR_HIP, "wide node capacity exceeded");
+    }
+    out.n_nodes = host_counters[0];
+    out.n_leaves = host_counters[2];
+    out.max_depth = depth;
+    out.nodes.alloc((size_t)out.n_nodes * 256);
+    CRT_HIP_CHECK(hipMemcpyAsync(out.nodes.p, nodes_tmp.p, (size_t)out.n_nodes * 256, hipMemcpyDeviceToDevice, st));
+    out.tris.alloc((size_t)n * 48);
+    hipLaunchKernelGGL(k_emit, dim3(cdiv(n, BLK)), dim3(BLK), 0, st, d_vertices, d_indices, n, sv,
+                       out.tris.as<float4>());
+    CRT_HIP_CHECK(hipGetLastError());
+    CRT_HIP_CHECK(hipEventRecord(e1, st));
+    uint32_t hb[12];
+    CRT_HIP_CHECK(hipMemcpyAsync(hb, bounds.p, sizeof(hb), hipMemcpyDeviceToHost, st));
+    CRT_HIP_CHECK(hipStreamSynchronize(st));
+    float ms = 0.f;
+    CRT_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    out.build_ms = ms;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    for (int k = 0; k < 6; ++k) out.bounds[k] = o2f(hb[6 + k]);
+
+    // SAH cost of the wide BVH (node visit cost 1, triangle cost 1), host side
+    std::vector<float4> hn((size_t)out.n_nodes * 16);
+    CRT_HIP_CHECK(hipMemcpy(hn.data(), out.nodes.p, hn.size() * 16, hipMemcpyDeviceToHost));
+    auto ha = [](const float4& lo, const float4& hi) {
+        const double dx = std::max(0.0, (double)hi.x - lo.x), dy = std::max(0.0, (double)hi.y - lo.y),
+                     dz = std::max(0.0, (double)hi.z - lo.z);
+        return dx * dy + dy * dz + dz * dx;
+    };
+    double cost = 0.0, root_area = 0.0;
+    {
+        float4 rl = make_float4(INFINITY, INFINITY, INFINITY, 0), rh = make_float4(-INFINITY, -INFINITY, -INFINITY, 0);
+        for (int k = 0; k < 8; ++k) {
+            const float4& l = hn[2 * k];
+            const float4& h = hn[2 * k + 1];
+            uint32_t ref;
+            memcpy(&ref, &l.w, 4);
+            if (ref == REF_EMPTY) continue;
+            rl.x = std::min(rl.x, l.x); rl.y = std::min(rl.y, l.y); rl.z = std::min(rl.z, l.z);
+            rh.x = std::max(rh.x, h.x); rh.y = std::max(rh.y, h.y); rh.z = std::max(rh.z, h.z);
+        }
+        root_area = ha(rl, rh);
+    }
+    cost = root_area;   // root visit
+    for (size_t i = 0; i < (size_t)out.n_nodes * 8; ++i) {
+        const float4& l = hn[2 * i];
+        const float4& h = hn[2 * i + 1];
+        uint32_t ref;
+        memcpy(&ref, &l.w, 4);
+        if (ref == REF_EMPTY) continue;
+        const double a = ha(l, h);
+        cost += (ref & REF_LEAF) ? a * (double)leaf_count(ref) : a;
+    }
+    out.sah_cost = root_area > 0 ? cost / root_area : 0.0;
+}
+
+}  // namespace crt

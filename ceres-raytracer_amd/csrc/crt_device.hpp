@@ -1,0 +1,313 @@
+// crt_device.hpp -- device-side data layout and the octet traversal core.
+//
+// BVH layout in HBM ("wide node"): 8 child records of 32 B each, 256 B per
+// node, 256-B aligned.  Child record k = { lo.xyz, ref | hi.xyz, meta }.
+// A ray is traced by one *octet*: 8 consecutive lanes of a 64-wide wave.
+// Lane k of the octet reads child record k (two 16-B loads), so the octet
+// reads the node as one contiguous 256-B block (two 128-B cache lines) and
+// tests all 8 child boxes in one step.  In leaves, lane k tests triangle k.
+//
+// ref encoding (also the traversal stack entry):
+//   inner child : wide node index                      (bit 31 clear)
+//   leaf child  : 0x80000000 | first << 4 | (count-1)  (first < 2^27, count <= 16)
+//   empty slot  : 0xFFFFFFFF (its box is empty too)
+//
+// Triangles are stored in leaf order as 3 float4 (48 B): {p0, orig index
+// bits}, {p1, 0}, {p2, 0}, so a leaf of 8 triangles is 384 contiguous bytes.
+//
+// The triangle test is the watertight test of Woop, Benthin and Wald (JCGT
+// 2013): shear/scale into the ray frame, exact-sign 2D edge functions with an
+// FP64 fall-back when an edge function is exactly zero.  Acceptance is
+// inclusive (edge functions of one sign or zero, tmin <= t <= tmax) like
+// lib/bvh/triangle.hpp:150-156.  Barycentrics follow the reference convention
+// hit = u*p1 + v*p2 + (1-u-v)*p0 (src/crt/passes.hpp:57).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace crt {
+
+constexpr uint32_t REF_EMPTY = 0xFFFFFFFFu;
+constexpr uint32_t REF_LEAF = 0x80000000u;
+constexpr int OCTET = 8;
+constexpr int TRACE_BLOCK = 256;                 // threads per block
+constexpr int OCTETS_PER_BLOCK = TRACE_BLOCK / OCTET;
+
+struct __align__(16) WideChild {
+    float lo[3];
+    uint32_t ref;
+    float hi[3];
+    uint32_t meta;
+};
+static_assert(sizeof(WideChild) == 32, "child record is 32 B");
+
+__host__ __device__ inline uint32_t make_leaf_ref(uint32_t first, uint32_t count) {
+    return REF_LEAF | (first << 4) | (count - 1u);
+}
+__host__ __device__ inline uint32_t leaf_first(uint32_t ref) { return (ref >> 4) & 0x07FFFFFFu; }
+__host__ __device__ inline uint32_t leaf_count(uint32_t ref) { return (ref & 15u) + 1u; }
+
+// Device view of a scene (passed by value to kernels).
+struct SceneView {
+    const float4* nodes;        // n_wide_nodes * 8 * 2 float4
+    const float4* tris;         // n_tris * 3 float4, leaf order
+    const uint32_t* tri_vidx;   // n_tris * 3, original order
+    const float* vnormals;      // n_vertices * 3 (may be null)
+    const uint32_t* tri_inst;   // n_tris, original order
+    const float* inst_color;    // n_instances * 3
+    const uint8_t* inst_smooth; // n_instances
+    uint32_t n_tris;
+    uint32_t n_instances;
+    uint32_t stack_cap;         // per-octet stack entries
+    uint32_t pad_;
+    double center[3];           // bounding sphere for FP64 rebasing
+    double radius;
+};
+
+// Closest-hit result of one traversal (local FP32 frame of the rebased ray).
+struct HitRec {
+    float t;
+    float u, v;
+    int32_t slot;   // triangle index in leaf order, -1 = miss
+};
+
+// ------------------------------------------------------------------------
+// small helpers
+// ------------------------------------------------------------------------
+__device__ inline float min3f(float a, float b, float c) { return fminf(a, fminf(b, c)); }
+__device__ inline float max3f(float a, float b, float c) { return fmaxf(a, fmaxf(b, c)); }
+
+__device__ inline float safe_rcp(float x) {
+    // Same intent as bvh::Vector::safe_inverse (lib/bvh/vector.hpp:75-80):
+    // tiny components are clamped so 1/x stays finite; the FP32 clamp keeps
+    // products with scene coordinates far below FLT_MAX.
+    const float eps = 1e-20f;
+    return 1.0f / (fabsf(x) < eps ? copysignf(eps, x) : x);
+}
+
+__device__ inline float sel3(float a, float b, float c, int k) {
+    return k == 0 ? a : (k == 1 ? b : c);
+}
+
+// Per-ray constants for the watertight test and the slab test.
+struct RayPre {
+    float ox, oy, oz;          // origin
+    float ix, iy, iz;          // 1/d (clamped)
+    float nx, ny, nz;          // -o/d
+    int kx, ky, kz;            // axis permutation, kz = dominant axis
+    float sx, sy, sz;          // shear constants
+};
+
+__device__ inline RayPre make_ray_pre(float ox, float oy, float oz, float dx, float dy, float dz) {
+    RayPre r;
+    r.ox = ox; r.oy = oy; r.oz = oz;
+    r.ix = safe_rcp(dx); r.iy = safe_rcp(dy); r.iz = safe_rcp(dz);
+    r.nx = -ox * r.ix; r.ny = -oy * r.iy; r.nz = -oz * r.iz;
+    const float ax = fabsf(dx), ay = fabsf(dy), az = fabsf(dz);
+    int kz = (ax >= ay) ? ((ax >= az) ? 0 : 2) : ((ay >= az) ? 1 : 2);
+    int kx = kz == 2 ? 0 : kz + 1;
+    int ky = kx == 2 ? 0 : kx + 1;
+    const float dkz = sel3(dx, dy, dz, kz);
+    if (dkz < 0.0f) { int t = kx; kx = ky; ky = t; }
+    r.kx = kx; r.ky = ky; r.kz = kz;
+    r.sx = sel3(dx, dy, dz, kx) / dkz;
+    r.sy = sel3(dx, dy, dz, ky) / dkz;
+    r.sz = 1.0f / dkz;
+    return r;
+}
+
+// Watertight ray/triangle test.  Returns true on hit with tmin <= t <= tmax;
+// u = weight of p1, v = weight of p2 (reference convention).
+__device__ inline bool tri_test(const RayPre& r, float4 a, float4 b, float4 c,
+                                float tmin, float tmax, float& t_out, float& u_out, float& v_out) {
+    // vertices relative to the origin, permuted into (kx, ky, kz)
+    const float Ax0 = a.x - r.ox, Ay0 = a.y - r.oy, Az0 = a.z - r.oz;
+    const float Bx0 = b.x - r.ox, By0 = b.y - r.oy, Bz0 = b.z - r.oz;
+    const float Cx0 = c.x - r.ox, Cy0 = c.y - r.oy, Cz0 = c.z - r.oz;
+    const float Akx = sel3(Ax0, Ay0, Az0, r.kx), Aky = sel3(Ax0, Ay0, Az0, r.ky), Akz = sel3(Ax0, Ay0, Az0, r.kz);
+    const float Bkx = sel3(Bx0, By0, Bz0, r.kx), Bky = sel3(Bx0, By0, Bz0, r.ky), Bkz = sel3(Bx0, By0, Bz0, r.kz);
+    const float Ckx = sel3(Cx0, Cy0, Cz0, r.kx), Cky = sel3(Cx0, Cy0, Cz0, r.ky), Ckz = sel3(Cx0, Cy0, Cz0, r.kz);
+    // shear (per vertex: identical bits for a vertex shared by two triangles)
+    const float Ax = __fsub_rn(Akx, __fmul_rn(r.sx, Akz));
+    const float Ay = __fsub_rn(Aky, __fmul_rn(r.sy, Akz));
+    const float Bx = __fsub_rn(Bkx, __fmul_rn(r.sx, Bkz));
+    const float By = __fsub_rn(Bky, __fmul_rn(r.sy, Bkz));
+    const float Cx = __fsub_rn(Ckx, __fmul_rn(r.sx, Ckz));
+    const float Cy = __fsub_rn(Cky, __fmul_rn(r.sy, Ckz));
+    // edge functions, never contracted into FMA (exact antisymmetry on shared edges)
+    float U = __fsub_rn(__fmul_rn(Cx, By), __fmul_rn(Cy, Bx));
+    float V = __fsub_rn(__fmul_rn(Ax, Cy), __fmul_rn(Ay, Cx));
+    float W = __fsub_rn(__fmul_rn(Bx, Ay), __fmul_rn(By, Ax));
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        const double CxBy = (double)Cx * (double)By, CyBx = (double)Cy * (double)Bx;
+        const double AxCy = (double)Ax * (double)Cy, AyCx = (double)Ay * (double)Cx;
+        const double BxAy = (double)Bx * (double)Ay, ByAx = (double)By * (double)Ax;
+        U = (float)(CxBy - CyBx);
+        V = (float)(AxCy - AyCx);
+        W = (float)(BxAy - ByAx);
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    const float det = U + V + W;
+    if (det == 0.0f) return false;
+    const float Az = r.sz * Akz, Bz = r.sz * Bkz, Cz = r.sz * Ckz;
+    const float T = U * Az + V * Bz + W * Cz;
+    const float rcp = 1.0f / det;
+    const float t = T * rcp;
+    if (!(t >= tmin && t <= tmax)) return false;
+    t_out = t;
+    u_out = V * rcp;
+    v_out = W * rcp;
+    return true;
+}
+
+// ------------------------------------------------------------------------
+// Octet traversal.  All 8 lanes of an octet call this with the same ray and
+// stay converged inside (every branch below is octet-uniform).  `stk_ref` /
+// `stk_t` point at this octet's LDS stack of `cap` entries.
+// ANY_HIT: returns at the first accepted hit (shadow rays), hit.slot >= 0.
+// ------------------------------------------------------------------------
+template <bool ANY_HIT>
+__device__ inline HitRec trace_octet(const SceneView& s, const RayPre& r, float tmin, float tmax,
+                                     uint32_t* stk_ref, float* stk_t, int sub, int lane_base) {
+    HitRec hit;
+    hit.t = tmax; hit.u = 0.f; hit.v = 0.f; hit.slot = -1;
+    const float4* __restrict__ nodes = s.nodes;
+    const float4* __restrict__ tris = s.tris;
+    const uint32_t cap = s.stack_cap;
+    float best = tmax;
+    uint32_t cur = 0u;           // root wide node
+    uint32_t sp = 0u;
+    while (true) {
+        if (!(cur & REF_LEAF)) {
+            // ---- inner node: lane `sub` tests child record `sub` ----
+            const float4 q0 = nodes[(size_t)cur * 16u + 2u * sub];
+            const float4 q1 = nodes[(size_t)cur * 16u + 2u * sub + 1u];
+            const uint32_t ref = __float_as_uint(q0.w);
+            const float tx0 = fmaf(q0.x, r.ix, r.nx), tx1 = fmaf(q1.x, r.ix, r.nx);
+            const float ty0 = fmaf(q0.y, r.iy, r.ny), ty1 = fmaf(q1.y, r.iy, r.ny);
+            const float tz0 = fmaf(q0.z, r.iz, r.nz), tz1 = fmaf(q1.z, r.iz, r.nz);
+            const float tnear = max3f(fminf(tx0, tx1), fminf(ty0, ty1), fmaxf(fminf(tz0, tz1), tmin));
+            const float tfar = min3f(fmaxf(tx0, tx1), fmaxf(ty0, ty1), fminf(fmaxf(tz0, tz1), best));
+            const bool h = (ref != REF_EMPTY) && (tnear <= tfar * 1.00000024f);
+            const uint32_t m = (uint32_t)((__ballot(h) >> lane_base) & 0xFFull);
+            if (m != 0u) {
+                const int nh = __popc(m);
+                // rank of this lane among hit children by (tnear, lane)
+                int rank = 0;
+#pragma unroll
+                for (int j = 1; j < OCTET; ++j) {
+                    const int o = (sub + j) & 7;
+                    const float ot = __shfl(tnear, lane_base + o, 64);
+                    rank += (((m >> o) & 1u) && (ot < tnear || (ot == tnear && o < sub))) ? 1 : 0;
+                }
+                const uint32_t m0 = (uint32_t)((__ballot(h && rank == 0) >> lane_base) & 0xFFull);
+                const int first_lane = __ffs(m0) - 1;
+                const uint32_t next = (uint32_t)__shfl((int)ref, lane_base + first_lane, 64);
+                if (h && rank > 0) {
+                    const uint32_t pos = sp + (uint32_t)(nh - 1 - rank);
+                    if (pos < cap) { stk_ref[pos] = ref; stk_t[pos] = tnear; }
+                }
+                sp += (uint32_t)(nh - 1);
+                if (sp > cap) sp = cap;   // never happens: cap >= 7 * depth (checked at build)
+                cur = next;
+                continue;
+            }
+        } else {
+            // ---- leaf: lane `sub` tests triangles first+sub, first+sub+8, ... ----
+            const uint32_t first = leaf_first(cur);
+            const uint32_t count = leaf_count(cur);
+            for (uint32_t base = 0; base < count; base += OCTET) {
+                const uint32_t k = base + (uint32_t)sub;
+                float t = __int_as_float(0x7f800000);
+                float u = 0.f, v = 0.f;
+                bool ok = false;
+                if (k < count) {
+                    const size_t ti = (size_t)(first + k) * 3u;
+                    const float4 a = tris[ti];
+                    const float4 b = tris[ti + 1];
+                    const float4 c = tris[ti + 2];
+                    ok = tri_test(r, a, b, c, tmin, best, t, u, v);
+                    if (!ok) t = __int_as_float(0x7f800000);
+                }
+                if (ANY_HIT) {
+                    const uint32_t mh = (uint32_t)((__ballot(ok) >> lane_base) & 0xFFull);
+                    if (mh != 0u) {
+                        const int l = __ffs(mh) - 1;
+                        hit.t = __shfl(t, lane_base + l, 64);
+                        hit.slot = (int)(first + base) + l;
+                        return hit;
+                    }
+                } else {
+                    // octet min-reduce of (t, k); exact ties go to the later
+                    // triangle, like the "last tested wins" rule of
+                    // lib/bvh/single_ray_traverser.hpp:55-60
+                    float bt = t;
+                    int bl = sub;
+#pragma unroll
+                    for (int w = 1; w < OCTET; w <<= 1) {
+                        const float ot = __shfl_xor(bt, w, 64);
+                        const int ol = __shfl_xor(bl, w, 64);
+                        if (ot < bt || (ot == bt && ol > bl)) { bt = ot; bl = ol; }
+                    }
+                    const uint32_t mh = (uint32_t)((__ballot(ok) >> lane_base) & 0xFFull);
+                    if (mh != 0u && bt <= best) {   // some lane accepted a hit
+                        best = bt;
+                        hit.t = bt;
+                        hit.u = __shfl(u, lane_base + bl, 64);
+                        hit.v = __shfl(v, lane_base + bl, 64);
+                        hit.slot = (int)(first + base) + bl;
+                    }
+                }
+            }
+        }
+        // ---- pop the next entry that can still hold a closer hit ----
+        bool found = false;
+        while (sp > 0u) {
+            --sp;
+            const uint32_t e = stk_ref[sp];
+            const float te = stk_t[sp];
+            if (te <= best) { cur = e; found = true; break; }
+        }
+        if (!found) break;
+    }
+    return hit;
+}
+
+// ------------------------------------------------------------------------
+// FP64 helpers: camera rays and rebasing
+// ------------------------------------------------------------------------
+struct CamView {
+    double pos[3];
+    double rot[9];       // row-major R; world dir = R^T * cam dir
+    double cx, cy, sx, sy, f;   // centre, scale (res/sensor), +/-focal
+    int w, h;
+};
+
+// src/crt/cameras/simple_camera.hpp:61-81 -- pixel (u,v) -> unit direction.
+__device__ inline void camera_dir(const CamView& c, double u, double v, double d[3]) {
+    double x = (-c.cx + u) / c.sx;
+    double y = (c.cy - v) / c.sy;
+    double z = c.f;
+    const double inv = 1.0 / sqrt(x * x + y * y + z * z);
+    x *= inv; y *= inv; z *= inv;
+    d[0] = c.rot[0] * x + c.rot[3] * y + c.rot[6] * z;
+    d[1] = c.rot[1] * x + c.rot[4] * y + c.rot[7] * z;
+    d[2] = c.rot[2] * x + c.rot[5] * y + c.rot[8] * z;
+}
+
+// Distance along a unit FP64 ray to the entry of the scene's bounding sphere;
+// returns false when the ray misses the sphere (no triangle can be hit).
+__device__ inline bool rebase_t0(const SceneView& s, const double o[3], const double d[3], double& t0) {
+    const double px = o[0] - s.center[0], py = o[1] - s.center[1], pz = o[2] - s.center[2];
+    const double b = px * d[0] + py * d[1] + pz * d[2];
+    const double c = px * px + py * py + pz * pz - s.radius * s.radius;
+    const double disc = b * b - c;
+    if (disc < 0.0) return false;
+    const double sq = sqrt(disc);
+    if (-b + sq < 0.0) return false;     // sphere entirely behind the origin
+    const double te = -b - sq;
+    t0 = te > 0.0 ? te : 0.0;
+    return true;
+}
+
+}  // namespace crt

@@ -1,0 +1,100 @@
+// crt_internal.hpp -- host-side internals shared by the .hip translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "crt_device.hpp"
+#include "../../include/crt_hip.h"
+
+namespace crt {
+
+// Error carried from deep inside the library to the C-ABI boundary, where it
+// becomes a status code + crt_last_error() text.
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define CRT_HIP_CHECK(expr)                                                              \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess) {                                                          \
+            (void)hipGetLastError();                                                     \
+            throw ::crt::Error(e_ == hipErrorOutOfMemory ? CRT_ERR_OOM : CRT_ERR_HIP,    \
+                               std::string(#expr) + ": " + hipGetErrorString(e_));       \
+        }                                                                                \
+    } while (0)
+
+#define CRT_REQUIRE(cond, code, msg)                     \
+    do {                                                 \
+        if (!(cond)) throw ::crt::Error((code), (msg));  \
+    } while (0)
+
+// Owning device buffer.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    void alloc(size_t b) {
+        release();
+        if (b == 0) return;
+        CRT_HIP_CHECK(hipMalloc(&p, b));
+        bytes = b;
+    }
+    // grow-only reallocation (contents not preserved)
+    void reserve(size_t b) {
+        if (b > bytes) alloc(b);
+    }
+    template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+struct BuildResult {
+    DevBuf nodes;        // wide nodes
+    DevBuf tris;         // leaf-ordered triangles (3 float4 each)
+    uint32_t n_nodes = 0;
+    uint32_t n_leaves = 0;
+    uint32_t max_depth = 0;
+    double build_ms = 0.0;
+    double sah_cost = 0.0;
+    float bounds[6] = {0, 0, 0, 0, 0, 0};
+};
+
+// GPU BVH build: LBVH (63-bit Morton codes, LDS radix sort, Karras
+// hierarchy, bottom-up refit) collapsed into 8-wide nodes.
+void build_bvh(const float* d_vertices, const uint32_t* d_indices, uint32_t n_tris,
+               uint32_t max_leaf, hipStream_t stream, BuildResult& out);
+
+// Scratch buffers reused across calls on one scene (grow-only).
+struct Workspace {
+    DevBuf a, b, c, d, e, f, g, h;
+    DevBuf counters;
+};
+
+}  // namespace crt
+
+struct crt_scene {
+    int device = 0;
+    uint32_t n_tris = 0, n_vertices = 0, n_instances = 0;
+    crt::DevBuf vertices, indices, tri_vidx, vnormals, tri_inst, inst_color, inst_smooth;
+    crt::BuildResult bvh;
+    crt::SceneView view{};
+    crt_bvh_info info{};
+    crt::Workspace ws;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+};

@@ -1,0 +1,637 @@
+// trace.hip -- scene lifetime, camera passes, general ray queries and the
+// primary + sun-shadow pass, exported through the C ABI of crt_hip.h.
+//
+// Every kernel here is "octet per ray": 8 lanes of a wave trace one ray
+// through the 8-wide BVH (crt_device.hpp).  A 256-thread block holds 32
+// octets; each octet owns a slice of the block's dynamic LDS for its
+// traversal stack (stack_cap refs + stack_cap distances).
+#include "crt_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+
+namespace crt {
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+template <typename F>
+int guarded(F&& f) {
+    try {
+        f();
+        return CRT_OK;
+    } catch (const Error& e) {
+        return fail(e.code, e.what());
+    } catch (const std::bad_alloc&) {
+        return fail(CRT_ERR_OOM, "host allocation failed");
+    } catch (const std::exception& e) {
+        return fail(CRT_ERR_HIP, e.what());
+    }
+}
+
+inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+size_t stack_lds_bytes(const SceneView& s) { return (size_t)OCTETS_PER_BLOCK * s.stack_cap * 8u; }
+
+CamView make_cam(const crt_camera& c) {
+    CamView v;
+    for (int k = 0; k < 3; ++k) v.pos[k] = c.position[k];
+    for (int k = 0; k < 9; ++k) v.rot[k] = c.rotation[k];
+    // src/crt/cameras/simple_camera.hpp:23-26: centre = res/2, scale = res/sensor
+    v.cx = c.resolution[0] / 2.0;
+    v.cy = c.resolution[1] / 2.0;
+    v.sx = c.resolution[0] / c.sensor_size[0];
+    v.sy = c.resolution[1] / c.sensor_size[1];
+    v.f = c.z_positive ? c.focal_length : -c.focal_length;
+    // src/pybind.cpp:534-535: (size_t) floor(resolution)
+    v.w = (int)std::floor(c.resolution[0]);
+    v.h = (int)std::floor(c.resolution[1]);
+    return v;
+}
+
+void check_camera(const crt_camera* c) {
+    CRT_REQUIRE(c != nullptr, CRT_ERR_INVALID, "camera is NULL");
+    CRT_REQUIRE(c->resolution[0] >= 1 && c->resolution[1] >= 1, CRT_ERR_INVALID, "camera resolution must be >= 1");
+    CRT_REQUIRE(c->resolution[0] * c->resolution[1] <= 1.0e9, CRT_ERR_INVALID, "camera resolution too large");
+    CRT_REQUIRE(c->sensor_size[0] > 0 && c->sensor_size[1] > 0, CRT_ERR_INVALID, "sensor size must be > 0");
+    CRT_REQUIRE(std::isfinite(c->focal_length), CRT_ERR_INVALID, "focal length must be finite");
+}
+
+// Octet coordinates of the calling lane.
+struct OctetId {
+    int sub, lane_base, local;   // lane in octet, first lane of octet in wave, octet in block
+};
+__device__ inline OctetId octet_id() {
+    OctetId o;
+    const int lane = threadIdx.x & 63;
+    o.sub = lane & 7;
+    o.lane_base = lane & ~7;
+    o.local = threadIdx.x >> 3;
+    return o;
+}
+
+// Closest/any hit of one FP64 ray (unit direction d) through the scene:
+// rebase in FP64, trace in FP32.  Returns false on miss; otherwise t (FP64,
+// from o), u, v (FP32) and the leaf-order slot.
+template <bool ANY>
+__device__ inline bool trace_f64(const SceneView& s, const double o[3], const double d[3], double tmin,
+                                 double tmax, uint32_t* sref, float* st, const OctetId& id, double& t_out,
+                                 float& u, float& v, int& slot) {
+    double t0;
+    if (!rebase_t0(s, o, d, t0)) return false;
+    if (t0 > tmax) return false;
+    const double ro[3] = {o[0] + t0 * d[0], o[1] + t0 * d[1], o[2] + t0 * d[2]};
+    const RayPre r = make_ray_pre((float)ro[0], (float)ro[1], (float)ro[2], (float)d[0], (float)d[1], (float)d[2]);
+    const float ltmin = (float)fmax(tmin - t0, 0.0);
+    const float ltmax = tmax >= 3.0e38 ? __int_as_float(0x7f800000) : (float)(tmax - t0);
+    const HitRec h = trace_octet<ANY>(s, r, ltmin, ltmax, sref, st, id.sub, id.lane_base);
+    if (h.slot < 0) return false;
+    t_out = t0 + (double)h.t;
+    u = h.u;
+    v = h.v;
+    slot = h.slot;
+    return true;
+}
+
+// Hit point from barycentrics in FP64 (src/crt/passes.hpp:53-58).
+__device__ inline void bary_point(const SceneView& s, int slot, float u, float v, double p[3]) {
+    const float4 a = s.tris[3 * slot], b = s.tris[3 * slot + 1], c = s.tris[3 * slot + 2];
+    const double du = u, dv = v, dw = 1.0 - du - dv;
+    p[0] = du * b.x + dv * c.x + dw * a.x;
+    p[1] = du * b.y + dv * c.y + dw * a.y;
+    p[2] = du * b.z + dv * c.z + dw * a.z;
+}
+
+// Normal pass value (src/crt/passes.hpp:176-181): flat = normalize((p0-p1)x(p2-p0))
+// (inward for CCW-outward meshes), smooth = normalize(u vn1 + v vn2 + w vn0).
+__device__ inline void hit_normal(const SceneView& s, int slot, float u, float v, double n[3]) {
+    const float4 a = s.tris[3 * slot], b = s.tris[3 * slot + 1], c = s.tris[3 * slot + 2];
+    const uint32_t orig = __float_as_uint(a.w);
+    const uint32_t inst = s.tri_inst[orig];
+    const bool smooth = s.vnormals != nullptr && inst >= 1 && inst <= s.n_instances && s.inst_smooth[inst - 1];
+    double x, y, z;
+    if (smooth) {
+        const uint32_t i0 = s.tri_vidx[3 * orig], i1 = s.tri_vidx[3 * orig + 1], i2 = s.tri_vidx[3 * orig + 2];
+        const double du = u, dv = v, dw = 1.0 - du - dv;
+        x = du * s.vnormals[3 * i1] + dv * s.vnormals[3 * i2] + dw * s.vnormals[3 * i0];
+        y = du * s.vnormals[3 * i1 + 1] + dv * s.vnormals[3 * i2 + 1] + dw * s.vnormals[3 * i0 + 1];
+        z = du * s.vnormals[3 * i1 + 2] + dv * s.vnormals[3 * i2 + 2] + dw * s.vnormals[3 * i0 + 2];
+    } else {
+        const double e1x = (double)a.x - b.x, e1y = (double)a.y - b.y, e1z = (double)a.z - b.z;
+        const double e2x = (double)c.x - a.x, e2y = (double)c.y - a.y, e2z = (double)c.z - a.z;
+        x = e1y * e2z - e1z * e2y;
+        y = e1z * e2x - e1x * e2z;
+        z = e1x * e2y - e1y * e2x;
+    }
+    const double inv = 1.0 / sqrt(x * x + y * y + z * z);
+    n[0] = x * inv; n[1] = y * inv; n[2] = z * inv;
+}
+
+// Pixel of octet `oct` in block `blk`: blocks tile the image in 8x4 tiles.
+__device__ inline void tile_pixel(int blk, int oct, int w, int& px, int& py) {
+    const int tiles_x = (w + 7) >> 3;
+    const int tx = blk % tiles_x, ty = blk / tiles_x;
+    px = tx * 8 + (oct & 7);
+    py = ty * 4 + (oct >> 3);
+}
+
+struct PassOut {
+    double* xyz;
+    double* normal;
+    uint32_t* instance;
+    double* depth;
+    int32_t* tri;
+};
+
+__global__ __launch_bounds__(TRACE_BLOCK) void k_camera_pass(SceneView s, CamView cam, PassOut out) {
+    extern __shared__ uint32_t smem[];
+    const OctetId id = octet_id();
+    uint32_t* sref = smem + (size_t)id.local * s.stack_cap;
+    float* st = reinterpret_cast<float*>(smem + (size_t)OCTETS_PER_BLOCK * s.stack_cap) + (size_t)id.local * s.stack_cap;
+    int px, py;
+    tile_pixel(blockIdx.x, id.local, cam.w, px, py);
+    if (px >= cam.w || py >= cam.h) return;   // octet-uniform
+    double d[3];
+    // integer pixel coordinates, like src/crt/passes.hpp:46
+    camera_dir(cam, (double)px, (double)py, d);
+    double t;
+    float u = 0.f, v = 0.f;
+    int slot = -1;
+    const bool hit = trace_f64<false>(s, cam.pos, d, 0.0, 1.0e300, sref, st, id, t, u, v, slot);
+    const size_t pix = (size_t)py * cam.w + px;
+    if (id.sub == 0) {
+        if (out.instance) out.instance[pix] = hit ? s.tri_inst[__float_as_uint(s.tris[3 * slot].w)] : 0u;
+        if (out.tri) out.tri[pix] = hit ? (int32_t)__float_as_uint(s.tris[3 * slot].w) : -1;
+    }
+    if (out.xyz || out.depth) {
+        double p[3] = {0.0, 0.0, 0.0};
+        if (hit) bary_point(s, slot, u, v, p);
+        if (out.xyz && id.sub < 3) out.xyz[3 * pix + id.sub] = p[id.sub == 0 ? 0 : (id.sub == 1 ? 1 : 2)];
+        if (out.depth && id.sub == 3) {
+            double dd = 0.0;
+            if (hit) {
+                const double dx = p[0] - cam.pos[0], dy = p[1] - cam.pos[1], dz = p[2] - cam.pos[2];
+                dd = sqrt(dx * dx + dy * dy + dz * dz);
+            }
+            out.depth[pix] = dd;
+        }
+    }
+    if (out.normal) {
+        double n[3] = {0.0, 0.0, 0.0};
+        if (hit) hit_normal(s, slot, u, v, n);
+        if (id.sub >= 4 && id.sub < 7) out.normal[3 * pix + (id.sub - 4)] = n[id.sub - 4];
+    }
+}
+
+struct RayIn {
+    const double* o;
+    const double* d;
+    const double* tmin;
+    const double* tmax;
+};
+struct RayOut {
+    double* t;
+    int32_t* tri;
+    double* u;
+    double* v;
+};
+
+template <bool ANY>
+__global__ __launch_bounds__(TRACE_BLOCK) void k_trace_rays(SceneView s, RayIn in, RayOut out, uint64_t n) {
+    extern __shared__ uint32_t smem[];
+    const OctetId id = octet_id();
+    uint32_t* sref = smem + (size_t)id.local * s.stack_cap;
+    float* st = reinterpret_cast<float*>(smem + (size_t)OCTETS_PER_BLOCK * s.stack_cap) + (size_t)id.local * s.stack_cap;
+    const uint64_t ray = (uint64_t)blockIdx.x * OCTETS_PER_BLOCK + id.local;
+    if (ray >= n) return;
+    double o[3] = {in.o[3 * ray], in.o[3 * ray + 1], in.o[3 * ray + 2]};
+    double d[3] = {in.d[3 * ray], in.d[3 * ray + 1], in.d[3 * ray + 2]};
+    const double len = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    double tmin = in.tmin ? in.tmin[ray] : 0.0;
+    double tmax = in.tmax ? in.tmax[ray] : 1.0e300;
+    bool hit = false;
+    double t = 0.0;
+    float u = 0.f, v = 0.f;
+    int slot = -1;
+    if (len > 0.0 && !(tmax < tmin)) {
+        d[0] /= len; d[1] /= len; d[2] /= len;
+        // distances along the unit direction
+        tmin *= len;
+        tmax = tmax >= 1.0e300 ? tmax : tmax * len;
+        hit = trace_f64<ANY>(s, o, d, tmin, tmax, sref, st, id, t, u, v, slot);
+        if (hit) t /= len;
+    }
+    if (id.sub == 0) {
+        if (out.t) out.t[ray] = hit ? t : -1.0;
+        if (out.tri) out.tri[ray] = hit ? (int32_t)__float_as_uint(s.tris[3 * slot].w) : -1;
+        if (out.u) out.u[ray] = hit ? (double)u : 0.0;
+        if (out.v) out.v[ray] = hit ? (double)v : 0.0;
+    }
+}
+
+// ---- primary + sun shadow (wavefront: closest hit -> compacted shadow queue -> any hit)
+struct ShadowRay {
+    float4 o;       // origin xyz, pixel index (bits)
+};
+
+__global__ __launch_bounds__(TRACE_BLOCK) void k_primary_sun(SceneView s, CamView cam, float3 sun, double* depth,
+                                                             int32_t* tri, uint8_t* lit, ShadowRay* queue,
+                                                             uint32_t* qcount, float offset) {
+    extern __shared__ uint32_t smem[];
+    const OctetId id = octet_id();
+    uint32_t* sref = smem + (size_t)id.local * s.stack_cap;
+    float* st = reinterpret_cast<float*>(smem + (size_t)OCTETS_PER_BLOCK * s.stack_cap) + (size_t)id.local * s.stack_cap;
+    int px, py;
+    tile_pixel(blockIdx.x, id.local, cam.w, px, py);
+    const bool active = px < cam.w && py < cam.h;
+    bool hit = false;
+    double p[3] = {0.0, 0.0, 0.0};
+    float nx = 0.f, ny = 0.f, nz = 0.f;
+    size_t pix = 0;
+    if (active) {
+        double d[3];
+        camera_dir(cam, (double)px, (double)py, d);
+        double t;
+        float u = 0.f, v = 0.f;
+        int slot = -1;
+        hit = trace_f64<false>(s, cam.pos, d, 0.0, 1.0e300, sref, st, id, t, u, v, slot);
+        pix = (size_t)py * cam.w + px;
+        if (hit) {
+            bary_point(s, slot, u, v, p);
+            double n[3];
+            hit_normal(s, slot, u, v, n);
+            nx = (float)n[0]; ny = (float)n[1]; nz = (float)n[2];
+        }
+        if (id.sub == 0) {
+            if (depth) {
+                double dd = 0.0;
+                if (hit) {
+                    const double dx = p[0] - cam.pos[0], dy = p[1] - cam.pos[1], dz = p[2] - cam.pos[2];
+                    dd = sqrt(dx * dx + dy * dy + dz * dz);
+                }
+                depth[pix] = dd;
+            }
+            if (tri) tri[pix] = hit ? (int32_t)__float_as_uint(s.tris[3 * slot].w) : -1;
+            if (lit) lit[pix] = 0;
+        }
+    }
+    // one shadow ray per hit, compacted with a wave ballot + one atomic per wave
+    // (lane sub==0 of each octet represents its ray)
+    const bool emit = active && hit && id.sub == 0;
+    const uint64_t m = __ballot(emit);
+    const int lane = threadIdx.x & 63;
+    uint32_t base = 0;
+    const int leader = __ffsll((long long)m) - 1;
+    if (m != 0ull && lane == leader) base = atomicAdd(qcount, (uint32_t)__popcll(m));
+    base = __shfl(base, leader < 0 ? 0 : leader, 64);
+    if (emit) {
+        const uint32_t slot_q = base + (uint32_t)__popcll(m & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
+        // offset off the surface along the outward normal (the geometric normal is inward)
+        const float ox = (float)p[0] - offset * nx, oy = (float)p[1] - offset * ny, oz = (float)p[2] - offset * nz;
+        queue[slot_q].o = make_float4(ox, oy, oz, __uint_as_float((uint32_t)pix));
+    }
+    (void)sun;
+}
+
+__global__ __launch_bounds__(TRACE_BLOCK) void k_shadow_sun(SceneView s, const ShadowRay* queue,
+                                                            const uint32_t* qcount, float3 sun, uint8_t* lit) {
+    extern __shared__ uint32_t smem[];
+    const OctetId id = octet_id();
+    uint32_t* sref = smem + (size_t)id.local * s.stack_cap;
+    float* st = reinterpret_cast<float*>(smem + (size_t)OCTETS_PER_BLOCK * s.stack_cap) + (size_t)id.local * s.stack_cap;
+    const uint32_t n = *qcount;
+    const uint32_t q = blockIdx.x * OCTETS_PER_BLOCK + id.local;
+    if (q >= n) return;
+    const float4 o = queue[q].o;
+    const RayPre r = make_ray_pre(o.x, o.y, o.z, sun.x, sun.y, sun.z);
+    const HitRec h = trace_octet<true>(s, r, 0.0f, __int_as_float(0x7f800000), sref, st, id.sub, id.lane_base);
+    if (id.sub == 0 && lit) lit[__float_as_uint(o.w)] = h.slot < 0 ? 1 : 0;
+}
+
+struct EventTimer {
+    hipEvent_t a = nullptr, b = nullptr, c = nullptr, d = nullptr;
+    EventTimer() {
+        CRT_HIP_CHECK(hipEventCreate(&a));
+        CRT_HIP_CHECK(hipEventCreate(&b));
+        CRT_HIP_CHECK(hipEventCreate(&c));
+        CRT_HIP_CHECK(hipEventCreate(&d));
+    }
+    ~EventTimer() {
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        (void)hipEventDestroy(c);
+        (void)hipEventDestroy(d);
+    }
+    static double ms(hipEvent_t x, hipEvent_t y) {
+        float f = 0.f;
+        CRT_HIP_CHECK(hipEventElapsedTime(&f, x, y));
+        return f;
+    }
+};
+
+// Device output staging: use the caller's device pointer directly, or a
+// workspace buffer followed by a copy to the host pointer.
+struct OutSlot {
+    void* host = nullptr;
+    void* dev = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace
+
+// ---- host helpers shared with other translation units ----
+void scene_make_view(crt_scene* s);
+
+}  // namespace crt
+
+using namespace crt;
+
+void crt::scene_make_view(crt_scene* s) {
+    SceneView& v = s->view;
+    v.nodes = s->bvh.nodes.as<float4>();
+    v.tris = s->bvh.tris.as<float4>();
+    v.tri_vidx = s->tri_vidx.as<uint32_t>();
+    v.vnormals = s->vnormals.p ? s->vnormals.as<float>() : nullptr;
+    v.tri_inst = s->tri_inst.as<uint32_t>();
+    v.inst_color = s->inst_color.as<float>();
+    v.inst_smooth = s->inst_smooth.as<uint8_t>();
+    v.n_tris = s->n_tris;
+    v.n_instances = s->n_instances;
+    v.stack_cap = std::max<uint32_t>(8u, 7u * s->bvh.max_depth + 1u);
+    const double* bb = s->info.bounds;
+    const double cx = 0.5 * (bb[0] + bb[3]), cy = 0.5 * (bb[1] + bb[4]), cz = 0.5 * (bb[2] + bb[5]);
+    const double hx = 0.5 * (bb[3] - bb[0]), hy = 0.5 * (bb[4] - bb[1]), hz = 0.5 * (bb[5] - bb[2]);
+    double r = std::sqrt(hx * hx + hy * hy + hz * hz);
+    r = r * (1.0 + 1e-3) + 1e-6 * (std::fabs(cx) + std::fabs(cy) + std::fabs(cz) + r) + 1e-9;
+    v.center[0] = cx; v.center[1] = cy; v.center[2] = cz;
+    v.radius = r;
+    s->info.sphere[0] = cx; s->info.sphere[1] = cy; s->info.sphere[2] = cz; s->info.sphere[3] = r;
+    s->info.stack_capacity = v.stack_cap;
+}
+
+extern "C" {
+
+int crt_version(void) { return 100; }
+
+const char* crt_last_error(void) { return g_err.c_str(); }
+
+int crt_device_init(int device) {
+    return guarded([&] {
+        int n = 0;
+        CRT_HIP_CHECK(hipGetDeviceCount(&n));
+        CRT_REQUIRE(n > 0, CRT_ERR_HIP, "no HIP device available");
+        CRT_REQUIRE(device >= 0 && device < n, CRT_ERR_INVALID, "device index out of range");
+        CRT_HIP_CHECK(hipSetDevice(device));
+    });
+}
+
+int crt_scene_create(const float* vertices, uint32_t n_vertices, const uint32_t* indices, uint32_t n_tris,
+                     const uint32_t* tri_instance, const float* vertex_normals, uint32_t n_instances,
+                     const float* instance_color, const uint8_t* instance_smooth, const crt_build_opts* opts,
+                     crt_scene** out_scene) {
+    return guarded([&] {
+        CRT_REQUIRE(out_scene != nullptr, CRT_ERR_INVALID, "out_scene is NULL");
+        *out_scene = nullptr;
+        CRT_REQUIRE(vertices && indices, CRT_ERR_INVALID, "vertices/indices are NULL");
+        CRT_REQUIRE(n_tris > 0, CRT_ERR_INVALID, "scene has no triangles");
+        CRT_REQUIRE(n_vertices > 0, CRT_ERR_INVALID, "scene has no vertices");
+        for (uint64_t i = 0; i < 3ull * n_tris; ++i)
+            CRT_REQUIRE(indices[i] < n_vertices, CRT_ERR_INVALID, "triangle index out of range");
+        for (uint64_t i = 0; i < 3ull * n_vertices; ++i)
+            CRT_REQUIRE(std::isfinite(vertices[i]), CRT_ERR_INVALID, "vertex coordinate is not finite");
+        const uint32_t ni = std::max(n_instances, 1u);
+        if (tri_instance)
+            for (uint32_t i = 0; i < n_tris; ++i)
+                CRT_REQUIRE(tri_instance[i] >= 1 && tri_instance[i] <= ni, CRT_ERR_INVALID,
+                            "triangle instance id out of range 1..n_instances");
+        std::unique_ptr<crt_scene> s(new crt_scene());
+        CRT_HIP_CHECK(hipGetDevice(&s->device));
+        s->n_tris = n_tris;
+        s->n_vertices = n_vertices;
+        s->n_instances = ni;
+        s->vertices.alloc((size_t)n_vertices * 12);
+        s->indices.alloc((size_t)n_tris * 12);
+        s->tri_vidx.alloc((size_t)n_tris * 12);
+        s->tri_inst.alloc((size_t)n_tris * 4);
+        s->inst_color.alloc((size_t)ni * 12);
+        s->inst_smooth.alloc((size_t)ni);
+        CRT_HIP_CHECK(hipMemcpy(s->vertices.p, vertices, (size_t)n_vertices * 12, hipMemcpyHostToDevice));
+        CRT_HIP_CHECK(hipMemcpy(s->indices.p, indices, (size_t)n_tris * 12, hipMemcpyHostToDevice));
+        CRT_HIP_CHECK(hipMemcpy(s->tri_vidx.p, indices, (size_t)n_tris * 12, hipMemcpyHostToDevice));
+        if (tri_instance) {
+            CRT_HIP_CHECK(hipMemcpy(s->tri_inst.p, tri_instance, (size_t)n_tris * 4, hipMemcpyHostToDevice));
+        } else {
+            std::vector<uint32_t> ones(n_tris, 1u);
+            CRT_HIP_CHECK(hipMemcpy(s->tri_inst.p, ones.data(), (size_t)n_tris * 4, hipMemcpyHostToDevice));
+        }
+        std::vector<float> col((size_t)ni * 3, 1.0f);
+        if (instance_color) std::memcpy(col.data(), instance_color, (size_t)n_instances * 12);
+        CRT_HIP_CHECK(hipMemcpy(s->inst_color.p, col.data(), col.size() * 4, hipMemcpyHostToDevice));
+        std::vector<uint8_t> sm(ni, 0);
+        if (instance_smooth) std::memcpy(sm.data(), instance_smooth, n_instances);
+        CRT_HIP_CHECK(hipMemcpy(s->inst_smooth.p, sm.data(), sm.size(), hipMemcpyHostToDevice));
+        if (vertex_normals) {
+            s->vnormals.alloc((size_t)n_vertices * 12);
+            CRT_HIP_CHECK(hipMemcpy(s->vnormals.p, vertex_normals, (size_t)n_vertices * 12, hipMemcpyHostToDevice));
+        }
+        const uint32_t max_leaf = opts && opts->max_leaf_size ? opts->max_leaf_size : 8u;
+        CRT_REQUIRE(max_leaf <= 8u, CRT_ERR_INVALID, "max_leaf_size must be in 1..8");
+        build_bvh(s->vertices.as<float>(), s->indices.as<uint32_t>(), n_tris, max_leaf, nullptr, s->bvh);
+        crt_bvh_info& inf = s->info;
+        inf.n_tris = n_tris;
+        inf.n_wide_nodes = s->bvh.n_nodes;
+        inf.n_leaves = s->bvh.n_leaves;
+        inf.max_depth = s->bvh.max_depth;
+        inf.build_ms = s->bvh.build_ms;
+        inf.sah_cost = s->bvh.sah_cost;
+        for (int k = 0; k < 6; ++k) inf.bounds[k] = s->bvh.bounds[k];   // min xyz, max xyz
+        inf.device_bytes = s->vertices.bytes + s->indices.bytes + s->tri_vidx.bytes + s->tri_inst.bytes +
+                           s->inst_color.bytes + s->inst_smooth.bytes + s->vnormals.bytes + s->bvh.nodes.bytes +
+                           s->bvh.tris.bytes;
+        scene_make_view(s.get());
+        CRT_REQUIRE(stack_lds_bytes(s->view) <= 64u * 1024u, CRT_ERR_INVALID,
+                    "BVH too deep for the LDS traversal stack");
+        *out_scene = s.release();
+    });
+}
+
+int crt_scene_info(const crt_scene* scene, crt_bvh_info* info) {
+    return guarded([&] {
+        CRT_REQUIRE(scene && info, CRT_ERR_INVALID, "NULL argument");
+        *info = scene->info;
+    });
+}
+
+void crt_scene_destroy(crt_scene* scene) { delete scene; }
+
+int crt_pass(crt_scene* scene, const crt_camera* camera, double* xyz, double* normal, uint32_t* instance,
+             double* depth, int32_t* tri_id, uint32_t flags, void* stream, crt_timing* timing) {
+    return guarded([&] {
+        CRT_REQUIRE(scene != nullptr, CRT_ERR_INVALID, "scene is NULL");
+        check_camera(camera);
+        hipStream_t st = (hipStream_t)stream;
+        const CamView cam = make_cam(*camera);
+        const size_t npix = (size_t)cam.w * cam.h;
+        const bool dev = (flags & CRT_OUT_DEVICE) != 0;
+        Workspace& ws = scene->ws;
+        PassOut o{nullptr, nullptr, nullptr, nullptr, nullptr};
+        if (xyz) { if (dev) o.xyz = xyz; else { ws.a.reserve(npix * 24); o.xyz = ws.a.as<double>(); } }
+        if (normal) { if (dev) o.normal = normal; else { ws.b.reserve(npix * 24); o.normal = ws.b.as<double>(); } }
+        if (instance) { if (dev) o.instance = instance; else { ws.c.reserve(npix * 4); o.instance = ws.c.as<uint32_t>(); } }
+        if (depth) { if (dev) o.depth = depth; else { ws.d.reserve(npix * 8); o.depth = ws.d.as<double>(); } }
+        if (tri_id) { if (dev) o.tri = tri_id; else { ws.e.reserve(npix * 4); o.tri = ws.e.as<int32_t>(); } }
+        EventTimer tm;
+        const uint32_t blocks = cdiv((uint64_t)cam.w, 8) * cdiv((uint64_t)cam.h, 4);
+        CRT_HIP_CHECK(hipEventRecord(tm.a, st));
+        hipLaunchKernelGGL(k_camera_pass, dim3(blocks), dim3(TRACE_BLOCK), stack_lds_bytes(scene->view), st,
+                           scene->view, cam, o);
+        CRT_HIP_CHECK(hipGetLastError());
+        CRT_HIP_CHECK(hipEventRecord(tm.b, st));
+        if (!dev) {
+            if (xyz) CRT_HIP_CHECK(hipMemcpyAsync(xyz, o.xyz, npix * 24, hipMemcpyDeviceToHost, st));
+            if (normal) CRT_HIP_CHECK(hipMemcpyAsync(normal, o.normal, npix * 24, hipMemcpyDeviceToHost, st));
+            if (instance) CRT_HIP_CHECK(hipMemcpyAsync(instance, o.instance, npix * 4, hipMemcpyDeviceToHost, st));
+            if (depth) CRT_HIP_CHECK(hipMemcpyAsync(depth, o.depth, npix * 8, hipMemcpyDeviceToHost, st));
+            if (tri_id) CRT_HIP_CHECK(hipMemcpyAsync(tri_id, o.tri, npix * 4, hipMemcpyDeviceToHost, st));
+        }
+        CRT_HIP_CHECK(hipStreamSynchronize(st));
+        if (timing) {
+            std::memset(timing, 0, sizeof(*timing));
+            timing->trace_ms = EventTimer::ms(tm.a, tm.b);
+            timing->total_ms = timing->trace_ms;
+            timing->rays_traced = npix;
+        }
+    });
+}
+
+int crt_trace_rays(crt_scene* scene, uint64_t n, const double* origins, const double* dirs, const double* tmin,
+                   const double* tmax, double* t, int32_t* tri, double* u, double* v, uint32_t flags, void* stream,
+                   crt_timing* timing) {
+    return guarded([&] {
+        CRT_REQUIRE(scene != nullptr, CRT_ERR_INVALID, "scene is NULL");
+        CRT_REQUIRE(origins && dirs, CRT_ERR_INVALID, "origins/dirs are NULL");
+        if (n == 0) return;
+        CRT_REQUIRE(n <= (1ull << 31), CRT_ERR_INVALID, "too many rays in one call");
+        hipStream_t st = (hipStream_t)stream;
+        const bool dev = (flags & CRT_OUT_DEVICE) != 0;
+        const bool any = (flags & CRT_ANY_HIT) != 0;
+        Workspace& ws = scene->ws;
+        RayIn in{origins, dirs, tmin, tmax};
+        RayOut out{t, tri, u, v};
+        if (!dev) {
+            // stage inputs: o(3n) d(3n) tmin(n) tmax(n) in ws.f; outputs in ws.g
+            const size_t in_bytes = n * 8 * 8;
+            ws.f.reserve(in_bytes);
+            double* base = ws.f.as<double>();
+            CRT_HIP_CHECK(hipMemcpyAsync(base, origins, n * 24, hipMemcpyHostToDevice, st));
+            CRT_HIP_CHECK(hipMemcpyAsync(base + 3 * n, dirs, n * 24, hipMemcpyHostToDevice, st));
+            in.o = base;
+            in.d = base + 3 * n;
+            if (tmin) { CRT_HIP_CHECK(hipMemcpyAsync(base + 6 * n, tmin, n * 8, hipMemcpyHostToDevice, st)); in.tmin = base + 6 * n; }
+            if (tmax) { CRT_HIP_CHECK(hipMemcpyAsync(base + 7 * n, tmax, n * 8, hipMemcpyHostToDevice, st)); in.tmax = base + 7 * n; }
+            ws.g.reserve(n * 28);
+            char* ob = ws.g.as<char>();
+            out.t = t ? reinterpret_cast<double*>(ob) : nullptr;
+            out.u = u ? reinterpret_cast<double*>(ob + n * 8) : nullptr;
+            out.v = v ? reinterpret_cast<double*>(ob + n * 16) : nullptr;
+            out.tri = tri ? reinterpret_cast<int32_t*>(ob + n * 24) : nullptr;
+        }
+        EventTimer tm;
+        const uint32_t blocks = cdiv(n, OCTETS_PER_BLOCK);
+        CRT_HIP_CHECK(hipEventRecord(tm.a, st));
+        if (any)
+            hipLaunchKernelGGL(k_trace_rays<true>, dim3(blocks), dim3(TRACE_BLOCK), stack_lds_bytes(scene->view), st,
+                               scene->view, in, out, n);
+        else
+            hipLaunchKernelGGL(k_trace_rays<false>, dim3(blocks), dim3(TRACE_BLOCK), stack_lds_bytes(scene->view), st,
+                               scene->view, in, out, n);
+        CRT_HIP_CHECK(hipGetLastError());
+        CRT_HIP_CHECK(hipEventRecord(tm.b, st));
+        if (!dev) {
+            if (t) CRT_HIP_CHECK(hipMemcpyAsync(t, out.t, n * 8, hipMemcpyDeviceToHost, st));
+            if (u) CRT_HIP_CHECK(hipMemcpyAsync(u, out.u, n * 8, hipMemcpyDeviceToHost, st));
+            if (v) CRT_HIP_CHECK(hipMemcpyAsync(v, out.v, n * 8, hipMemcpyDeviceToHost, st));
+            if (tri) CRT_HIP_CHECK(hipMemcpyAsync(tri, out.tri, n * 4, hipMemcpyDeviceToHost, st));
+        }
+        CRT_HIP_CHECK(hipStreamSynchronize(st));
+        if (timing) {
+            std::memset(timing, 0, sizeof(*timing));
+            const double ms = EventTimer::ms(tm.a, tm.b);
+            if (any) { timing->shadow_ms = ms; timing->shadow_rays = n; }
+            else { timing->trace_ms = ms; timing->rays_traced = n; }
+            timing->total_ms = ms;
+        }
+    });
+}
+
+int crt_sun_pass(crt_scene* scene, const crt_camera* camera, const double sun_dir[3], double* depth, int32_t* tri_id,
+                 uint8_t* lit, uint32_t flags, void* stream, crt_timing* timing) {
+    return guarded([&] {
+        CRT_REQUIRE(scene != nullptr, CRT_ERR_INVALID, "scene is NULL");
+        check_camera(camera);
+        CRT_REQUIRE(sun_dir != nullptr, CRT_ERR_INVALID, "sun_dir is NULL");
+        const double sl = std::sqrt(sun_dir[0] * sun_dir[0] + sun_dir[1] * sun_dir[1] + sun_dir[2] * sun_dir[2]);
+        CRT_REQUIRE(sl > 0 && std::isfinite(sl), CRT_ERR_INVALID, "sun_dir must be a finite non-zero vector");
+        const float3 sun = make_float3((float)(sun_dir[0] / sl), (float)(sun_dir[1] / sl), (float)(sun_dir[2] / sl));
+        hipStream_t st = (hipStream_t)stream;
+        const CamView cam = make_cam(*camera);
+        const size_t npix = (size_t)cam.w * cam.h;
+        const bool dev = (flags & CRT_OUT_DEVICE) != 0;
+        Workspace& ws = scene->ws;
+        double* d_depth = depth;
+        int32_t* d_tri = tri_id;
+        uint8_t* d_lit = lit;
+        if (!dev) {
+            if (depth) { ws.d.reserve(npix * 8); d_depth = ws.d.as<double>(); }
+            if (tri_id) { ws.e.reserve(npix * 4); d_tri = ws.e.as<int32_t>(); }
+            if (lit) { ws.c.reserve(npix); d_lit = ws.c.as<uint8_t>(); }
+        }
+        ws.h.reserve(npix * sizeof(ShadowRay));
+        ws.counters.reserve(256);
+        uint32_t* qcount = ws.counters.as<uint32_t>();
+        // scale-aware self-intersection offset: reference uses 1e-4 absolute
+        // (src/crt/path_tracing/unidirectional.hpp:61); FP32 needs >= a few ulp
+        const double ext = std::max({std::fabs(scene->info.bounds[0]), std::fabs(scene->info.bounds[1]),
+                                     std::fabs(scene->info.bounds[2]), std::fabs(scene->info.bounds[3]),
+                                     std::fabs(scene->info.bounds[4]), std::fabs(scene->info.bounds[5])});
+        const float offset = (float)std::max(1e-4, 1e-6 * ext);
+        EventTimer tm;
+        const uint32_t blocks = cdiv((uint64_t)cam.w, 8) * cdiv((uint64_t)cam.h, 4);
+        CRT_HIP_CHECK(hipMemsetAsync(qcount, 0, 4, st));
+        CRT_HIP_CHECK(hipEventRecord(tm.a, st));
+        hipLaunchKernelGGL(k_primary_sun, dim3(blocks), dim3(TRACE_BLOCK), stack_lds_bytes(scene->view), st,
+                           scene->view, cam, sun, d_depth, d_tri, d_lit, ws.h.as<ShadowRay>(), qcount, offset);
+        CRT_HIP_CHECK(hipEventRecord(tm.b, st));
+        hipLaunchKernelGGL(k_shadow_sun, dim3(cdiv(npix, OCTETS_PER_BLOCK)), dim3(TRACE_BLOCK),
+                           stack_lds_bytes(scene->view), st, scene->view, ws.h.as<ShadowRay>(), qcount, sun, d_lit);
+        CRT_HIP_CHECK(hipGetLastError());
+        CRT_HIP_CHECK(hipEventRecord(tm.c, st));
+        uint32_t nshadow = 0;
+        CRT_HIP_CHECK(hipMemcpyAsync(&nshadow, qcount, 4, hipMemcpyDeviceToHost, st));
+        if (!dev) {
+            if (depth) CRT_HIP_CHECK(hipMemcpyAsync(depth, d_depth, npix * 8, hipMemcpyDeviceToHost, st));
+            if (tri_id) CRT_HIP_CHECK(hipMemcpyAsync(tri_id, d_tri, npix * 4, hipMemcpyDeviceToHost, st));
+            if (lit) CRT_HIP_CHECK(hipMemcpyAsync(lit, d_lit, npix, hipMemcpyDeviceToHost, st));
+        }
+        CRT_HIP_CHECK(hipStreamSynchronize(st));
+        if (timing) {
+            std::memset(timing, 0, sizeof(*timing));
+            timing->trace_ms = EventTimer::ms(tm.a, tm.b);
+            timing->shadow_ms = EventTimer::ms(tm.b, tm.c);
+            timing->total_ms = EventTimer::ms(tm.a, tm.c);
+            timing->rays_traced = npix;
+            timing->shadow_rays = nshadow;
+        }
+    });
+}
+
+}  // extern "C"

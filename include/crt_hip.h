@@ -1,0 +1,190 @@
+/*
+ * crt_hip.h -- C ABI of the MI355X ray-tracing core (libcrt_hip.so).
+ *
+ * Plain C: POD structs, raw pointers and sizes, int status codes.  Every
+ * entry point returns CRT_OK (0) or an error code; crt_last_error() returns a
+ * thread-local message for the last failure.  No entry point aborts the
+ * process on bad input.
+ *
+ * Precision contract: poses, camera rays and outputs are FP64 at this
+ * boundary.  Inside, each ray is generated in FP64, its origin is moved in
+ * FP64 to the entry point of the scene's bounding sphere ("rebasing"), and
+ * only then cast to FP32 for traversal and the watertight triangle test.
+ * Distances are returned as FP64 (rebase distance + FP32 local distance).
+ *
+ * Memory: unless a flag says otherwise, output pointers are HOST pointers and
+ * the call copies results back before returning.  With CRT_OUT_DEVICE the
+ * output pointers are device pointers (e.g. from hipMalloc or a torch CUDA
+ * tensor) and the call only enqueues work on `stream` (hipStream_t, may be
+ * NULL for the null stream).
+ *
+ * Reference interfaces replaced (ceres-raytracer v0.6a4 paths):
+ *   crt_scene_create/build  <- BodyFixedGroup ctor: transform + SweepSAH + PRO + NLO
+ *                              (src/crt/rendering_body_fixed/body_fixed_group.hpp:45-89)
+ *   crt_pass                <- get_inetersections / get_instances / get_normals
+ *                              (src/crt/passes.hpp:13-198) via BodyFixedGroup::
+ *                              intersection_pass/instance_pass/normal_pass
+ *                              (body_fixed_group.hpp:111-124, src/pybind.cpp:527-578)
+ *   crt_trace_rays          <- SingleRayTraverser::traverse + Closest/AnyPrimitiveIntersector
+ *                              (lib/bvh/single_ray_traverser.hpp:145-163,
+ *                               lib/bvh/primitive_intersectors.hpp:33-76)
+ *   crt_lidar_batch         <- do_lidar / do_batch_lidar + SimpleLidar::cast_rays /
+ *                              batch_cast_rays (src/crt/do_lidar.hpp:13-155,
+ *                              src/crt/lidars/simple_lidar.hpp:42-96)
+ *   crt_render              <- do_render + unidirectional (src/crt/do_render.hpp:27-141,
+ *                              src/crt/path_tracing/unidirectional.hpp:11-96)
+ *   crt_flash_lidar, crt_radiation_pressure, crt_sun_pass: no reference
+ *                              counterpart (new, additive API).
+ */
+#ifndef CRT_HIP_H
+#define CRT_HIP_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define CRT_OK               0
+#define CRT_ERR_INVALID      1   /* bad argument (maps to ValueError)        */
+#define CRT_ERR_TYPE         2   /* unknown camera/light/lidar type          */
+#define CRT_ERR_OOM          3   /* device allocation failed (MemoryError)   */
+#define CRT_ERR_HIP          4   /* HIP runtime error (RuntimeError)         */
+#define CRT_ERR_STATE        5   /* scene not built / wrong call order       */
+#define CRT_ERR_ADMISSION    6   /* serving: request exceeds memory budget   */
+
+/* ---- flags ------------------------------------------------------------- */
+#define CRT_OUT_DEVICE       1u  /* output (and input ray) pointers are device pointers */
+#define CRT_ANY_HIT          2u  /* crt_trace_rays: stop at the first hit (shadow query) */
+
+/* ---- light types ------------------------------------------------------- */
+#define CRT_LIGHT_POINT        0 /* src/crt/lights/point_light.hpp        */
+#define CRT_LIGHT_AREA         1 /* src/crt/lights/area_light.hpp         */
+#define CRT_LIGHT_DIRECTIONAL  2 /* new: sun at infinity                  */
+
+typedef struct crt_scene crt_scene;   /* opaque: device mesh + BVH, resident in HBM */
+
+/* Pinhole camera (src/crt/cameras/simple_camera.hpp:14-27,61-81).
+ * rotation is row-major R; camera rays are rotated by R^T like the reference. */
+typedef struct {
+    double position[3];
+    double rotation[9];
+    double focal_length;
+    double resolution[2];     /* [x, y] = [W, H] */
+    double sensor_size[2];
+    int32_t z_positive;
+    int32_t pad_;
+} crt_camera;
+
+typedef struct {
+    int32_t type;             /* CRT_LIGHT_* */
+    int32_t pad_;
+    double intensity;
+    double position[3];
+    double rotation[9];       /* row-major R (area light samples use R, area_light.hpp:63) */
+    double size[2];           /* area light side lengths */
+    double direction[3];      /* directional: unit vector from the scene towards the light */
+} crt_light;
+
+typedef struct {
+    uint32_t max_leaf_size;   /* triangles per leaf, 1..8 (default 8)  */
+    uint32_t flags;           /* reserved, 0 */
+} crt_build_opts;
+
+typedef struct {
+    uint32_t n_tris;
+    uint32_t n_wide_nodes;
+    uint32_t n_leaves;
+    uint32_t max_depth;       /* wide levels, root = 1 */
+    uint32_t stack_capacity;  /* per-ray traversal stack entries */
+    uint32_t pad_;
+    double   build_ms;        /* device time of the whole build */
+    double   sah_cost;        /* SAH cost of the wide BVH (node cost 1, triangle cost 1) */
+    double   bounds[6];       /* xmin,ymin,zmin,xmax,ymax,zmax */
+    double   sphere[4];       /* bounding sphere centre xyz + radius used for rebasing */
+    uint64_t device_bytes;    /* HBM held by the scene (mesh + BVH) */
+} crt_bvh_info;
+
+/* Kernel timings of the last call, measured with hipEvents on the launch stream. */
+typedef struct {
+    double total_ms;          /* first launch start .. last launch end          */
+    double trace_ms;          /* dominant traversal kernel (closest hit)        */
+    double shadow_ms;         /* any-hit kernel, if any                         */
+    uint64_t rays_traced;     /* closest-hit rays                               */
+    uint64_t shadow_rays;     /* any-hit rays                                   */
+} crt_timing;
+
+typedef struct {
+    int32_t  min_samples;
+    int32_t  max_samples;
+    double   noise_threshold;
+    int32_t  num_bounces;
+    uint32_t pad_;
+    uint64_t seed;            /* counter-based RNG key: results depend only on
+                                 (seed, pixel, sample, bounce), not on batching */
+} crt_render_opts;
+
+/* ---- device / errors --------------------------------------------------- */
+int         crt_device_init(int device);
+const char* crt_last_error(void);
+int         crt_version(void);
+
+/* ---- scene ------------------------------------------------------------- */
+/* Upload a merged triangle mesh (all entities already transformed to the
+ * scene frame) and build the BVH on the GPU.
+ *   vertices        : n_vertices x 3 float
+ *   indices         : n_tris x 3 uint32 (vertex indices)
+ *   tri_instance    : n_tris uint32 instance ids (1-based, like src/pybind.cpp:110-120)
+ *   vertex_normals  : n_vertices x 3 float or NULL (then smooth shading falls back to flat)
+ *   instance_color  : n_instances x 3 float (RGB albedo), may be NULL (grey 1,1,1)
+ *   instance_smooth : n_instances uint8, may be NULL (flat)                              */
+int crt_scene_create(const float* vertices, uint32_t n_vertices,
+                     const uint32_t* indices, uint32_t n_tris,
+                     const uint32_t* tri_instance,
+                     const float* vertex_normals,
+                     uint32_t n_instances,
+                     const float* instance_color,
+                     const uint8_t* instance_smooth,
+                     const crt_build_opts* opts,
+                     crt_scene** out_scene);
+int  crt_scene_info(const crt_scene* scene, crt_bvh_info* info);
+void crt_scene_destroy(crt_scene* scene);
+
+/* ---- camera passes (src/crt/passes.hpp:13-198) --------------------------
+ * One closest-hit ray per pixel at integer pixel coordinates.  Outputs are
+ * row-major (H, W): xyz / normal f64 [H][W][3], instance u32 [H][W]; misses
+ * give zeros, like the reference.  Any output may be NULL.  `depth` (f64
+ * [H][W], distance camera->hit, 0 on miss) and `tri_id` (int32, -1 on miss)
+ * are additive outputs.                                                      */
+int crt_pass(crt_scene* scene, const crt_camera* camera,
+             double* xyz, double* normal, uint32_t* instance,
+             double* depth, int32_t* tri_id,
+             uint32_t flags, void* stream, crt_timing* timing);
+
+/* ---- general ray queries ----------------------------------------------- *
+ * n rays given in FP64: origins/dirs n x 3, tmin/tmax n (NULL -> 0 / +inf).
+ * Closest hit: t (in units of |dir|), tri (original triangle index, -1 miss),
+ * u, v (reference barycentrics: hit = u p1 + v p2 + (1-u-v) p0).  With
+ * CRT_ANY_HIT only `t` (any hit distance, -1 if none) and `tri` (>=0 if hit)
+ * are meaningful.  Any output may be NULL.                                    */
+int crt_trace_rays(crt_scene* scene, uint64_t n,
+                   const double* origins, const double* dirs,
+                   const double* tmin, const double* tmax,
+                   double* t, int32_t* tri, double* u, double* v,
+                   uint32_t flags, void* stream, crt_timing* timing);
+
+/* ---- primary + directional-sun shadow pass (new) ------------------------ *
+ * The camera's closest hits plus one any-hit shadow ray per hit towards the
+ * sun direction (unit vector from the scene towards the sun).  Shadow rays
+ * are compacted on the device.  Outputs (any may be NULL): depth f64 [H][W]
+ * (0 on miss), tri_id int32 [H][W], lit uint8 [H][W] (1 = hit and sunlit).  */
+int crt_sun_pass(crt_scene* scene, const crt_camera* camera, const double sun_dir[3],
+                 double* depth, int32_t* tri_id, uint8_t* lit,
+                 uint32_t flags, void* stream, crt_timing* timing);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CRT_HIP_H */

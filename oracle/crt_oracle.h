@@ -1,0 +1,67 @@
+/* crt_oracle.h -- FP64 CPU restatement of the ceres-raytracer hot path.
+ * TEST INFRASTRUCTURE ONLY (the checker): used by tests/, the smoke check and
+ * the CPU-baseline leg of bench.py; never by the product path. */
+#ifndef CRT_ORACLE_H
+#define CRT_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_scene orc_scene;
+
+typedef struct {
+    double position[3];
+    double rotation[9];      /* row-major R, rays rotated by R^T */
+    double focal_length;
+    double resolution[2];
+    double sensor_size[2];
+    int32_t z_positive;
+    int32_t pad_;
+} crt_oracle_camera;
+
+typedef struct {
+    int32_t type;            /* 0 point, 1 area, 2 directional */
+    int32_t pad_;
+    double intensity;
+    double position[3];
+    double rotation[9];
+    double size[2];
+    double direction[3];
+} crt_oracle_light;
+
+orc_scene* crt_oracle_scene_create(const float* vertices, int64_t nv, const uint32_t* indices, int64_t nt,
+                                   const uint32_t* tri_inst, const float* vnormals, int64_t ni,
+                                   const float* colors, const uint8_t* smooth);
+void crt_oracle_scene_destroy(orc_scene* s);
+
+int crt_oracle_tri_intersect(const double p0[3], const double p1[3], const double p2[3], const double o[3],
+                             const double d[3], double tmin, double tmax, double out[3]);
+int64_t crt_oracle_closest_brute(const orc_scene* s, const double o[3], const double d[3], double tmin, double tmax,
+                                 double out[3]);
+void crt_oracle_trace(const orc_scene* s, int64_t n, const double* o, const double* d, const double* tmin,
+                      const double* tmax, int any_hit, double* t, int32_t* tri, double* u, double* v);
+void crt_oracle_tie_flags(const orc_scene* s, int64_t n, const double* o, const double* d, double rel_t,
+                          double bary_eps, uint8_t* flags);
+void crt_oracle_edge_flags(const orc_scene* s, int64_t n, const double* o, const double* d, const int32_t* other,
+                           double rel_t, double bary_eps, uint8_t* flags);
+void crt_oracle_camera_rays(const crt_oracle_camera* c, double* o, double* d);
+void crt_oracle_pass(const orc_scene* s, const crt_oracle_camera* c, double* xyz, double* normal, uint32_t* instance,
+                     double* depth, int32_t* tri);
+void crt_oracle_lidar_batch(const orc_scene* s, int64_t n, const double* pos, const double* rot, int z_positive,
+                            double* range);
+double crt_oracle_u01(uint64_t seed, uint64_t a, uint64_t b, uint64_t c);
+void crt_oracle_render(const orc_scene* s, const crt_oracle_camera* c, const crt_oracle_light* lights, int nl,
+                       int min_samples, int max_samples, double noise_threshold, int num_bounces, uint64_t seed,
+                       uint64_t key_offset, float* rgb, int32_t* spp_used);
+void crt_oracle_radiation_pressure(const orc_scene* s, const double sun[3], double pressure, double rho_s,
+                                   double rho_d, int64_t grid, const double center[3], double radius,
+                                   const double ref[3], double force[3], double torque[3], int64_t* n_hit);
+void crt_oracle_vertex_normals(const float* vertices, int64_t nv, const uint32_t* indices, int64_t nt, float* out);
+int crt_oracle_num_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

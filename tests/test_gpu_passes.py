@@ -1,0 +1,175 @@
+"""GPU parity of the camera passes and general ray queries against the FP64 oracle.
+
+Tolerances (from the task statement for this path): triangle / instance ids
+bit-exact except for rays the oracle classifies as exact ties or edge hits
+(reported as a fraction); depth and hit points within 1e-5 relative.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from crt import capi, scenes
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-5
+
+
+def _cornell():
+    sc = scenes.cornell_box(0)
+    v, i, inst = sc.merged()
+    return sc, v, i, inst
+
+
+@pytest.fixture(scope="module")
+def cornell():
+    sc, v, i, inst = _cornell()
+    vn = oracle.vertex_normals(v, i)
+    gpu = capi.Scene(v, i, inst, vn, colors=np.array(sc.colors), smooth=np.array(sc.smooth, dtype=np.uint8))
+    cpu = oracle.Scene(v, i, inst, vn, colors=np.array(sc.colors), smooth=np.array(sc.smooth, dtype=np.uint8))
+    return gpu, cpu, v, i
+
+
+@pytest.fixture(scope="module")
+def icosphere():
+    m = scenes.displaced_icosphere(8)
+    gpu = capi.Scene(m.vertices, m.indices)
+    cpu = oracle.Scene(m.vertices, m.indices)
+    return gpu, cpu, m
+
+
+def _compare_pass(gpu_out, cpu_out, cpu, cam, max_mismatch_frac):
+    o, d = oracle.camera_rays(cam)
+    tri_g = gpu_out["tri"].ravel()
+    tri_c = cpu_out["tri"].ravel()
+    mism = tri_g != tri_c
+    ties = (cpu.tie_flags(o[mism], d[mism]) | cpu.edge_flags(o[mism], d[mism], tri_g[mism])) \
+        if mism.any() else np.zeros(0, bool)
+    hard = int(mism.sum() - ties.sum())
+    n = tri_c.size
+    print("pixels %d  mismatched ids %d  tie/edge %d  unexplained %d" % (n, mism.sum(), ties.sum(), hard))
+    assert hard <= max_mismatch_frac * n
+    same = ~mism
+    hit = same & (tri_c >= 0)
+    dg = gpu_out["depth"].ravel()[hit]
+    dc = cpu_out["depth"].ravel()[hit]
+    rel = np.abs(dg - dc) / np.maximum(dc, 1e-30)
+    print("depth max rel err %.3e" % (rel.max() if rel.size else 0))
+    assert rel.size == 0 or rel.max() < REL
+    xg = gpu_out["xyz"].reshape(-1, 3)[hit]
+    xc = cpu_out["xyz"].reshape(-1, 3)[hit]
+    err = np.linalg.norm(xg - xc, axis=1) / np.maximum(dc, 1e-30)
+    assert err.size == 0 or err.max() < REL
+    # misses are zeros, like the reference (src/crt/passes.hpp:60-63)
+    miss = same & (tri_c < 0)
+    assert np.all(gpu_out["xyz"].reshape(-1, 3)[miss] == 0.0)
+    assert np.all(gpu_out["depth"].ravel()[miss] == 0.0)
+    return mism.mean(), ties.mean() if ties.size else 0.0
+
+
+def test_cornell_passes_match_oracle(cornell):
+    gpu, cpu, v, i = cornell
+    cam = scenes.cornell_camera()
+    g = gpu.camera_pass(cam)
+    c = cpu.camera_pass(cam)
+    _compare_pass(g, c, cpu, cam, max_mismatch_frac=0.0)
+    same = g["tri"] == c["tri"]
+    assert np.array_equal(g["instance"][same], c["instance"][same])
+    hit = same & (c["tri"] >= 0)
+    assert np.max(np.abs(g["normal"][hit] - c["normal"][hit])) < 1e-6
+    assert np.all(g["instance"][c["tri"] < 0] == 0)
+
+
+def test_cornell_smooth_normals(cornell):
+    sc, v, i, inst = _cornell()
+    vn = oracle.vertex_normals(v, i)
+    smooth = np.ones(len(sc.meshes), dtype=np.uint8)
+    gpu = capi.Scene(v, i, inst, vn, colors=np.array(sc.colors), smooth=smooth)
+    cpu = oracle.Scene(v, i, inst, vn, colors=np.array(sc.colors), smooth=smooth)
+    cam = scenes.cornell_camera()
+    g = gpu.camera_pass(cam)
+    c = cpu.camera_pass(cam)
+    hit = (g["tri"] == c["tri"]) & (c["tri"] >= 0)
+    assert np.max(np.abs(g["normal"][hit] - c["normal"][hit])) < 1e-5
+
+
+def test_icosphere_primary_subsample(icosphere):
+    gpu, cpu, m = icosphere
+    cam = scenes.small_body_camera(m, resolution=512, seed=1)
+    g = gpu.camera_pass(cam)
+    c = cpu.camera_pass(cam)
+    frac, tie = _compare_pass(g, c, cpu, cam, max_mismatch_frac=1e-4)
+    print("icosphere 512^2: id mismatch fraction %.2e (tie/edge %.2e)" % (frac, tie))
+
+
+def test_trace_rays_closest_and_any(icosphere):
+    gpu, cpu, m = icosphere
+    rng = np.random.default_rng(7)
+    n = 20000
+    # rays from a shell at 3-6 km towards random points near the body
+    o = rng.normal(size=(n, 3))
+    o *= (rng.uniform(3000, 6000, size=n) / np.linalg.norm(o, axis=1))[:, None]
+    tgt = rng.normal(size=(n, 3)) * 800.0
+    d = tgt - o
+    d *= rng.uniform(0.5, 3.0, size=n)[:, None]     # non-unit directions: t is in units of |d|
+    tg, trg, ug, vg = gpu.trace(o, d)
+    tc, trc, uc, vc = cpu.trace(o, d)
+    mism = trg != trc
+    ties = (cpu.tie_flags(o[mism], d[mism]) | cpu.edge_flags(o[mism], d[mism], trg[mism])) \
+        if mism.any() else np.zeros(0, bool)
+    assert mism.sum() - ties.sum() <= 2
+    hit = (~mism) & (trc >= 0)
+    assert np.max(np.abs(tg[hit] - tc[hit]) / tc[hit]) < REL
+    assert np.max(np.abs(ug[hit] - uc[hit])) < 1e-3 and np.max(np.abs(vg[hit] - vc[hit])) < 1e-3
+    # any-hit with tmax just before / after the closest hit
+    sel = np.where(hit)[0][:2000]
+    tmax_short = tc[sel] * (1 - 1e-4)
+    tmax_long = tc[sel] * (1 + 1e-4)
+    _, a_short, _, _ = gpu.trace(o[sel], d[sel], tmax=tmax_short, any_hit=True)
+    _, a_long, _, _ = gpu.trace(o[sel], d[sel], tmax=tmax_long, any_hit=True)
+    _, c_short, _, _ = cpu.trace(o[sel], d[sel], tmax=tmax_short, any_hit=True)
+    assert np.all(a_long >= 0)
+    assert np.array_equal(a_short >= 0, c_short >= 0)
+
+
+def test_sun_pass_shadows(icosphere):
+    gpu, cpu, m = icosphere
+    cam = scenes.small_body_camera(m, resolution=256, seed=1)
+    sun = scenes.sun_direction(2)
+    r = gpu.sun_pass(cam, sun)
+    c = cpu.camera_pass(cam)
+    hit = c["tri"] >= 0
+    assert r["timing"]["shadow_rays"] == int((r["tri"] >= 0).sum())
+    # oracle shadow rays from the oracle hit points, offset like the GPU
+    p = c["xyz"].reshape(-1, 3)[hit.ravel()]
+    n = c["normal"].reshape(-1, 3)[hit.ravel()]
+    off = max(1e-4, 1e-6 * 2300.0)
+    o = p - off * n
+    dd = np.repeat(np.asarray(sun)[None, :], o.shape[0], axis=0)
+    _, occ, _, _ = cpu.trace(o, dd, any_hit=True)
+    lit_c = occ < 0
+    lit_g = r["lit"].ravel()[hit.ravel()].astype(bool)
+    same_id = r["tri"].ravel()[hit.ravel()] == c["tri"].ravel()[hit.ravel()]
+    disagree = (lit_c != lit_g) & same_id
+    print("shadow disagreement %d of %d" % (disagree.sum(), hit.sum()))
+    assert disagree.mean() < 2e-3
+    assert np.all(r["lit"][~hit] == 0)
+
+
+def test_empty_scene_rejected():
+    with pytest.raises(ValueError):
+        capi.Scene(np.zeros((0, 3), np.float32), np.zeros((0, 3), np.uint32))
+
+
+def test_bad_index_rejected():
+    with pytest.raises(ValueError):
+        capi.Scene(np.zeros((3, 3), np.float32), np.array([[0, 1, 5]], np.uint32))
+
+
+def test_single_triangle_scene():
+    v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    i = np.array([[0, 1, 2]], np.uint32)
+    s = capi.Scene(v, i)
+    t, tri, u, vv = s.trace(np.array([[0.25, 0.25, 1.0]]), np.array([[0.0, 0.0, -1.0]]))
+    assert tri[0] == 0 and abs(t[0] - 1.0) < 1e-6
+    assert abs(u[0] - 0.25) < 1e-6 and abs(vv[0] - 0.25) < 1e-6
