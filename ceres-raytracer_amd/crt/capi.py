@@ -69,6 +69,17 @@ class RenderOpts(C.Structure):
                 ("num_bounces", C.c_int32), ("pad_", C.c_uint32), ("seed", C.c_uint64)]
 
 
+class FlashOpts(C.Structure):
+    _fields_ = [("beams", Camera), ("sub_rays", C.c_uint32), ("pad_", C.c_uint32), ("sigma", C.c_double),
+                ("seed", C.c_uint64)]
+
+
+class FluxOpts(C.Structure):
+    _fields_ = [("sun_dir", C.c_double * 3), ("pressure", C.c_double), ("rho_s", C.c_double),
+                ("rho_d", C.c_double), ("grid", C.c_uint32), ("pad_", C.c_uint32), ("center", C.c_double * 3),
+                ("radius", C.c_double), ("ref_point", C.c_double * 3)]
+
+
 _p = C.c_void_p
 _lib = None
 
@@ -84,6 +95,10 @@ SIGNATURES = {
     "crt_pass": (C.c_int, [_p, _p, _p, _p, _p, _p, _p, C.c_uint32, _p, _p]),
     "crt_trace_rays": (C.c_int, [_p, C.c_uint64, _p, _p, _p, _p, _p, _p, _p, _p, C.c_uint32, _p, _p]),
     "crt_sun_pass": (C.c_int, [_p, _p, _p, _p, _p, _p, C.c_uint32, _p, _p]),
+    "crt_render": (C.c_int, [_p, _p, _p, C.c_uint32, _p, _p, _p, _p, C.c_uint32, _p, _p]),
+    "crt_lidar_batch": (C.c_int, [_p, C.c_uint64, _p, _p, C.c_int32, _p, C.c_uint32, _p, _p]),
+    "crt_flash_lidar": (C.c_int, [_p, _p, _p, _p, _p, _p, C.c_uint32, _p, _p]),
+    "crt_radiation_pressure": (C.c_int, [_p, _p, _p, _p, _p, _p, _p]),
 }
 
 
@@ -246,3 +261,56 @@ class Scene:
         lit = np.zeros((h, w), dtype=np.uint8)
         _check(load().crt_sun_pass(self.h, C.byref(c), s, ptr(depth), ptr(tri), ptr(lit), 0, stream, C.byref(tm)))
         return {"depth": depth, "tri": tri, "lit": lit, "timing": tm.as_dict()}
+
+    # ---- render / sensors -------------------------------------------------
+    def render(self, cam: dict, lights, min_samples=1, max_samples=1, noise_threshold=1.0, num_bounces=1, seed=0):
+        """Path-traced radiance: returns (rgba f32 (H,W,4), rgba uint8 (H,W,4), spp_used, timing)."""
+        c = make_camera(cam)
+        arr = (Light * max(len(lights), 1))(*[make_light(l) for l in lights])
+        opts = RenderOpts(int(min_samples), int(max_samples), float(noise_threshold), int(num_bounces), 0, int(seed))
+        w, h = int(np.floor(cam["resolution"][0])), int(np.floor(cam["resolution"][1]))
+        rgba = np.zeros((h, w, 4), dtype=np.float32)
+        u8 = np.zeros((h, w, 4), dtype=np.uint8)
+        spp = np.zeros((h, w), dtype=np.int32)
+        tm = Timing()
+        _check(load().crt_render(self.h, C.byref(c), arr, len(lights), C.byref(opts), ptr(rgba), ptr(u8), ptr(spp),
+                                 0, None, C.byref(tm)))
+        return rgba, u8, spp, tm.as_dict()
+
+    def lidar_batch(self, positions, rotations, z_positive=False):
+        pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
+        rot = np.ascontiguousarray(rotations, dtype=np.float64).reshape(-1, 9)
+        out = np.zeros(pos.shape[0])
+        tm = Timing()
+        _check(load().crt_lidar_batch(self.h, pos.shape[0], ptr(pos), ptr(rot), int(bool(z_positive)), ptr(out), 0,
+                                      None, C.byref(tm)))
+        return out
+
+    def flash_lidar(self, beams: dict, sub_rays=8, sigma=5e-4, seed=0):
+        o = FlashOpts()
+        o.beams = make_camera(beams)
+        o.sub_rays = int(sub_rays)
+        o.sigma = float(sigma)
+        o.seed = int(seed)
+        w, h = int(np.floor(beams["resolution"][0])), int(np.floor(beams["resolution"][1]))
+        single, first, last = np.zeros((h, w)), np.zeros((h, w)), np.zeros((h, w))
+        nret = np.zeros((h, w), dtype=np.uint8)
+        tm = Timing()
+        _check(load().crt_flash_lidar(self.h, C.byref(o), ptr(single), ptr(first), ptr(last), ptr(nret), 0, None,
+                                      C.byref(tm)))
+        return {"single": single, "first": first, "last": last, "n_returns": nret, "timing": tm.as_dict()}
+
+    def radiation_pressure(self, sun_dir, pressure, rho_s, rho_d, grid, center=(0, 0, 0), radius=0.0,
+                           ref_point=(0, 0, 0)):
+        o = FluxOpts()
+        o.sun_dir[:] = [float(x) for x in sun_dir]
+        o.pressure, o.rho_s, o.rho_d, o.grid = float(pressure), float(rho_s), float(rho_d), int(grid)
+        o.center[:] = [float(x) for x in center]
+        o.radius = float(radius)
+        o.ref_point[:] = [float(x) for x in ref_point]
+        f = (C.c_double * 3)()
+        t = (C.c_double * 3)()
+        nh = C.c_uint64(0)
+        tm = Timing()
+        _check(load().crt_radiation_pressure(self.h, C.byref(o), f, t, C.byref(nh), None, C.byref(tm)))
+        return np.array(f[:]), np.array(t[:]), nh.value, tm.as_dict()

@@ -4,6 +4,7 @@
 #include <stdint.h>
 #include <stdexcept>
 #include <string>
+#include <new>
 #include <vector>
 
 #include "crt_device.hpp"
@@ -32,6 +33,27 @@ struct Error : std::runtime_error {
     do {                                                 \
         if (!(cond)) throw ::crt::Error((code), (msg));  \
     } while (0)
+
+// Thread-local text of the last error, returned by crt_last_error().
+inline thread_local std::string g_last_error;
+
+// Run `f`, mapping exceptions to C-ABI status codes.
+template <typename F>
+int guarded(F&& f) {
+    try {
+        f();
+        return CRT_OK;
+    } catch (const Error& e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "host allocation failed";
+        return CRT_ERR_OOM;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return CRT_ERR_HIP;
+    }
+}
 
 // Owning device buffer.
 struct DevBuf {
@@ -79,6 +101,12 @@ struct BuildResult {
 // hierarchy, bottom-up refit) collapsed into 8-wide nodes.
 void build_bvh(const float* d_vertices, const uint32_t* d_indices, uint32_t n_tris,
                uint32_t max_leaf, hipStream_t stream, BuildResult& out);
+
+// Shared helpers of the trace / render / sensor translation units.
+CamView make_cam_view(const crt_camera& c);
+void check_camera_args(const crt_camera* c);
+size_t trace_lds_bytes(const SceneView& s);
+float self_offset(const crt_scene* s);
 
 // Scratch buffers reused across calls on one scene (grow-only).
 struct Workspace {

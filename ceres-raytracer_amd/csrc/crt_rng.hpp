@@ -1,0 +1,36 @@
+// crt_rng.hpp -- counter-based random numbers (no generator state).
+//
+// u01(seed, a, b, c) hashes a 4-tuple with the splitmix64 finaliser chain and
+// returns 53 random bits as a double in [0, 1).  Keys used by the tracer:
+//   a = key_offset + pixel (or beam), b = sample (1-based) or sub-ray,
+//   c = 64 * bounce + dim, dims: 0,1 pixel jitter; 2+2l, 3+2l area-light
+//   sample of light l; 62, 63 cosine (BRDF) sample; 0,1 LiDAR sub-ray offsets.
+// Any result therefore depends only on its key, never on launch order or
+// batching.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace crt {
+
+constexpr unsigned RNG_JITTER = 0u;
+constexpr unsigned RNG_LIGHT = 2u;
+constexpr unsigned RNG_BRDF = 62u;
+
+__host__ __device__ inline uint64_t rng_mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__host__ __device__ inline double rng_u01(uint64_t seed, uint64_t a, uint64_t b, uint64_t c) {
+    uint64_t h = rng_mix64(seed + 0x9E3779B97F4A7C15ull);
+    h = rng_mix64(h ^ a);
+    h = rng_mix64(h ^ (b + 0x632BE59BD9B4E019ull));
+    h = rng_mix64(h ^ (c * 0x8CB92BA72F3D8DD7ull + 1ull));
+    return (double)(h >> 11) * (1.0 / 9007199254740992.0);
+}
+
+__host__ __device__ inline uint64_t rng_dim(int bounce, unsigned dim) { return (uint64_t)bounce * 64u + dim; }
+
+}  // namespace crt

@@ -1,0 +1,422 @@
+// sensors.hip -- LiDAR (simple + batch + flash) and radiation pressure.
+//
+// crt_lidar_batch: the reference LiDAR casts ONE ray per pose along +/-z
+//   rotated by R^T (src/crt/lidars/simple_lidar.hpp:42-96) and returns
+//   |hit - position| with the hit point from barycentrics, 0 on a miss
+//   (src/crt/do_lidar.hpp:43-70, 111-143).  One octet per pose.
+// crt_flash_lidar (new): a beam grid laid out like a pinhole camera; every
+//   beam casts its boresight ray (the "single" return, equal to the reference
+//   LiDAR pointed along that beam) plus `sub_rays` rays whose directions are
+//   perturbed by two independent N(0, sigma^2) angles in the beam's tangent
+//   plane (Box-Muller on the counter RNG, key = beam, sub-ray).  first / last
+//   = min / max range over the sub-rays that hit; n_returns counts them.
+// crt_radiation_pressure (new): parallel rays on an N x N grid over the
+//   square circumscribing the bounding disk perpendicular to the sun; per hit
+//   on a sun-facing triangle (flat outward normal n, cos = n.s > 0):
+//     dF = P A_ray [ (1 - rs) d - 2 (rs cos + rd / 3) n ],  d = -s,
+//     dT = (x - ref) x dF.
+//   Per-ray terms in FP32; block partials and the final sum in FP64 with
+//   TwoSum compensation (error-free transformation, Knuth), so the result is
+//   independent of summation order to well below 1e-12 relative.
+#include "crt_internal.hpp"
+#include "crt_rng.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace crt {
+namespace {
+
+inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+struct OctetLanes {
+    int sub, lane_base, local;
+};
+__device__ inline OctetLanes lanes() {
+    OctetLanes o;
+    const int lane = threadIdx.x & 63;
+    o.sub = lane & 7;
+    o.lane_base = lane & ~7;
+    o.local = threadIdx.x >> 3;
+    return o;
+}
+
+// Closest hit of a unit FP64 ray; returns range |hit - o| (FP64 from
+// barycentrics) or 0 on a miss.
+__device__ inline double range_f64(const SceneView& s, const double o[3], const double d[3], uint32_t* sref,
+                                   float* sd, const OctetLanes& id) {
+    double t0;
+    if (!rebase_t0(s, o, d, t0)) return 0.0;
+    const RayPre r = make_ray_pre((float)(o[0] + t0 * d[0]), (float)(o[1] + t0 * d[1]), (float)(o[2] + t0 * d[2]),
+                                  (float)d[0], (float)d[1], (float)d[2]);
+    const HitRec h = trace_octet<false>(s, r, 0.0f, __int_as_float(0x7f800000), sref, sd, id.sub, id.lane_base);
+    if (h.slot < 0) return 0.0;
+    const float4 a = s.tris[3 * h.slot], b = s.tris[3 * h.slot + 1], c = s.tris[3 * h.slot + 2];
+    const double u = h.u, v = h.v, w = 1.0 - u - v;
+    const double px = u * b.x + v * c.x + w * a.x - o[0];
+    const double py = u * b.y + v * c.y + w * a.y - o[1];
+    const double pz = u * b.z + v * c.z + w * a.z - o[2];
+    return sqrt(px * px + py * py + pz * pz);
+}
+
+__global__ __launch_bounds__(TRACE_BLOCK) void k_lidar_batch(SceneView s, const double* __restrict__ pos,
+                                                             const double* __restrict__ rot, double zsign,
+                                                             double* __restrict__ range, uint64_t n) {
+    extern __shared__ uint32_t smem[];
+    const OctetLanes id = lanes();
+    uint32_t* sref = smem + (size_t)id.local * s.stack_cap;
+    float* sd = reinterpret_cast<float*>(smem + (size_t)OCTETS_PER_BLOCK * s.stack_cap) + (size_t)id.local * s.stack_cap;
+    const uint64_t i = (uint64_t)blockIdx.x * OCTETS_PER_BLOCK + id.local;
+    if (i >= n) return;
+    const double* R = rot + 9 * i;
+    // R^T (0, 0, +/-1) = +/- third row of R
+    const double d[3] = {R[6] * zsign, R[7] * zsign, R[8] * zsign};
+    const double o[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+    const double r = range_f64(s, o, d, sref, sd, id);
+    if (id.sub == 0) range[i] = r;
+}
+
+struct FlashParams {
+    CamView cam;
+    uint32_t sub_rays;
+    double sigma;
+    uint64_t seed;
+};
+
+// One octet per (beam, ray) with ray 0 = boresight, 1..sub_rays perturbed.
+__global__ __launch_bounds__(TRACE_BLOCK) void k_flash(SceneView s, FlashParams F, double* __restrict__ ranges,
+                                                       uint64_t n) {
+    extern __shared__ uint32_t smem[];
+    const OctetLanes id = lanes();
+    uint32_t* sref = smem + (size_t)id.local * s.stack_cap;
+    float* sd = reinterpret_cast<float*>(smem + (size_t)OCTETS_PER_BLOCK * s.stack_cap) + (size_t)id.local * s.stack_cap;
+    const uint64_t q = (uint64_t)blockIdx.x * OCTETS_PER_BLOCK + id.local;
+    if (q >= n) return;
+    const uint32_t per = F.sub_rays + 1;
+    const uint64_t beam = q / per;
+    const uint32_t k = (uint32_t)(q % per);
+    const int px = (int)(beam % (uint64_t)F.cam.w), py = (int)(beam / (uint64_t)F.cam.w);
+    double d[3];
+    camera_dir(F.cam, (double)px, (double)py, d);
+    if (k > 0) {
+        // tangent frame of the beam
+        double ax[3] = {1.0, 0.0, 0.0};
+        if (fabs(d[0]) > 0.9) { ax[0] = 0.0; ax[1] = 1.0; }
+        double e1[3] = {d[1] * ax[2] - d[2] * ax[1], d[2] * ax[0] - d[0] * ax[2], d[0] * ax[1] - d[1] * ax[0]};
+        const double l1 = sqrt(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]);
+        e1[0] /= l1; e1[1] /= l1; e1[2] /= l1;
+        const double e2[3] = {d[1] * e1[2] - d[2] * e1[1], d[2] * e1[0] - d[0] * e1[2], d[0] * e1[1] - d[1] * e1[0]};
+        const double r1 = rng_u01(F.seed, beam, k, 0), r2 = rng_u01(F.seed, beam, k, 1);
+        const double rad = sqrt(-2.0 * log(1.0 - r1));
+        const double g1 = rad * cos(2.0 * M_PI * r2), g2 = rad * sin(2.0 * M_PI * r2);
+        const double a1 = tan(F.sigma * g1), a2 = tan(F.sigma * g2);
+        double nd[3] = {d[0] + a1 * e1[0] + a2 * e2[0], d[1] + a1 * e1[1] + a2 * e2[1], d[2] + a1 * e1[2] + a2 * e2[2]};
+        const double ln = sqrt(nd[0] * nd[0] + nd[1] * nd[1] + nd[2] * nd[2]);
+        d[0] = nd[0] / ln; d[1] = nd[1] / ln; d[2] = nd[2] / ln;
+    }
+    const double r = range_f64(s, F.cam.pos, d, sref, sd, id);
+    if (id.sub == 0) ranges[q] = r;
+}
+
+__global__ __launch_bounds__(256) void k_flash_reduce(const double* __restrict__ ranges, uint32_t sub_rays,
+                                                      uint64_t n_beams, double* __restrict__ single,
+                                                      double* __restrict__ first, double* __restrict__ last,
+                                                      uint8_t* __restrict__ n_returns) {
+    const uint64_t b = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (b >= n_beams) return;
+    const double* r = ranges + b * (sub_rays + 1);
+    double mn = 0.0, mx = 0.0;
+    uint32_t cnt = 0;
+    for (uint32_t k = 1; k <= sub_rays; ++k) {
+        const double x = r[k];
+        if (x > 0.0) {
+            mn = cnt == 0 ? x : fmin(mn, x);
+            mx = cnt == 0 ? x : fmax(mx, x);
+            ++cnt;
+        }
+    }
+    if (single) single[b] = r[0];
+    if (first) first[b] = mn;
+    if (last) last[b] = mx;
+    if (n_returns) n_returns[b] = (uint8_t)min(cnt, 255u);
+}
+
+// ---- radiation pressure ----
+struct Comp {   // compensated FP64 accumulator (value, error)
+    double s, c;
+};
+__device__ inline Comp two_sum(double a, double b) {
+    const double s = a + b;
+    const double bp = s - a;
+    const double e = (a - (s - bp)) + (b - bp);
+    return Comp{s, e};
+}
+__device__ inline Comp comp_add(Comp x, Comp y) {
+    const Comp t = two_sum(x.s, y.s);
+    return Comp{t.s, t.c + x.c + y.c};
+}
+
+struct FluxParams {
+    double sun[3], e1[3], e2[3], center[3], ref[3];
+    double radius, h, pressure, rho_s, rho_d;
+    double origin_back;    // distance of the grid plane from the centre, sunward
+    uint32_t grid;
+};
+
+// Block partials: 32 octets (rays) per block; 7 compensated sums each
+// (Fx,Fy,Fz,Tx,Ty,Tz, hits).
+__global__ __launch_bounds__(TRACE_BLOCK) void k_flux(SceneView s, FluxParams F, Comp* __restrict__ partial,
+                                                      uint64_t n) {
+    extern __shared__ uint32_t smem[];
+    __shared__ Comp red[7][OCTETS_PER_BLOCK];
+    const OctetLanes id = lanes();
+    uint32_t* sref = smem + (size_t)id.local * s.stack_cap;
+    float* sd = reinterpret_cast<float*>(smem + (size_t)OCTETS_PER_BLOCK * s.stack_cap) + (size_t)id.local * s.stack_cap;
+    const uint64_t q = (uint64_t)blockIdx.x * OCTETS_PER_BLOCK + id.local;
+    double term[7] = {0, 0, 0, 0, 0, 0, 0};
+    if (q < n) {
+        const uint32_t i = (uint32_t)(q % F.grid), j = (uint32_t)(q / F.grid);
+        const double x = -F.radius + (i + 0.5) * F.h, y = -F.radius + (j + 0.5) * F.h;
+        double o[3], d[3];
+        for (int k = 0; k < 3; ++k) {
+            o[k] = F.center[k] + F.origin_back * F.sun[k] + x * F.e1[k] + y * F.e2[k];
+            d[k] = -F.sun[k];
+        }
+        double t0;
+        if (rebase_t0(s, o, d, t0)) {
+            const RayPre r = make_ray_pre((float)(o[0] + t0 * d[0]), (float)(o[1] + t0 * d[1]),
+                                          (float)(o[2] + t0 * d[2]), (float)d[0], (float)d[1], (float)d[2]);
+            const HitRec h = trace_octet<false>(s, r, 0.0f, __int_as_float(0x7f800000), sref, sd, id.sub,
+                                                id.lane_base);
+            if (h.slot >= 0) {
+                const float4 a = s.tris[3 * h.slot], b = s.tris[3 * h.slot + 1], c = s.tris[3 * h.slot + 2];
+                // outward normal = -normalize((p0 - p1) x (p2 - p0))  (reference normal is inward)
+                const float e1x = a.x - b.x, e1y = a.y - b.y, e1z = a.z - b.z;
+                const float e2x = c.x - a.x, e2y = c.y - a.y, e2z = c.z - a.z;
+                float nx = -(e1y * e2z - e1z * e2y), ny = -(e1z * e2x - e1x * e2z), nz = -(e1x * e2y - e1y * e2x);
+                const float inv = 1.0f / sqrtf(nx * nx + ny * ny + nz * nz);
+                nx *= inv; ny *= inv; nz *= inv;
+                const float sx = (float)F.sun[0], sy = (float)F.sun[1], sz = (float)F.sun[2];
+                const float cs = nx * sx + ny * sy + nz * sz;
+                if (cs > 0.0f) {
+                    const float pa = (float)(F.pressure * F.h * F.h);
+                    const float kd = pa * (1.0f - (float)F.rho_s);
+                    const float kn = pa * 2.0f * ((float)F.rho_s * cs + (float)F.rho_d / 3.0f);
+                    const float fx = -sx * kd - kn * nx, fy = -sy * kd - kn * ny, fz = -sz * kd - kn * nz;
+                    const double u = h.u, v = h.v, w = 1.0 - u - v;
+                    const double lx = u * b.x + v * c.x + w * a.x - F.ref[0];
+                    const double ly = u * b.y + v * c.y + w * a.y - F.ref[1];
+                    const double lz = u * b.z + v * c.z + w * a.z - F.ref[2];
+                    term[0] = fx; term[1] = fy; term[2] = fz;
+                    term[3] = ly * fz - lz * fy;
+                    term[4] = lz * fx - lx * fz;
+                    term[5] = lx * fy - ly * fx;
+                    term[6] = 1.0;
+                }
+            }
+        }
+    }
+    if (id.sub == 0)
+        for (int k = 0; k < 7; ++k) red[k][id.local] = Comp{term[k], 0.0};
+    __syncthreads();
+    for (int w = OCTETS_PER_BLOCK / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int k = 0; k < 7; ++k) red[k][threadIdx.x] = comp_add(red[k][threadIdx.x], red[k][threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x < 7) partial[(size_t)blockIdx.x * 7 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(256) void k_flux_final(const Comp* __restrict__ partial, uint32_t nblocks,
+                                                    double* __restrict__ out) {
+    __shared__ Comp red[256];
+    const int k = blockIdx.x;   // component 0..6
+    Comp acc{0.0, 0.0};
+    for (uint32_t b = threadIdx.x; b < nblocks; b += 256) acc = comp_add(acc, partial[(size_t)b * 7 + k]);
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] = comp_add(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[k] = red[0].s + red[0].c;
+}
+
+struct Timer {
+    hipEvent_t a = nullptr, b = nullptr;
+    Timer() {
+        CRT_HIP_CHECK(hipEventCreate(&a));
+        CRT_HIP_CHECK(hipEventCreate(&b));
+    }
+    ~Timer() {
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+    }
+    double ms() const {
+        float f = 0.f;
+        CRT_HIP_CHECK(hipEventElapsedTime(&f, a, b));
+        return f;
+    }
+};
+
+}  // namespace
+}  // namespace crt
+
+using namespace crt;
+
+extern "C" {
+
+int crt_lidar_batch(crt_scene* scene, uint64_t n, const double* positions, const double* rotations,
+                    int32_t z_positive, double* ranges, uint32_t flags, void* stream, crt_timing* timing) {
+    return guarded([&] {
+        CRT_REQUIRE(scene != nullptr, CRT_ERR_INVALID, "scene is NULL");
+        if (n == 0) return;
+        CRT_REQUIRE(positions && rotations && ranges, CRT_ERR_INVALID, "NULL argument");
+        CRT_REQUIRE(n <= (1ull << 31), CRT_ERR_INVALID, "too many poses in one call");
+        hipStream_t st = (hipStream_t)stream;
+        const bool dev = (flags & CRT_OUT_DEVICE) != 0;
+        Workspace& ws = scene->ws;
+        const double* dp = positions;
+        const double* dr = rotations;
+        double* dout = ranges;
+        if (!dev) {
+            ws.f.reserve(n * 12 * 8);
+            double* base = ws.f.as<double>();
+            CRT_HIP_CHECK(hipMemcpyAsync(base, positions, n * 24, hipMemcpyHostToDevice, st));
+            CRT_HIP_CHECK(hipMemcpyAsync(base + 3 * n, rotations, n * 72, hipMemcpyHostToDevice, st));
+            dp = base;
+            dr = base + 3 * n;
+            ws.g.reserve(n * 8);
+            dout = ws.g.as<double>();
+        }
+        Timer tm;
+        CRT_HIP_CHECK(hipEventRecord(tm.a, st));
+        hipLaunchKernelGGL(k_lidar_batch, dim3(cdiv(n, OCTETS_PER_BLOCK)), dim3(TRACE_BLOCK),
+                           trace_lds_bytes(scene->view), st, scene->view, dp, dr, z_positive ? 1.0 : -1.0, dout, n);
+        CRT_HIP_CHECK(hipGetLastError());
+        CRT_HIP_CHECK(hipEventRecord(tm.b, st));
+        if (!dev) CRT_HIP_CHECK(hipMemcpyAsync(ranges, dout, n * 8, hipMemcpyDeviceToHost, st));
+        CRT_HIP_CHECK(hipStreamSynchronize(st));
+        if (timing) {
+            std::memset(timing, 0, sizeof(*timing));
+            timing->trace_ms = timing->total_ms = tm.ms();
+            timing->rays_traced = n;
+        }
+    });
+}
+
+int crt_flash_lidar(crt_scene* scene, const crt_flash_opts* opts, double* single, double* first, double* last,
+                    uint8_t* n_returns, uint32_t flags, void* stream, crt_timing* timing) {
+    return guarded([&] {
+        CRT_REQUIRE(scene != nullptr && opts != nullptr, CRT_ERR_INVALID, "NULL argument");
+        check_camera_args(&opts->beams);
+        CRT_REQUIRE(opts->sub_rays <= 64, CRT_ERR_INVALID, "at most 64 sub-rays per beam");
+        CRT_REQUIRE(opts->sigma >= 0.0 && std::isfinite(opts->sigma), CRT_ERR_INVALID, "sigma must be >= 0");
+        hipStream_t st = (hipStream_t)stream;
+        FlashParams F;
+        F.cam = make_cam_view(opts->beams);
+        F.sub_rays = opts->sub_rays;
+        F.sigma = opts->sigma;
+        F.seed = opts->seed;
+        const uint64_t nb = (uint64_t)F.cam.w * F.cam.h;
+        const uint64_t nr = nb * (F.sub_rays + 1);
+        CRT_REQUIRE(nr < (1ull << 32), CRT_ERR_INVALID, "flash LiDAR too large");
+        const bool dev = (flags & CRT_OUT_DEVICE) != 0;
+        Workspace& ws = scene->ws;
+        ws.f.reserve(nr * 8);
+        double* d_single = single;
+        double* d_first = first;
+        double* d_last = last;
+        uint8_t* d_n = n_returns;
+        if (!dev) {
+            ws.g.reserve(nb * 25);
+            char* base = ws.g.as<char>();
+            d_single = single ? reinterpret_cast<double*>(base) : nullptr;
+            d_first = first ? reinterpret_cast<double*>(base + nb * 8) : nullptr;
+            d_last = last ? reinterpret_cast<double*>(base + nb * 16) : nullptr;
+            d_n = n_returns ? reinterpret_cast<uint8_t*>(base + nb * 24) : nullptr;
+        }
+        Timer tm;
+        CRT_HIP_CHECK(hipEventRecord(tm.a, st));
+        hipLaunchKernelGGL(k_flash, dim3(cdiv(nr, OCTETS_PER_BLOCK)), dim3(TRACE_BLOCK), trace_lds_bytes(scene->view),
+                           st, scene->view, F, ws.f.as<double>(), nr);
+        hipLaunchKernelGGL(k_flash_reduce, dim3(cdiv(nb, 256)), dim3(256), 0, st, ws.f.as<double>(), F.sub_rays, nb,
+                           d_single, d_first, d_last, d_n);
+        CRT_HIP_CHECK(hipGetLastError());
+        CRT_HIP_CHECK(hipEventRecord(tm.b, st));
+        if (!dev) {
+            if (single) CRT_HIP_CHECK(hipMemcpyAsync(single, d_single, nb * 8, hipMemcpyDeviceToHost, st));
+            if (first) CRT_HIP_CHECK(hipMemcpyAsync(first, d_first, nb * 8, hipMemcpyDeviceToHost, st));
+            if (last) CRT_HIP_CHECK(hipMemcpyAsync(last, d_last, nb * 8, hipMemcpyDeviceToHost, st));
+            if (n_returns) CRT_HIP_CHECK(hipMemcpyAsync(n_returns, d_n, nb, hipMemcpyDeviceToHost, st));
+        }
+        CRT_HIP_CHECK(hipStreamSynchronize(st));
+        if (timing) {
+            std::memset(timing, 0, sizeof(*timing));
+            timing->trace_ms = timing->total_ms = tm.ms();
+            timing->rays_traced = nr;
+        }
+    });
+}
+
+int crt_radiation_pressure(crt_scene* scene, const crt_flux_opts* opts, double force[3], double torque[3],
+                           uint64_t* n_hits, void* stream, crt_timing* timing) {
+    return guarded([&] {
+        CRT_REQUIRE(scene != nullptr && opts != nullptr && force && torque, CRT_ERR_INVALID, "NULL argument");
+        CRT_REQUIRE(opts->grid >= 1 && opts->grid <= 65536, CRT_ERR_INVALID, "grid must be in 1..65536");
+        const double sl = std::sqrt(opts->sun_dir[0] * opts->sun_dir[0] + opts->sun_dir[1] * opts->sun_dir[1] +
+                                    opts->sun_dir[2] * opts->sun_dir[2]);
+        CRT_REQUIRE(sl > 0 && std::isfinite(sl), CRT_ERR_INVALID, "sun_dir must be a finite non-zero vector");
+        hipStream_t st = (hipStream_t)stream;
+        FluxParams F;
+        for (int k = 0; k < 3; ++k) F.sun[k] = opts->sun_dir[k] / sl;
+        const double ax[3] = {std::fabs(F.sun[0]) < 0.9 ? 1.0 : 0.0, std::fabs(F.sun[0]) < 0.9 ? 0.0 : 1.0, 0.0};
+        double e1[3] = {F.sun[1] * ax[2] - F.sun[2] * ax[1], F.sun[2] * ax[0] - F.sun[0] * ax[2],
+                        F.sun[0] * ax[1] - F.sun[1] * ax[0]};
+        const double l1 = std::sqrt(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]);
+        for (int k = 0; k < 3; ++k) F.e1[k] = e1[k] / l1;
+        F.e2[0] = F.sun[1] * F.e1[2] - F.sun[2] * F.e1[1];
+        F.e2[1] = F.sun[2] * F.e1[0] - F.sun[0] * F.e1[2];
+        F.e2[2] = F.sun[0] * F.e1[1] - F.sun[1] * F.e1[0];
+        if (opts->radius > 0) {
+            for (int k = 0; k < 3; ++k) F.center[k] = opts->center[k];
+            F.radius = opts->radius;
+        } else {
+            for (int k = 0; k < 3; ++k) F.center[k] = scene->view.center[k];
+            F.radius = scene->view.radius;
+        }
+        for (int k = 0; k < 3; ++k) F.ref[k] = opts->ref_point[k];
+        F.grid = opts->grid;
+        F.h = 2.0 * F.radius / (double)F.grid;
+        F.pressure = opts->pressure;
+        F.rho_s = opts->rho_s;
+        F.rho_d = opts->rho_d;
+        F.origin_back = 2.0 * F.radius;
+        const uint64_t n = (uint64_t)F.grid * F.grid;
+        const uint32_t nblocks = cdiv(n, OCTETS_PER_BLOCK);
+        Workspace& ws = scene->ws;
+        ws.h.reserve((size_t)nblocks * 7 * sizeof(Comp));
+        ws.counters.reserve(256);
+        double* out = ws.counters.as<double>();
+        Timer tm;
+        CRT_HIP_CHECK(hipEventRecord(tm.a, st));
+        hipLaunchKernelGGL(k_flux, dim3(nblocks), dim3(TRACE_BLOCK), trace_lds_bytes(scene->view), st, scene->view, F,
+                           ws.h.as<Comp>(), n);
+        hipLaunchKernelGGL(k_flux_final, dim3(7), dim3(256), 0, st, ws.h.as<Comp>(), nblocks, out);
+        CRT_HIP_CHECK(hipGetLastError());
+        CRT_HIP_CHECK(hipEventRecord(tm.b, st));
+        double h[7];
+        CRT_HIP_CHECK(hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, st));
+        CRT_HIP_CHECK(hipStreamSynchronize(st));
+        for (int k = 0; k < 3; ++k) { force[k] = h[k]; torque[k] = h[3 + k]; }
+        if (n_hits) *n_hits = (uint64_t)std::llround(h[6]);
+        if (timing) {
+            std::memset(timing, 0, sizeof(*timing));
+            timing->trace_ms = timing->total_ms = tm.ms();
+            timing->rays_traced = n;
+        }
+    });
+}
+
+}  // extern "C"

@@ -17,32 +17,13 @@
 namespace crt {
 namespace {
 
-thread_local std::string g_err;
-
-int fail(int code, const std::string& msg) {
-    g_err = msg;
-    return code;
-}
-
-template <typename F>
-int guarded(F&& f) {
-    try {
-        f();
-        return CRT_OK;
-    } catch (const Error& e) {
-        return fail(e.code, e.what());
-    } catch (const std::bad_alloc&) {
-        return fail(CRT_ERR_OOM, "host allocation failed");
-    } catch (const std::exception& e) {
-        return fail(CRT_ERR_HIP, e.what());
-    }
-}
-
 inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
-size_t stack_lds_bytes(const SceneView& s) { return (size_t)OCTETS_PER_BLOCK * s.stack_cap * 8u; }
+}  // namespace
 
-CamView make_cam(const crt_camera& c) {
+size_t trace_lds_bytes(const SceneView& s) { return (size_t)OCTETS_PER_BLOCK * s.stack_cap * 8u; }
+
+CamView make_cam_view(const crt_camera& c) {
     CamView v;
     for (int k = 0; k < 3; ++k) v.pos[k] = c.position[k];
     for (int k = 0; k < 9; ++k) v.rot[k] = c.rotation[k];
@@ -58,13 +39,24 @@ CamView make_cam(const crt_camera& c) {
     return v;
 }
 
-void check_camera(const crt_camera* c) {
+void check_camera_args(const crt_camera* c) {
     CRT_REQUIRE(c != nullptr, CRT_ERR_INVALID, "camera is NULL");
     CRT_REQUIRE(c->resolution[0] >= 1 && c->resolution[1] >= 1, CRT_ERR_INVALID, "camera resolution must be >= 1");
     CRT_REQUIRE(c->resolution[0] * c->resolution[1] <= 1.0e9, CRT_ERR_INVALID, "camera resolution too large");
     CRT_REQUIRE(c->sensor_size[0] > 0 && c->sensor_size[1] > 0, CRT_ERR_INVALID, "sensor size must be > 0");
     CRT_REQUIRE(std::isfinite(c->focal_length), CRT_ERR_INVALID, "focal length must be finite");
 }
+
+// Self-intersection offset: 1e-4 absolute like the reference
+// (src/crt/path_tracing/unidirectional.hpp:61), raised to 1e-6 of the scene
+// extent so that it stays ~8 FP32 ulp above the coordinates' resolution.
+float self_offset(const crt_scene* s) {
+    double ext = 0.0;
+    for (int k = 0; k < 6; ++k) ext = std::max(ext, std::fabs(s->info.bounds[k]));
+    return (float)std::max(1e-4, 1e-6 * ext);
+}
+
+namespace {
 
 // Octet coordinates of the calling lane.
 struct OctetId {
@@ -382,7 +374,7 @@ extern "C" {
 
 int crt_version(void) { return 100; }
 
-const char* crt_last_error(void) { return g_err.c_str(); }
+const char* crt_last_error(void) { return g_last_error.c_str(); }
 
 int crt_device_init(int device) {
     return guarded([&] {
@@ -458,7 +450,7 @@ int crt_scene_create(const float* vertices, uint32_t n_vertices, const uint32_t*
                            s->inst_color.bytes + s->inst_smooth.bytes + s->vnormals.bytes + s->bvh.nodes.bytes +
                            s->bvh.tris.bytes;
         scene_make_view(s.get());
-        CRT_REQUIRE(stack_lds_bytes(s->view) <= 64u * 1024u, CRT_ERR_INVALID,
+        CRT_REQUIRE(trace_lds_bytes(s->view) <= 64u * 1024u, CRT_ERR_INVALID,
                     "BVH too deep for the LDS traversal stack");
         *out_scene = s.release();
     });
@@ -477,9 +469,9 @@ int crt_pass(crt_scene* scene, const crt_camera* camera, double* xyz, double* no
              double* depth, int32_t* tri_id, uint32_t flags, void* stream, crt_timing* timing) {
     return guarded([&] {
         CRT_REQUIRE(scene != nullptr, CRT_ERR_INVALID, "scene is NULL");
-        check_camera(camera);
+        check_camera_args(camera);
         hipStream_t st = (hipStream_t)stream;
-        const CamView cam = make_cam(*camera);
+        const CamView cam = make_cam_view(*camera);
         const size_t npix = (size_t)cam.w * cam.h;
         const bool dev = (flags & CRT_OUT_DEVICE) != 0;
         Workspace& ws = scene->ws;
@@ -492,7 +484,7 @@ int crt_pass(crt_scene* scene, const crt_camera* camera, double* xyz, double* no
         EventTimer tm;
         const uint32_t blocks = cdiv((uint64_t)cam.w, 8) * cdiv((uint64_t)cam.h, 4);
         CRT_HIP_CHECK(hipEventRecord(tm.a, st));
-        hipLaunchKernelGGL(k_camera_pass, dim3(blocks), dim3(TRACE_BLOCK), stack_lds_bytes(scene->view), st,
+        hipLaunchKernelGGL(k_camera_pass, dim3(blocks), dim3(TRACE_BLOCK), trace_lds_bytes(scene->view), st,
                            scene->view, cam, o);
         CRT_HIP_CHECK(hipGetLastError());
         CRT_HIP_CHECK(hipEventRecord(tm.b, st));
@@ -549,10 +541,10 @@ int crt_trace_rays(crt_scene* scene, uint64_t n, const double* origins, const do
         const uint32_t blocks = cdiv(n, OCTETS_PER_BLOCK);
         CRT_HIP_CHECK(hipEventRecord(tm.a, st));
         if (any)
-            hipLaunchKernelGGL(k_trace_rays<true>, dim3(blocks), dim3(TRACE_BLOCK), stack_lds_bytes(scene->view), st,
+            hipLaunchKernelGGL(k_trace_rays<true>, dim3(blocks), dim3(TRACE_BLOCK), trace_lds_bytes(scene->view), st,
                                scene->view, in, out, n);
         else
-            hipLaunchKernelGGL(k_trace_rays<false>, dim3(blocks), dim3(TRACE_BLOCK), stack_lds_bytes(scene->view), st,
+            hipLaunchKernelGGL(k_trace_rays<false>, dim3(blocks), dim3(TRACE_BLOCK), trace_lds_bytes(scene->view), st,
                                scene->view, in, out, n);
         CRT_HIP_CHECK(hipGetLastError());
         CRT_HIP_CHECK(hipEventRecord(tm.b, st));
@@ -577,13 +569,13 @@ int crt_sun_pass(crt_scene* scene, const crt_camera* camera, const double sun_di
                  uint8_t* lit, uint32_t flags, void* stream, crt_timing* timing) {
     return guarded([&] {
         CRT_REQUIRE(scene != nullptr, CRT_ERR_INVALID, "scene is NULL");
-        check_camera(camera);
+        check_camera_args(camera);
         CRT_REQUIRE(sun_dir != nullptr, CRT_ERR_INVALID, "sun_dir is NULL");
         const double sl = std::sqrt(sun_dir[0] * sun_dir[0] + sun_dir[1] * sun_dir[1] + sun_dir[2] * sun_dir[2]);
         CRT_REQUIRE(sl > 0 && std::isfinite(sl), CRT_ERR_INVALID, "sun_dir must be a finite non-zero vector");
         const float3 sun = make_float3((float)(sun_dir[0] / sl), (float)(sun_dir[1] / sl), (float)(sun_dir[2] / sl));
         hipStream_t st = (hipStream_t)stream;
-        const CamView cam = make_cam(*camera);
+        const CamView cam = make_cam_view(*camera);
         const size_t npix = (size_t)cam.w * cam.h;
         const bool dev = (flags & CRT_OUT_DEVICE) != 0;
         Workspace& ws = scene->ws;
@@ -598,21 +590,16 @@ int crt_sun_pass(crt_scene* scene, const crt_camera* camera, const double sun_di
         ws.h.reserve(npix * sizeof(ShadowRay));
         ws.counters.reserve(256);
         uint32_t* qcount = ws.counters.as<uint32_t>();
-        // scale-aware self-intersection offset: reference uses 1e-4 absolute
-        // (src/crt/path_tracing/unidirectional.hpp:61); FP32 needs >= a few ulp
-        const double ext = std::max({std::fabs(scene->info.bounds[0]), std::fabs(scene->info.bounds[1]),
-                                     std::fabs(scene->info.bounds[2]), std::fabs(scene->info.bounds[3]),
-                                     std::fabs(scene->info.bounds[4]), std::fabs(scene->info.bounds[5])});
-        const float offset = (float)std::max(1e-4, 1e-6 * ext);
+        const float offset = self_offset(scene);
         EventTimer tm;
         const uint32_t blocks = cdiv((uint64_t)cam.w, 8) * cdiv((uint64_t)cam.h, 4);
         CRT_HIP_CHECK(hipMemsetAsync(qcount, 0, 4, st));
         CRT_HIP_CHECK(hipEventRecord(tm.a, st));
-        hipLaunchKernelGGL(k_primary_sun, dim3(blocks), dim3(TRACE_BLOCK), stack_lds_bytes(scene->view), st,
+        hipLaunchKernelGGL(k_primary_sun, dim3(blocks), dim3(TRACE_BLOCK), trace_lds_bytes(scene->view), st,
                            scene->view, cam, sun, d_depth, d_tri, d_lit, ws.h.as<ShadowRay>(), qcount, offset);
         CRT_HIP_CHECK(hipEventRecord(tm.b, st));
         hipLaunchKernelGGL(k_shadow_sun, dim3(cdiv(npix, OCTETS_PER_BLOCK)), dim3(TRACE_BLOCK),
-                           stack_lds_bytes(scene->view), st, scene->view, ws.h.as<ShadowRay>(), qcount, sun, d_lit);
+                           trace_lds_bytes(scene->view), st, scene->view, ws.h.as<ShadowRay>(), qcount, sun, d_lit);
         CRT_HIP_CHECK(hipGetLastError());
         CRT_HIP_CHECK(hipEventRecord(tm.c, st));
         uint32_t nshadow = 0;

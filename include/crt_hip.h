@@ -184,6 +184,63 @@ int crt_sun_pass(crt_scene* scene, const crt_camera* camera, const double sun_di
                  double* depth, int32_t* tri_id, uint8_t* lit,
                  uint32_t flags, void* stream, crt_timing* timing);
 
+/* ---- path-traced render (src/crt/do_render.hpp:27-141) -----------------
+ * Lambertian multi-bounce path tracing with next-event estimation, adaptive
+ * sampling (min/max samples, noise threshold) and the reference's weights.
+ * Outputs (any may be NULL): rgba f32 [H][W][4] radiance (alpha 1), rgba_u8
+ * [H][W][4] = trunc(clamp(256 x, 0, 255)) like the reference's uint8 image,
+ * spp_used int32 [H][W].  At most 8 lights.                                */
+int crt_render(crt_scene* scene, const crt_camera* camera, const crt_light* lights, uint32_t n_lights,
+               const crt_render_opts* opts, float* rgba, uint8_t* rgba_u8, int32_t* spp_used,
+               uint32_t flags, void* stream, crt_timing* timing);
+
+/* ---- LiDAR (src/crt/do_lidar.hpp:13-155) ------------------------------
+ * One ray per pose along +z (z_positive) or -z rotated by R^T; range =
+ * |hit - position| (0 on a miss).  positions n x 3, rotations n x 9
+ * (row-major R per pose), ranges n.  A single-pose call is the reference's
+ * simulate_lidar; n poses is batch_simulate_lidar without its 100000 cap.   */
+int crt_lidar_batch(crt_scene* scene, uint64_t n, const double* positions, const double* rotations,
+                    int32_t z_positive, double* ranges, uint32_t flags, void* stream, crt_timing* timing);
+
+/* ---- flash LiDAR (new) --------------------------------------------------
+ * Beam grid = pinhole camera `beams`; per beam the boresight ray ("single"
+ * return) plus `sub_rays` rays with Gaussian angular offsets of std `sigma`
+ * (rad) per tangent axis (counter RNG keyed by seed, beam, sub-ray).
+ * Outputs [H][W] (any may be NULL): single, first (min), last (max) range over
+ * the sub-rays that hit (0 if none), n_returns = number of sub-ray hits.     */
+typedef struct {
+    crt_camera beams;
+    uint32_t sub_rays;        /* <= 64 */
+    uint32_t pad_;
+    double sigma;
+    uint64_t seed;
+} crt_flash_opts;
+int crt_flash_lidar(crt_scene* scene, const crt_flash_opts* opts, double* single, double* first, double* last,
+                    uint8_t* n_returns, uint32_t flags, void* stream, crt_timing* timing);
+
+/* ---- radiation pressure / flux (new) --------------------------------------
+ * Parallel rays on a grid x grid lattice over the square circumscribing the
+ * disk of radius `radius` around `center` (radius <= 0: the scene's bounding
+ * sphere), perpendicular to sun_dir (unit vector towards the sun).  Per hit
+ * on a sun-facing triangle with flat outward normal n, cos = n.s:
+ *   dF = pressure * A_ray * [(1 - rho_s) (-s) - 2 (rho_s cos + rho_d / 3) n]
+ *   dT = (x_hit - ref_point) x dF
+ * Per-ray terms FP32, compensated FP64 reduction.  force/torque in N, N m
+ * for pressure in N/m^2 and lengths in m.                                   */
+typedef struct {
+    double sun_dir[3];
+    double pressure;
+    double rho_s;
+    double rho_d;
+    uint32_t grid;
+    uint32_t pad_;
+    double center[3];
+    double radius;
+    double ref_point[3];
+} crt_flux_opts;
+int crt_radiation_pressure(crt_scene* scene, const crt_flux_opts* opts, double force[3], double torque[3],
+                           uint64_t* n_hits, void* stream, crt_timing* timing);
+
 #ifdef __cplusplus
 }
 #endif

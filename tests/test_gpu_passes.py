@@ -77,7 +77,7 @@ def test_cornell_passes_match_oracle(cornell):
     assert np.array_equal(g["instance"][same], c["instance"][same])
     hit = same & (c["tri"] >= 0)
     assert np.max(np.abs(g["normal"][hit] - c["normal"][hit])) < 1e-6
-    assert np.all(g["instance"][c["tri"] < 0] == 0)
+    assert np.all(g["instance"][same & (c["tri"] < 0)] == 0)
 
 
 def test_cornell_smooth_normals(cornell):

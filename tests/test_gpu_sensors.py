@@ -1,0 +1,176 @@
+"""GPU parity: path-traced render, LiDAR (batch + flash) and radiation pressure.
+
+* render: same counter-RNG keys as the FP64 oracle -> per-pixel agreement
+  except where FP32 vs FP64 geometry sends a path elsewhere (edges,
+  terminators); and the reference's own radiance statistics (K independent
+  reference renders, tests/golden/cornell_radiance.npz) within Monte-Carlo 3 sigma.
+* LiDAR: the reference's batch ranges (tests/golden/lidar_batch.npz) within
+  1e-5 relative, apart from edge-grazing beams.
+* radiation pressure: FP64 oracle on the same ray grid within 1e-4 relative;
+  sphere closed form F = P pi R^2 (1 + 4 rho_d / 9).
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from crt import capi, scenes
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.fixture(scope="module")
+def cornell():
+    sc = scenes.cornell_box(0)
+    v, i, inst = sc.merged()
+    vn = oracle.vertex_normals(v, i)
+    col = np.array(sc.colors)
+    gpu = capi.Scene(v, i, inst, vn, colors=col)
+    cpu = oracle.Scene(v, i, inst, vn, colors=col)
+    return gpu, cpu
+
+
+@pytest.fixture(scope="module")
+def icosphere():
+    m = scenes.displaced_icosphere(8)
+    return m, capi.Scene(m.vertices, m.indices), oracle.Scene(m.vertices, m.indices)
+
+
+def _light():
+    return dict(scenes.cornell_light(), type="area")
+
+
+def test_render_matches_oracle_same_keys(cornell):
+    gpu, cpu = cornell
+    cam = scenes.cornell_camera()
+    cam["resolution"] = [128, 128]
+    rgba, u8, spp, tm = gpu.render(cam, [_light()], 16, 16, 0.0, 4, seed=7)
+    rgb_c, spp_c = cpu.render(cam, [_light()], 16, 16, 0.0, 4, seed=7)
+    g = rgba[..., :3]
+    assert np.all(spp == 16) and np.all(rgba[..., 3] == 1.0)
+    diff = np.abs(g - rgb_c)
+    scale = np.maximum(np.abs(rgb_c), 1e-3)
+    close = np.all(diff <= 1e-3 * scale + 1e-5, axis=-1)
+    print("render same-key agreement: %.4f of pixels, mean |diff| %.3e" % (close.mean(), diff.mean()))
+    assert close.mean() > 0.97
+    assert abs(g.mean() - rgb_c.mean()) < 2e-3 * rgb_c.mean()
+    # uint8 image = trunc(clamp(256 x, 0, 255)), alpha 255
+    assert np.array_equal(u8[..., :3], np.clip(g * 256.0, 0, 255).astype(np.uint8))
+    assert np.all(u8[..., 3] == 255)
+
+
+def test_render_adaptive_sampling_matches_oracle(cornell):
+    gpu, cpu = cornell
+    cam = scenes.cornell_camera()
+    cam["resolution"] = [64, 64]
+    rgba, _, spp, _ = gpu.render(cam, [_light()], 2, 12, 0.05, 3, seed=3)
+    rgb_c, spp_c = cpu.render(cam, [_light()], 2, 12, 0.05, 3, seed=3)
+    agree = spp == spp_c
+    print("adaptive: spp agreement %.4f, mean spp %.2f" % (agree.mean(), spp.mean()))
+    assert agree.mean() > 0.95
+    assert spp.min() >= 2 and spp.max() <= 12 and spp.min() < 12
+
+
+def test_render_statistics_match_reference(cornell):
+    path = os.path.join(GOLD, "cornell_radiance.npz")
+    if not os.path.exists(path):
+        pytest.skip("reference radiance statistics not generated")
+    g = np.load(path)
+    gpu, _ = cornell
+    cam = scenes.cornell_camera()
+    M = 16
+    runs = np.stack([gpu.render(cam, [_light()], 16, 16, 0.0, 4, seed=100 + k)[0][..., :3] for k in range(M)])
+    mu_g, var_g = runs.mean(0), runs.var(0, ddof=1)
+    K = int(g["k_runs"])
+    mu_r, var_r = g["mean"].astype(np.float64), g["var"].astype(np.float64)
+    se = np.sqrt(var_g / M + var_r / K)
+    ok = se > 0
+    z = np.abs(mu_g - mu_r)[ok] / se[ok]
+    frac = (z > 3.0).mean()
+    zero_both = (~ok) & (np.abs(mu_g - mu_r) > 1e-6)
+    print("radiance vs reference: |z|>3 fraction %.4f (MC expectation 0.0027), image means %.5f vs %.5f"
+          % (frac, mu_g.mean(), mu_r.mean()))
+    assert frac < 0.01
+    assert zero_both.sum() == 0
+    # whole-image mean within 3 standard errors
+    se_img = math.sqrt(var_g.sum() / M + var_r.sum() / K) / mu_g.size
+    assert abs(mu_g.mean() - mu_r.mean()) < 3 * se_img + 1e-6
+
+
+def test_render_point_and_directional_lights(cornell):
+    gpu, cpu = cornell
+    cam = scenes.cornell_camera()
+    cam["resolution"] = [64, 64]
+    lights = [{"type": "point", "intensity": 0.05, "position": np.array([0.5, 0.9, 0.5])},
+              {"type": "directional", "intensity": 0.2, "direction": np.array([0.3, 0.2, 1.0])}]
+    rgba, _, _, _ = gpu.render(cam, lights, 4, 4, 0.0, 2, seed=1)
+    rgb_c, _ = cpu.render(cam, lights, 4, 4, 0.0, 2, seed=1)
+    close = np.all(np.abs(rgba[..., :3] - rgb_c) <= 1e-3 * np.maximum(np.abs(rgb_c), 1e-3) + 1e-5, axis=-1)
+    assert close.mean() > 0.97
+
+
+def test_lidar_batch_matches_reference(icosphere):
+    m, gpu, cpu = icosphere
+    g = np.load(os.path.join(GOLD, "lidar_batch.npz"))
+    r = gpu.lidar_batch(g["position"], g["rotation"].reshape(-1, 9))
+    ref = g["range"]
+    hit_both = (r > 0) & (ref > 0)
+    rel = np.abs(r[hit_both] - ref[hit_both]) / ref[hit_both]
+    bad = np.where((r > 0) != (ref > 0))[0]
+    print("lidar: %d beams, hit-status disagreements %d, max rel range err %.2e" % (r.size, bad.size, rel.max()))
+    assert bad.size <= 1
+    assert rel.max() < 1e-5
+    assert np.allclose(r[:8], g["single_range"], rtol=1e-5)
+
+
+def test_flash_lidar(icosphere):
+    m, gpu, cpu = icosphere
+    beams = scenes.small_body_camera(m, resolution=128, seed=3)
+    out = gpu.flash_lidar(beams, sub_rays=8, sigma=5e-4, seed=4)
+    # single return == boresight closest hit (reference LiDAR semantics)
+    c = cpu.camera_pass(beams)
+    hit = (c["tri"] >= 0) & (out["single"] > 0)
+    rel = np.abs(out["single"][hit] - c["depth"][hit]) / c["depth"][hit]
+    assert rel.max() < 1e-5
+    n = out["n_returns"]
+    assert n.max() <= 8
+    any_ret = n > 0
+    assert np.all(out["first"][any_ret] <= out["last"][any_ret])
+    assert np.all(out["first"][~any_ret] == 0) and np.all(out["last"][~any_ret] == 0)
+    # sub-ray spread: 0.5 mrad at ~100 km is ~50 m on the surface -> first/last bracket the single return
+    both = any_ret & (out["single"] > 0) & (n == 8)
+    spread = out["last"][both] - out["first"][both]
+    assert np.median(spread) > 0
+
+
+def test_radiation_pressure_matches_oracle(icosphere):
+    m, gpu, cpu = icosphere
+    sun = scenes.sun_direction(2)
+    info = gpu.info()
+    ctr, rad = info["sphere"][:3], info["sphere"][3]
+    P, rs, rd = 4.56e-6, 0.3, 0.6
+    F, T, nh, tm = gpu.radiation_pressure(sun, P, rs, rd, 1024, ctr, rad, (0, 0, 0))
+    Fc, Tc, nhc = cpu.radiation_pressure(sun, P, rs, rd, 1024, ctr, rad, (0, 0, 0))
+    print("SRP: F gpu %s oracle %s; hits %d vs %d" % (F, Fc, nh, nhc))
+    assert abs(nh - nhc) <= 2e-4 * nhc
+    assert np.linalg.norm(F - Fc) < 1e-4 * np.linalg.norm(Fc)
+    assert np.linalg.norm(T - Tc) < 1e-4 * np.linalg.norm(Fc) * 2400.0
+
+
+def test_radiation_pressure_sphere_closed_form():
+    m = scenes.displaced_icosphere(7, radius=1000.0, amplitude=0.0)
+    gpu = capi.Scene(m.vertices, m.indices)
+    sun = np.array([0.3, -0.5, 0.8])
+    sun /= np.linalg.norm(sun)
+    P, rs, rd = 4.56e-6, 0.3, 0.6
+    F, T, nh, _ = gpu.radiation_pressure(sun, P, rs, rd, 2048, (0, 0, 0), 1010.0, (0, 0, 0))
+    Fexp = P * math.pi * 1000.0 ** 2 * (1 + 4 * rd / 9)
+    along = -F @ sun
+    assert along == pytest.approx(Fexp, rel=2e-3)
+    assert np.linalg.norm(F + along * sun) < 1e-3 * Fexp
+    # linear in the pressure
+    F2, _, _, _ = gpu.radiation_pressure(sun, 2 * P, rs, rd, 2048, (0, 0, 0), 1010.0, (0, 0, 0))
+    assert np.allclose(F2, 2 * F, rtol=1e-6)
