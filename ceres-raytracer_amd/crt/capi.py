@@ -74,6 +74,11 @@ class FlashOpts(C.Structure):
                 ("seed", C.c_uint64)]
 
 
+class BatchOpts(C.Structure):
+    _fields_ = [("samples_per_pass", C.c_uint32), ("use_graph", C.c_uint32), ("persistent", C.c_uint32),
+                ("blocks_per_cu", C.c_uint32), ("grab", C.c_uint32), ("pad_", C.c_uint32)]
+
+
 class FluxOpts(C.Structure):
     _fields_ = [("sun_dir", C.c_double * 3), ("pressure", C.c_double), ("rho_s", C.c_double),
                 ("rho_d", C.c_double), ("grid", C.c_uint32), ("pad_", C.c_uint32), ("center", C.c_double * 3),
@@ -96,6 +101,8 @@ SIGNATURES = {
     "crt_trace_rays": (C.c_int, [_p, C.c_uint64, _p, _p, _p, _p, _p, _p, _p, _p, C.c_uint32, _p, _p]),
     "crt_sun_pass": (C.c_int, [_p, _p, _p, _p, _p, _p, C.c_uint32, _p, _p]),
     "crt_render": (C.c_int, [_p, _p, _p, C.c_uint32, _p, _p, _p, _p, C.c_uint32, _p, _p]),
+    "crt_render_batch": (C.c_int, [_p, C.c_uint32, _p, _p, _p, _p, C.c_uint32, _p, _p, _p, _p, _p, C.c_uint32,
+                                   _p, _p]),
     "crt_lidar_batch": (C.c_int, [_p, C.c_uint64, _p, _p, C.c_int32, _p, C.c_uint32, _p, _p]),
     "crt_flash_lidar": (C.c_int, [_p, _p, _p, _p, _p, _p, C.c_uint32, _p, _p]),
     "crt_radiation_pressure": (C.c_int, [_p, _p, _p, _p, _p, _p, _p]),
@@ -314,3 +321,31 @@ class Scene:
         tm = Timing()
         _check(load().crt_radiation_pressure(self.h, C.byref(o), f, t, C.byref(nh), None, C.byref(tm)))
         return np.array(f[:]), np.array(t[:]), nh.value, tm.as_dict()
+
+    def render_batch(self, cams, lights, min_samples=1, max_samples=1, noise_threshold=0.0, num_bounces=1,
+                     seeds=None, spp=None, samples_per_pass=0, use_graph=False, persistent=False, blocks_per_cu=0,
+                     grab=0, outputs=True):
+        """Render many cameras in one set of launches.  Returns a list of
+        per-camera (rgba f32, spp_used) and the timing dict."""
+        n = len(cams)
+        arr = (Camera * n)(*[make_camera(c) for c in cams])
+        larr = (Light * max(len(lights), 1))(*[make_light(l) for l in lights])
+        opts = RenderOpts(int(min_samples), int(max_samples), float(noise_threshold), int(num_bounces), 0, 0)
+        bo = BatchOpts(int(samples_per_pass), int(bool(use_graph)), int(bool(persistent)), int(blocks_per_cu),
+                       int(grab), 0)
+        sd = None if seeds is None else np.ascontiguousarray(seeds, dtype=np.uint64)
+        ms = None if spp is None else np.ascontiguousarray(spp, dtype=np.int32)
+        sizes = [(int(np.floor(c["resolution"][1])), int(np.floor(c["resolution"][0]))) for c in cams]
+        total = sum(h * w for h, w in sizes)
+        rgba = np.zeros((total, 4), dtype=np.float32) if outputs else None
+        sppo = np.zeros(total, dtype=np.int32) if outputs else None
+        tm = Timing()
+        _check(load().crt_render_batch(self.h, n, arr, ptr(ms), ptr(sd), larr, len(lights), C.byref(opts),
+                                       C.byref(bo), ptr(rgba), None, ptr(sppo), 0, None, C.byref(tm)))
+        out = []
+        if outputs:
+            off = 0
+            for h, w in sizes:
+                out.append((rgba[off:off + h * w].reshape(h, w, 4), sppo[off:off + h * w].reshape(h, w)))
+                off += h * w
+        return out, tm.as_dict()

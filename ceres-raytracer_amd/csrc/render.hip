@@ -1,4 +1,5 @@
-// render.hip -- wavefront path tracer (crt_render).
+// render.hip -- wavefront path tracer for one camera or a batch of cameras
+// (crt_render, crt_render_batch).
 //
 // Semantics of do_render + unidirectional (src/crt/do_render.hpp:65-134,
 // src/crt/path_tracing/unidirectional.hpp:24-96), Lambertian material
@@ -12,18 +13,27 @@
 //     sum is clamped to [0,1] from bounce 1 on; radiance += sum * weight;
 //     then a Malley cosine sample around n_s and weight *= (1 - r1).
 //
-// MI355X structure: samples are processed in passes; each pass is a
-// wavefront over (pixel, sample) paths:
-//   k_raygen  -> FP64 camera ray, rebased, FP32 path state
-//   per bounce: k_extend (octet closest hit) -> k_shade (thread per path:
-//     light samples + shadow rays + next direction) -> k_shadow (octet any
-//     hit) -> k_accumulate (thread per path; survivors are compacted into the
-//     next queue with a wave ballot + one atomic per wave)
-//   k_fold    -> per pixel, folds the pass's samples into the running mean in
-//     sample order and applies the adaptive stop, so results do not depend on
-//     how samples are batched.
-// Randomness is counter-based: u01(seed, key_offset + pixel, sample,
-// 64 * bounce + dim), no shared generator state.
+// MI355X structure.  Work is organised in *passes*; a pass is a list of
+// segments (camera, pixels, samples [s_first, s_first + s_count)) and every
+// (pixel, sample) of every segment is one path.  One pass runs
+//   k_raygen   FP64 camera ray (jittered), rebased, FP32 path state; paths
+//              whose ray misses the scene's bounding sphere are not queued
+//   per bounce: k_extend (octet per path, closest hit) -> k_shade (thread per
+//              path: light samples, shadow rays, next direction) -> k_shadow
+//              (octet per shadow ray, any hit) -> k_accumulate (thread per
+//              path; surviving paths are compacted into the next queue with a
+//              wave ballot + one atomic per wave)
+//   k_fold     per pixel: folds the pass's samples into the running mean in
+//              sample order and applies the adaptive stop.
+// All kernels read their sizes from device memory, so a pass can be captured
+// once in a hipGraph and replayed (the host only rewrites the segment table
+// between passes).  Counters are cleared by a kernel, never by a memset node.
+// The traversal kernels exist in two forms: one octet per queue entry, or
+// persistent (a fixed grid whose octets claim `grab` entries per atomic).
+//
+// Randomness is counter-based: u01(seed_c, pixel, sample, 64*bounce + dim)
+// with a per-camera seed, so a camera rendered inside a batch is bit-for-bit
+// the same as the camera rendered alone with that seed.
 #include "crt_internal.hpp"
 #include "crt_rng.hpp"
 
@@ -39,6 +49,7 @@ inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b
 
 constexpr int MAX_LIGHTS = 8;
 constexpr int TB = 256;
+constexpr uint32_t NO_LIST = 0xFFFFFFFFu;
 
 struct LightDev {
     int type;
@@ -49,31 +60,74 @@ struct LightDev {
     float dir[3];
 };
 
-struct PathState {
-    float4 o;          // origin xyz, unused
-    float4 d;          // direction xyz, unused
-    float4 w;          // weight rgb, pending bounce factor
-    float4 L;          // radiance rgb, unused
-    uint32_t pixel;    // pixel index
-    uint32_t sample;   // 1-based sample index
-    int32_t slot;      // hit triangle (leaf order) or -1
-    float hu, hv;      // hit barycentrics
-    uint32_t pad[3];
+struct CamDev {
+    CamView v;
+    uint64_t seed;
+    uint32_t pix_base;   // first global pixel of this camera
+    uint32_t jitter;     // max_samples != 1
 };
 
-struct RenderParams {
+struct Segment {
+    uint64_t path_begin;   // first path of the segment in the pass
+    uint32_t fold_begin;   // first fold slot (pixel) of the segment in the pass
+    uint32_t cam;
+    uint32_t npix;
+    uint32_t s_first;      // 1-based
+    uint32_t s_count;
+    uint32_t plist_off;    // offset in the pixel list, NO_LIST = pixels 0..npix-1
+};
+
+struct PassCtl {
+    uint64_t npath;
+    uint32_t nfold;
+    uint32_t nseg;
+};
+
+struct PathState {
+    float4 o;          // origin
+    float4 d;          // direction
+    float4 w;          // weight rgb, pending bounce factor (1 - r1)
+    float4 L;          // radiance rgb
+    uint32_t cam;
+    uint32_t pix;      // camera-local pixel index
+    uint32_t sample;   // 1-based
+    int32_t slot;      // hit triangle (leaf order) or -1
+    float hu, hv;
+    uint32_t pad[2];
+};
+
+struct ShadowRec {
+    float4 o;      // origin, tmax
+    float4 d;      // direction
+    float4 c;      // colour * cos * irradiance, occluded flag (written by k_shadow)
+};
+
+// Everything a pass kernel needs, passed by value.
+struct Ctx {
     SceneView s;
-    CamView cam;
     LightDev lights[MAX_LIGHTS];
     int nl;
     int nb;
-    int jitter;
+    int min_samples;
+    float thr;
     float offset;
-    uint64_t seed;
-    uint64_t key_offset;
+    uint32_t grab;
+    const CamDev* cams;
+    const Segment* segs;
+    const PassCtl* ctl;
+    const uint32_t* plist;
+    PathState* st;
+    uint32_t* queue[2];
+    uint32_t* live;
+    ShadowRec* shadow;
+    unsigned long long* tally;   // [0] closest-hit rays, [1] shadow rays
+    float4* samples;
+    float4* mean;
+    int32_t* spp;
+    uint8_t* active;
+    uint32_t* cnt;   // [0],[1] queues, [2] shadow, [3] live, [4] active list, [5],[6] persistent work
 };
 
-__device__ inline float3 f3(float4 a) { return make_float3(a.x, a.y, a.z); }
 __device__ inline float dot3(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 __device__ inline float3 cross3(float3 a, float3 b) {
     return make_float3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
@@ -83,7 +137,7 @@ __device__ inline float3 norm3(float3 a) {
     return make_float3(a.x * inv, a.y * inv, a.z * inv);
 }
 
-// Wave-aggregated append: returns the slot of this lane (valid only if `want`).
+// Wave-aggregated append: slot of this lane (valid only if `want`).
 __device__ inline uint32_t wave_append(bool want, uint32_t* counter) {
     const uint64_t m = __ballot(want);
     const int lane = threadIdx.x & 63;
@@ -94,82 +148,129 @@ __device__ inline uint32_t wave_append(bool want, uint32_t* counter) {
     return base + (uint32_t)__popcll(m & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
 }
 
-// Paths of a pass: list of (pixel, first sample, count); path k of the pass
-// = pixel list[k / count], sample first + k % count.
-__global__ __launch_bounds__(TB) void k_raygen(RenderParams P, const uint32_t* __restrict__ pixels, uint32_t npix,
-                                               uint32_t s_first, uint32_t s_count, PathState* __restrict__ st,
-                                               uint32_t* __restrict__ queue, uint32_t* __restrict__ qcount,
-                                               float4* __restrict__ sample_out) {
+// largest segment index with key(seg) <= x
+template <typename F>
+__device__ inline uint32_t seg_search(uint32_t nseg, uint64_t x, F key) {
+    uint32_t lo = 0, hi = nseg;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (key(mid) <= x) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void k_clear(uint32_t* cnt, uint32_t mask) {
+    const uint32_t i = threadIdx.x;
+    if (i < 32 && ((mask >> i) & 1u)) cnt[i] = 0u;
+}
+
+__global__ __launch_bounds__(TB) void k_raygen(Ctx C) {
     const uint64_t k = (uint64_t)blockIdx.x * TB + threadIdx.x;
-    const uint64_t n = (uint64_t)npix * s_count;
+    const PassCtl ctl = *C.ctl;
     bool alive = false;
-    if (k < n) {
-        const uint32_t pix = pixels[k / s_count];
-        const uint32_t smp = s_first + (uint32_t)(k % s_count);
-        const int px = (int)(pix % (uint32_t)P.cam.w), py = (int)(pix / (uint32_t)P.cam.w);
+    if (k < ctl.npath) {
+        const uint32_t si = seg_search(ctl.nseg, k, [&](uint32_t i) { return C.segs[i].path_begin; });
+        const Segment sg = C.segs[si];
+        const uint64_t local = k - sg.path_begin;
+        const uint32_t i = (uint32_t)(local / sg.s_count);
+        const uint32_t smp = sg.s_first + (uint32_t)(local % sg.s_count);
+        const uint32_t pix = sg.plist_off == NO_LIST ? i : C.plist[sg.plist_off + i];
+        const CamDev& cd = C.cams[sg.cam];
+        const int px = (int)(pix % (uint32_t)cd.v.w), py = (int)(pix / (uint32_t)cd.v.w);
         double u = px, v = py;
-        const uint64_t a = P.key_offset + pix;
-        if (P.jitter) {
-            u += rng_u01(P.seed, a, smp, rng_dim(0, RNG_JITTER)) - 0.5;
-            v += rng_u01(P.seed, a, smp, rng_dim(0, RNG_JITTER + 1)) - 0.5;
+        if (cd.jitter) {
+            u += rng_u01(cd.seed, pix, smp, rng_dim(0, RNG_JITTER)) - 0.5;
+            v += rng_u01(cd.seed, pix, smp, rng_dim(0, RNG_JITTER + 1)) - 0.5;
         }
         double d[3];
-        camera_dir(P.cam, u, v, d);
+        camera_dir(cd.v, u, v, d);
         double t0 = 0.0;
         PathState ps;
-        ps.pixel = pix;
+        ps.cam = sg.cam;
+        ps.pix = pix;
         ps.sample = smp;
         ps.slot = -1;
         ps.hu = ps.hv = 0.f;
         const float tp = (float)(2.0 * 3.14159265358979323846);
         ps.w = make_float4(tp, tp, tp, 1.0f);
         ps.L = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (rebase_t0(P.s, P.cam.pos, d, t0)) {
-            ps.o = make_float4((float)(P.cam.pos[0] + t0 * d[0]), (float)(P.cam.pos[1] + t0 * d[1]),
-                               (float)(P.cam.pos[2] + t0 * d[2]), 0.f);
+        if (rebase_t0(C.s, cd.v.pos, d, t0)) {
+            ps.o = make_float4((float)(cd.v.pos[0] + t0 * d[0]), (float)(cd.v.pos[1] + t0 * d[1]),
+                               (float)(cd.v.pos[2] + t0 * d[2]), 0.f);
             ps.d = make_float4((float)d[0], (float)d[1], (float)d[2], 0.f);
             alive = true;
         } else {
             ps.o = make_float4(0.f, 0.f, 0.f, 0.f);
             ps.d = make_float4(0.f, 0.f, 1.f, 0.f);
+            C.samples[k] = make_float4(0.f, 0.f, 0.f, 0.f);   // camera ray misses the scene
         }
-        st[k] = ps;
-        if (!alive) sample_out[k] = make_float4(0.f, 0.f, 0.f, 0.f);   // camera ray misses the scene
+        C.st[k] = ps;
     }
-    const uint32_t slot = wave_append(alive, qcount);
-    if (alive) queue[slot] = (uint32_t)k;
+    const uint32_t slot = wave_append(alive, C.cnt + 0);
+    if (alive) C.queue[0][slot] = (uint32_t)k;
 }
 
-__global__ __launch_bounds__(TRACE_BLOCK) void k_extend(SceneView s, PathState* __restrict__ st,
-                                                        const uint32_t* __restrict__ queue,
-                                                        const uint32_t* __restrict__ qcount) {
-    extern __shared__ uint32_t smem[];
-    const int lane = threadIdx.x & 63;
-    const int sub = lane & 7, lane_base = lane & ~7, local = threadIdx.x >> 3;
-    uint32_t* sref = smem + (size_t)local * s.stack_cap;
-    float* sd = reinterpret_cast<float*>(smem + (size_t)OCTETS_PER_BLOCK * s.stack_cap) + (size_t)local * s.stack_cap;
-    const uint32_t q = blockIdx.x * OCTETS_PER_BLOCK + local;
-    if (q >= *qcount) return;
-    const uint32_t k = queue[q];
-    const float4 o = st[k].o, d = st[k].d;
+// ---- traversal kernels -------------------------------------------------
+__device__ inline void extend_one(const Ctx& C, uint32_t k, uint32_t* sref, float* sd, int sub, int lane_base) {
+    const float4 o = C.st[k].o, d = C.st[k].d;
     const RayPre r = make_ray_pre(o.x, o.y, o.z, d.x, d.y, d.z);
-    const HitRec h = trace_octet<false>(s, r, 0.0f, __int_as_float(0x7f800000), sref, sd, sub, lane_base);
+    const HitRec h = trace_octet<false>(C.s, r, 0.0f, __int_as_float(0x7f800000), sref, sd, sub, lane_base);
     if (sub == 0) {
-        st[k].slot = h.slot;
-        st[k].hu = h.u;
-        st[k].hv = h.v;
+        C.st[k].slot = h.slot;
+        C.st[k].hu = h.u;
+        C.st[k].hv = h.v;
     }
 }
 
-struct ShadowRec {
-    float4 o;      // origin, tmax
-    float4 d;      // direction, unused
-    float4 c;      // colour * cos * irradiance, occluded flag (set by k_shadow)
-};
+__device__ inline void shadow_one(const Ctx& C, uint32_t i, uint32_t* sref, float* sd, int sub, int lane_base) {
+    const float4 o = C.shadow[i].o, d = C.shadow[i].d;
+    const RayPre r = make_ray_pre(o.x, o.y, o.z, d.x, d.y, d.z);
+    const HitRec h = trace_octet<true>(C.s, r, 0.0f, o.w, sref, sd, sub, lane_base);
+    if (sub == 0) C.shadow[i].c.w = h.slot >= 0 ? 1.0f : 0.0f;
+}
 
-__device__ inline void light_sample(const RenderParams& P, int l, float3 p, uint64_t a, uint32_t smp, int bounce,
-                                    float3& dir, float& tmax, float& irr) {
-    const LightDev& L = P.lights[l];
+#define OCTET_SETUP(C)                                                                                      \
+    extern __shared__ uint32_t smem[];                                                                      \
+    const int lane = threadIdx.x & 63;                                                                      \
+    const int sub = lane & 7, lane_base = lane & ~7, local = threadIdx.x >> 3;                              \
+    uint32_t* sref = smem + (size_t)local * (C).s.stack_cap;                                                \
+    float* sd = reinterpret_cast<float*>(smem + (size_t)OCTETS_PER_BLOCK * (C).s.stack_cap) +                \
+                (size_t)local * (C).s.stack_cap;
+
+template <bool SHADOW>
+__global__ __launch_bounds__(TRACE_BLOCK) void k_trace(Ctx C, int qsel) {
+    OCTET_SETUP(C)
+    const uint32_t q = blockIdx.x * OCTETS_PER_BLOCK + local;
+    const uint32_t n = SHADOW ? C.cnt[2] : C.cnt[qsel];
+    if (q >= n) return;
+    if (SHADOW) shadow_one(C, q, sref, sd, sub, lane_base);
+    else extend_one(C, C.queue[qsel][q], sref, sd, sub, lane_base);
+}
+
+// Persistent form: octets claim `grab` consecutive entries per atomic.
+template <bool SHADOW>
+__global__ __launch_bounds__(TRACE_BLOCK) void k_trace_persistent(Ctx C, int qsel) {
+    OCTET_SETUP(C)
+    (void)local;
+    uint32_t* work = C.cnt + (SHADOW ? 6 : 5);
+    const uint32_t n = SHADOW ? C.cnt[2] : C.cnt[qsel];
+    while (true) {
+        uint32_t base = 0;
+        if (sub == 0) base = atomicAdd(work, C.grab);
+        base = (uint32_t)__shfl((int)base, lane_base, 64);
+        if (base >= n) break;
+        const uint32_t end = min(base + C.grab, n);
+        for (uint32_t q = base; q < end; ++q) {
+            if (SHADOW) shadow_one(C, q, sref, sd, sub, lane_base);
+            else extend_one(C, C.queue[qsel][q], sref, sd, sub, lane_base);
+        }
+    }
+}
+
+// ---- shading -----------------------------------------------------------
+__device__ inline void light_sample(const Ctx& C, int l, float3 p, uint64_t seed, uint32_t pix, uint32_t smp,
+                                    int bounce, float3& dir, float& tmax, float& irr) {
+    const LightDev& L = C.lights[l];
     if (L.type == CRT_LIGHT_POINT) {
         // src/crt/lights/point_light.hpp:32-39
         const float3 dd = make_float3(L.pos[0] - p.x, L.pos[1] - p.y, L.pos[2] - p.z);
@@ -181,8 +282,8 @@ __device__ inline void light_sample(const RenderParams& P, int l, float3 p, uint
     } else if (L.type == CRT_LIGHT_AREA) {
         // src/crt/lights/area_light.hpp:54-72: uniform point of the local XY
         // rectangle, placed with R (transform.hpp:31-40); I / d^2, no cosine
-        const float r1 = (float)rng_u01(P.seed, a, smp, rng_dim(bounce, RNG_LIGHT + 2 * l));
-        const float r2 = (float)rng_u01(P.seed, a, smp, rng_dim(bounce, RNG_LIGHT + 2 * l + 1));
+        const float r1 = (float)rng_u01(seed, pix, smp, rng_dim(bounce, RNG_LIGHT + 2 * l));
+        const float r2 = (float)rng_u01(seed, pix, smp, rng_dim(bounce, RNG_LIGHT + 2 * l + 1));
         const float x = L.size[0] * r1 - L.size[0] / 2, y = L.size[1] * r2 - L.size[1] / 2;
         const float3 sp = make_float3(L.rot[0] * x + L.rot[1] * y + L.pos[0], L.rot[3] * x + L.rot[4] * y + L.pos[1],
                                       L.rot[6] * x + L.rot[7] * y + L.pos[2]);
@@ -219,29 +320,22 @@ __device__ inline float3 cosine_sample(float3 n, float r1, float r2) {
     return make_float3(nx.x * x + ny.x * y + n.x * z, nx.y * x + ny.y * y + n.y * z, nx.z * x + ny.z * y + n.z * z);
 }
 
-__global__ __launch_bounds__(TB) void k_shade(RenderParams P, int bounce, PathState* __restrict__ st,
-                                              const uint32_t* __restrict__ queue, const uint32_t* __restrict__ qcount,
-                                              ShadowRec* __restrict__ shadow, uint32_t* __restrict__ squeue,
-                                              uint32_t* __restrict__ scount, float4* __restrict__ sample_out,
-                                              uint32_t* __restrict__ live_count, uint32_t* __restrict__ live) {
+__global__ __launch_bounds__(TB) void k_shade(Ctx C, int bounce, int qsel) {
     const uint32_t q = blockIdx.x * TB + threadIdx.x;
-    const uint32_t n = *qcount;
+    const uint32_t n = C.cnt[qsel];
     bool has_hit = false;
     uint32_t k = 0;
     PathState ps;
     if (q < n) {
-        k = queue[q];
-        ps = st[k];
-        if (ps.slot < 0) {
-            sample_out[k] = ps.L;     // path ends: no hit
-        } else {
-            has_hit = true;
-        }
+        k = C.queue[qsel][q];
+        ps = C.st[k];
+        if (ps.slot < 0) C.samples[k] = ps.L;   // path ends without a hit
+        else has_hit = true;
     }
-    const uint32_t ls = wave_append(has_hit, live_count);
-    if (has_hit) live[ls] = k;
+    const uint32_t ls = wave_append(has_hit, C.cnt + 3);   // live slot = shadow slots ls*nl .. ls*nl+nl-1
+    if (has_hit) C.live[ls] = k;
     if (!has_hit) return;
-    const SceneView& s = P.s;
+    const SceneView& s = C.s;
     const float4 a = s.tris[3 * ps.slot], b = s.tris[3 * ps.slot + 1], c = s.tris[3 * ps.slot + 2];
     const uint32_t orig = __float_as_uint(a.w);
     const uint32_t inst = s.tri_inst[orig];
@@ -258,67 +352,51 @@ __global__ __launch_bounds__(TB) void k_shade(RenderParams P, int bounce, PathSt
                                u * s.vnormals[3 * i1 + 2] + v * s.vnormals[3 * i2 + 2] + w * s.vnormals[3 * i0 + 2]));
     }
     float3 p = make_float3(u * b.x + v * c.x + w * a.x, u * b.y + v * c.y + w * a.y, u * b.z + v * c.z + w * a.z);
-    p = make_float3(p.x - P.offset * gn.x, p.y - P.offset * gn.y, p.z - P.offset * gn.z);
+    p = make_float3(p.x - C.offset * gn.x, p.y - C.offset * gn.y, p.z - C.offset * gn.z);
     const float* col = s.inst_color + 3 * (inst >= 1 && inst <= s.n_instances ? inst - 1 : 0);
-    const uint64_t key = P.key_offset + ps.pixel;
-    for (int l = 0; l < P.nl; ++l) {
+    const uint64_t seed = C.cams[ps.cam].seed;
+    for (int l = 0; l < C.nl; ++l) {
         float3 ld;
         float tmax, irr;
-        light_sample(P, l, p, key, ps.sample, bounce, ld, tmax, irr);
+        light_sample(C, l, p, seed, ps.pix, ps.sample, bounce, ld, tmax, irr);
         // ColoredLambertianMaterial::compute: colour * (-L.n_s), no max(0, .)
         const float ldn = -dot3(ld, sn);
         ShadowRec r;
         r.o = make_float4(p.x, p.y, p.z, tmax);
         r.d = make_float4(ld.x, ld.y, ld.z, 0.f);
         r.c = make_float4(col[0] * ldn * irr, col[1] * ldn * irr, col[2] * ldn * irr, 0.f);
-        shadow[(size_t)k * P.nl + l] = r;
+        C.shadow[ls * (uint32_t)C.nl + (uint32_t)l] = r;
     }
-    if (P.nl > 0) {
-        const uint32_t base = atomicAdd(scount, (uint32_t)P.nl);
-        for (int l = 0; l < P.nl; ++l) squeue[base + l] = (uint32_t)((size_t)k * P.nl + l);
-    }
-    if (bounce < P.nb - 1) {
-        const float r1 = (float)rng_u01(P.seed, key, ps.sample, rng_dim(bounce, RNG_BRDF));
-        const float r2 = (float)rng_u01(P.seed, key, ps.sample, rng_dim(bounce, RNG_BRDF + 1));
+    if (bounce < C.nb - 1) {
+        const float r1 = (float)rng_u01(seed, ps.pix, ps.sample, rng_dim(bounce, RNG_BRDF));
+        const float r2 = (float)rng_u01(seed, ps.pix, ps.sample, rng_dim(bounce, RNG_BRDF + 1));
         const float3 nd = cosine_sample(sn, r1, r2);
-        st[k].o = make_float4(p.x, p.y, p.z, 0.f);
-        st[k].d = make_float4(nd.x, nd.y, nd.z, 0.f);
-        st[k].w.w = 1.0f - r1;
+        C.st[k].o = make_float4(p.x, p.y, p.z, 0.f);
+        C.st[k].d = make_float4(nd.x, nd.y, nd.z, 0.f);
+        C.st[k].w.w = 1.0f - r1;
     }
 }
 
-__global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(SceneView s, ShadowRec* __restrict__ shadow,
-                                                        const uint32_t* __restrict__ squeue,
-                                                        const uint32_t* __restrict__ scount) {
-    extern __shared__ uint32_t smem[];
-    const int lane = threadIdx.x & 63;
-    const int sub = lane & 7, lane_base = lane & ~7, local = threadIdx.x >> 3;
-    uint32_t* sref = smem + (size_t)local * s.stack_cap;
-    float* sd = reinterpret_cast<float*>(smem + (size_t)OCTETS_PER_BLOCK * s.stack_cap) + (size_t)local * s.stack_cap;
-    const uint32_t q = blockIdx.x * OCTETS_PER_BLOCK + local;
-    if (q >= *scount) return;
-    const uint32_t i = squeue[q];
-    const float4 o = shadow[i].o, d = shadow[i].d;
-    const RayPre r = make_ray_pre(o.x, o.y, o.z, d.x, d.y, d.z);
-    const HitRec h = trace_octet<true>(s, r, 0.0f, o.w, sref, sd, sub, lane_base);
-    if (sub == 0) shadow[i].c.w = h.slot >= 0 ? 1.0f : 0.0f;
+// After k_shade: shadow rays = nl per live path; tally traced rays.
+__global__ void k_tally(Ctx C, int qsel) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        C.cnt[2] = C.cnt[3] * (uint32_t)C.nl;
+        C.tally[0] += C.cnt[qsel];
+        C.tally[1] += C.cnt[2];
+    }
 }
 
-__global__ __launch_bounds__(TB) void k_accumulate(RenderParams P, int bounce, PathState* __restrict__ st,
-                                                   const uint32_t* __restrict__ live, const uint32_t* __restrict__ lcount,
-                                                   const ShadowRec* __restrict__ shadow,
-                                                   uint32_t* __restrict__ next_queue, uint32_t* __restrict__ next_count,
-                                                   float4* __restrict__ sample_out) {
+__global__ __launch_bounds__(TB) void k_accumulate(Ctx C, int bounce, int next) {
     const uint32_t q = blockIdx.x * TB + threadIdx.x;
-    const uint32_t n = *lcount;
+    const uint32_t n = C.cnt[3];
     bool cont = false;
     uint32_t k = 0;
     if (q < n) {
-        k = live[q];
-        PathState& ps = st[k];
+        k = C.live[q];
+        PathState& ps = C.st[k];
         float lr[3] = {0.f, 0.f, 0.f};
-        for (int l = 0; l < P.nl; ++l) {
-            const float4 c = shadow[(size_t)k * P.nl + l].c;
+        for (int l = 0; l < C.nl; ++l) {
+            const float4 c = C.shadow[q * (uint32_t)C.nl + (uint32_t)l].c;
             if (c.w == 0.0f) { lr[0] += c.x; lr[1] += c.y; lr[2] += c.z; }
         }
         if (bounce >= 1)
@@ -329,44 +407,52 @@ __global__ __launch_bounds__(TB) void k_accumulate(RenderParams P, int bounce, P
         L.y += lr[1] * w.y;
         L.z += lr[2] * w.z;
         ps.L = L;
-        if (bounce == P.nb - 1) {
-            sample_out[k] = L;
+        if (bounce == C.nb - 1) {
+            C.samples[k] = L;
         } else {
             w.x *= w.w; w.y *= w.w; w.z *= w.w;
             ps.w = w;
             cont = true;
         }
     }
-    const uint32_t slot = wave_append(cont, next_count);
-    if (cont) next_queue[slot] = k;
+    const uint32_t slot = wave_append(cont, C.cnt + next);
+    if (cont) C.queue[next][slot] = k;
+}
+
+__global__ __launch_bounds__(TB) void k_zero_samples(Ctx C) {
+    const uint64_t k = (uint64_t)blockIdx.x * TB + threadIdx.x;
+    if (k < C.ctl->npath) C.samples[k] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
 // Fold the pass's samples into the running mean (do_render.hpp:100-108).
-__global__ __launch_bounds__(TB) void k_fold(const uint32_t* __restrict__ pixels, uint32_t npix, uint32_t s_first,
-                                             uint32_t s_count, int min_samples, float thr,
-                                             const float4* __restrict__ sample_out, float4* __restrict__ mean,
-                                             int32_t* __restrict__ spp_used, uint8_t* __restrict__ active) {
-    const uint32_t i = blockIdx.x * TB + threadIdx.x;
-    if (i >= npix) return;
-    const uint32_t pix = pixels[i];
-    float4 m = mean[pix];
-    int used = spp_used[pix];
+__global__ __launch_bounds__(TB) void k_fold(Ctx C) {
+    const uint32_t t = blockIdx.x * TB + threadIdx.x;
+    const PassCtl ctl = *C.ctl;
+    if (t >= ctl.nfold) return;
+    const uint32_t si = seg_search(ctl.nseg, t, [&](uint32_t i) { return (uint64_t)C.segs[i].fold_begin; });
+    const Segment sg = C.segs[si];
+    const uint32_t i = t - sg.fold_begin;
+    const uint32_t pix = sg.plist_off == NO_LIST ? i : C.plist[sg.plist_off + i];
+    const uint32_t g = C.cams[sg.cam].pix_base + pix;
+    float4 m = C.mean[g];
+    int used = C.spp[g];
     bool act = true;
-    for (uint32_t j = 0; j < s_count; ++j) {
-        const uint32_t smp = s_first + j;
-        const float4 x = sample_out[(size_t)i * s_count + j];
+    const uint64_t base = sg.path_begin + (uint64_t)i * sg.s_count;
+    for (uint32_t j = 0; j < sg.s_count; ++j) {
+        const uint32_t smp = sg.s_first + j;
+        const float4 x = C.samples[base + j];
         const float inv = 1.0f / (float)smp;
         const float rx = (x.x - m.x) * inv, ry = (x.y - m.y) * inv, rz = (x.z - m.z) * inv;
         m.x += rx; m.y += ry; m.z += rz;
         used = (int)smp;
-        if ((int)smp >= min_samples) {
+        if ((int)smp >= C.min_samples) {
             const float noise = sqrtf(rx * rx + ry * ry + rz * rz);
-            if (noise < thr) { act = false; break; }
+            if (noise < C.thr) { act = false; break; }
         }
     }
-    mean[pix] = m;
-    spp_used[pix] = used;
-    active[pix] = act ? 1 : 0;
+    C.mean[g] = m;
+    C.spp[g] = used;
+    C.active[g] = act ? 1 : 0;
 }
 
 __global__ __launch_bounds__(TB) void k_finish(const float4* __restrict__ mean, uint32_t npix, float* __restrict__ rgba,
@@ -382,12 +468,7 @@ __global__ __launch_bounds__(TB) void k_finish(const float4* __restrict__ mean, 
     }
 }
 
-__global__ void k_iota(uint32_t* p, uint32_t n) {
-    const uint32_t i = blockIdx.x * TB + threadIdx.x;
-    if (i < n) p[i] = i;
-}
-
-// compact active pixels into a list
+// Active pixels (global ids) of the cameras listed in `mask` (1 byte per pixel).
 __global__ __launch_bounds__(TB) void k_active_list(const uint8_t* __restrict__ active, uint32_t npix,
                                                     uint32_t* __restrict__ list, uint32_t* __restrict__ count) {
     const uint32_t i = blockIdx.x * TB + threadIdx.x;
@@ -396,32 +477,97 @@ __global__ __launch_bounds__(TB) void k_active_list(const uint8_t* __restrict__ 
     if (a) list[slot] = i;
 }
 
-}  // namespace
+__global__ __launch_bounds__(TB) void k_fill_u8(uint8_t* p, uint32_t n, uint8_t v) {
+    const uint32_t i = blockIdx.x * TB + threadIdx.x;
+    if (i < n) p[i] = v;
+}
 
-}  // namespace crt
+// Launch the kernels of one pass.  Grids are sized for `cap_paths` /
+// `cap_fold` so that the same sequence can be captured once and replayed.
+void launch_pass(const Ctx& C, uint64_t cap_paths, uint32_t cap_fold, const crt_batch_opts& B, int num_cus,
+                 hipStream_t st) {
+    const size_t lds = trace_lds_bytes(C.s);
+    const uint32_t gp = std::max(1u, cdiv(cap_paths, TB));
+    const uint32_t go = std::max(1u, cdiv(cap_paths, OCTETS_PER_BLOCK));
+    const uint32_t gs = std::max(1u, cdiv(cap_paths * std::max(1, C.nl), OCTETS_PER_BLOCK));
+    const uint32_t gpers = std::max(1u, B.blocks_per_cu) * (uint32_t)num_cus;
+    hipLaunchKernelGGL(k_clear, dim3(1), dim3(32), 0, st, C.cnt, 0x7Fu);
+    hipLaunchKernelGGL(k_raygen, dim3(gp), dim3(TB), 0, st, C);
+    for (int b = 0; b < C.nb; ++b) {
+        const int cur = b & 1, nxt = cur ^ 1;
+        hipLaunchKernelGGL(k_clear, dim3(1), dim3(32), 0, st, C.cnt, (1u << nxt) | (1u << 2) | (1u << 3) | (3u << 5));
+        if (B.persistent)
+            hipLaunchKernelGGL(k_trace_persistent<false>, dim3(std::min(gpers, go)), dim3(TRACE_BLOCK), lds, st, C, cur);
+        else
+            hipLaunchKernelGGL(k_trace<false>, dim3(go), dim3(TRACE_BLOCK), lds, st, C, cur);
+        hipLaunchKernelGGL(k_shade, dim3(gp), dim3(TB), 0, st, C, b, cur);
+        hipLaunchKernelGGL(k_tally, dim3(1), dim3(32), 0, st, C, cur);
+        if (C.nl > 0) {
+            if (B.persistent)
+                hipLaunchKernelGGL(k_trace_persistent<true>, dim3(std::min(gpers, gs)), dim3(TRACE_BLOCK), lds, st, C, 0);
+            else
+                hipLaunchKernelGGL(k_trace<true>, dim3(gs), dim3(TRACE_BLOCK), lds, st, C, 0);
+        }
+        hipLaunchKernelGGL(k_accumulate, dim3(gp), dim3(TB), 0, st, C, b, nxt);
+    }
+    if (C.nb == 0) hipLaunchKernelGGL(k_zero_samples, dim3(gp), dim3(TB), 0, st, C);
+    hipLaunchKernelGGL(k_fold, dim3(std::max(1u, cdiv(cap_fold, TB))), dim3(TB), 0, st, C);
+}
 
-using namespace crt;
+struct HostSeg {
+    Segment s;
+};
 
-extern "C" int crt_render(crt_scene* scene, const crt_camera* camera, const crt_light* lights, uint32_t n_lights,
-                          const crt_render_opts* opts, float* rgba, uint8_t* rgba_u8, int32_t* spp_used,
-                          uint32_t flags, void* stream, crt_timing* timing) {
-  return guarded([&] {
+int render_batch(crt_scene* scene, uint32_t n_cams, const crt_camera* cams, const int32_t* max_samples,
+                 const uint64_t* seeds, const crt_light* lights, uint32_t n_lights, const crt_render_opts* opts,
+                 const crt_batch_opts* bopts, float* rgba, uint8_t* rgba_u8, int32_t* spp_used, uint32_t flags,
+                 void* stream, crt_timing* timing) {
     CRT_REQUIRE(scene != nullptr, CRT_ERR_INVALID, "scene is NULL");
-    check_camera_args(camera);
+    CRT_REQUIRE(n_cams >= 1 && cams != nullptr, CRT_ERR_INVALID, "no cameras");
     CRT_REQUIRE(opts != nullptr, CRT_ERR_INVALID, "render options are NULL");
     CRT_REQUIRE(n_lights <= (uint32_t)MAX_LIGHTS, CRT_ERR_INVALID, "at most 8 lights per render");
     CRT_REQUIRE(n_lights == 0 || lights != nullptr, CRT_ERR_INVALID, "lights is NULL");
-    CRT_REQUIRE(opts->max_samples >= 1, CRT_ERR_INVALID, "max_samples must be >= 1");
     CRT_REQUIRE(opts->num_bounces >= 0, CRT_ERR_INVALID, "num_bounces must be >= 0");
+    crt_batch_opts B{};
+    if (bopts) B = *bopts;
+    if (B.grab == 0) B.grab = 4;
+    if (B.blocks_per_cu == 0) B.blocks_per_cu = 8;
     hipStream_t st = (hipStream_t)stream;
-    RenderParams P;
-    std::memset(&P, 0, sizeof(P));
-    P.s = scene->view;
-    P.cam = make_cam_view(*camera);
+    int dev = 0, num_cus = 0;
+    CRT_HIP_CHECK(hipGetDevice(&dev));
+    CRT_HIP_CHECK(hipDeviceGetAttribute(&num_cus, hipDeviceAttributeMultiprocessorCount, dev));
+
+    // ---- cameras ----
+    std::vector<CamDev> hc(n_cams);
+    std::vector<int> spp_c(n_cams);
+    uint64_t total_pix = 0;
+    for (uint32_t c = 0; c < n_cams; ++c) {
+        check_camera_args(&cams[c]);
+        hc[c].v = make_cam_view(cams[c]);
+        spp_c[c] = max_samples ? max_samples[c] : opts->max_samples;
+        CRT_REQUIRE(spp_c[c] >= 1, CRT_ERR_INVALID, "max_samples must be >= 1");
+        hc[c].seed = seeds ? seeds[c] : opts->seed + c;
+        hc[c].jitter = spp_c[c] != 1;
+        hc[c].pix_base = (uint32_t)total_pix;
+        total_pix += (uint64_t)hc[c].v.w * hc[c].v.h;
+        CRT_REQUIRE(total_pix < (1ull << 31), CRT_ERR_INVALID, "too many pixels in one batch");
+    }
+    const uint32_t P = (uint32_t)total_pix;
+    const int min_s = std::max(1, opts->min_samples);
+    bool adaptive = opts->noise_threshold > 0.0;
+    {
+        bool any = false;
+        for (uint32_t c = 0; c < n_cams; ++c) any |= spp_c[c] > min_s;
+        adaptive = adaptive && any;
+    }
+
+    Ctx C;
+    std::memset(&C, 0, sizeof(C));
+    C.s = scene->view;
     for (uint32_t l = 0; l < n_lights; ++l) {
         const crt_light& L = lights[l];
         CRT_REQUIRE(L.type >= 0 && L.type <= 2, CRT_ERR_TYPE, "unknown light type");
-        LightDev& D = P.lights[l];
+        LightDev& D = C.lights[l];
         D.type = L.type;
         D.intensity = (float)L.intensity;
         for (int k = 0; k < 3; ++k) D.pos[k] = (float)L.position[k];
@@ -435,134 +581,200 @@ extern "C" int crt_render(crt_scene* scene, const crt_camera* camera, const crt_
             for (int k = 0; k < 3; ++k) D.dir[k] = (float)(L.direction[k] / n);
         }
     }
-    P.nl = (int)n_lights;
-    P.nb = opts->num_bounces;
-    P.jitter = opts->max_samples != 1;
-    P.seed = opts->seed;
-    P.key_offset = 0;
-    P.offset = self_offset(scene);
-    const uint32_t npix = (uint32_t)P.cam.w * (uint32_t)P.cam.h;
-    const int min_s = std::max(1, std::min(opts->min_samples, opts->max_samples));
-    const int max_s = opts->max_samples;
-    const size_t budget = (size_t)1 << 22;   // paths per wavefront pass
+    C.nl = (int)n_lights;
+    C.nb = opts->num_bounces;
+    C.min_samples = min_s;
+    C.thr = (float)opts->noise_threshold;
+    C.offset = self_offset(scene);
+    C.grab = B.grab;
 
-    // persistent per-call buffers
-    DevBuf mean, spp, active, plist, counters;
-    mean.alloc((size_t)npix * 16);
-    spp.alloc((size_t)npix * 4);
-    active.alloc(npix);
-    plist.alloc((size_t)npix * 4);
-    counters.alloc(64);
+    // ---- pass planning ----
+    const uint64_t budget = (uint64_t)1 << 23;   // paths per pass
+    int S = (int)B.samples_per_pass;
+    if (S <= 0) {
+        int smax = 1;
+        for (int v : spp_c) smax = std::max(smax, v);
+        S = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)smax, budget / std::max<uint64_t>(P, 1)));
+    }
+    // capacity: the largest pass we will plan
+    const uint64_t cap_paths = std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)P * S, std::max(budget, (uint64_t)P)));
+
+    DevBuf d_cams, d_segs, d_ctl, d_plist, mean, spp, active, counters, tally, pstate, q0, q1, live, shadow, samples;
+    d_cams.alloc(sizeof(CamDev) * n_cams);
+    d_segs.alloc(sizeof(Segment) * (n_cams + 1));
+    d_ctl.alloc(sizeof(PassCtl));
+    d_plist.alloc((size_t)P * 4);
+    mean.alloc((size_t)P * 16);
+    spp.alloc((size_t)P * 4);
+    active.alloc(P);
+    counters.alloc(32 * 4);
+    pstate.alloc(cap_paths * sizeof(PathState));
+    q0.alloc(cap_paths * 4);
+    q1.alloc(cap_paths * 4);
+    live.alloc(cap_paths * 4);
+    shadow.alloc(cap_paths * std::max(1, C.nl) * sizeof(ShadowRec));
+    tally.alloc(16);
+    samples.alloc(cap_paths * 16);
+    CRT_HIP_CHECK(hipMemcpyAsync(d_cams.p, hc.data(), sizeof(CamDev) * n_cams, hipMemcpyHostToDevice, st));
     CRT_HIP_CHECK(hipMemsetAsync(mean.p, 0, mean.bytes, st));
     CRT_HIP_CHECK(hipMemsetAsync(spp.p, 0, spp.bytes, st));
-    uint32_t* cnt = counters.as<uint32_t>();   // [0] queue, [1] next queue, [2] shadow, [3] live, [4] active list
-    hipLaunchKernelGGL(k_iota, dim3(cdiv(npix, TB)), dim3(TB), 0, st, plist.as<uint32_t>(), npix);
+    CRT_HIP_CHECK(hipMemsetAsync(tally.p, 0, tally.bytes, st));
+    hipLaunchKernelGGL(k_fill_u8, dim3(std::max(1u, cdiv(P, TB))), dim3(TB), 0, st, active.as<uint8_t>(), P, (uint8_t)1);
+    C.cams = d_cams.as<CamDev>();
+    C.segs = d_segs.as<Segment>();
+    C.ctl = d_ctl.as<PassCtl>();
+    C.plist = d_plist.as<uint32_t>();
+    C.st = pstate.as<PathState>();
+    C.queue[0] = q0.as<uint32_t>();
+    C.queue[1] = q1.as<uint32_t>();
+    C.live = live.as<uint32_t>();
+    C.shadow = shadow.as<ShadowRec>();
+    C.tally = tally.as<unsigned long long>();
+    C.samples = samples.as<float4>();
+    C.mean = mean.as<float4>();
+    C.spp = spp.as<int32_t>();
+    C.active = active.as<uint8_t>();
+    C.cnt = counters.as<uint32_t>();
+
     hipEvent_t e0, e1;
     CRT_HIP_CHECK(hipEventCreate(&e0));
     CRT_HIP_CHECK(hipEventCreate(&e1));
     CRT_HIP_CHECK(hipEventRecord(e0, st));
-    uint64_t rays = 0, srays = 0;
-    uint32_t n_active = npix;
-    int s_done = 0;
-    DevBuf pstate, queue[2], live, shadow, squeue, samples;
-    while (s_done < max_s && n_active > 0) {
-        // samples of this pass
-        int s_count = (s_done == 0) ? min_s : std::min(max_s - s_done, std::max(1, (int)(budget / n_active)));
-        if (s_done == 0 && (size_t)n_active * s_count > budget * 4)
-            s_count = std::max(1, (int)(budget * 4 / n_active));
-        const uint32_t s_first = (uint32_t)s_done + 1;
-        const uint64_t npath = (uint64_t)n_active * s_count;
-        CRT_REQUIRE(npath < (1ull << 32), CRT_ERR_INVALID, "render pass too large");
-        pstate.reserve(npath * sizeof(PathState));
-        queue[0].reserve(npath * 4);
-        queue[1].reserve(npath * 4);
-        live.reserve(npath * 4);
-        shadow.reserve(npath * std::max(1, P.nl) * sizeof(ShadowRec));
-        squeue.reserve(npath * std::max(1, P.nl) * 4);
-        samples.reserve(npath * 16);
-        CRT_HIP_CHECK(hipMemsetAsync(cnt, 0, 64, st));
-        hipLaunchKernelGGL(k_raygen, dim3(cdiv(npath, TB)), dim3(TB), 0, st, P, plist.as<uint32_t>(), n_active,
-                           s_first, (uint32_t)s_count, pstate.as<PathState>(), queue[0].as<uint32_t>(), cnt + 0,
-                           samples.as<float4>());
-        int cur = 0;
-        for (int b = 0; b < P.nb; ++b) {
-            uint32_t* qc = cnt + (cur == 0 ? 0 : 1);
-            uint32_t* nc = cnt + (cur == 0 ? 1 : 0);
-            CRT_HIP_CHECK(hipMemsetAsync(nc, 0, 4, st));
-            CRT_HIP_CHECK(hipMemsetAsync(cnt + 2, 0, 8, st));   // shadow + live counters
-            hipLaunchKernelGGL(k_extend, dim3(cdiv(npath, OCTETS_PER_BLOCK)), dim3(TRACE_BLOCK), trace_lds_bytes(P.s),
-                               st, P.s, pstate.as<PathState>(), queue[cur].as<uint32_t>(), qc);
-            hipLaunchKernelGGL(k_shade, dim3(cdiv(npath, TB)), dim3(TB), 0, st, P, b, pstate.as<PathState>(),
-                               queue[cur].as<uint32_t>(), qc, shadow.as<ShadowRec>(), squeue.as<uint32_t>(), cnt + 2,
-                               samples.as<float4>(), cnt + 3, live.as<uint32_t>());
-            if (P.nl > 0)
-                hipLaunchKernelGGL(k_shadow, dim3(cdiv(npath * P.nl, OCTETS_PER_BLOCK)), dim3(TRACE_BLOCK),
-                                   trace_lds_bytes(P.s), st, P.s, shadow.as<ShadowRec>(), squeue.as<uint32_t>(),
-                                   cnt + 2);
-            hipLaunchKernelGGL(k_accumulate, dim3(cdiv(npath, TB)), dim3(TB), 0, st, P, b, pstate.as<PathState>(),
-                               live.as<uint32_t>(), cnt + 3, shadow.as<ShadowRec>(), queue[cur ^ 1].as<uint32_t>(), nc,
-                               samples.as<float4>());
-            uint32_t hc[4];
-            CRT_HIP_CHECK(hipMemcpyAsync(hc, cnt, 16, hipMemcpyDeviceToHost, st));
+
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    const bool use_graph = B.use_graph && !adaptive;
+    std::vector<int> done(n_cams, 0);
+    std::vector<uint32_t> n_act(n_cams);
+    std::vector<uint32_t> list_off(n_cams, NO_LIST);
+    for (uint32_t c = 0; c < n_cams; ++c) n_act[c] = (uint32_t)hc[c].v.w * hc[c].v.h;
+    uint64_t passes = 0;
+    while (true) {
+        // segments of this pass
+        std::vector<Segment> segs;
+        uint64_t npath = 0;
+        uint32_t nfold = 0;
+        for (uint32_t c = 0; c < n_cams; ++c) {
+            if (done[c] >= spp_c[c] || n_act[c] == 0) continue;
+            Segment sg;
+            sg.path_begin = npath;
+            sg.fold_begin = nfold;
+            sg.cam = c;
+            sg.npix = n_act[c];
+            sg.s_first = (uint32_t)done[c] + 1;
+            sg.s_count = (uint32_t)std::min(S, spp_c[c] - done[c]);
+            sg.plist_off = list_off[c];
+            if (npath + (uint64_t)sg.npix * sg.s_count > cap_paths) {
+                if (!segs.empty()) break;                 // this camera waits for the next pass
+                sg.s_count = (uint32_t)std::max<uint64_t>(1, cap_paths / sg.npix);
+                CRT_REQUIRE((uint64_t)sg.npix * sg.s_count <= cap_paths, CRT_ERR_INVALID, "camera too large");
+            }
+            npath += (uint64_t)sg.npix * sg.s_count;
+            nfold += sg.npix;
+            segs.push_back(sg);
+        }
+        if (segs.empty()) break;
+        PassCtl ctl{npath, nfold, (uint32_t)segs.size()};
+        CRT_HIP_CHECK(hipMemcpyAsync(d_segs.p, segs.data(), sizeof(Segment) * segs.size(), hipMemcpyHostToDevice, st));
+        CRT_HIP_CHECK(hipMemcpyAsync(d_ctl.p, &ctl, sizeof(ctl), hipMemcpyHostToDevice, st));
+        CRT_HIP_CHECK(hipStreamSynchronize(st));   // host tables are reused next pass
+        if (use_graph) {
+            if (!gexec) {
+                CRT_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                launch_pass(C, cap_paths, P, B, num_cus, st);
+                CRT_HIP_CHECK(hipStreamEndCapture(st, &graph));
+                CRT_HIP_CHECK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+            }
+            CRT_HIP_CHECK(hipGraphLaunch(gexec, st));
+        } else {
+            launch_pass(C, npath, nfold, B, num_cus, st);
+        }
+        CRT_HIP_CHECK(hipGetLastError());
+        ++passes;
+        for (const Segment& sg : segs) done[sg.cam] += (int)sg.s_count;
+        if (adaptive) {
+            // remaining active pixels, per camera, in increasing pixel order
+            CRT_HIP_CHECK(hipMemsetAsync(C.cnt + 4, 0, 4, st));
+            hipLaunchKernelGGL(k_active_list, dim3(std::max(1u, cdiv(P, TB))), dim3(TB), 0, st, C.active, P,
+                               d_plist.as<uint32_t>(), C.cnt + 4);
+            uint32_t na = 0;
+            CRT_HIP_CHECK(hipMemcpyAsync(&na, C.cnt + 4, 4, hipMemcpyDeviceToHost, st));
             CRT_HIP_CHECK(hipStreamSynchronize(st));
-            rays += hc[cur == 0 ? 0 : 1];
-            srays += hc[2];
-            cur ^= 1;
-            if (hc[cur == 0 ? 0 : 1] == 0) break;   // no path continues
-        }
-        if (P.nb == 0) {
-            // no bounce: every path contributes zero radiance
-            CRT_HIP_CHECK(hipMemsetAsync(samples.p, 0, npath * 16, st));
-        }
-        hipLaunchKernelGGL(k_fold, dim3(cdiv(n_active, TB)), dim3(TB), 0, st, plist.as<uint32_t>(), n_active, s_first,
-                           (uint32_t)s_count, min_s, (float)opts->noise_threshold, samples.as<float4>(),
-                           mean.as<float4>(), spp.as<int32_t>(), active.as<uint8_t>());
-        s_done += s_count;
-        if (s_done >= max_s) break;
-        // next pass: remaining active pixels
-        CRT_HIP_CHECK(hipMemsetAsync(cnt + 4, 0, 4, st));
-        hipLaunchKernelGGL(k_active_list, dim3(cdiv(npix, TB)), dim3(TB), 0, st, active.as<uint8_t>(), npix,
-                           plist.as<uint32_t>(), cnt + 4);
-        uint32_t na = 0;
-        CRT_HIP_CHECK(hipMemcpyAsync(&na, cnt + 4, 4, hipMemcpyDeviceToHost, st));
-        CRT_HIP_CHECK(hipStreamSynchronize(st));
-        // keep the pixel list sorted so that pass-to-pass order is stable
-        if (na > 0) {
             std::vector<uint32_t> hl(na);
-            CRT_HIP_CHECK(hipMemcpy(hl.data(), plist.p, (size_t)na * 4, hipMemcpyDeviceToHost));
+            if (na) CRT_HIP_CHECK(hipMemcpy(hl.data(), d_plist.p, (size_t)na * 4, hipMemcpyDeviceToHost));
             std::sort(hl.begin(), hl.end());
-            CRT_HIP_CHECK(hipMemcpy(plist.p, hl.data(), (size_t)na * 4, hipMemcpyHostToDevice));
+            std::vector<uint32_t> local(na);
+            size_t pos = 0;
+            for (uint32_t c = 0; c < n_cams; ++c) {
+                const uint32_t lo = hc[c].pix_base, hi = lo + (uint32_t)hc[c].v.w * hc[c].v.h;
+                const size_t start = pos;
+                while (pos < hl.size() && hl[pos] < hi) { local[pos] = hl[pos] - lo; ++pos; }
+                n_act[c] = (uint32_t)(pos - start);
+                list_off[c] = (uint32_t)start;
+            }
+            if (na) CRT_HIP_CHECK(hipMemcpy(d_plist.p, local.data(), (size_t)na * 4, hipMemcpyHostToDevice));
         }
-        n_active = na;
     }
-    const bool dev = (flags & CRT_OUT_DEVICE) != 0;
+    const bool dev_out = (flags & CRT_OUT_DEVICE) != 0;
     DevBuf o_rgba, o_u8;
     float* d_rgba = rgba;
     uint8_t* d_u8 = rgba_u8;
-    if (!dev) {
-        if (rgba) { o_rgba.alloc((size_t)npix * 16); d_rgba = o_rgba.as<float>(); }
-        if (rgba_u8) { o_u8.alloc((size_t)npix * 4); d_u8 = o_u8.as<uint8_t>(); }
+    if (!dev_out) {
+        if (rgba) { o_rgba.alloc((size_t)P * 16); d_rgba = o_rgba.as<float>(); }
+        if (rgba_u8) { o_u8.alloc((size_t)P * 4); d_u8 = o_u8.as<uint8_t>(); }
     }
-    hipLaunchKernelGGL(k_finish, dim3(cdiv(npix, TB)), dim3(TB), 0, st, mean.as<float4>(), npix, d_rgba, d_u8);
+    hipLaunchKernelGGL(k_finish, dim3(std::max(1u, cdiv(P, TB))), dim3(TB), 0, st, C.mean, P, d_rgba, d_u8);
     CRT_HIP_CHECK(hipGetLastError());
     CRT_HIP_CHECK(hipEventRecord(e1, st));
-    if (!dev) {
-        if (rgba) CRT_HIP_CHECK(hipMemcpyAsync(rgba, d_rgba, (size_t)npix * 16, hipMemcpyDeviceToHost, st));
-        if (rgba_u8) CRT_HIP_CHECK(hipMemcpyAsync(rgba_u8, d_u8, (size_t)npix * 4, hipMemcpyDeviceToHost, st));
-        if (spp_used) CRT_HIP_CHECK(hipMemcpyAsync(spp_used, spp.p, (size_t)npix * 4, hipMemcpyDeviceToHost, st));
+    if (!dev_out) {
+        if (rgba) CRT_HIP_CHECK(hipMemcpyAsync(rgba, d_rgba, (size_t)P * 16, hipMemcpyDeviceToHost, st));
+        if (rgba_u8) CRT_HIP_CHECK(hipMemcpyAsync(rgba_u8, d_u8, (size_t)P * 4, hipMemcpyDeviceToHost, st));
+        if (spp_used) CRT_HIP_CHECK(hipMemcpyAsync(spp_used, spp.p, (size_t)P * 4, hipMemcpyDeviceToHost, st));
     } else if (spp_used) {
-        CRT_HIP_CHECK(hipMemcpyAsync(spp_used, spp.p, (size_t)npix * 4, hipMemcpyDeviceToDevice, st));
+        CRT_HIP_CHECK(hipMemcpyAsync(spp_used, spp.p, (size_t)P * 4, hipMemcpyDeviceToDevice, st));
     }
+    unsigned long long ht[2] = {0, 0};
+    CRT_HIP_CHECK(hipMemcpyAsync(ht, tally.p, 16, hipMemcpyDeviceToHost, st));
     CRT_HIP_CHECK(hipStreamSynchronize(st));
     float ms = 0.f;
     CRT_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
     if (timing) {
         std::memset(timing, 0, sizeof(*timing));
         timing->total_ms = ms;
-        timing->rays_traced = rays;
-        timing->shadow_rays = srays;
+        timing->rays_traced = ht[0];
+        timing->shadow_rays = ht[1];
+        (void)passes;
     }
-  });
+    return CRT_OK;
+}
+
+}  // namespace
+}  // namespace crt
+
+using namespace crt;
+
+extern "C" int crt_render(crt_scene* scene, const crt_camera* camera, const crt_light* lights, uint32_t n_lights,
+                          const crt_render_opts* opts, float* rgba, uint8_t* rgba_u8, int32_t* spp_used,
+                          uint32_t flags, void* stream, crt_timing* timing) {
+    return guarded([&] {
+        CRT_REQUIRE(opts != nullptr, CRT_ERR_INVALID, "render options are NULL");
+        const uint64_t seed = opts->seed;
+        render_batch(scene, 1, camera, nullptr, &seed, lights, n_lights, opts, nullptr, rgba, rgba_u8, spp_used,
+                     flags, stream, timing);
+    });
+}
+
+extern "C" int crt_render_batch(crt_scene* scene, uint32_t n_cams, const crt_camera* cams,
+                                const int32_t* max_samples, const uint64_t* seeds, const crt_light* lights,
+                                uint32_t n_lights, const crt_render_opts* opts, const crt_batch_opts* bopts,
+                                float* rgba, uint8_t* rgba_u8, int32_t* spp_used, uint32_t flags, void* stream,
+                                crt_timing* timing) {
+    return guarded([&] {
+        render_batch(scene, n_cams, cams, max_samples, seeds, lights, n_lights, opts, bopts, rgba, rgba_u8, spp_used,
+                     flags, stream, timing);
+    });
 }

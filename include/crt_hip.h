@@ -194,6 +194,28 @@ int crt_render(crt_scene* scene, const crt_camera* camera, const crt_light* ligh
                const crt_render_opts* opts, float* rgba, uint8_t* rgba_u8, int32_t* spp_used,
                uint32_t flags, void* stream, crt_timing* timing);
 
+/* ---- batched render (new) ------------------------------------------------
+ * n_cams cameras of one scene in one set of launches.  Camera c uses
+ * max_samples[c] (NULL: opts->max_samples) and RNG seed seeds[c] (NULL:
+ * opts->seed + c); its image is bit-for-bit the crt_render image of that
+ * camera with that seed.  Outputs are packed in camera order, each camera
+ * row-major [H][W][4] (rgba, rgba_u8) or [H][W] (spp_used).
+ * Knobs (crt_batch_opts, zeros = defaults): samples per pass, hipGraph replay
+ * of a pass (non-adaptive renders only), persistent traversal kernels with
+ * blocks_per_cu x CUs blocks claiming `grab` rays per atomic.              */
+typedef struct {
+    uint32_t samples_per_pass;
+    uint32_t use_graph;
+    uint32_t persistent;
+    uint32_t blocks_per_cu;
+    uint32_t grab;
+    uint32_t pad_;
+} crt_batch_opts;
+int crt_render_batch(crt_scene* scene, uint32_t n_cams, const crt_camera* cams, const int32_t* max_samples,
+                     const uint64_t* seeds, const crt_light* lights, uint32_t n_lights,
+                     const crt_render_opts* opts, const crt_batch_opts* bopts, float* rgba, uint8_t* rgba_u8,
+                     int32_t* spp_used, uint32_t flags, void* stream, crt_timing* timing);
+
 /* ---- LiDAR (src/crt/do_lidar.hpp:13-155) ------------------------------
  * One ray per pose along +z (z_positive) or -z rotated by R^T; range =
  * |hit - position| (0 on a miss).  positions n x 3, rotations n x 9
