@@ -28,6 +28,10 @@ import numpy as np  # noqa: E402
 # the reference CPU tracer (SweepSAH + PRO + NLO BVH, SingleRayTraverser
 # Statistics) by tests/golden/make_golden.py; used for the algorithmic bytes.
 STATS_PATH = os.path.join(ROOT, "tests", "golden", "c2_reference_stats.json")
+# Measured HBM traffic of the dominant kernel at this workload: rocprofv3
+# FETCH_SIZE / WRITE_SIZE per dispatch, scaled by the same counters read on a
+# plain copy of known size (bench.py --copy-calibration).
+TRAFFIC_PATH = os.path.join(ROOT, "profiles", "k_primary_sun_icosphere_L8_2048_traffic.json")
 
 
 def log(msg):
@@ -90,6 +94,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--resolution", type=int, default=2048)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--out", default=None, help="also write the JSON line to this file")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain 1 GiB device copy (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the path shards by image tiles with no exchange); "
@@ -98,6 +105,16 @@ def main():
 
     import torch
     from crt import capi, scenes
+
+    if args.copy_calibration:
+        n = 1 << 28   # 2^28 float32 = 1 GiB read + 1 GiB written per copy
+        x = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "copies": 5}))
+        return
 
     log("generating icosphere L8")
     mesh = scenes.displaced_icosphere(8, 2000.0, 0.2, 6, 0)
@@ -145,9 +162,13 @@ def main():
             stats = json.load(fh)
         bc, ba = algorithmic_bytes(stats)
         achieved = prim_rays * bc / (avg_trace * 1e-3) / 1e9
+        traffic = None
+        if os.path.exists(TRAFFIC_PATH):
+            with open(TRAFFIC_PATH) as fh:
+                traffic = json.load(fh).get("hbm_bytes_per_launch")
         roof = {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s", "frac": achieved / 8000.0,
-                "traffic": None, "kernel": "k_primary_sun", "bytes_per_ray": bc,
-                "kernel_ms": avg_trace}
+                "traffic": traffic, "kernel": "k_primary_sun", "bytes_per_ray": bc,
+                "algorithmic_bytes_per_launch": prim_rays * bc, "kernel_ms": avg_trace}
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline")
@@ -181,6 +202,9 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+    if args.out:
+        with open(args.out, "w") as fh:
+            fh.write(json.dumps(line) + "\n")
 
 
 if __name__ == "__main__":

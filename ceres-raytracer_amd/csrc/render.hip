@@ -40,6 +40,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <vector>
 
 namespace crt {
@@ -514,14 +515,34 @@ void launch_pass(const Ctx& C, uint64_t cap_paths, uint32_t cap_fold, const crt_
     hipLaunchKernelGGL(k_fold, dim3(std::max(1u, cdiv(cap_fold, TB))), dim3(TB), 0, st, C);
 }
 
-struct HostSeg {
-    Segment s;
+}  // namespace
+
+// Grow-only device buffers of one render caller (a scene's default
+// workspace, or one per serving worker so that workers never allocate).
+struct RenderWorkspace {
+    DevBuf d_cams, d_segs, d_ctl, d_plist, mean, spp, active, counters, tally, pstate, q0, q1, live, shadow, samples,
+        o_rgba, o_u8;
 };
+
+RenderWorkspace* new_render_workspace() { return new RenderWorkspace(); }
+void delete_render_workspace(RenderWorkspace* w) { delete w; }
+
+size_t render_bytes_estimate(uint64_t pixels, int spp, int n_lights) {
+    const uint64_t paths = std::min<uint64_t>(pixels * (uint64_t)std::max(spp, 1), (uint64_t)1 << 23);
+    return (size_t)(paths * (sizeof(PathState) + 12 + 16 + std::max(n_lights, 1) * sizeof(ShadowRec)) +
+                    pixels * (16 + 4 + 1 + 4 + 16 + 4));
+}
 
 int render_batch(crt_scene* scene, uint32_t n_cams, const crt_camera* cams, const int32_t* max_samples,
                  const uint64_t* seeds, const crt_light* lights, uint32_t n_lights, const crt_render_opts* opts,
                  const crt_batch_opts* bopts, float* rgba, uint8_t* rgba_u8, int32_t* spp_used, uint32_t flags,
-                 void* stream, crt_timing* timing) {
+                 void* stream, crt_timing* timing, RenderWorkspace* wsp) {
+    std::unique_ptr<RenderWorkspace> own;
+    if (!wsp) {
+        own.reset(new RenderWorkspace());
+        wsp = own.get();
+    }
+    RenderWorkspace& W = *wsp;
     CRT_REQUIRE(scene != nullptr, CRT_ERR_INVALID, "scene is NULL");
     CRT_REQUIRE(n_cams >= 1 && cams != nullptr, CRT_ERR_INVALID, "no cameras");
     CRT_REQUIRE(opts != nullptr, CRT_ERR_INVALID, "render options are NULL");
@@ -599,26 +620,28 @@ int render_batch(crt_scene* scene, uint32_t n_cams, const crt_camera* cams, cons
     // capacity: the largest pass we will plan
     const uint64_t cap_paths = std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)P * S, std::max(budget, (uint64_t)P)));
 
-    DevBuf d_cams, d_segs, d_ctl, d_plist, mean, spp, active, counters, tally, pstate, q0, q1, live, shadow, samples;
-    d_cams.alloc(sizeof(CamDev) * n_cams);
-    d_segs.alloc(sizeof(Segment) * (n_cams + 1));
-    d_ctl.alloc(sizeof(PassCtl));
-    d_plist.alloc((size_t)P * 4);
-    mean.alloc((size_t)P * 16);
-    spp.alloc((size_t)P * 4);
-    active.alloc(P);
-    counters.alloc(32 * 4);
-    pstate.alloc(cap_paths * sizeof(PathState));
-    q0.alloc(cap_paths * 4);
-    q1.alloc(cap_paths * 4);
-    live.alloc(cap_paths * 4);
-    shadow.alloc(cap_paths * std::max(1, C.nl) * sizeof(ShadowRec));
-    tally.alloc(16);
-    samples.alloc(cap_paths * 16);
+    DevBuf &d_cams = W.d_cams, &d_segs = W.d_segs, &d_ctl = W.d_ctl, &d_plist = W.d_plist, &mean = W.mean,
+           &spp = W.spp, &active = W.active, &counters = W.counters, &tally = W.tally, &pstate = W.pstate,
+           &q0 = W.q0, &q1 = W.q1, &live = W.live, &shadow = W.shadow, &samples = W.samples;
+    d_cams.reserve(sizeof(CamDev) * n_cams);
+    d_segs.reserve(sizeof(Segment) * (n_cams + 1));
+    d_ctl.reserve(sizeof(PassCtl));
+    d_plist.reserve((size_t)P * 4);
+    mean.reserve((size_t)P * 16);
+    spp.reserve((size_t)P * 4);
+    active.reserve(P);
+    counters.reserve(32 * 4);
+    pstate.reserve(cap_paths * sizeof(PathState));
+    q0.reserve(cap_paths * 4);
+    q1.reserve(cap_paths * 4);
+    live.reserve(cap_paths * 4);
+    shadow.reserve(cap_paths * std::max(1, C.nl) * sizeof(ShadowRec));
+    tally.reserve(16);
+    samples.reserve(cap_paths * 16);
     CRT_HIP_CHECK(hipMemcpyAsync(d_cams.p, hc.data(), sizeof(CamDev) * n_cams, hipMemcpyHostToDevice, st));
-    CRT_HIP_CHECK(hipMemsetAsync(mean.p, 0, mean.bytes, st));
-    CRT_HIP_CHECK(hipMemsetAsync(spp.p, 0, spp.bytes, st));
-    CRT_HIP_CHECK(hipMemsetAsync(tally.p, 0, tally.bytes, st));
+    CRT_HIP_CHECK(hipMemsetAsync(mean.p, 0, (size_t)P * 16, st));
+    CRT_HIP_CHECK(hipMemsetAsync(spp.p, 0, (size_t)P * 4, st));
+    CRT_HIP_CHECK(hipMemsetAsync(tally.p, 0, 16, st));
     hipLaunchKernelGGL(k_fill_u8, dim3(std::max(1u, cdiv(P, TB))), dim3(TB), 0, st, active.as<uint8_t>(), P, (uint8_t)1);
     C.cams = d_cams.as<CamDev>();
     C.segs = d_segs.as<Segment>();
@@ -716,12 +739,11 @@ int render_batch(crt_scene* scene, uint32_t n_cams, const crt_camera* cams, cons
         }
     }
     const bool dev_out = (flags & CRT_OUT_DEVICE) != 0;
-    DevBuf o_rgba, o_u8;
     float* d_rgba = rgba;
     uint8_t* d_u8 = rgba_u8;
     if (!dev_out) {
-        if (rgba) { o_rgba.alloc((size_t)P * 16); d_rgba = o_rgba.as<float>(); }
-        if (rgba_u8) { o_u8.alloc((size_t)P * 4); d_u8 = o_u8.as<uint8_t>(); }
+        if (rgba) { W.o_rgba.reserve((size_t)P * 16); d_rgba = W.o_rgba.as<float>(); }
+        if (rgba_u8) { W.o_u8.reserve((size_t)P * 4); d_u8 = W.o_u8.as<uint8_t>(); }
     }
     hipLaunchKernelGGL(k_finish, dim3(std::max(1u, cdiv(P, TB))), dim3(TB), 0, st, C.mean, P, d_rgba, d_u8);
     CRT_HIP_CHECK(hipGetLastError());
@@ -752,7 +774,6 @@ int render_batch(crt_scene* scene, uint32_t n_cams, const crt_camera* cams, cons
     return CRT_OK;
 }
 
-}  // namespace
 }  // namespace crt
 
 using namespace crt;
@@ -764,7 +785,7 @@ extern "C" int crt_render(crt_scene* scene, const crt_camera* camera, const crt_
         CRT_REQUIRE(opts != nullptr, CRT_ERR_INVALID, "render options are NULL");
         const uint64_t seed = opts->seed;
         render_batch(scene, 1, camera, nullptr, &seed, lights, n_lights, opts, nullptr, rgba, rgba_u8, spp_used,
-                     flags, stream, timing);
+                     flags, stream, timing, nullptr);
     });
 }
 
@@ -775,6 +796,6 @@ extern "C" int crt_render_batch(crt_scene* scene, uint32_t n_cams, const crt_cam
                                 crt_timing* timing) {
     return guarded([&] {
         render_batch(scene, n_cams, cams, max_samples, seeds, lights, n_lights, opts, bopts, rgba, rgba_u8, spp_used,
-                     flags, stream, timing);
+                     flags, stream, timing, nullptr);
     });
 }
