@@ -85,6 +85,29 @@ class FluxOpts(C.Structure):
                 ("radius", C.c_double), ("ref_point", C.c_double * 3)]
 
 
+class ServerOpts(C.Structure):
+    _fields_ = [("memory_budget", C.c_uint64), ("n_workers", C.c_uint32), ("max_batch", C.c_uint32),
+                ("max_batch_paths", C.c_uint64), ("batching", C.c_uint32), ("pad_", C.c_uint32)]
+
+
+class Request(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("scene", C.c_uint32), ("camera", Camera), ("spp", C.c_int32),
+                ("bounces", C.c_int32), ("seed", C.c_uint64), ("n_lights", C.c_uint32), ("sub_rays", C.c_uint32),
+                ("lights", Light * 2), ("sigma", C.c_double), ("flux", FluxOpts),
+                ("rgba", C.c_void_p), ("lidar", C.c_void_p), ("force_torque", C.c_void_p)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("completed", C.c_uint64), ("rejected", C.c_uint64), ("rays", C.c_uint64), ("batches", C.c_uint64),
+                ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("mean_ms", C.c_double), ("wall_s", C.c_double),
+                ("mrays_per_s", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+REQ_RENDER, REQ_LIDAR, REQ_FLUX = 0, 1, 2
+
 _p = C.c_void_p
 _lib = None
 
@@ -103,6 +126,13 @@ SIGNATURES = {
     "crt_render": (C.c_int, [_p, _p, _p, C.c_uint32, _p, _p, _p, _p, C.c_uint32, _p, _p]),
     "crt_render_batch": (C.c_int, [_p, C.c_uint32, _p, _p, _p, _p, C.c_uint32, _p, _p, _p, _p, _p, C.c_uint32,
                                    _p, _p]),
+    "crt_server_create": (C.c_int, [_p, _p]),
+    "crt_server_add_scene": (C.c_int, [_p, _p, _p]),
+    "crt_server_submit": (C.c_int, [_p, _p, _p]),
+    "crt_server_wait": (C.c_int, [_p, C.c_uint64, _p]),
+    "crt_server_drain": (C.c_int, [_p]),
+    "crt_server_stats_get": (C.c_int, [_p, _p]),
+    "crt_server_destroy": (None, [_p]),
     "crt_lidar_batch": (C.c_int, [_p, C.c_uint64, _p, _p, C.c_int32, _p, C.c_uint32, _p, _p]),
     "crt_flash_lidar": (C.c_int, [_p, _p, _p, _p, _p, _p, C.c_uint32, _p, _p]),
     "crt_radiation_pressure": (C.c_int, [_p, _p, _p, _p, _p, _p, _p]),

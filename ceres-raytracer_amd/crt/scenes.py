@@ -347,3 +347,44 @@ def small_body_camera(mesh: Mesh, resolution=2048, seed=1, range_km=(50.0, 200.0
 def sun_direction(seed=2):
     """Unit vector from the body towards the sun (uniform on the sphere)."""
     return random_unit(np.random.default_rng(seed))
+
+
+def geodesic_sphere(k: int, radius: float = 2000.0, amplitude: float = 0.2, octaves: int = 6,
+                    seed: int = 0) -> Mesh:
+    """Geodesic sphere of frequency k: every icosahedron face split into k^2
+    triangles (20 k^2 in total), vertices projected to the sphere and displaced
+    like displaced_icosphere.  Points on shared edges are computed from the
+    same integer barycentric weights summed in global-vertex order, so they are
+    bit-identical from both faces and deduplicate exactly (no cracks)."""
+    v0, f0 = _icosahedron()
+    ii, jj = np.meshgrid(np.arange(k + 1), np.arange(k + 1), indexing="ij")
+    m = (ii + jj) <= k
+    gi, gj = ii[m], jj[m]                     # grid points of one face
+    lid = -np.ones((k + 1, k + 1), dtype=np.int64)
+    lid[gi, gj] = np.arange(gi.size)
+    ui, uj = np.meshgrid(np.arange(k), np.arange(k), indexing="ij")
+    um = (ui + uj) <= k - 1
+    ui, uj = ui[um], uj[um]
+    up = np.stack([lid[ui, uj], lid[ui + 1, uj], lid[ui, uj + 1]], 1)
+    dm = (ui + uj) < k - 1
+    di, dj = ui[dm], uj[dm]
+    down = np.stack([lid[di + 1, dj], lid[di + 1, dj + 1], lid[di, dj + 1]], 1)
+    local_tris = np.concatenate([up, down])
+    pts, tris = [], []
+    for fi in range(f0.shape[0]):
+        a, b, c = f0[fi]
+        w = np.stack([k - gi - gj, gi, gj], 1).astype(np.float64)   # weights of a, b, c
+        order = np.argsort([a, b, c])
+        verts = np.array([a, b, c])[order]
+        ww = w[:, order]
+        p = (ww[:, 0:1] * v0[verts[0]] + ww[:, 1:2] * v0[verts[1]]) + ww[:, 2:3] * v0[verts[2]]
+        pts.append(p / k)
+        tris.append(local_tris + fi * gi.size)
+    P = np.concatenate(pts)
+    T = np.concatenate(tris)
+    uniq, inv = np.unique(P, axis=0, return_inverse=True)
+    inv = inv.reshape(-1)
+    T = inv[T]
+    n = uniq / np.linalg.norm(uniq, axis=1, keepdims=True)
+    r = radius * (1.0 + amplitude * fbm(n, octaves=octaves, seed=seed)) if amplitude else np.full(n.shape[0], radius)
+    return Mesh((n * r[:, None]).astype(np.float32), T.astype(np.uint32), "geodesic_k%d" % k)

@@ -1,0 +1,154 @@
+"""Serving: resident scenes, memory-based admission, continuous batching.
+
+Thin Python view of the crt_server_* C ABI plus the mixed-request workload of
+the served benchmark (Poisson arrivals of renders, LiDAR scans and flux
+grids).  The server itself (admission, batching, worker streams) is native.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import threading
+import time
+
+import numpy as np
+
+from crt import capi
+
+
+class Server:
+    def __init__(self, batching=True, n_workers=3, memory_budget=16 << 30, max_batch=64,
+                 max_batch_paths=1 << 23):
+        capi.init()
+        o = capi.ServerOpts(int(memory_budget), int(n_workers), int(max_batch), int(max_batch_paths),
+                            int(bool(batching)), 0)
+        h = C.c_void_p()
+        capi._check(capi.load().crt_server_create(C.byref(o), C.byref(h)))
+        self.h = h
+        self._keep = {}
+
+    def add_scene(self, scene: capi.Scene) -> int:
+        sid = C.c_uint32()
+        capi._check(capi.load().crt_server_add_scene(self.h, scene.h, C.byref(sid)))
+        self._keep[("scene", sid.value)] = scene
+        return sid.value
+
+    def submit(self, req: dict) -> int:
+        """req: kind ('render'|'lidar'|'flux'), scene, and the kind's fields.
+        Output arrays are allocated here and returned by result()."""
+        r = capi.Request()
+        kind = req["kind"]
+        r.kind = {"render": capi.REQ_RENDER, "lidar": capi.REQ_LIDAR, "flux": capi.REQ_FLUX}[kind]
+        r.scene = int(req["scene"])
+        out = {}
+        if kind in ("render", "lidar"):
+            r.camera = capi.make_camera(req["camera"])
+            h, w = int(np.floor(req["camera"]["resolution"][1])), int(np.floor(req["camera"]["resolution"][0]))
+        if kind == "render":
+            r.spp = int(req["spp"])
+            r.bounces = int(req.get("bounces", 1))
+            r.seed = int(req.get("seed", 0))
+            lights = req.get("lights", [])
+            r.n_lights = len(lights)
+            for k, l in enumerate(lights):
+                r.lights[k] = capi.make_light(l)
+            out["rgba"] = np.zeros((h, w, 4), dtype=np.float32)
+            r.rgba = out["rgba"].ctypes.data
+        elif kind == "lidar":
+            r.sub_rays = int(req.get("sub_rays", 8))
+            r.sigma = float(req.get("sigma", 5e-4))
+            r.seed = int(req.get("seed", 0))
+            out["lidar"] = np.zeros((3, h, w))
+            r.lidar = out["lidar"].ctypes.data
+        else:
+            f = capi.FluxOpts()
+            f.sun_dir[:] = [float(x) for x in req["sun_dir"]]
+            f.pressure, f.rho_s, f.rho_d = float(req["pressure"]), float(req["rho_s"]), float(req["rho_d"])
+            f.grid = int(req["grid"])
+            f.radius = 0.0
+            r.flux = f
+            out["force_torque"] = np.zeros(6)
+            r.force_torque = out["force_torque"].ctypes.data
+        t = C.c_uint64()
+        capi._check(capi.load().crt_server_submit(self.h, C.byref(r), C.byref(t)))
+        self._keep[t.value] = out
+        return t.value
+
+    def result(self, ticket: int):
+        lat = C.c_double()
+        capi._check(capi.load().crt_server_wait(self.h, ticket, C.byref(lat)))
+        return self._keep.pop(ticket), lat.value
+
+    def drain(self):
+        capi._check(capi.load().crt_server_drain(self.h))
+
+    def stats(self) -> dict:
+        s = capi.ServerStats()
+        capi._check(capi.load().crt_server_stats_get(self.h, C.byref(s)))
+        return s.as_dict()
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            capi.load().crt_server_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def mixed_workload(meshes, n_requests, rate_hz, seed=0, max_render=1024, max_lidar=512, max_flux=2048):
+    """Poisson stream: 60% renders (128^2-1024^2, 1-16 spp), 30% flash-LiDAR
+    scans (64^2-512^2 beams, 8 sub-rays), 10% flux grids (0.25-4M rays), over
+    the given resident meshes; poses at 20-500 km."""
+    from crt import scenes
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    reqs = []
+    for k in range(n_requests):
+        t += rng.exponential(1.0 / rate_hz)
+        sid = int(rng.integers(0, len(meshes)))
+        u = rng.uniform()
+        cam_seed = int(rng.integers(0, 2 ** 31))
+        sun = scenes.random_unit(rng)
+        if u < 0.6:
+            res = int(2 ** rng.uniform(7, np.log2(max_render)))
+            cam = scenes.small_body_camera(meshes[sid], resolution=res, seed=cam_seed, range_km=(20.0, 500.0))
+            req = {"kind": "render", "scene": sid, "camera": cam, "spp": int(rng.integers(1, 17)), "bounces": 2,
+                   "seed": cam_seed, "lights": [{"type": "directional", "intensity": 1.0, "direction": sun}]}
+        elif u < 0.9:
+            res = int(2 ** rng.uniform(6, np.log2(max_lidar)))
+            cam = scenes.small_body_camera(meshes[sid], resolution=res, seed=cam_seed, range_km=(20.0, 500.0))
+            req = {"kind": "lidar", "scene": sid, "camera": cam, "sub_rays": 8, "sigma": 5e-4, "seed": cam_seed}
+        else:
+            grid = int(2 ** rng.uniform(9, np.log2(max_flux)))
+            req = {"kind": "flux", "scene": sid, "sun_dir": sun, "pressure": 4.56e-6, "rho_s": 0.3, "rho_d": 0.6,
+                   "grid": grid}
+        reqs.append((t, req))
+    return reqs
+
+
+def run_stream(server: Server, reqs, realtime=True):
+    """Submit (arrival_time, request) pairs at their arrival times from a
+    feeder thread; returns (results by index, latencies ms, stats)."""
+    tickets = [None] * len(reqs)
+    t0 = time.perf_counter()
+
+    def feed():
+        for i, (ta, req) in enumerate(reqs):
+            if realtime:
+                dt = ta - (time.perf_counter() - t0)
+                if dt > 0:
+                    time.sleep(dt)
+            tickets[i] = server.submit(req)
+
+    th = threading.Thread(target=feed)
+    th.start()
+    th.join()
+    results, lat = [], []
+    for t in tickets:
+        out, l = server.result(t)
+        results.append(out)
+        lat.append(l)
+    return results, np.array(lat), server.stats()

@@ -124,6 +124,11 @@ struct Workspace {
     DevBuf counters;
 };
 
+void flash_lidar_impl(crt_scene* scene, const crt_flash_opts* opts, double* single, double* first, double* last,
+                      uint8_t* n_returns, uint32_t flags, void* stream, crt_timing* timing, Workspace& ws);
+void radiation_pressure_impl(crt_scene* scene, const crt_flux_opts* opts, double force[3], double torque[3],
+                             uint64_t* n_hits, void* stream, crt_timing* timing, Workspace& ws);
+
 }  // namespace crt
 
 struct crt_scene {

@@ -306,9 +306,12 @@ int crt_lidar_batch(crt_scene* scene, uint64_t n, const double* positions, const
     });
 }
 
-int crt_flash_lidar(crt_scene* scene, const crt_flash_opts* opts, double* single, double* first, double* last,
-                    uint8_t* n_returns, uint32_t flags, void* stream, crt_timing* timing) {
-    return guarded([&] {
+}  // extern "C"
+
+namespace crt {
+void flash_lidar_impl(crt_scene* scene, const crt_flash_opts* opts, double* single, double* first, double* last,
+                      uint8_t* n_returns, uint32_t flags, void* stream, crt_timing* timing, Workspace& ws) {
+    {
         CRT_REQUIRE(scene != nullptr && opts != nullptr, CRT_ERR_INVALID, "NULL argument");
         check_camera_args(&opts->beams);
         CRT_REQUIRE(opts->sub_rays <= 64, CRT_ERR_INVALID, "at most 64 sub-rays per beam");
@@ -323,7 +326,6 @@ int crt_flash_lidar(crt_scene* scene, const crt_flash_opts* opts, double* single
         const uint64_t nr = nb * (F.sub_rays + 1);
         CRT_REQUIRE(nr < (1ull << 32), CRT_ERR_INVALID, "flash LiDAR too large");
         const bool dev = (flags & CRT_OUT_DEVICE) != 0;
-        Workspace& ws = scene->ws;
         ws.f.reserve(nr * 8);
         double* d_single = single;
         double* d_first = first;
@@ -357,12 +359,12 @@ int crt_flash_lidar(crt_scene* scene, const crt_flash_opts* opts, double* single
             timing->trace_ms = timing->total_ms = tm.ms();
             timing->rays_traced = nr;
         }
-    });
+    }
 }
 
-int crt_radiation_pressure(crt_scene* scene, const crt_flux_opts* opts, double force[3], double torque[3],
-                           uint64_t* n_hits, void* stream, crt_timing* timing) {
-    return guarded([&] {
+void radiation_pressure_impl(crt_scene* scene, const crt_flux_opts* opts, double force[3], double torque[3],
+                             uint64_t* n_hits, void* stream, crt_timing* timing, Workspace& ws) {
+    {
         CRT_REQUIRE(scene != nullptr && opts != nullptr && force && torque, CRT_ERR_INVALID, "NULL argument");
         CRT_REQUIRE(opts->grid >= 1 && opts->grid <= 65536, CRT_ERR_INVALID, "grid must be in 1..65536");
         const double sl = std::sqrt(opts->sun_dir[0] * opts->sun_dir[0] + opts->sun_dir[1] * opts->sun_dir[1] +
@@ -395,7 +397,6 @@ int crt_radiation_pressure(crt_scene* scene, const crt_flux_opts* opts, double f
         F.origin_back = 2.0 * F.radius;
         const uint64_t n = (uint64_t)F.grid * F.grid;
         const uint32_t nblocks = cdiv(n, OCTETS_PER_BLOCK);
-        Workspace& ws = scene->ws;
         ws.h.reserve((size_t)nblocks * 7 * sizeof(Comp));
         ws.counters.reserve(256);
         double* out = ws.counters.as<double>();
@@ -416,6 +417,25 @@ int crt_radiation_pressure(crt_scene* scene, const crt_flux_opts* opts, double f
             timing->trace_ms = timing->total_ms = tm.ms();
             timing->rays_traced = n;
         }
+    }
+}
+}  // namespace crt
+
+extern "C" {
+
+int crt_flash_lidar(crt_scene* scene, const crt_flash_opts* opts, double* single, double* first, double* last,
+                    uint8_t* n_returns, uint32_t flags, void* stream, crt_timing* timing) {
+    return guarded([&] {
+        CRT_REQUIRE(scene != nullptr, CRT_ERR_INVALID, "scene is NULL");
+        flash_lidar_impl(scene, opts, single, first, last, n_returns, flags, stream, timing, scene->ws);
+    });
+}
+
+int crt_radiation_pressure(crt_scene* scene, const crt_flux_opts* opts, double force[3], double torque[3],
+                           uint64_t* n_hits, void* stream, crt_timing* timing) {
+    return guarded([&] {
+        CRT_REQUIRE(scene != nullptr, CRT_ERR_INVALID, "scene is NULL");
+        radiation_pressure_impl(scene, opts, force, torque, n_hits, stream, timing, scene->ws);
     });
 }
 

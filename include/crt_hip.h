@@ -263,6 +263,63 @@ typedef struct {
 int crt_radiation_pressure(crt_scene* scene, const crt_flux_opts* opts, double force[3], double torque[3],
                            uint64_t* n_hits, void* stream, crt_timing* timing);
 
+/* ---- serving (new) ---------------------------------------------------------
+ * A server keeps scenes resident, admits requests by device-memory budget and
+ * runs them on n_workers streams; each worker batches every queued request
+ * that can share its launches (continuous batching).  batching = 0 is the
+ * plain baseline: one worker, one request per launch sequence.  Outputs are
+ * host buffers owned by the caller until crt_server_wait returns.
+ * Results depend only on the request (its seed), never on the mix.           */
+#define CRT_REQ_RENDER 0u
+#define CRT_REQ_LIDAR  1u
+#define CRT_REQ_FLUX   2u
+
+typedef struct crt_server crt_server;
+
+typedef struct {
+    uint64_t memory_budget;     /* bytes of working memory admitted requests may hold (0: 16 GiB) */
+    uint32_t n_workers;         /* concurrent batches / streams (0: 3)          */
+    uint32_t max_batch;         /* requests per batch (0: 64)                   */
+    uint64_t max_batch_paths;   /* camera paths (pixels x spp) per render batch */
+    uint32_t batching;          /* 0: plain per-request launches                */
+    uint32_t pad_;
+} crt_server_opts;
+
+typedef struct {
+    uint32_t kind;              /* CRT_REQ_*                                        */
+    uint32_t scene;             /* id from crt_server_add_scene                     */
+    crt_camera camera;          /* render camera or LiDAR beam grid                 */
+    int32_t spp;                /* render samples per pixel                         */
+    int32_t bounces;            /* render bounces                                   */
+    uint64_t seed;              /* render / LiDAR RNG seed                          */
+    uint32_t n_lights;          /* render lights (<= 2)                             */
+    uint32_t sub_rays;          /* LiDAR sub-rays per beam                          */
+    crt_light lights[2];
+    double sigma;               /* LiDAR sub-ray angular sigma (rad)                */
+    crt_flux_opts flux;         /* radiation-pressure request                       */
+    float* rgba;                /* render output [H][W][4] f32                      */
+    double* lidar;              /* LiDAR output: single, first, last, each [H][W]    */
+    double* force_torque;       /* flux output: force xyz, torque xyz               */
+} crt_request;
+
+typedef struct {
+    uint64_t completed;
+    uint64_t rejected;
+    uint64_t rays;              /* closest-hit + shadow rays traced                 */
+    uint64_t batches;
+    double p50_ms, p99_ms, mean_ms;   /* submit -> completion latency           */
+    double wall_s;              /* first submission .. last completion              */
+    double mrays_per_s;         /* rays / wall_s                                    */
+} crt_server_stats;
+
+int  crt_server_create(const crt_server_opts* opts, crt_server** out);
+int  crt_server_add_scene(crt_server* server, crt_scene* scene, uint32_t* scene_id);
+int  crt_server_submit(crt_server* server, const crt_request* request, uint64_t* ticket);
+int  crt_server_wait(crt_server* server, uint64_t ticket, double* latency_ms);
+int  crt_server_drain(crt_server* server);
+int  crt_server_stats_get(crt_server* server, crt_server_stats* stats);
+void crt_server_destroy(crt_server* server);
+
 #ifdef __cplusplus
 }
 #endif

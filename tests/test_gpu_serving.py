@@ -1,0 +1,59 @@
+"""Serving: each result is the same alone and in a mixed, batched stream;
+admission rejects what can never fit the memory budget."""
+import numpy as np
+import pytest
+
+from crt import capi, scenes, serving
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def meshes():
+    ms = [scenes.displaced_icosphere(6, seed=1), scenes.geodesic_sphere(40, seed=2)]
+    return ms, [capi.Scene(m.vertices, m.indices) for m in ms]
+
+
+def test_alone_vs_mix_identical(meshes):
+    ms, sc = meshes
+    reqs = serving.mixed_workload(ms, 40, rate_hz=1e6, seed=3, max_render=256, max_lidar=128, max_flux=512)
+    srv = serving.Server(batching=True, n_workers=3)
+    for s in sc:
+        srv.add_scene(s)
+    mixed, lat, st = serving.run_stream(srv, reqs, realtime=False)
+    assert st["completed"] == len(reqs)
+    assert st["batches"] < len(reqs)          # requests were batched
+    alone_srv = serving.Server(batching=False)
+    for s in sc:
+        alone_srv.add_scene(s)
+    for i in range(0, len(reqs), 3):
+        t = alone_srv.submit(reqs[i][1])
+        out, _ = alone_srv.result(t)
+        for k in out:
+            assert np.array_equal(out[k], mixed[i][k]), (i, reqs[i][1]["kind"], k)
+    # plain render through the C ABI directly gives the same image too
+    for i, (_, r) in enumerate(reqs):
+        if r["kind"] == "render":
+            rgba, _, _, _ = sc[r["scene"]].render(r["camera"], r["lights"], r["spp"], r["spp"], 0.0, r["bounces"],
+                                                  seed=r["seed"])
+            assert np.array_equal(rgba, mixed[i]["rgba"])
+            break
+    srv.close()
+    alone_srv.close()
+
+
+def test_admission_rejects_oversized_request(meshes):
+    ms, sc = meshes
+    srv = serving.Server(batching=True, memory_budget=1 << 20)
+    srv.add_scene(sc[0])
+    cam = scenes.small_body_camera(ms[0], resolution=512, seed=1)
+    with pytest.raises(capi.CrtError) as e:
+        srv.submit({"kind": "render", "scene": 0, "camera": cam, "spp": 16, "bounces": 2, "seed": 1, "lights": []})
+    assert e.value.code == capi.CRT_ERR_ADMISSION
+    # a small request still goes through under the same budget
+    small = scenes.small_body_camera(ms[0], resolution=16, seed=1)
+    t = srv.submit({"kind": "render", "scene": 0, "camera": small, "spp": 1, "bounces": 1, "seed": 1, "lights": []})
+    out, lat = srv.result(t)
+    assert out["rgba"].shape == (16, 16, 4) and lat > 0
+    assert srv.stats()["rejected"] == 1
+    srv.close()
