@@ -554,6 +554,19 @@ int render_batch(crt_scene* scene, uint32_t n_cams, const crt_camera* cams, cons
     if (B.grab == 0) B.grab = 4;
     if (B.blocks_per_cu == 0) B.blocks_per_cu = 8;
     hipStream_t st = (hipStream_t)stream;
+    // stream capture is not allowed on the legacy null stream: graph replay
+    // on a null stream runs on a private stream (this call synchronises at
+    // its end anyway)
+    struct OwnStream {
+        hipStream_t s = nullptr;
+        ~OwnStream() {
+            if (s) (void)hipStreamDestroy(s);
+        }
+    } own_stream;
+    if (B.use_graph && st == nullptr) {
+        CRT_HIP_CHECK(hipStreamCreateWithFlags(&own_stream.s, hipStreamNonBlocking));
+        st = own_stream.s;
+    }
     int dev = 0, num_cus = 0;
     CRT_HIP_CHECK(hipGetDevice(&dev));
     CRT_HIP_CHECK(hipDeviceGetAttribute(&num_cus, hipDeviceAttributeMultiprocessorCount, dev));
