@@ -87,6 +87,101 @@ def cpu_baseline(mesh, cam, sun, budget_s=15.0):
                       "cached BVH (build not timed), %.1f s" % (total_rays, t_sum)}
 
 
+def run_batched(args, capi, scenes):
+    """256 cameras on the L8 body, sides 2^U(6,10) px, FP64 poses at 20-500 km,
+    8 spp, 3 bounces, directional sun: one batched render (autotuned launch
+    parameters, hipGraph pass replay) vs per-camera plain launches."""
+    from crt import autotune
+    mesh = scenes.displaced_icosphere(8, 2000.0, 0.2, 6, 0)
+    scene = capi.Scene(mesh.vertices, mesh.indices)
+    rng = np.random.default_rng(5)
+    cams = []
+    for k in range(args.cameras):
+        side = int(2 ** rng.uniform(6, 10))
+        cams.append(scenes.small_body_camera(mesh, resolution=side, seed=10000 + k, range_km=(20.0, 500.0)))
+    sun = {"type": "directional", "intensity": 1.0, "direction": scenes.sun_direction(2)}
+    spp, nb = 8, 3
+    log("autotuning on a subset")
+    best = autotune.tune(scene, cams, [sun], spp, nb, log=log)
+    log("best launch parameters: %s" % best)
+    seeds = list(range(len(cams)))
+    scene.render_batch(cams[:8], [sun], spp, spp, 0.0, nb, seeds=seeds[:8], outputs=False, **best)
+    t0 = time.perf_counter()
+    _, tmb = scene.render_batch(cams, [sun], spp, spp, 0.0, nb, seeds=seeds, outputs=False, **best)
+    tb = time.perf_counter() - t0
+    rays = tmb["rays_traced"] + tmb["shadow_rays"]
+    log("batched: %.1f ms, %d rays" % (tb * 1e3, rays))
+    t0 = time.perf_counter()
+    prays = 0
+    for k, c in enumerate(cams):
+        _, tm = scene.render_batch([c], [sun], spp, spp, 0.0, nb, seeds=[k], outputs=False)
+        prays += tm["rays_traced"] + tm["shadow_rays"]
+    tp = time.perf_counter() - t0
+    log("plain per-camera: %.1f ms" % (tp * 1e3))
+    # identity check on a few cameras
+    sel = [0, len(cams) // 2, len(cams) - 1]
+    outb, _ = scene.render_batch([cams[i] for i in sel], [sun], spp, spp, 0.0, nb, seeds=[seeds[i] for i in sel], **best)
+    same = all(np.array_equal(outb[j][0], scene.render_batch([cams[i]], [sun], spp, spp, 0.0, nb, seeds=[seeds[i]])[0][0][0])
+               for j, i in enumerate(sel))
+    pix = sum(int(c["resolution"][0]) * int(c["resolution"][1]) for c in cams)
+    return {"metric": "batched render Mrays/s (256 cameras, 8 spp, 3 bounces, L8 body)", "value": rays / tb / 1e6,
+            "unit": "Mrays/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "C4 batched: %d cameras, %d pixels, spp 8, bounces 3, sun" % (len(cams), pix)},
+            "batched_ms": tb * 1e3, "plain_ms": tp * 1e3, "plain_mrays_s": prays / tp / 1e6,
+            "speedup_vs_plain": tp / tb, "launch_params": best, "batch_equals_plain": bool(same)}
+
+
+def run_served(args, capi, scenes):
+    """Mixed Poisson stream (60% renders 128^2-1024^2 x 1-16 spp, 30% flash
+    LiDAR 64^2-512^2 x 8 sub-rays, 10% flux grids 0.25-4M rays) over 4
+    resident geodesic spheres of 1.05M-10M triangles: continuous batching
+    server vs plain per-request launches, same requests (same per-pixel
+    error), saturated throughput and latency at a fixed arrival rate."""
+    from crt import serving
+    ks = [229, 400, 560, 707] if not args.small else [40, 64, 80, 100]
+    meshes, sc = [], []
+    for i, k in enumerate(ks):
+        log("geodesic sphere k=%d" % k)
+        m = scenes.geodesic_sphere(k, seed=i)
+        meshes.append(m)
+        sc.append(capi.Scene(m.vertices, m.indices))
+    resident = sum(s.info()["device_bytes"] for s in sc)
+    out = {"metric": "served Mrays/s (mixed render/LiDAR/flux stream, 4 resident meshes)", "unit": "Mrays/s",
+           "n_gpus": 1, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "C5 served: meshes %s tris, resident %.2f GB" % (
+               [m.n_tris for m in meshes], resident / 1e9), "requests": args.requests}}
+    for mode, batching in (("served", True), ("plain", False)):
+        srv = serving.Server(batching=batching, n_workers=3)
+        for s in sc:
+            srv.add_scene(s)
+        warm = serving.mixed_workload(meshes, 8, 1e6, seed=99)
+        serving.run_stream(srv, warm, realtime=False)
+        srv.close()
+        srv = serving.Server(batching=batching, n_workers=3)
+        for s in sc:
+            srv.add_scene(s)
+        reqs = serving.mixed_workload(meshes, args.requests, 1e6, seed=7)
+        log("%s: saturated stream of %d requests" % (mode, len(reqs)))
+        _, lat, st = serving.run_stream(srv, reqs, realtime=False)
+        out[mode] = {"mrays_s": st["mrays_per_s"], "wall_s": st["wall_s"], "batches": st["batches"],
+                     "rays": st["rays"], "p50_ms_saturated": st["p50_ms"], "p99_ms_saturated": st["p99_ms"]}
+        srv.close()
+    rate = args.rate
+    for mode, batching in (("served", True), ("plain", False)):
+        srv = serving.Server(batching=batching, n_workers=3)
+        for s in sc:
+            srv.add_scene(s)
+        reqs = serving.mixed_workload(meshes, args.requests, rate, seed=8)
+        log("%s: %d requests at %.0f req/s" % (mode, len(reqs), rate))
+        _, lat, st = serving.run_stream(srv, reqs, realtime=True)
+        out[mode].update({"rate_hz": rate, "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
+                          "mrays_s_at_rate": st["mrays_per_s"]})
+        srv.close()
+    out["value"] = out["served"]["mrays_s"]
+    out["speedup_vs_plain"] = out["served"]["mrays_s"] / max(out["plain"]["mrays_s"], 1e-9)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,6 +190,13 @@ def main():
     ap.add_argument("--resolution", type=int, default=2048)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--out", default=None, help="also write the JSON line to this file")
+    ap.add_argument("--workload", default="c2", choices=["c2", "batched", "served"],
+                    help="c2 (headline, default): closest hit + sun shadow on the 1.31M-tri body; "
+                         "batched: 256-camera path tracing; served: mixed request stream")
+    ap.add_argument("--cameras", type=int, default=256)
+    ap.add_argument("--requests", type=int, default=300)
+    ap.add_argument("--rate", type=float, default=100.0)
+    ap.add_argument("--small", action="store_true", help="served: small meshes (quick check)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
@@ -114,6 +216,14 @@ def main():
             y.copy_(x)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "copies": 5}))
+        return
+
+    if args.workload != "c2":
+        line = run_batched(args, capi, scenes) if args.workload == "batched" else run_served(args, capi, scenes)
+        print(json.dumps(line), flush=True)
+        if args.out:
+            with open(args.out, "w") as fh:
+                fh.write(json.dumps(line) + "\n")
         return
 
     log("generating icosphere L8")
