@@ -1,8 +1,8 @@
 """Autotuner for batched rendering launch parameters.
 
 Tunes crt_batch_opts -- samples per pass, hipGraph replay of a pass,
-persistent traversal kernels, persistent blocks per CU and rays claimed per
-atomic -- by timing a representative subset of the batch (the real cameras,
+persistent traversal kernels, persistent blocks per CU, rays claimed per
+atomic and the traversal form of extension rays -- by timing a representative subset of the batch (the real cameras,
 resolution-capped) and keeps the winner in a JSON cache keyed by device,
 scene size, bounces and light count.  Every candidate renders bit-identical
 images (the RNG is keyed per camera/pixel/sample), so only speed is compared.
@@ -22,6 +22,7 @@ CACHE = os.environ.get("CRT_AUTOTUNE_CACHE", os.path.join(os.path.expanduser("~"
 CANDIDATES = [
     dict(samples_per_pass=0, use_graph=False, persistent=False),
     dict(samples_per_pass=0, use_graph=True, persistent=False),
+    dict(samples_per_pass=0, use_graph=True, persistent=False, extend_mode=2),
     dict(samples_per_pass=1, use_graph=True, persistent=False),
     dict(samples_per_pass=2, use_graph=True, persistent=False),
     dict(samples_per_pass=0, use_graph=True, persistent=True, blocks_per_cu=4, grab=4),

@@ -105,7 +105,9 @@ void build_bvh(const float* d_vertices, const uint32_t* d_indices, uint32_t n_tr
 // Shared helpers of the trace / render / sensor translation units.
 CamView make_cam_view(const crt_camera& c);
 void check_camera_args(const crt_camera* c);
-size_t trace_lds_bytes(const SceneView& s);
+size_t trace_lds_bytes(const SceneView& s);   // octet kernels
+size_t lane_lds_bytes(const SceneView& s);    // lane kernels
+int device_cus();                             // compute units of the current device
 float self_offset(const crt_scene* s);
 
 // Render entry shared by crt_render, crt_render_batch and the server.

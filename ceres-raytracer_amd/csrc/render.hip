@@ -231,7 +231,7 @@ __device__ inline void shadow_one(const Ctx& C, uint32_t i, uint32_t* sref, floa
 }
 
 #define OCTET_SETUP(C)                                                                                      \
-    extern __shared__ uint32_t smem[];                                                                      \
+    uint32_t* smem = traversal_lds((C).s);                                                                  \
     const int lane = threadIdx.x & 63;                                                                      \
     const int sub = lane & 7, lane_base = lane & ~7, local = threadIdx.x >> 3;                              \
     uint32_t* sref = smem + (size_t)local * (C).s.stack_cap;                                                \
@@ -246,6 +246,20 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace(Ctx C, int qsel) {
     if (q >= n) return;
     if (SHADOW) shadow_one(C, q, sref, sd, sub, lane_base);
     else extend_one(C, C.queue[qsel][q], sref, sd, sub, lane_base);
+}
+
+// One extension ray per lane (coherent rays: camera rays).
+__global__ __launch_bounds__(TRACE_BLOCK) void k_extend_lane(Ctx C, int qsel) {
+    const LaneStack stk = lane_stack(C.s);
+    const uint32_t q = blockIdx.x * TRACE_BLOCK + threadIdx.x;
+    if (q >= C.cnt[qsel]) return;
+    const uint32_t k = C.queue[qsel][q];
+    const float4 o = C.st[k].o, d = C.st[k].d;
+    const RayPre r = make_ray_pre(o.x, o.y, o.z, d.x, d.y, d.z);
+    const HitRec h = trace_lane<false>(C.s, r, 0.0f, __int_as_float(0x7f800000), stk);
+    C.st[k].slot = h.slot;
+    C.st[k].hu = h.u;
+    C.st[k].hv = h.v;
 }
 
 // Persistent form: octets claim `grab` consecutive entries per atomic.
@@ -489,6 +503,7 @@ void launch_pass(const Ctx& C, uint64_t cap_paths, uint32_t cap_fold, const crt_
                  hipStream_t st) {
     const size_t lds = trace_lds_bytes(C.s);
     const uint32_t gp = std::max(1u, cdiv(cap_paths, TB));
+    static_assert(TB == TRACE_BLOCK, "lane kernels share the path grid");
     const uint32_t go = std::max(1u, cdiv(cap_paths, OCTETS_PER_BLOCK));
     const uint32_t gs = std::max(1u, cdiv(cap_paths * std::max(1, C.nl), OCTETS_PER_BLOCK));
     const uint32_t gpers = std::max(1u, B.blocks_per_cu) * (uint32_t)num_cus;
@@ -497,7 +512,10 @@ void launch_pass(const Ctx& C, uint64_t cap_paths, uint32_t cap_fold, const crt_
     for (int b = 0; b < C.nb; ++b) {
         const int cur = b & 1, nxt = cur ^ 1;
         hipLaunchKernelGGL(k_clear, dim3(1), dim3(32), 0, st, C.cnt, (1u << nxt) | (1u << 2) | (1u << 3) | (3u << 5));
-        if (B.persistent)
+        const bool lane = B.extend_mode == 2;
+        if (lane)
+            hipLaunchKernelGGL(k_extend_lane, dim3(gp), dim3(TRACE_BLOCK), lane_lds_bytes(C.s), st, C, cur);
+        else if (B.persistent)
             hipLaunchKernelGGL(k_trace_persistent<false>, dim3(std::min(gpers, go)), dim3(TRACE_BLOCK), lds, st, C, cur);
         else
             hipLaunchKernelGGL(k_trace<false>, dim3(go), dim3(TRACE_BLOCK), lds, st, C, cur);

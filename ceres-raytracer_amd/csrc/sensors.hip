@@ -3,7 +3,7 @@
 // crt_lidar_batch: the reference LiDAR casts ONE ray per pose along +/-z
 //   rotated by R^T (src/crt/lidars/simple_lidar.hpp:42-96) and returns
 //   |hit - position| with the hit point from barycentrics, 0 on a miss
-//   (src/crt/do_lidar.hpp:43-70, 111-143).  One octet per pose.
+//   (src/crt/do_lidar.hpp:43-70, 111-143).  One lane per pose.
 // crt_flash_lidar (new): a beam grid laid out like a pinhole camera; every
 //   beam casts its boresight ray (the "single" return, equal to the reference
 //   LiDAR pointed along that beam) plus `sub_rays` rays whose directions are
@@ -30,27 +30,14 @@ namespace {
 
 inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
-struct OctetLanes {
-    int sub, lane_base, local;
-};
-__device__ inline OctetLanes lanes() {
-    OctetLanes o;
-    const int lane = threadIdx.x & 63;
-    o.sub = lane & 7;
-    o.lane_base = lane & ~7;
-    o.local = threadIdx.x >> 3;
-    return o;
-}
-
-// Closest hit of a unit FP64 ray; returns range |hit - o| (FP64 from
-// barycentrics) or 0 on a miss.
-__device__ inline double range_f64(const SceneView& s, const double o[3], const double d[3], uint32_t* sref,
-                                   float* sd, const OctetLanes& id) {
+// Closest hit of a unit FP64 ray (one ray per lane); returns range
+// |hit - o| (FP64 from barycentrics) or 0 on a miss.
+__device__ inline double range_f64(const SceneView& s, const double o[3], const double d[3], const LaneStack& stk) {
     double t0;
     if (!rebase_t0(s, o, d, t0)) return 0.0;
     const RayPre r = make_ray_pre((float)(o[0] + t0 * d[0]), (float)(o[1] + t0 * d[1]), (float)(o[2] + t0 * d[2]),
                                   (float)d[0], (float)d[1], (float)d[2]);
-    const HitRec h = trace_octet<false>(s, r, 0.0f, __int_as_float(0x7f800000), sref, sd, id.sub, id.lane_base);
+    const HitRec h = trace_lane<false>(s, r, 0.0f, __int_as_float(0x7f800000), stk);
     if (h.slot < 0) return 0.0;
     const float4 a = s.tris[3 * h.slot], b = s.tris[3 * h.slot + 1], c = s.tris[3 * h.slot + 2];
     const double u = h.u, v = h.v, w = 1.0 - u - v;
@@ -63,18 +50,14 @@ __device__ inline double range_f64(const SceneView& s, const double o[3], const 
 __global__ __launch_bounds__(TRACE_BLOCK) void k_lidar_batch(SceneView s, const double* __restrict__ pos,
                                                              const double* __restrict__ rot, double zsign,
                                                              double* __restrict__ range, uint64_t n) {
-    extern __shared__ uint32_t smem[];
-    const OctetLanes id = lanes();
-    uint32_t* sref = smem + (size_t)id.local * s.stack_cap;
-    float* sd = reinterpret_cast<float*>(smem + (size_t)OCTETS_PER_BLOCK * s.stack_cap) + (size_t)id.local * s.stack_cap;
-    const uint64_t i = (uint64_t)blockIdx.x * OCTETS_PER_BLOCK + id.local;
+    const LaneStack stk = lane_stack(s);
+    const uint64_t i = (uint64_t)blockIdx.x * TRACE_BLOCK + threadIdx.x;
     if (i >= n) return;
     const double* R = rot + 9 * i;
     // R^T (0, 0, +/-1) = +/- third row of R
     const double d[3] = {R[6] * zsign, R[7] * zsign, R[8] * zsign};
     const double o[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
-    const double r = range_f64(s, o, d, sref, sd, id);
-    if (id.sub == 0) range[i] = r;
+    range[i] = range_f64(s, o, d, stk);
 }
 
 struct FlashParams {
@@ -87,11 +70,8 @@ struct FlashParams {
 // One octet per (beam, ray) with ray 0 = boresight, 1..sub_rays perturbed.
 __global__ __launch_bounds__(TRACE_BLOCK) void k_flash(SceneView s, FlashParams F, double* __restrict__ ranges,
                                                        uint64_t n) {
-    extern __shared__ uint32_t smem[];
-    const OctetLanes id = lanes();
-    uint32_t* sref = smem + (size_t)id.local * s.stack_cap;
-    float* sd = reinterpret_cast<float*>(smem + (size_t)OCTETS_PER_BLOCK * s.stack_cap) + (size_t)id.local * s.stack_cap;
-    const uint64_t q = (uint64_t)blockIdx.x * OCTETS_PER_BLOCK + id.local;
+    const LaneStack stk = lane_stack(s);
+    const uint64_t q = (uint64_t)blockIdx.x * TRACE_BLOCK + threadIdx.x;
     if (q >= n) return;
     const uint32_t per = F.sub_rays + 1;
     const uint64_t beam = q / per;
@@ -115,8 +95,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_flash(SceneView s, FlashParams 
         const double ln = sqrt(nd[0] * nd[0] + nd[1] * nd[1] + nd[2] * nd[2]);
         d[0] = nd[0] / ln; d[1] = nd[1] / ln; d[2] = nd[2] / ln;
     }
-    const double r = range_f64(s, F.cam.pos, d, sref, sd, id);
-    if (id.sub == 0) ranges[q] = r;
+    ranges[q] = range_f64(s, F.cam.pos, d, stk);
 }
 
 __global__ __launch_bounds__(256) void k_flash_reduce(const double* __restrict__ ranges, uint32_t sub_rays,
@@ -164,16 +143,14 @@ struct FluxParams {
     uint32_t grid;
 };
 
-// Block partials: 32 octets (rays) per block; 7 compensated sums each
-// (Fx,Fy,Fz,Tx,Ty,Tz, hits).
+// Block partials: 256 rays per block (one per lane); 7 compensated sums
+// each (Fx,Fy,Fz,Tx,Ty,Tz, hits), reduced per wave with shuffles, then
+// over the block's 4 waves.
 __global__ __launch_bounds__(TRACE_BLOCK) void k_flux(SceneView s, FluxParams F, Comp* __restrict__ partial,
                                                       uint64_t n) {
-    extern __shared__ uint32_t smem[];
-    __shared__ Comp red[7][OCTETS_PER_BLOCK];
-    const OctetLanes id = lanes();
-    uint32_t* sref = smem + (size_t)id.local * s.stack_cap;
-    float* sd = reinterpret_cast<float*>(smem + (size_t)OCTETS_PER_BLOCK * s.stack_cap) + (size_t)id.local * s.stack_cap;
-    const uint64_t q = (uint64_t)blockIdx.x * OCTETS_PER_BLOCK + id.local;
+    const LaneStack stk = lane_stack(s);
+    __shared__ Comp red[7][TRACE_BLOCK / 64];
+    const uint64_t q = (uint64_t)blockIdx.x * TRACE_BLOCK + threadIdx.x;
     double term[7] = {0, 0, 0, 0, 0, 0, 0};
     if (q < n) {
         const uint32_t i = (uint32_t)(q % F.grid), j = (uint32_t)(q / F.grid);
@@ -187,8 +164,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_flux(SceneView s, FluxParams F,
         if (rebase_t0(s, o, d, t0)) {
             const RayPre r = make_ray_pre((float)(o[0] + t0 * d[0]), (float)(o[1] + t0 * d[1]),
                                           (float)(o[2] + t0 * d[2]), (float)d[0], (float)d[1], (float)d[2]);
-            const HitRec h = trace_octet<false>(s, r, 0.0f, __int_as_float(0x7f800000), sref, sd, id.sub,
-                                                id.lane_base);
+            const HitRec h = trace_lane<false>(s, r, 0.0f, __int_as_float(0x7f800000), stk);
             if (h.slot >= 0) {
                 const float4 a = s.tris[3 * h.slot], b = s.tris[3 * h.slot + 1], c = s.tris[3 * h.slot + 2];
                 // outward normal = -normalize((p0 - p1) x (p2 - p0))  (reference normal is inward)
@@ -217,15 +193,20 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_flux(SceneView s, FluxParams F,
             }
         }
     }
-    if (id.sub == 0)
-        for (int k = 0; k < 7; ++k) red[k][id.local] = Comp{term[k], 0.0};
+    Comp acc[7];
+    for (int k = 0; k < 7; ++k) acc[k] = Comp{term[k], 0.0};
+    for (int w = 32; w > 0; w >>= 1)
+        for (int k = 0; k < 7; ++k)
+            acc[k] = comp_add(acc[k], Comp{__shfl_xor(acc[k].s, w, 64), __shfl_xor(acc[k].c, w, 64)});
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 7; ++k) red[k][wave] = acc[k];
     __syncthreads();
-    for (int w = OCTETS_PER_BLOCK / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w)
-            for (int k = 0; k < 7; ++k) red[k][threadIdx.x] = comp_add(red[k][threadIdx.x], red[k][threadIdx.x + w]);
-        __syncthreads();
+    if (threadIdx.x < 7) {
+        Comp t = red[threadIdx.x][0];
+        for (int w = 1; w < TRACE_BLOCK / 64; ++w) t = comp_add(t, red[threadIdx.x][w]);
+        partial[(size_t)blockIdx.x * 7 + threadIdx.x] = t;
     }
-    if (threadIdx.x < 7) partial[(size_t)blockIdx.x * 7 + threadIdx.x] = red[threadIdx.x][0];
 }
 
 __global__ __launch_bounds__(256) void k_flux_final(const Comp* __restrict__ partial, uint32_t nblocks,
@@ -292,8 +273,8 @@ int crt_lidar_batch(crt_scene* scene, uint64_t n, const double* positions, const
         }
         Timer tm;
         CRT_HIP_CHECK(hipEventRecord(tm.a, st));
-        hipLaunchKernelGGL(k_lidar_batch, dim3(cdiv(n, OCTETS_PER_BLOCK)), dim3(TRACE_BLOCK),
-                           trace_lds_bytes(scene->view), st, scene->view, dp, dr, z_positive ? 1.0 : -1.0, dout, n);
+        hipLaunchKernelGGL(k_lidar_batch, dim3(cdiv(n, TRACE_BLOCK)), dim3(TRACE_BLOCK),
+                           lane_lds_bytes(scene->view), st, scene->view, dp, dr, z_positive ? 1.0 : -1.0, dout, n);
         CRT_HIP_CHECK(hipGetLastError());
         CRT_HIP_CHECK(hipEventRecord(tm.b, st));
         if (!dev) CRT_HIP_CHECK(hipMemcpyAsync(ranges, dout, n * 8, hipMemcpyDeviceToHost, st));
@@ -341,7 +322,7 @@ void flash_lidar_impl(crt_scene* scene, const crt_flash_opts* opts, double* sing
         }
         Timer tm;
         CRT_HIP_CHECK(hipEventRecord(tm.a, st));
-        hipLaunchKernelGGL(k_flash, dim3(cdiv(nr, OCTETS_PER_BLOCK)), dim3(TRACE_BLOCK), trace_lds_bytes(scene->view),
+        hipLaunchKernelGGL(k_flash, dim3(cdiv(nr, TRACE_BLOCK)), dim3(TRACE_BLOCK), lane_lds_bytes(scene->view),
                            st, scene->view, F, ws.f.as<double>(), nr);
         hipLaunchKernelGGL(k_flash_reduce, dim3(cdiv(nb, 256)), dim3(256), 0, st, ws.f.as<double>(), F.sub_rays, nb,
                            d_single, d_first, d_last, d_n);
@@ -396,13 +377,13 @@ void radiation_pressure_impl(crt_scene* scene, const crt_flux_opts* opts, double
         F.rho_d = opts->rho_d;
         F.origin_back = 2.0 * F.radius;
         const uint64_t n = (uint64_t)F.grid * F.grid;
-        const uint32_t nblocks = cdiv(n, OCTETS_PER_BLOCK);
+        const uint32_t nblocks = cdiv(n, TRACE_BLOCK);
         ws.h.reserve((size_t)nblocks * 7 * sizeof(Comp));
         ws.counters.reserve(256);
         double* out = ws.counters.as<double>();
         Timer tm;
         CRT_HIP_CHECK(hipEventRecord(tm.a, st));
-        hipLaunchKernelGGL(k_flux, dim3(nblocks), dim3(TRACE_BLOCK), trace_lds_bytes(scene->view), st, scene->view, F,
+        hipLaunchKernelGGL(k_flux, dim3(nblocks), dim3(TRACE_BLOCK), lane_lds_bytes(scene->view), st, scene->view, F,
                            ws.h.as<Comp>(), n);
         hipLaunchKernelGGL(k_flux_final, dim3(7), dim3(256), 0, st, ws.h.as<Comp>(), nblocks, out);
         CRT_HIP_CHECK(hipGetLastError());

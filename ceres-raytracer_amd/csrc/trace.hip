@@ -22,6 +22,22 @@ inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b
 }  // namespace
 
 size_t trace_lds_bytes(const SceneView& s) { return (size_t)OCTETS_PER_BLOCK * s.stack_cap * 8u; }
+size_t lane_lds_bytes(const SceneView& s) { return (size_t)TRACE_BLOCK * s.lane_cap * 8u; }
+
+int device_cus() {
+    static std::mutex mu;
+    static int cached[64] = {0};
+    int dev = 0;
+    CRT_HIP_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(mu);
+    if (dev < 0 || dev >= 64) dev = 0;
+    if (cached[dev] == 0) {
+        int v = 0;
+        CRT_HIP_CHECK(hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev));
+        cached[dev] = std::max(v, 1);
+    }
+    return cached[dev];
+}
 
 CamView make_cam_view(const crt_camera& c) {
     CamView v;
@@ -58,26 +74,12 @@ float self_offset(const crt_scene* s) {
 
 namespace {
 
-// Octet coordinates of the calling lane.
-struct OctetId {
-    int sub, lane_base, local;   // lane in octet, first lane of octet in wave, octet in block
-};
-__device__ inline OctetId octet_id() {
-    OctetId o;
-    const int lane = threadIdx.x & 63;
-    o.sub = lane & 7;
-    o.lane_base = lane & ~7;
-    o.local = threadIdx.x >> 3;
-    return o;
-}
-
 // Closest/any hit of one FP64 ray (unit direction d) through the scene:
-// rebase in FP64, trace in FP32.  Returns false on miss; otherwise t (FP64,
-// from o), u, v (FP32) and the leaf-order slot.
+// rebase in FP64, trace in FP32 (one ray per lane).  Returns false on miss;
+// otherwise t (FP64, from o), u, v (FP32) and the leaf-order slot.
 template <bool ANY>
 __device__ inline bool trace_f64(const SceneView& s, const double o[3], const double d[3], double tmin,
-                                 double tmax, uint32_t* sref, float* st, const OctetId& id, double& t_out,
-                                 float& u, float& v, int& slot) {
+                                 double tmax, const LaneStack& stk, double& t_out, float& u, float& v, int& slot) {
     double t0;
     if (!rebase_t0(s, o, d, t0)) return false;
     if (t0 > tmax) return false;
@@ -85,7 +87,7 @@ __device__ inline bool trace_f64(const SceneView& s, const double o[3], const do
     const RayPre r = make_ray_pre((float)ro[0], (float)ro[1], (float)ro[2], (float)d[0], (float)d[1], (float)d[2]);
     const float ltmin = (float)fmax(tmin - t0, 0.0);
     const float ltmax = tmax >= 3.0e38 ? __int_as_float(0x7f800000) : (float)(tmax - t0);
-    const HitRec h = trace_octet<ANY>(s, r, ltmin, ltmax, sref, st, id.sub, id.lane_base);
+    const HitRec h = trace_lane<ANY>(s, r, ltmin, ltmax, stk);
     if (h.slot < 0) return false;
     t_out = t0 + (double)h.t;
     u = h.u;
@@ -128,12 +130,15 @@ __device__ inline void hit_normal(const SceneView& s, int slot, float u, float v
     n[0] = x * inv; n[1] = y * inv; n[2] = z * inv;
 }
 
-// Pixel of octet `oct` in block `blk`: blocks tile the image in 8x4 tiles.
-__device__ inline void tile_pixel(int blk, int oct, int w, int& px, int& py) {
-    const int tiles_x = (w + 7) >> 3;
+// Pixel of thread `t` in block `blk`: a block covers 16x16 pixels, each wave
+// an 8x8 quarter of it (rays of a wave stay close on the image).
+constexpr int PIX_TILE = 16;
+__device__ inline void tile_pixel(int blk, int t, int w, int& px, int& py) {
+    const int tiles_x = (w + PIX_TILE - 1) / PIX_TILE;
     const int tx = blk % tiles_x, ty = blk / tiles_x;
-    px = tx * 8 + (oct & 7);
-    py = ty * 4 + (oct >> 3);
+    const int wave = t >> 6, lane = t & 63;
+    px = tx * PIX_TILE + (wave & 1) * 8 + (lane & 7);
+    py = ty * PIX_TILE + (wave >> 1) * 8 + (lane >> 3);
 }
 
 struct PassOut {
@@ -145,30 +150,29 @@ struct PassOut {
 };
 
 __global__ __launch_bounds__(TRACE_BLOCK) void k_camera_pass(SceneView s, CamView cam, PassOut out) {
-    extern __shared__ uint32_t smem[];
-    const OctetId id = octet_id();
-    uint32_t* sref = smem + (size_t)id.local * s.stack_cap;
-    float* st = reinterpret_cast<float*>(smem + (size_t)OCTETS_PER_BLOCK * s.stack_cap) + (size_t)id.local * s.stack_cap;
+    const LaneStack stk = lane_stack(s);
     int px, py;
-    tile_pixel(blockIdx.x, id.local, cam.w, px, py);
-    if (px >= cam.w || py >= cam.h) return;   // octet-uniform
+    tile_pixel(blockIdx.x, threadIdx.x, cam.w, px, py);
+    if (px >= cam.w || py >= cam.h) return;
     double d[3];
     // integer pixel coordinates, like src/crt/passes.hpp:46
     camera_dir(cam, (double)px, (double)py, d);
     double t;
     float u = 0.f, v = 0.f;
     int slot = -1;
-    const bool hit = trace_f64<false>(s, cam.pos, d, 0.0, 1.0e300, sref, st, id, t, u, v, slot);
+    const bool hit = trace_f64<false>(s, cam.pos, d, 0.0, 1.0e300, stk, t, u, v, slot);
     const size_t pix = (size_t)py * cam.w + px;
-    if (id.sub == 0) {
-        if (out.instance) out.instance[pix] = hit ? s.tri_inst[__float_as_uint(s.tris[3 * slot].w)] : 0u;
-        if (out.tri) out.tri[pix] = hit ? (int32_t)__float_as_uint(s.tris[3 * slot].w) : -1;
-    }
+    if (out.instance) out.instance[pix] = hit ? s.tri_inst[__float_as_uint(s.tris[3 * slot].w)] : 0u;
+    if (out.tri) out.tri[pix] = hit ? (int32_t)__float_as_uint(s.tris[3 * slot].w) : -1;
     if (out.xyz || out.depth) {
         double p[3] = {0.0, 0.0, 0.0};
         if (hit) bary_point(s, slot, u, v, p);
-        if (out.xyz && id.sub < 3) out.xyz[3 * pix + id.sub] = p[id.sub == 0 ? 0 : (id.sub == 1 ? 1 : 2)];
-        if (out.depth && id.sub == 3) {
+        if (out.xyz) {
+            out.xyz[3 * pix] = p[0];
+            out.xyz[3 * pix + 1] = p[1];
+            out.xyz[3 * pix + 2] = p[2];
+        }
+        if (out.depth) {
             double dd = 0.0;
             if (hit) {
                 const double dx = p[0] - cam.pos[0], dy = p[1] - cam.pos[1], dz = p[2] - cam.pos[2];
@@ -180,7 +184,9 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_camera_pass(SceneView s, CamVie
     if (out.normal) {
         double n[3] = {0.0, 0.0, 0.0};
         if (hit) hit_normal(s, slot, u, v, n);
-        if (id.sub >= 4 && id.sub < 7) out.normal[3 * pix + (id.sub - 4)] = n[id.sub - 4];
+        out.normal[3 * pix] = n[0];
+        out.normal[3 * pix + 1] = n[1];
+        out.normal[3 * pix + 2] = n[2];
     }
 }
 
@@ -199,11 +205,8 @@ struct RayOut {
 
 template <bool ANY>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_rays(SceneView s, RayIn in, RayOut out, uint64_t n) {
-    extern __shared__ uint32_t smem[];
-    const OctetId id = octet_id();
-    uint32_t* sref = smem + (size_t)id.local * s.stack_cap;
-    float* st = reinterpret_cast<float*>(smem + (size_t)OCTETS_PER_BLOCK * s.stack_cap) + (size_t)id.local * s.stack_cap;
-    const uint64_t ray = (uint64_t)blockIdx.x * OCTETS_PER_BLOCK + id.local;
+    const LaneStack stk = lane_stack(s);
+    const uint64_t ray = (uint64_t)blockIdx.x * TRACE_BLOCK + threadIdx.x;
     if (ray >= n) return;
     double o[3] = {in.o[3 * ray], in.o[3 * ray + 1], in.o[3 * ray + 2]};
     double d[3] = {in.d[3 * ray], in.d[3 * ray + 1], in.d[3 * ray + 2]};
@@ -219,15 +222,13 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_rays(SceneView s, RayIn i
         // distances along the unit direction
         tmin *= len;
         tmax = tmax >= 1.0e300 ? tmax : tmax * len;
-        hit = trace_f64<ANY>(s, o, d, tmin, tmax, sref, st, id, t, u, v, slot);
+        hit = trace_f64<ANY>(s, o, d, tmin, tmax, stk, t, u, v, slot);
         if (hit) t /= len;
     }
-    if (id.sub == 0) {
-        if (out.t) out.t[ray] = hit ? t : -1.0;
-        if (out.tri) out.tri[ray] = hit ? (int32_t)__float_as_uint(s.tris[3 * slot].w) : -1;
-        if (out.u) out.u[ray] = hit ? (double)u : 0.0;
-        if (out.v) out.v[ray] = hit ? (double)v : 0.0;
-    }
+    if (out.t) out.t[ray] = hit ? t : -1.0;
+    if (out.tri) out.tri[ray] = hit ? (int32_t)__float_as_uint(s.tris[3 * slot].w) : -1;
+    if (out.u) out.u[ray] = hit ? (double)u : 0.0;
+    if (out.v) out.v[ray] = hit ? (double)v : 0.0;
 }
 
 // ---- primary + sun shadow (wavefront: closest hit -> compacted shadow queue -> any hit)
@@ -235,15 +236,12 @@ struct ShadowRay {
     float4 o;       // origin xyz, pixel index (bits)
 };
 
-__global__ __launch_bounds__(TRACE_BLOCK) void k_primary_sun(SceneView s, CamView cam, float3 sun, double* depth,
-                                                             int32_t* tri, uint8_t* lit, ShadowRay* queue,
-                                                             uint32_t* qcount, float offset) {
-    extern __shared__ uint32_t smem[];
-    const OctetId id = octet_id();
-    uint32_t* sref = smem + (size_t)id.local * s.stack_cap;
-    float* st = reinterpret_cast<float*>(smem + (size_t)OCTETS_PER_BLOCK * s.stack_cap) + (size_t)id.local * s.stack_cap;
+__global__ __launch_bounds__(TRACE_BLOCK) void k_primary_sun(SceneView s, CamView cam, double* depth, int32_t* tri,
+                                                             uint8_t* lit, ShadowRay* queue, uint32_t* qcount,
+                                                             float offset) {
+    const LaneStack stk = lane_stack(s);
     int px, py;
-    tile_pixel(blockIdx.x, id.local, cam.w, px, py);
+    tile_pixel(blockIdx.x, threadIdx.x, cam.w, px, py);
     const bool active = px < cam.w && py < cam.h;
     bool hit = false;
     double p[3] = {0.0, 0.0, 0.0};
@@ -255,7 +253,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_primary_sun(SceneView s, CamVie
         double t;
         float u = 0.f, v = 0.f;
         int slot = -1;
-        hit = trace_f64<false>(s, cam.pos, d, 0.0, 1.0e300, sref, st, id, t, u, v, slot);
+        hit = trace_f64<false>(s, cam.pos, d, 0.0, 1.0e300, stk, t, u, v, slot);
         pix = (size_t)py * cam.w + px;
         if (hit) {
             bary_point(s, slot, u, v, p);
@@ -263,50 +261,50 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_primary_sun(SceneView s, CamVie
             hit_normal(s, slot, u, v, n);
             nx = (float)n[0]; ny = (float)n[1]; nz = (float)n[2];
         }
-        if (id.sub == 0) {
-            if (depth) {
-                double dd = 0.0;
-                if (hit) {
-                    const double dx = p[0] - cam.pos[0], dy = p[1] - cam.pos[1], dz = p[2] - cam.pos[2];
-                    dd = sqrt(dx * dx + dy * dy + dz * dz);
-                }
-                depth[pix] = dd;
+        if (depth) {
+            double dd = 0.0;
+            if (hit) {
+                const double dx = p[0] - cam.pos[0], dy = p[1] - cam.pos[1], dz = p[2] - cam.pos[2];
+                dd = sqrt(dx * dx + dy * dy + dz * dz);
             }
-            if (tri) tri[pix] = hit ? (int32_t)__float_as_uint(s.tris[3 * slot].w) : -1;
-            if (lit) lit[pix] = 0;
+            depth[pix] = dd;
         }
+        if (tri) tri[pix] = hit ? (int32_t)__float_as_uint(s.tris[3 * slot].w) : -1;
+        if (lit) lit[pix] = 0;
     }
     // one shadow ray per hit, compacted with a wave ballot + one atomic per wave
-    // (lane sub==0 of each octet represents its ray)
-    const bool emit = active && hit && id.sub == 0;
-    const uint64_t m = __ballot(emit);
+    const uint64_t m = __ballot(active && hit);
+    if (m == 0ull) return;
     const int lane = threadIdx.x & 63;
-    uint32_t base = 0;
     const int leader = __ffsll((long long)m) - 1;
-    if (m != 0ull && lane == leader) base = atomicAdd(qcount, (uint32_t)__popcll(m));
-    base = __shfl(base, leader < 0 ? 0 : leader, 64);
-    if (emit) {
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(qcount, (uint32_t)__popcll(m));
+    base = (uint32_t)__shfl((int)base, leader, 64);
+    if (active && hit) {
         const uint32_t slot_q = base + (uint32_t)__popcll(m & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
         // offset off the surface along the outward normal (the geometric normal is inward)
         const float ox = (float)p[0] - offset * nx, oy = (float)p[1] - offset * ny, oz = (float)p[2] - offset * nz;
         queue[slot_q].o = make_float4(ox, oy, oz, __uint_as_float((uint32_t)pix));
     }
-    (void)sun;
 }
 
+// Shadow rays are any-hit: octet traversal (8 rays per wave, children tested
+// in parallel) beats one ray per lane here, whose waves wait on their
+// longest unoccluded ray.
 __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow_sun(SceneView s, const ShadowRay* queue,
                                                             const uint32_t* qcount, float3 sun, uint8_t* lit) {
-    extern __shared__ uint32_t smem[];
-    const OctetId id = octet_id();
-    uint32_t* sref = smem + (size_t)id.local * s.stack_cap;
-    float* st = reinterpret_cast<float*>(smem + (size_t)OCTETS_PER_BLOCK * s.stack_cap) + (size_t)id.local * s.stack_cap;
+    uint32_t* smem = traversal_lds(s);
+    const int lane = threadIdx.x & 63;
+    const int sub = lane & 7, lane_base = lane & ~7, local = threadIdx.x >> 3;
+    uint32_t* sref = smem + (size_t)local * s.stack_cap;
+    float* st = reinterpret_cast<float*>(smem + (size_t)OCTETS_PER_BLOCK * s.stack_cap) + (size_t)local * s.stack_cap;
     const uint32_t n = *qcount;
-    const uint32_t q = blockIdx.x * OCTETS_PER_BLOCK + id.local;
+    const uint32_t q = blockIdx.x * OCTETS_PER_BLOCK + local;
     if (q >= n) return;
     const float4 o = queue[q].o;
     const RayPre r = make_ray_pre(o.x, o.y, o.z, sun.x, sun.y, sun.z);
-    const HitRec h = trace_octet<true>(s, r, 0.0f, __int_as_float(0x7f800000), sref, st, id.sub, id.lane_base);
-    if (id.sub == 0 && lit) lit[__float_as_uint(o.w)] = h.slot < 0 ? 1 : 0;
+    const HitRec h = trace_octet<true>(s, r, 0.0f, __int_as_float(0x7f800000), sref, st, sub, lane_base);
+    if (sub == 0 && lit) lit[__float_as_uint(o.w)] = h.slot < 0 ? 1 : 0;
 }
 
 struct EventTimer {
@@ -341,13 +339,13 @@ struct OutSlot {
 }  // namespace
 
 // ---- host helpers shared with other translation units ----
-void scene_make_view(crt_scene* s);
+void scene_make_view(crt_scene* s, const float* host_vertices);
 
 }  // namespace crt
 
 using namespace crt;
 
-void crt::scene_make_view(crt_scene* s) {
+void crt::scene_make_view(crt_scene* s, const float* host_vertices) {
     SceneView& v = s->view;
     v.nodes = s->bvh.nodes.as<float4>();
     v.tris = s->bvh.tris.as<float4>();
@@ -359,10 +357,19 @@ void crt::scene_make_view(crt_scene* s) {
     v.n_tris = s->n_tris;
     v.n_instances = s->n_instances;
     v.stack_cap = std::max<uint32_t>(8u, 7u * s->bvh.max_depth + 1u);
+    v.lane_cap = s->bvh.max_depth + 1u;
+    // Bounding sphere for FP64 rebasing: centre of the box, radius = the
+    // farthest vertex, padded so FP32 rounding of rebased origins stays
+    // outside every triangle.
     const double* bb = s->info.bounds;
     const double cx = 0.5 * (bb[0] + bb[3]), cy = 0.5 * (bb[1] + bb[4]), cz = 0.5 * (bb[2] + bb[5]);
-    const double hx = 0.5 * (bb[3] - bb[0]), hy = 0.5 * (bb[4] - bb[1]), hz = 0.5 * (bb[5] - bb[2]);
-    double r = std::sqrt(hx * hx + hy * hy + hz * hz);
+    double r2 = 0.0;
+    for (uint64_t i = 0; i < s->n_vertices; ++i) {
+        const double dx = host_vertices[3 * i] - cx, dy = host_vertices[3 * i + 1] - cy,
+                     dz = host_vertices[3 * i + 2] - cz;
+        r2 = std::max(r2, dx * dx + dy * dy + dz * dz);
+    }
+    double r = std::sqrt(r2);
     r = r * (1.0 + 1e-3) + 1e-6 * (std::fabs(cx) + std::fabs(cy) + std::fabs(cz) + r) + 1e-9;
     v.center[0] = cx; v.center[1] = cy; v.center[2] = cz;
     v.radius = r;
@@ -449,8 +456,9 @@ int crt_scene_create(const float* vertices, uint32_t n_vertices, const uint32_t*
         inf.device_bytes = s->vertices.bytes + s->indices.bytes + s->tri_vidx.bytes + s->tri_inst.bytes +
                            s->inst_color.bytes + s->inst_smooth.bytes + s->vnormals.bytes + s->bvh.nodes.bytes +
                            s->bvh.tris.bytes;
-        scene_make_view(s.get());
-        CRT_REQUIRE(trace_lds_bytes(s->view) <= 64u * 1024u, CRT_ERR_INVALID,
+        scene_make_view(s.get(), vertices);
+        CRT_REQUIRE(s->bvh.n_nodes < (1u << 23), CRT_ERR_INVALID, "BVH exceeds 2^23 wide nodes");
+        CRT_REQUIRE(trace_lds_bytes(s->view) <= 64u * 1024u && lane_lds_bytes(s->view) <= 64u * 1024u, CRT_ERR_INVALID,
                     "BVH too deep for the LDS traversal stack");
         *out_scene = s.release();
     });
@@ -482,9 +490,9 @@ int crt_pass(crt_scene* scene, const crt_camera* camera, double* xyz, double* no
         if (depth) { if (dev) o.depth = depth; else { ws.d.reserve(npix * 8); o.depth = ws.d.as<double>(); } }
         if (tri_id) { if (dev) o.tri = tri_id; else { ws.e.reserve(npix * 4); o.tri = ws.e.as<int32_t>(); } }
         EventTimer tm;
-        const uint32_t blocks = cdiv((uint64_t)cam.w, 8) * cdiv((uint64_t)cam.h, 4);
+        const uint32_t blocks = cdiv((uint64_t)cam.w, PIX_TILE) * cdiv((uint64_t)cam.h, PIX_TILE);
         CRT_HIP_CHECK(hipEventRecord(tm.a, st));
-        hipLaunchKernelGGL(k_camera_pass, dim3(blocks), dim3(TRACE_BLOCK), trace_lds_bytes(scene->view), st,
+        hipLaunchKernelGGL(k_camera_pass, dim3(blocks), dim3(TRACE_BLOCK), lane_lds_bytes(scene->view), st,
                            scene->view, cam, o);
         CRT_HIP_CHECK(hipGetLastError());
         CRT_HIP_CHECK(hipEventRecord(tm.b, st));
@@ -538,13 +546,13 @@ int crt_trace_rays(crt_scene* scene, uint64_t n, const double* origins, const do
             out.tri = tri ? reinterpret_cast<int32_t*>(ob + n * 24) : nullptr;
         }
         EventTimer tm;
-        const uint32_t blocks = cdiv(n, OCTETS_PER_BLOCK);
+        const uint32_t blocks = cdiv(n, TRACE_BLOCK);
         CRT_HIP_CHECK(hipEventRecord(tm.a, st));
         if (any)
-            hipLaunchKernelGGL(k_trace_rays<true>, dim3(blocks), dim3(TRACE_BLOCK), trace_lds_bytes(scene->view), st,
+            hipLaunchKernelGGL(k_trace_rays<true>, dim3(blocks), dim3(TRACE_BLOCK), lane_lds_bytes(scene->view), st,
                                scene->view, in, out, n);
         else
-            hipLaunchKernelGGL(k_trace_rays<false>, dim3(blocks), dim3(TRACE_BLOCK), trace_lds_bytes(scene->view), st,
+            hipLaunchKernelGGL(k_trace_rays<false>, dim3(blocks), dim3(TRACE_BLOCK), lane_lds_bytes(scene->view), st,
                                scene->view, in, out, n);
         CRT_HIP_CHECK(hipGetLastError());
         CRT_HIP_CHECK(hipEventRecord(tm.b, st));
@@ -592,11 +600,11 @@ int crt_sun_pass(crt_scene* scene, const crt_camera* camera, const double sun_di
         uint32_t* qcount = ws.counters.as<uint32_t>();
         const float offset = self_offset(scene);
         EventTimer tm;
-        const uint32_t blocks = cdiv((uint64_t)cam.w, 8) * cdiv((uint64_t)cam.h, 4);
+        const uint32_t blocks = cdiv((uint64_t)cam.w, PIX_TILE) * cdiv((uint64_t)cam.h, PIX_TILE);
         CRT_HIP_CHECK(hipMemsetAsync(qcount, 0, 4, st));
         CRT_HIP_CHECK(hipEventRecord(tm.a, st));
-        hipLaunchKernelGGL(k_primary_sun, dim3(blocks), dim3(TRACE_BLOCK), trace_lds_bytes(scene->view), st,
-                           scene->view, cam, sun, d_depth, d_tri, d_lit, ws.h.as<ShadowRay>(), qcount, offset);
+        hipLaunchKernelGGL(k_primary_sun, dim3(blocks), dim3(TRACE_BLOCK), lane_lds_bytes(scene->view), st,
+                           scene->view, cam, d_depth, d_tri, d_lit, ws.h.as<ShadowRay>(), qcount, offset);
         CRT_HIP_CHECK(hipEventRecord(tm.b, st));
         hipLaunchKernelGGL(k_shadow_sun, dim3(cdiv(npix, OCTETS_PER_BLOCK)), dim3(TRACE_BLOCK),
                            trace_lds_bytes(scene->view), st, scene->view, ws.h.as<ShadowRay>(), qcount, sun, d_lit);
