@@ -388,6 +388,68 @@ __global__ __launch_bounds__(BLK) void k_emit(const float* __restrict__ V, const
     tris[3 * k + 2] = make_float4(V[3 * c], V[3 * c + 1], V[3 * c + 2], 0.f);
 }
 
+// Compressed copy of every wide node (layout in crt_device.hpp): origin =
+// min corner of the children's union, per-axis power-of-two step such that
+// the union spans at most 255 steps, child bounds rounded outwards to steps.
+__global__ __launch_bounds__(BLK) void k_compress(const float4* __restrict__ nodes, uint32_t n,
+                                                  float4* __restrict__ cnodes) {
+    const uint32_t i = blockIdx.x * BLK + threadIdx.x;
+    if (i >= n) return;
+    const float4* nd = nodes + (size_t)i * 16u;
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    uint32_t refs[8];
+    for (int k = 0; k < 8; ++k) {
+        const float4 l = nd[2 * k], h = nd[2 * k + 1];
+        refs[k] = __float_as_uint(l.w);
+        if (refs[k] == REF_EMPTY) continue;
+        lo[0] = fminf(lo[0], l.x); lo[1] = fminf(lo[1], l.y); lo[2] = fminf(lo[2], l.z);
+        hi[0] = fmaxf(hi[0], h.x); hi[1] = fmaxf(hi[1], h.y); hi[2] = fmaxf(hi[2], h.z);
+    }
+    int e[3];
+    double step[3];
+    for (int a = 0; a < 3; ++a) {
+        if (!(lo[a] <= hi[a])) { lo[a] = 0.f; hi[a] = 0.f; }
+        const double ext = (double)hi[a] - (double)lo[a];
+        int ea = -100;
+        if (ext > 0.0) {
+            ea = (int)ceil(log2(ext / 255.0));
+            if (ea < -100) ea = -100;
+            while (ldexp(255.0, ea) < ext) ++ea;
+            while (ea > -100 && ldexp(255.0, ea - 1) >= ext) --ea;
+        }
+        e[a] = ea;
+        step[a] = ldexp(1.0, ea);
+    }
+    uint32_t q[6][2] = {{0u, 0u}, {0u, 0u}, {0u, 0u}, {0u, 0u}, {0u, 0u}, {0u, 0u}};   // lo xyz, hi xyz
+    for (int k = 0; k < 8; ++k) {
+        if (refs[k] == REF_EMPTY) continue;
+        const float4 l = nd[2 * k], h = nd[2 * k + 1];
+        const float cl[3] = {l.x, l.y, l.z}, ch[3] = {h.x, h.y, h.z};
+        for (int a = 0; a < 3; ++a) {
+            double ql = floor(((double)cl[a] - (double)lo[a]) / step[a]);
+            double qh = ceil(((double)ch[a] - (double)lo[a]) / step[a]);
+            ql = fmin(fmax(ql, 0.0), 255.0);
+            qh = fmin(fmax(qh, 0.0), 255.0);
+            q[a][k >> 2] |= (uint32_t)ql << (8 * (k & 3));
+            q[3 + a][k >> 2] |= (uint32_t)qh << (8 * (k & 3));
+        }
+    }
+    const uint32_t ew = ((uint32_t)(uint8_t)(int8_t)e[0]) | ((uint32_t)(uint8_t)(int8_t)e[1] << 8) |
+                        ((uint32_t)(uint8_t)(int8_t)e[2] << 16);
+    float4* o = cnodes + (size_t)i * 6u;
+    o[0] = make_float4(lo[0], lo[1], lo[2], __uint_as_float(ew));
+    o[1] = make_float4(__uint_as_float(q[0][0]), __uint_as_float(q[0][1]), __uint_as_float(q[1][0]),
+                       __uint_as_float(q[1][1]));
+    o[2] = make_float4(__uint_as_float(q[2][0]), __uint_as_float(q[2][1]), __uint_as_float(q[3][0]),
+                       __uint_as_float(q[3][1]));
+    o[3] = make_float4(__uint_as_float(q[4][0]), __uint_as_float(q[4][1]), __uint_as_float(q[5][0]),
+                       __uint_as_float(q[5][1]));
+    o[4] = make_float4(__uint_as_float(refs[0]), __uint_as_float(refs[1]), __uint_as_float(refs[2]),
+                       __uint_as_float(refs[3]));
+    o[5] = make_float4(__uint_as_float(refs[4]), __uint_as_float(refs[5]), __uint_as_float(refs[6]),
+                       __uint_as_float(refs[7]));
+}
+
 inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
 }  // namespace
@@ -494,6 +556,9 @@ void build_bvh(const float* d_vertices, const uint32_t* d_indices, uint32_t n_tr
     out.max_depth = depth;
     out.nodes.alloc((size_t)out.n_nodes * 256);
     CRT_HIP_CHECK(hipMemcpyAsync(out.nodes.p, nodes_tmp.p, (size_t)out.n_nodes * 256, hipMemcpyDeviceToDevice, st));
+    out.cnodes.alloc((size_t)out.n_nodes * 96);
+    hipLaunchKernelGGL(k_compress, dim3(cdiv(out.n_nodes, BLK)), dim3(BLK), 0, st, out.nodes.as<float4>(), out.n_nodes,
+                       out.cnodes.as<float4>());
     out.tris.alloc((size_t)n * 48);
     hipLaunchKernelGGL(k_emit, dim3(cdiv(n, BLK)), dim3(BLK), 0, st, d_vertices, d_indices, n, sv,
                        out.tris.as<float4>());

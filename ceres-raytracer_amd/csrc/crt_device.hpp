@@ -32,6 +32,13 @@ constexpr uint32_t REF_LEAF = 0x80000000u;
 constexpr int OCTET = 8;
 constexpr int TRACE_BLOCK = 256;                 // threads per block
 constexpr int OCTETS_PER_BLOCK = TRACE_BLOCK / OCTET;
+// Lane kernels run one wave per block: a block's LDS stack is released as
+// soon as its wave ends, so short waves (rays that miss) never hold a
+// long-running neighbour's slot.
+constexpr int LANE_BLOCK = 64;
+// Lane kernels ask for 8 waves per SIMD (at most 64 VGPRs): the traversal is
+// latency-bound and needs every resident wave it can get.
+#define CRT_LANE_KERNEL __global__ __launch_bounds__(LANE_BLOCK) __attribute__((amdgpu_waves_per_eu(8)))
 
 struct __align__(16) WideChild {
     float lo[3];
@@ -50,6 +57,7 @@ __host__ __device__ inline uint32_t leaf_count(uint32_t ref) { return (ref & 15u
 // Device view of a scene (passed by value to kernels).
 struct SceneView {
     const float4* nodes;        // n_wide_nodes * 8 * 2 float4
+    const float4* cnodes;       // n_wide_nodes * 6 float4 (compressed, lane traversal)
     const float4* tris;         // n_tris * 3 float4, leaf order
     const uint32_t* tri_vidx;   // n_tris * 3, original order
     const float* vnormals;      // n_vertices * 3 (may be null)
@@ -319,7 +327,7 @@ __device__ inline HitRec trace_octet(const SceneView& s, const RayPre& r, float 
 // ------------------------------------------------------------------------
 // Lane traversal: one ray per lane, 64 independent rays per wave.
 //
-// A lane evaluates all 8 child records of a wide node itself.  The stack
+// A lane evaluates all 8 children of a (compressed) wide node itself.  The stack
 // holds node *groups*: (node << 8 | mask of children still to visit, nearest
 // tnear among them).  At a node the nearest hit child is entered and the
 // other hit children are pushed as one group; a popped group whose nearest
@@ -346,92 +354,81 @@ __device__ inline LaneStack lane_stack(const SceneView& s) {
     return k;
 }
 
-// Traversal state of one ray in one lane.  step() does one unit of work (a
-// node group, or a leaf plus the pops after it) and returns false once the
-// ray is finished; `hit` then holds the result.
-struct LaneTrace {
-    RayPre r;
-    float tmin, best;
-    uint32_t cur, sp;
-    HitRec hit;
+// Compressed wide node (lane traversal), 96 B = 6 x 16 B, written by the
+// build from the 256-B node with the same child order:
+//   [0] origin p.xyz (float), w = exponents ex | ey << 8 | ez << 16 (int8)
+//   [1] qlo_x[0..7] | qlo_y[0..7]      (u8, child k in byte k)
+//   [2] qlo_z[0..7] | qhi_x[0..7]
+//   [3] qhi_y[0..7] | qhi_z[0..7]
+//   [4] ref[0..3]   [5] ref[4..7]      (same refs as the 256-B node)
+// Child k's box on axis a is [p_a + 2^e_a qlo_a[k], p_a + 2^e_a qhi_a[k]],
+// rounded outwards from the padded FP32 box, so it always contains it.
+// A lane reads the whole node with 6 loads instead of 16; the slab test per
+// plane stays one FMA, t = q * (2^e / d) + (p - o) / d, and the ray octant
+// picks near and far planes per axis for 4 children at a time.
+__device__ inline float cvt_u8(uint32_t w, uint32_t byte) {
+    return (float)((w >> (8u * byte)) & 0xFFu);
+}
 
-    __device__ inline void init(const RayPre& ray, float t0, float t1) {
-        r = ray;
-        tmin = t0;
-        best = t1;
-        cur = 0xFFu;           // root group: node 0, all slots (empty slots never hit)
-        sp = 0u;
-        hit.t = t1; hit.u = 0.f; hit.v = 0.f; hit.slot = -1;
-    }
-
-    template <bool ANY_HIT>
-    __device__ inline bool step(const SceneView& s, const LaneStack& stk) {
-        if (!(cur & REF_LEAF)) {
-            // ---- node group: test the masked children against [tmin, best] ----
-            const float4* __restrict__ nd = s.nodes + (size_t)(cur >> 8) * 16u;
-            const uint32_t mask = cur & 0xFFu;
-            float tn0 = __int_as_float(0x7f800000), tn1 = tn0;   // nearest, and nearest of the rest
-            uint32_t k0 = 8u, ref0 = 0u, hits = 0u;
+// Test the children of compressed node `cn` in `mask` against [tmin, best].
+// Returns the hit mask; k0/ref0/tn0 = nearest hit child, tn1 = nearest of
+// the other hit children.
+__device__ inline uint32_t cnode_test(const float4* __restrict__ cn, const RayPre& r, float tmin, float best,
+                                      uint32_t mask, uint32_t& k0, uint32_t& ref0, float& tn0, float& tn1) {
+    const float4 h = cn[0];
+    const uint4 a = __builtin_bit_cast(uint4, cn[1]);
+    const uint4 b = __builtin_bit_cast(uint4, cn[2]);
+    const uint4 c = __builtin_bit_cast(uint4, cn[3]);
+    const uint4 r0 = __builtin_bit_cast(uint4, cn[4]);
+    const uint4 r1 = __builtin_bit_cast(uint4, cn[5]);
+    const uint32_t ew = __float_as_uint(h.w);
+    const float sx = __int_as_float(((int)(int8_t)(ew & 0xFFu) + 127) << 23);
+    const float sy = __int_as_float(((int)(int8_t)((ew >> 8) & 0xFFu) + 127) << 23);
+    const float sz = __int_as_float(((int)(int8_t)((ew >> 16) & 0xFFu) + 127) << 23);
+    const float Ax = sx * r.ix, Ay = sy * r.iy, Az = sz * r.iz;
+    const float Bx = (h.x - r.ox) * r.ix, By = (h.y - r.oy) * r.iy, Bz = (h.z - r.oz) * r.iz;
+    // near / far planes: lo for a positive direction component, hi for a negative one
+    const bool nx = r.ix < 0.f, ny = r.iy < 0.f, nz = r.iz < 0.f;
+    const uint32_t nx0 = nx ? b.z : a.x, nx1 = nx ? b.w : a.y, fx0 = nx ? a.x : b.z, fx1 = nx ? a.y : b.w;
+    const uint32_t ny0 = ny ? c.x : a.z, ny1 = ny ? c.y : a.w, fy0 = ny ? a.z : c.x, fy1 = ny ? a.w : c.y;
+    const uint32_t nz0 = nz ? c.z : b.x, nz1 = nz ? c.w : b.y, fz0 = nz ? b.x : c.z, fz1 = nz ? b.y : c.w;
+    const uint32_t refs[8] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
+    tn0 = __int_as_float(0x7f800000);
+    tn1 = tn0;
+    k0 = 8u;
+    ref0 = 0u;
+    uint32_t hits = 0u;
 #pragma unroll
-            for (uint32_t k = 0; k < 8u; ++k) {
-                const float4 q0 = nd[2u * k];
-                const float4 q1 = nd[2u * k + 1u];
-                const float tx0 = fmaf(q0.x, r.ix, r.nx), tx1 = fmaf(q1.x, r.ix, r.nx);
-                const float ty0 = fmaf(q0.y, r.iy, r.ny), ty1 = fmaf(q1.y, r.iy, r.ny);
-                const float tz0 = fmaf(q0.z, r.iz, r.nz), tz1 = fmaf(q1.z, r.iz, r.nz);
-                const float tnear = max3f(fminf(tx0, tx1), fminf(ty0, ty1), fmaxf(fminf(tz0, tz1), tmin));
-                const float tfar = min3f(fmaxf(tx0, tx1), fmaxf(ty0, ty1), fminf(fmaxf(tz0, tz1), best));
-                const uint32_t ref = __float_as_uint(q0.w);
-                const bool h = ((mask >> k) & 1u) && (ref != REF_EMPTY) && (tnear <= tfar * 1.00000024f);
-                if (h) {
-                    hits |= 1u << k;
-                    if (tnear < tn0) { tn1 = tn0; tn0 = tnear; k0 = k; ref0 = ref; }
-                    else tn1 = fminf(tn1, tnear);
-                }
-            }
-            if (hits != 0u) {
-                const uint32_t rest = hits & ~(1u << k0);
-                if (rest != 0u && sp < s.lane_cap) {   // sp < lane_cap always: one entry per level
-                    stk.g[sp * stk.stride] = (cur & ~0xFFu) | rest;
-                    stk.t[sp * stk.stride] = tn1;
-                    ++sp;
-                }
-                cur = (ref0 & REF_LEAF) ? ref0 : ((ref0 << 8) | 0xFFu);
-                return true;
-            }
-        } else {
-            // ---- leaf: test its triangles in order ----
-            const uint32_t first = leaf_first(cur);
-            const uint32_t count = leaf_count(cur);
-            const float4* __restrict__ tp = s.tris + (size_t)first * 3u;
-            for (uint32_t k = 0; k < count; ++k) {
-                float t, u, v;
-                if (tri_test(r, tp[3u * k], tp[3u * k + 1u], tp[3u * k + 2u], tmin, best, t, u, v)) {
-                    // t <= best: exact ties go to the later triangle, like the
-                    // "last tested wins" rule of lib/bvh/single_ray_traverser.hpp:55-60
-                    hit.t = t; hit.u = u; hit.v = v; hit.slot = (int)(first + k);
-                    if (ANY_HIT) return false;
-                    best = t;
-                }
-            }
+    for (uint32_t k = 0; k < 8u; ++k) {
+        const uint32_t byte = k & 3u;
+        const bool hi4 = k >= 4u;
+        const float tnx = fmaf(cvt_u8(hi4 ? nx1 : nx0, byte), Ax, Bx);
+        const float tny = fmaf(cvt_u8(hi4 ? ny1 : ny0, byte), Ay, By);
+        const float tnz = fmaf(cvt_u8(hi4 ? nz1 : nz0, byte), Az, Bz);
+        const float tfx = fmaf(cvt_u8(hi4 ? fx1 : fx0, byte), Ax, Bx);
+        const float tfy = fmaf(cvt_u8(hi4 ? fy1 : fy0, byte), Ay, By);
+        const float tfz = fmaf(cvt_u8(hi4 ? fz1 : fz0, byte), Az, Bz);
+        const float tnear = fmaxf(max3f(tnx, tny, tnz), tmin);
+        const float tfar = fminf(min3f(tfx, tfy, tfz), best);
+        const uint32_t ref = refs[k];
+        const bool hk = ((mask >> k) & 1u) && (ref != REF_EMPTY) && (tnear <= tfar * 1.00000024f);
+        if (hk) {
+            hits |= 1u << k;
+            if (tnear < tn0) { tn1 = tn0; tn0 = tnear; k0 = k; ref0 = ref; }
+            else tn1 = fminf(tn1, tnear);
         }
-        // ---- pop the next group that can still hold a closer hit ----
-        while (sp > 0u) {
-            --sp;
-            if (stk.t[sp * stk.stride] <= best) { cur = stk.g[sp * stk.stride]; return true; }
-        }
-        return false;
     }
-};
+    return hits;
+}
 
-// "While-while" form of the same traversal (Aila and Laine, HPG 2009): a
-// lane that reaches a leaf parks it and keeps descending until it meets a
-// second leaf or every lane of the wave holds one; then the wave tests
-// triangles together.  Inner-node and leaf work of different lanes are thus
-// not interleaved step by step.
+// Lane traversal in "while-while" form (Aila and Laine, HPG 2009): a lane
+// that reaches a leaf parks it and keeps descending until it meets a second
+// leaf or every lane of the wave holds one; then the wave tests triangles
+// together, so inner-node and leaf work of different lanes are not
+// interleaved step by step.
 template <bool ANY_HIT>
-__device__ inline HitRec trace_lane_ww(const SceneView& s, const RayPre& r, float tmin, float tmax,
-                                       const LaneStack& stk) {
+__device__ inline HitRec trace_lane(const SceneView& s, const RayPre& r, float tmin, float tmax,
+                                    const LaneStack& stk) {
     HitRec hit;
     hit.t = tmax; hit.u = 0.f; hit.v = 0.f; hit.slot = -1;
     float best = tmax;
@@ -441,27 +438,10 @@ __device__ inline HitRec trace_lane_ww(const SceneView& s, const RayPre& r, floa
     while (cur != 0u || leaf != 0u) {
         // ---- inner nodes ----
         while (cur != 0u && !(cur & REF_LEAF)) {
-            const float4* __restrict__ nd = s.nodes + (size_t)(cur >> 8) * 16u;
-            const uint32_t mask = cur & 0xFFu;
-            float tn0 = __int_as_float(0x7f800000), tn1 = tn0;
-            uint32_t k0 = 8u, ref0 = 0u, hits = 0u;
-#pragma unroll
-            for (uint32_t k = 0; k < 8u; ++k) {
-                const float4 q0 = nd[2u * k];
-                const float4 q1 = nd[2u * k + 1u];
-                const float tx0 = fmaf(q0.x, r.ix, r.nx), tx1 = fmaf(q1.x, r.ix, r.nx);
-                const float ty0 = fmaf(q0.y, r.iy, r.ny), ty1 = fmaf(q1.y, r.iy, r.ny);
-                const float tz0 = fmaf(q0.z, r.iz, r.nz), tz1 = fmaf(q1.z, r.iz, r.nz);
-                const float tnear = max3f(fminf(tx0, tx1), fminf(ty0, ty1), fmaxf(fminf(tz0, tz1), tmin));
-                const float tfar = min3f(fmaxf(tx0, tx1), fmaxf(ty0, ty1), fminf(fmaxf(tz0, tz1), best));
-                const uint32_t ref = __float_as_uint(q0.w);
-                const bool h = ((mask >> k) & 1u) && (ref != REF_EMPTY) && (tnear <= tfar * 1.00000024f);
-                if (h) {
-                    hits |= 1u << k;
-                    if (tnear < tn0) { tn1 = tn0; tn0 = tnear; k0 = k; ref0 = ref; }
-                    else tn1 = fminf(tn1, tnear);
-                }
-            }
+            uint32_t k0, ref0;
+            float tn0, tn1;
+            const uint32_t hits = cnode_test(s.cnodes + (size_t)(cur >> 8) * 6u, r, tmin, best, cur & 0xFFu, k0,
+                                             ref0, tn0, tn1);
             uint32_t next = 0u;
             if (hits != 0u) {
                 const uint32_t rest = hits & ~(1u << k0);
@@ -512,20 +492,6 @@ __device__ inline HitRec trace_lane_ww(const SceneView& s, const RayPre& r, floa
         }
     }
     return hit;
-}
-
-// Lanes run the while-while form (measured ~7% faster than the step form
-// on coherent camera rays).
-constexpr bool LANE_WHILE_WHILE = true;
-
-template <bool ANY_HIT>
-__device__ inline HitRec trace_lane(const SceneView& s, const RayPre& r, float tmin, float tmax,
-                                    const LaneStack& stk) {
-    if (LANE_WHILE_WHILE) return trace_lane_ww<ANY_HIT>(s, r, tmin, tmax, stk);
-    LaneTrace L;
-    L.init(r, tmin, tmax);
-    while (L.step<ANY_HIT>(s, stk)) {}
-    return L.hit;
 }
 
 // ------------------------------------------------------------------------

@@ -87,7 +87,8 @@ struct DevBuf {
 };
 
 struct BuildResult {
-    DevBuf nodes;        // wide nodes
+    DevBuf nodes;        // wide nodes (256 B)
+    DevBuf cnodes;       // the same nodes compressed (96 B)
     DevBuf tris;         // leaf-ordered triangles (3 float4 each)
     uint32_t n_nodes = 0;
     uint32_t n_leaves = 0;

@@ -249,9 +249,9 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace(Ctx C, int qsel) {
 }
 
 // One extension ray per lane (coherent rays: camera rays).
-__global__ __launch_bounds__(TRACE_BLOCK) void k_extend_lane(Ctx C, int qsel) {
+CRT_LANE_KERNEL void k_extend_lane(Ctx C, int qsel) {
     const LaneStack stk = lane_stack(C.s);
-    const uint32_t q = blockIdx.x * TRACE_BLOCK + threadIdx.x;
+    const uint32_t q = blockIdx.x * LANE_BLOCK + threadIdx.x;
     if (q >= C.cnt[qsel]) return;
     const uint32_t k = C.queue[qsel][q];
     const float4 o = C.st[k].o, d = C.st[k].d;
@@ -503,7 +503,6 @@ void launch_pass(const Ctx& C, uint64_t cap_paths, uint32_t cap_fold, const crt_
                  hipStream_t st) {
     const size_t lds = trace_lds_bytes(C.s);
     const uint32_t gp = std::max(1u, cdiv(cap_paths, TB));
-    static_assert(TB == TRACE_BLOCK, "lane kernels share the path grid");
     const uint32_t go = std::max(1u, cdiv(cap_paths, OCTETS_PER_BLOCK));
     const uint32_t gs = std::max(1u, cdiv(cap_paths * std::max(1, C.nl), OCTETS_PER_BLOCK));
     const uint32_t gpers = std::max(1u, B.blocks_per_cu) * (uint32_t)num_cus;
@@ -514,7 +513,8 @@ void launch_pass(const Ctx& C, uint64_t cap_paths, uint32_t cap_fold, const crt_
         hipLaunchKernelGGL(k_clear, dim3(1), dim3(32), 0, st, C.cnt, (1u << nxt) | (1u << 2) | (1u << 3) | (3u << 5));
         const bool lane = B.extend_mode == 2;
         if (lane)
-            hipLaunchKernelGGL(k_extend_lane, dim3(gp), dim3(TRACE_BLOCK), lane_lds_bytes(C.s), st, C, cur);
+            hipLaunchKernelGGL(k_extend_lane, dim3(std::max(1u, cdiv(cap_paths, LANE_BLOCK))), dim3(LANE_BLOCK),
+                               lane_lds_bytes(C.s), st, C, cur);
         else if (B.persistent)
             hipLaunchKernelGGL(k_trace_persistent<false>, dim3(std::min(gpers, go)), dim3(TRACE_BLOCK), lds, st, C, cur);
         else

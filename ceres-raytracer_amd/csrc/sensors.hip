@@ -47,11 +47,11 @@ __device__ inline double range_f64(const SceneView& s, const double o[3], const 
     return sqrt(px * px + py * py + pz * pz);
 }
 
-__global__ __launch_bounds__(TRACE_BLOCK) void k_lidar_batch(SceneView s, const double* __restrict__ pos,
+CRT_LANE_KERNEL void k_lidar_batch(SceneView s, const double* __restrict__ pos,
                                                              const double* __restrict__ rot, double zsign,
                                                              double* __restrict__ range, uint64_t n) {
     const LaneStack stk = lane_stack(s);
-    const uint64_t i = (uint64_t)blockIdx.x * TRACE_BLOCK + threadIdx.x;
+    const uint64_t i = (uint64_t)blockIdx.x * LANE_BLOCK + threadIdx.x;
     if (i >= n) return;
     const double* R = rot + 9 * i;
     // R^T (0, 0, +/-1) = +/- third row of R
@@ -68,10 +68,10 @@ struct FlashParams {
 };
 
 // One octet per (beam, ray) with ray 0 = boresight, 1..sub_rays perturbed.
-__global__ __launch_bounds__(TRACE_BLOCK) void k_flash(SceneView s, FlashParams F, double* __restrict__ ranges,
+CRT_LANE_KERNEL void k_flash(SceneView s, FlashParams F, double* __restrict__ ranges,
                                                        uint64_t n) {
     const LaneStack stk = lane_stack(s);
-    const uint64_t q = (uint64_t)blockIdx.x * TRACE_BLOCK + threadIdx.x;
+    const uint64_t q = (uint64_t)blockIdx.x * LANE_BLOCK + threadIdx.x;
     if (q >= n) return;
     const uint32_t per = F.sub_rays + 1;
     const uint64_t beam = q / per;
@@ -143,14 +143,12 @@ struct FluxParams {
     uint32_t grid;
 };
 
-// Block partials: 256 rays per block (one per lane); 7 compensated sums
-// each (Fx,Fy,Fz,Tx,Ty,Tz, hits), reduced per wave with shuffles, then
-// over the block's 4 waves.
-__global__ __launch_bounds__(TRACE_BLOCK) void k_flux(SceneView s, FluxParams F, Comp* __restrict__ partial,
+// Block partials: one wave of 64 rays per block; 7 compensated sums each
+// (Fx,Fy,Fz,Tx,Ty,Tz, hits), reduced across the wave with shuffles.
+CRT_LANE_KERNEL void k_flux(SceneView s, FluxParams F, Comp* __restrict__ partial,
                                                       uint64_t n) {
     const LaneStack stk = lane_stack(s);
-    __shared__ Comp red[7][TRACE_BLOCK / 64];
-    const uint64_t q = (uint64_t)blockIdx.x * TRACE_BLOCK + threadIdx.x;
+    const uint64_t q = (uint64_t)blockIdx.x * LANE_BLOCK + threadIdx.x;
     double term[7] = {0, 0, 0, 0, 0, 0, 0};
     if (q < n) {
         const uint32_t i = (uint32_t)(q % F.grid), j = (uint32_t)(q / F.grid);
@@ -198,15 +196,8 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_flux(SceneView s, FluxParams F,
     for (int w = 32; w > 0; w >>= 1)
         for (int k = 0; k < 7; ++k)
             acc[k] = comp_add(acc[k], Comp{__shfl_xor(acc[k].s, w, 64), __shfl_xor(acc[k].c, w, 64)});
-    const int wave = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0)
-        for (int k = 0; k < 7; ++k) red[k][wave] = acc[k];
-    __syncthreads();
-    if (threadIdx.x < 7) {
-        Comp t = red[threadIdx.x][0];
-        for (int w = 1; w < TRACE_BLOCK / 64; ++w) t = comp_add(t, red[threadIdx.x][w]);
-        partial[(size_t)blockIdx.x * 7 + threadIdx.x] = t;
-    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 7; ++k) partial[(size_t)blockIdx.x * 7 + k] = acc[k];
 }
 
 __global__ __launch_bounds__(256) void k_flux_final(const Comp* __restrict__ partial, uint32_t nblocks,
@@ -273,7 +264,7 @@ int crt_lidar_batch(crt_scene* scene, uint64_t n, const double* positions, const
         }
         Timer tm;
         CRT_HIP_CHECK(hipEventRecord(tm.a, st));
-        hipLaunchKernelGGL(k_lidar_batch, dim3(cdiv(n, TRACE_BLOCK)), dim3(TRACE_BLOCK),
+        hipLaunchKernelGGL(k_lidar_batch, dim3(cdiv(n, LANE_BLOCK)), dim3(LANE_BLOCK),
                            lane_lds_bytes(scene->view), st, scene->view, dp, dr, z_positive ? 1.0 : -1.0, dout, n);
         CRT_HIP_CHECK(hipGetLastError());
         CRT_HIP_CHECK(hipEventRecord(tm.b, st));
@@ -322,7 +313,7 @@ void flash_lidar_impl(crt_scene* scene, const crt_flash_opts* opts, double* sing
         }
         Timer tm;
         CRT_HIP_CHECK(hipEventRecord(tm.a, st));
-        hipLaunchKernelGGL(k_flash, dim3(cdiv(nr, TRACE_BLOCK)), dim3(TRACE_BLOCK), lane_lds_bytes(scene->view),
+        hipLaunchKernelGGL(k_flash, dim3(cdiv(nr, LANE_BLOCK)), dim3(LANE_BLOCK), lane_lds_bytes(scene->view),
                            st, scene->view, F, ws.f.as<double>(), nr);
         hipLaunchKernelGGL(k_flash_reduce, dim3(cdiv(nb, 256)), dim3(256), 0, st, ws.f.as<double>(), F.sub_rays, nb,
                            d_single, d_first, d_last, d_n);
@@ -377,13 +368,13 @@ void radiation_pressure_impl(crt_scene* scene, const crt_flux_opts* opts, double
         F.rho_d = opts->rho_d;
         F.origin_back = 2.0 * F.radius;
         const uint64_t n = (uint64_t)F.grid * F.grid;
-        const uint32_t nblocks = cdiv(n, TRACE_BLOCK);
+        const uint32_t nblocks = cdiv(n, LANE_BLOCK);
         ws.h.reserve((size_t)nblocks * 7 * sizeof(Comp));
         ws.counters.reserve(256);
         double* out = ws.counters.as<double>();
         Timer tm;
         CRT_HIP_CHECK(hipEventRecord(tm.a, st));
-        hipLaunchKernelGGL(k_flux, dim3(nblocks), dim3(TRACE_BLOCK), lane_lds_bytes(scene->view), st, scene->view, F,
+        hipLaunchKernelGGL(k_flux, dim3(nblocks), dim3(LANE_BLOCK), lane_lds_bytes(scene->view), st, scene->view, F,
                            ws.h.as<Comp>(), n);
         hipLaunchKernelGGL(k_flux_final, dim3(7), dim3(256), 0, st, ws.h.as<Comp>(), nblocks, out);
         CRT_HIP_CHECK(hipGetLastError());

@@ -22,7 +22,7 @@ inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b
 }  // namespace
 
 size_t trace_lds_bytes(const SceneView& s) { return (size_t)OCTETS_PER_BLOCK * s.stack_cap * 8u; }
-size_t lane_lds_bytes(const SceneView& s) { return (size_t)TRACE_BLOCK * s.lane_cap * 8u; }
+size_t lane_lds_bytes(const SceneView& s) { return (size_t)LANE_BLOCK * s.lane_cap * 8u; }
 
 int device_cus() {
     static std::mutex mu;
@@ -130,15 +130,13 @@ __device__ inline void hit_normal(const SceneView& s, int slot, float u, float v
     n[0] = x * inv; n[1] = y * inv; n[2] = z * inv;
 }
 
-// Pixel of thread `t` in block `blk`: a block covers 16x16 pixels, each wave
-// an 8x8 quarter of it (rays of a wave stay close on the image).
-constexpr int PIX_TILE = 16;
+// Pixel of lane `t` in block `blk`: a block (one wave) covers an 8x8 pixel
+// tile, tiles run row-major.
+constexpr int PIX_TILE = 8;
 __device__ inline void tile_pixel(int blk, int t, int w, int& px, int& py) {
     const int tiles_x = (w + PIX_TILE - 1) / PIX_TILE;
-    const int tx = blk % tiles_x, ty = blk / tiles_x;
-    const int wave = t >> 6, lane = t & 63;
-    px = tx * PIX_TILE + (wave & 1) * 8 + (lane & 7);
-    py = ty * PIX_TILE + (wave >> 1) * 8 + (lane >> 3);
+    px = (blk % tiles_x) * PIX_TILE + (t & 7);
+    py = (blk / tiles_x) * PIX_TILE + (t >> 3);
 }
 
 struct PassOut {
@@ -149,7 +147,7 @@ struct PassOut {
     int32_t* tri;
 };
 
-__global__ __launch_bounds__(TRACE_BLOCK) void k_camera_pass(SceneView s, CamView cam, PassOut out) {
+CRT_LANE_KERNEL void k_camera_pass(SceneView s, CamView cam, PassOut out) {
     const LaneStack stk = lane_stack(s);
     int px, py;
     tile_pixel(blockIdx.x, threadIdx.x, cam.w, px, py);
@@ -204,9 +202,9 @@ struct RayOut {
 };
 
 template <bool ANY>
-__global__ __launch_bounds__(TRACE_BLOCK) void k_trace_rays(SceneView s, RayIn in, RayOut out, uint64_t n) {
+CRT_LANE_KERNEL void k_trace_rays(SceneView s, RayIn in, RayOut out, uint64_t n) {
     const LaneStack stk = lane_stack(s);
-    const uint64_t ray = (uint64_t)blockIdx.x * TRACE_BLOCK + threadIdx.x;
+    const uint64_t ray = (uint64_t)blockIdx.x * LANE_BLOCK + threadIdx.x;
     if (ray >= n) return;
     double o[3] = {in.o[3 * ray], in.o[3 * ray + 1], in.o[3 * ray + 2]};
     double d[3] = {in.d[3 * ray], in.d[3 * ray + 1], in.d[3 * ray + 2]};
@@ -236,7 +234,7 @@ struct ShadowRay {
     float4 o;       // origin xyz, pixel index (bits)
 };
 
-__global__ __launch_bounds__(TRACE_BLOCK) void k_primary_sun(SceneView s, CamView cam, double* depth, int32_t* tri,
+CRT_LANE_KERNEL void k_primary_sun(SceneView s, CamView cam, double* depth, int32_t* tri,
                                                              uint8_t* lit, ShadowRay* queue, uint32_t* qcount,
                                                              float offset) {
     const LaneStack stk = lane_stack(s);
@@ -348,6 +346,7 @@ using namespace crt;
 void crt::scene_make_view(crt_scene* s, const float* host_vertices) {
     SceneView& v = s->view;
     v.nodes = s->bvh.nodes.as<float4>();
+    v.cnodes = s->bvh.cnodes.as<float4>();
     v.tris = s->bvh.tris.as<float4>();
     v.tri_vidx = s->tri_vidx.as<uint32_t>();
     v.vnormals = s->vnormals.p ? s->vnormals.as<float>() : nullptr;
@@ -406,7 +405,8 @@ int crt_scene_create(const float* vertices, uint32_t n_vertices, const uint32_t*
         for (uint64_t i = 0; i < 3ull * n_tris; ++i)
             CRT_REQUIRE(indices[i] < n_vertices, CRT_ERR_INVALID, "triangle index out of range");
         for (uint64_t i = 0; i < 3ull * n_vertices; ++i)
-            CRT_REQUIRE(std::isfinite(vertices[i]), CRT_ERR_INVALID, "vertex coordinate is not finite");
+            CRT_REQUIRE(std::isfinite(vertices[i]) && std::fabs(vertices[i]) <= 1e18f, CRT_ERR_INVALID,
+                        "vertex coordinate is not finite or exceeds 1e18");
         const uint32_t ni = std::max(n_instances, 1u);
         if (tri_instance)
             for (uint32_t i = 0; i < n_tris; ++i)
@@ -455,7 +455,7 @@ int crt_scene_create(const float* vertices, uint32_t n_vertices, const uint32_t*
         for (int k = 0; k < 6; ++k) inf.bounds[k] = s->bvh.bounds[k];   // min xyz, max xyz
         inf.device_bytes = s->vertices.bytes + s->indices.bytes + s->tri_vidx.bytes + s->tri_inst.bytes +
                            s->inst_color.bytes + s->inst_smooth.bytes + s->vnormals.bytes + s->bvh.nodes.bytes +
-                           s->bvh.tris.bytes;
+                           s->bvh.cnodes.bytes + s->bvh.tris.bytes;
         scene_make_view(s.get(), vertices);
         CRT_REQUIRE(s->bvh.n_nodes < (1u << 23), CRT_ERR_INVALID, "BVH exceeds 2^23 wide nodes");
         CRT_REQUIRE(trace_lds_bytes(s->view) <= 64u * 1024u && lane_lds_bytes(s->view) <= 64u * 1024u, CRT_ERR_INVALID,
@@ -492,7 +492,7 @@ int crt_pass(crt_scene* scene, const crt_camera* camera, double* xyz, double* no
         EventTimer tm;
         const uint32_t blocks = cdiv((uint64_t)cam.w, PIX_TILE) * cdiv((uint64_t)cam.h, PIX_TILE);
         CRT_HIP_CHECK(hipEventRecord(tm.a, st));
-        hipLaunchKernelGGL(k_camera_pass, dim3(blocks), dim3(TRACE_BLOCK), lane_lds_bytes(scene->view), st,
+        hipLaunchKernelGGL(k_camera_pass, dim3(blocks), dim3(LANE_BLOCK), lane_lds_bytes(scene->view), st,
                            scene->view, cam, o);
         CRT_HIP_CHECK(hipGetLastError());
         CRT_HIP_CHECK(hipEventRecord(tm.b, st));
@@ -546,13 +546,13 @@ int crt_trace_rays(crt_scene* scene, uint64_t n, const double* origins, const do
             out.tri = tri ? reinterpret_cast<int32_t*>(ob + n * 24) : nullptr;
         }
         EventTimer tm;
-        const uint32_t blocks = cdiv(n, TRACE_BLOCK);
+        const uint32_t blocks = cdiv(n, LANE_BLOCK);
         CRT_HIP_CHECK(hipEventRecord(tm.a, st));
         if (any)
-            hipLaunchKernelGGL(k_trace_rays<true>, dim3(blocks), dim3(TRACE_BLOCK), lane_lds_bytes(scene->view), st,
+            hipLaunchKernelGGL(k_trace_rays<true>, dim3(blocks), dim3(LANE_BLOCK), lane_lds_bytes(scene->view), st,
                                scene->view, in, out, n);
         else
-            hipLaunchKernelGGL(k_trace_rays<false>, dim3(blocks), dim3(TRACE_BLOCK), lane_lds_bytes(scene->view), st,
+            hipLaunchKernelGGL(k_trace_rays<false>, dim3(blocks), dim3(LANE_BLOCK), lane_lds_bytes(scene->view), st,
                                scene->view, in, out, n);
         CRT_HIP_CHECK(hipGetLastError());
         CRT_HIP_CHECK(hipEventRecord(tm.b, st));
@@ -603,7 +603,7 @@ int crt_sun_pass(crt_scene* scene, const crt_camera* camera, const double sun_di
         const uint32_t blocks = cdiv((uint64_t)cam.w, PIX_TILE) * cdiv((uint64_t)cam.h, PIX_TILE);
         CRT_HIP_CHECK(hipMemsetAsync(qcount, 0, 4, st));
         CRT_HIP_CHECK(hipEventRecord(tm.a, st));
-        hipLaunchKernelGGL(k_primary_sun, dim3(blocks), dim3(TRACE_BLOCK), lane_lds_bytes(scene->view), st,
+        hipLaunchKernelGGL(k_primary_sun, dim3(blocks), dim3(LANE_BLOCK), lane_lds_bytes(scene->view), st,
                            scene->view, cam, d_depth, d_tri, d_lit, ws.h.as<ShadowRay>(), qcount, offset);
         CRT_HIP_CHECK(hipEventRecord(tm.b, st));
         hipLaunchKernelGGL(k_shadow_sun, dim3(cdiv(npix, OCTETS_PER_BLOCK)), dim3(TRACE_BLOCK),
