@@ -252,7 +252,10 @@ __device__ inline HitRec trace_octet(const SceneView& s, const RayPre& r, float 
                 const uint32_t next = (uint32_t)__shfl((int)ref, lane_base + first_lane, 64);
                 if (h && rank > 0) {
                     const uint32_t pos = sp + (uint32_t)(nh - 1 - rank);
-                    if (pos < cap) { stk_ref[pos] = ref; stk_t[pos] = tnear; }
+                    if (pos < cap) {
+                        stk_ref[pos] = ref;
+                        if (!ANY_HIT) stk_t[pos] = tnear;   // any hit never tightens the bound
+                    }
                 }
                 sp += (uint32_t)(nh - 1);
                 if (sp > cap) sp = cap;   // never happens: cap >= 7 * depth (checked at build)
@@ -260,58 +263,57 @@ __device__ inline HitRec trace_octet(const SceneView& s, const RayPre& r, float 
                 continue;
             }
         } else {
-            // ---- leaf: lane `sub` tests triangles first+sub, first+sub+8, ... ----
+            // ---- leaf (at most 8 triangles, checked at scene creation):
+            //      lane `sub` tests triangle first + sub ----
             const uint32_t first = leaf_first(cur);
             const uint32_t count = leaf_count(cur);
-            for (uint32_t base = 0; base < count; base += OCTET) {
-                const uint32_t k = base + (uint32_t)sub;
-                float t = __int_as_float(0x7f800000);
-                float u = 0.f, v = 0.f;
-                bool ok = false;
-                if (k < count) {
-                    const size_t ti = (size_t)(first + k) * 3u;
-                    const float4 a = tris[ti];
-                    const float4 b = tris[ti + 1];
-                    const float4 c = tris[ti + 2];
-                    ok = tri_test(r, a, b, c, tmin, best, t, u, v);
-                    if (!ok) t = __int_as_float(0x7f800000);
+            float t = __int_as_float(0x7f800000);
+            float u = 0.f, v = 0.f;
+            bool ok = false;
+            if ((uint32_t)sub < count) {
+                const size_t ti = (size_t)(first + (uint32_t)sub) * 3u;
+                ok = tri_test(r, tris[ti], tris[ti + 1], tris[ti + 2], tmin, best, t, u, v);
+                if (!ok) t = __int_as_float(0x7f800000);
+            }
+            const uint32_t mh = (uint32_t)((__ballot(ok) >> lane_base) & 0xFFull);
+            if (ANY_HIT) {
+                if (mh != 0u) {
+                    const int l = __ffs(mh) - 1;
+                    hit.t = __shfl(t, lane_base + l, 64);
+                    hit.slot = (int)first + l;
+                    return hit;
                 }
-                if (ANY_HIT) {
-                    const uint32_t mh = (uint32_t)((__ballot(ok) >> lane_base) & 0xFFull);
-                    if (mh != 0u) {
-                        const int l = __ffs(mh) - 1;
-                        hit.t = __shfl(t, lane_base + l, 64);
-                        hit.slot = (int)(first + base) + l;
-                        return hit;
-                    }
-                } else {
-                    // octet min-reduce of (t, k); exact ties go to the later
-                    // triangle, like the "last tested wins" rule of
-                    // lib/bvh/single_ray_traverser.hpp:55-60
-                    // (t >= 0 or +inf: its bits, -0 folded to +0, order like the value)
-                    uint32_t bk = __float_as_uint(t) & 0x7FFFFFFFu;
-                    uint32_t bl = (uint32_t)sub;
-                    {
-                        uint32_t ok_ = dpp<DPP_XOR1>(bk), ol = dpp<DPP_XOR1>(bl);
-                        if (ok_ < bk || (ok_ == bk && ol > bl)) { bk = ok_; bl = ol; }
-                        ok_ = dpp<DPP_XOR2>(bk); ol = dpp<DPP_XOR2>(bl);
-                        if (ok_ < bk || (ok_ == bk && ol > bl)) { bk = ok_; bl = ol; }
-                        ok_ = dpp<DPP_HALF_MIRROR>(bk); ol = dpp<DPP_HALF_MIRROR>(bl);
-                        if (ok_ < bk || (ok_ == bk && ol > bl)) { bk = ok_; bl = ol; }
-                    }
-                    const float bt = __uint_as_float(bk);
-                    const uint32_t mh = (uint32_t)((__ballot(ok) >> lane_base) & 0xFFull);
-                    if (mh != 0u && bt <= best) {   // some lane accepted a hit
-                        best = bt;
-                        hit.t = bt;
-                        hit.u = __shfl(u, lane_base + (int)bl, 64);
-                        hit.v = __shfl(v, lane_base + (int)bl, 64);
-                        hit.slot = (int)(first + base) + (int)bl;
-                    }
+            } else if (mh != 0u) {
+                // octet min-reduce of (t, lane); exact ties go to the later
+                // triangle, like the "last tested wins" rule of
+                // lib/bvh/single_ray_traverser.hpp:55-60
+                // (t >= 0 or +inf: its bits, -0 folded to +0, order like the value)
+                uint32_t bk = __float_as_uint(t) & 0x7FFFFFFFu;
+                uint32_t bl = (uint32_t)sub;
+                {
+                    uint32_t ok_ = dpp<DPP_XOR1>(bk), ol = dpp<DPP_XOR1>(bl);
+                    if (ok_ < bk || (ok_ == bk && ol > bl)) { bk = ok_; bl = ol; }
+                    ok_ = dpp<DPP_XOR2>(bk); ol = dpp<DPP_XOR2>(bl);
+                    if (ok_ < bk || (ok_ == bk && ol > bl)) { bk = ok_; bl = ol; }
+                    ok_ = dpp<DPP_HALF_MIRROR>(bk); ol = dpp<DPP_HALF_MIRROR>(bl);
+                    if (ok_ < bk || (ok_ == bk && ol > bl)) { bk = ok_; bl = ol; }
+                }
+                const float bt = __uint_as_float(bk);
+                if (bt <= best) {
+                    best = bt;
+                    hit.t = bt;
+                    hit.u = __shfl(u, lane_base + (int)bl, 64);
+                    hit.v = __shfl(v, lane_base + (int)bl, 64);
+                    hit.slot = (int)first + (int)bl;
                 }
             }
         }
         // ---- pop the next entry that can still hold a closer hit ----
+        if (ANY_HIT) {
+            if (sp == 0u) break;
+            cur = stk_ref[--sp];
+            continue;
+        }
         bool found = false;
         while (sp > 0u) {
             --sp;
