@@ -11,7 +11,10 @@
 //                   scatter whose ranks come from 64-lane ballot matching
 //   4. k_karras   : binary radix-tree emission (Karras 2012); every internal
 //                   node covers a contiguous range of sorted triangles
-//   5. k_refit    : bottom-up AABB refit with one atomic arrival flag per node
+//   5. k_seg_level + k_range_boxes : every internal node covers a contiguous
+//                   range of sorted leaves, so its box is a range union over a
+//                   segment tree of the leaf boxes (O(log n) blocks per node;
+//                   no inter-thread communication, exact min/max)
 //   6. k_collapse : top-down, level-synchronous collapse into 8-wide nodes;
 //                   each wide node greedily opens its largest-area inner
 //                   children until 8 slots are used; subtrees with at most
@@ -134,41 +137,66 @@ __global__ __launch_bounds__(BLK) void k_rs_hist(const uint64_t* __restrict__ ke
     hist[threadIdx.x * nblocks + blockIdx.x] = cnt[threadIdx.x];
 }
 
-// Exclusive scan of `total` counters in place, one block of 1024 threads.
-__global__ __launch_bounds__(1024) void k_rs_scan(uint32_t* __restrict__ data, uint32_t total) {
-    __shared__ uint32_t part[1024];
-    const uint32_t per = (total + 1023) / 1024;
-    const uint32_t beg = threadIdx.x * per;
-    const uint32_t end = min(beg + per, total);
-    uint32_t s = 0;
-    for (uint32_t i = beg; i < end; ++i) s += data[i];
-    part[threadIdx.x] = s;
+// Inclusive scan across a wave (64 lanes).
+__device__ inline uint32_t wave_scan_incl(uint32_t v, int lane) {
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(v, o, 64);
+        if (lane >= o) v += y;
+    }
+    return v;
+}
+
+// Exclusive scan of each digit's row of per-block counts (one block per
+// digit, hist is digit-major); the row totals go to `totals`.
+__global__ __launch_bounds__(BLK) void k_rs_scan_rows(uint32_t* __restrict__ hist, uint32_t nb,
+                                                      uint32_t* __restrict__ totals) {
+    __shared__ uint32_t wsum[BLK / 64];
+    __shared__ uint32_t carry;
+    uint32_t* row = hist + (size_t)blockIdx.x * nb;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (threadIdx.x == 0) carry = 0u;
     __syncthreads();
-    // Hillis-Steele inclusive scan over 1024 partial sums
-    for (int off = 1; off < 1024; off <<= 1) {
-        const uint32_t x = threadIdx.x >= (uint32_t)off ? part[threadIdx.x - off] : 0u;
+    for (uint32_t base = 0; base < nb; base += BLK) {
+        const uint32_t i = base + threadIdx.x;
+        const uint32_t x = i < nb ? row[i] : 0u;
+        const uint32_t v = wave_scan_incl(x, lane);
+        if (lane == 63) wsum[wave] = v;
         __syncthreads();
-        part[threadIdx.x] += x;
+        uint32_t pre = carry;
+        for (int w = 0; w < wave; ++w) pre += wsum[w];
+        if (i < nb) row[i] = pre + v - x;
+        __syncthreads();
+        if (threadIdx.x == BLK - 1) carry = pre + v;
         __syncthreads();
     }
-    uint32_t run = threadIdx.x ? part[threadIdx.x - 1] : 0u;
-    for (uint32_t i = beg; i < end; ++i) {
-        const uint32_t x = data[i];
-        data[i] = run;
-        run += x;
-    }
+    if (threadIdx.x == 0) totals[blockIdx.x] = carry;
+}
+
+// Exclusive scan of the 256 digit totals in place (one block of 256).
+__global__ __launch_bounds__(BLK) void k_rs_scan_digits(uint32_t* __restrict__ totals) {
+    __shared__ uint32_t wsum[BLK / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t x = totals[threadIdx.x];
+    const uint32_t v = wave_scan_incl(x, lane);
+    if (lane == 63) wsum[wave] = v;
+    __syncthreads();
+    uint32_t pre = 0u;
+    for (int w = 0; w < wave; ++w) pre += wsum[w];
+    totals[threadIdx.x] = pre + v - x;
 }
 
 __global__ __launch_bounds__(BLK) void k_rs_scatter(const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                     uint64_t* __restrict__ kout, uint32_t* __restrict__ vout,
                                                     uint32_t n, int shift, uint32_t nblocks,
-                                                    const uint32_t* __restrict__ hist) {
+                                                    const uint32_t* __restrict__ hist,
+                                                    const uint32_t* __restrict__ digit_base) {
     __shared__ uint32_t run[256];
     __shared__ uint32_t wcnt[BLK / 64][256];
     const int tid = threadIdx.x;
     const int wave = tid >> 6;
     const int lane = tid & 63;
-    run[tid] = hist[tid * nblocks + blockIdx.x];
+    run[tid] = hist[tid * nblocks + blockIdx.x] + digit_base[tid];
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const uint32_t base = blockIdx.x * RS_TILE;
     for (int it = 0; it < RS_ITEMS; ++it) {
@@ -245,50 +273,54 @@ __global__ __launch_bounds__(BLK) void k_karras(const uint64_t* __restrict__ key
     rlast[i] = hi;
 }
 
-// Bottom-up refit.  blo/bhi hold boxes of all 2n-1 unified nodes (leaves
-// pre-filled from sorted triangle boxes).
-__global__ __launch_bounds__(BLK) void k_refit(int n, const int* __restrict__ left, const int* __restrict__ right,
-                                               const int* __restrict__ parent, float4* blo, float4* bhi,
-                                               uint32_t* __restrict__ flags) {
-    const int k = blockIdx.x * BLK + threadIdx.x;
-    if (k >= n) return;
-    int node = n - 1 + k;
-    int p = parent[node];
-    while (p >= 0) {
-        __threadfence();
-        const uint32_t old = atomicAdd(&flags[p], 1u);
-        if (old == 0u) return;          // first arrival: sibling not ready yet
-        __threadfence();
-        const int a = left[p], b = right[p];
-        // coherent reads of boxes written by other threads
-        float4 la, ha, lb, hb;
-        const float* pla = reinterpret_cast<const float*>(&blo[a]);
-        const float* pha = reinterpret_cast<const float*>(&bhi[a]);
-        const float* plb = reinterpret_cast<const float*>(&blo[b]);
-        const float* phb = reinterpret_cast<const float*>(&bhi[b]);
-        la.x = __hip_atomic_load(pla + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        la.y = __hip_atomic_load(pla + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        la.z = __hip_atomic_load(pla + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        ha.x = __hip_atomic_load(pha + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        ha.y = __hip_atomic_load(pha + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        ha.z = __hip_atomic_load(pha + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        lb.x = __hip_atomic_load(plb + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        lb.y = __hip_atomic_load(plb + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        lb.z = __hip_atomic_load(plb + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        hb.x = __hip_atomic_load(phb + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        hb.y = __hip_atomic_load(phb + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        hb.z = __hip_atomic_load(phb + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        float* plp = reinterpret_cast<float*>(&blo[p]);
-        float* php = reinterpret_cast<float*>(&bhi[p]);
-        __hip_atomic_store(plp + 0, fminf(la.x, lb.x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(plp + 1, fminf(la.y, lb.y), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(plp + 2, fminf(la.z, lb.z), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(php + 0, fmaxf(ha.x, hb.x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(php + 1, fmaxf(ha.y, hb.y), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(php + 2, fmaxf(ha.z, hb.z), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        node = p;
-        p = parent[node];
+// Segment tree over the sorted leaf boxes.  Level 0 is the leaves
+// themselves (blo/bhi[n-1 .. 2n-2]); entry i of level L >= 1 is the union of
+// entries 2i and 2i+1 of level L-1, i.e. of leaves [i 2^L, (i+1) 2^L).
+struct SegLevels {
+    uint32_t off[40];    // first entry of level L in the seg arrays (L >= 1)
+    uint32_t count;      // number of levels including level 0
+};
+
+__global__ __launch_bounds__(BLK) void k_seg_level(const float4* __restrict__ lo_in, const float4* __restrict__ hi_in,
+                                                   uint32_t n_in, float4* __restrict__ lo_out,
+                                                   float4* __restrict__ hi_out) {
+    const uint32_t i = blockIdx.x * BLK + threadIdx.x;
+    if (2u * i >= n_in) return;
+    float4 lo = lo_in[2 * i], hi = hi_in[2 * i];
+    if (2u * i + 1u < n_in) {
+        const float4 l2 = lo_in[2 * i + 1], h2 = hi_in[2 * i + 1];
+        lo.x = fminf(lo.x, l2.x); lo.y = fminf(lo.y, l2.y); lo.z = fminf(lo.z, l2.z);
+        hi.x = fmaxf(hi.x, h2.x); hi.y = fmaxf(hi.y, h2.y); hi.z = fmaxf(hi.z, h2.z);
     }
+    lo_out[i] = lo;
+    hi_out[i] = hi;
+}
+
+// Box of every internal node = union of its leaf range [rfirst, rlast],
+// gathered from O(log n) aligned segment-tree blocks.
+__global__ __launch_bounds__(BLK) void k_range_boxes(int n, const int* __restrict__ rfirst,
+                                                     const int* __restrict__ rlast, const float4* __restrict__ seg_lo,
+                                                     const float4* __restrict__ seg_hi, SegLevels L, float4* blo,
+                                                     float4* bhi) {
+    const int i = blockIdx.x * BLK + threadIdx.x;
+    if (i >= n - 1) return;
+    float4 lo = make_float4(INFINITY, INFINITY, INFINITY, 0.f), hi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.f);
+    auto take = [&](uint32_t level, uint32_t e) {
+        const float4 a = level == 0 ? blo[n - 1 + e] : seg_lo[L.off[level] + e];
+        const float4 b = level == 0 ? bhi[n - 1 + e] : seg_hi[L.off[level] + e];
+        lo.x = fminf(lo.x, a.x); lo.y = fminf(lo.y, a.y); lo.z = fminf(lo.z, a.z);
+        hi.x = fmaxf(hi.x, b.x); hi.y = fmaxf(hi.y, b.y); hi.z = fmaxf(hi.z, b.z);
+    };
+    uint32_t l = (uint32_t)rfirst[i], r = (uint32_t)rlast[i] + 1u, level = 0;
+    while (l < r) {
+        if (l & 1u) take(level, l++);
+        if (r & 1u) take(level, --r);
+        l >>= 1;
+        r >>= 1;
+        ++level;
+    }
+    blo[i] = lo;
+    bhi[i] = hi;
 }
 
 // Leaf boxes in sorted order: blo[n-1+k] = box of triangle vals[k].
@@ -487,23 +519,25 @@ void build_bvh(const float* d_vertices, const uint32_t* d_indices, uint32_t n_tr
     hipLaunchKernelGGL(k_morton, dim3(cdiv(n, BLK)), dim3(BLK), 0, st, cen.as<float4>(), n,
                        bounds.as<uint32_t>(), keys[0].as<uint64_t>(), vals[0].as<uint32_t>());
     const uint32_t nb = cdiv(n, RS_TILE);
-    hist.alloc((size_t)256 * nb * 4);
+    hist.alloc((size_t)256 * (nb + 1) * 4);   // per-block counts, then the 256 digit totals
     int cur = 0;
     for (int pass = 0; pass < 8; ++pass) {
         const int shift = pass * 8;
         hipLaunchKernelGGL(k_rs_hist, dim3(nb), dim3(BLK), 0, st, keys[cur].as<uint64_t>(), n, shift, nb,
                            hist.as<uint32_t>());
-        hipLaunchKernelGGL(k_rs_scan, dim3(1), dim3(1024), 0, st, hist.as<uint32_t>(), 256u * nb);
+        hipLaunchKernelGGL(k_rs_scan_rows, dim3(256), dim3(BLK), 0, st, hist.as<uint32_t>(), nb,
+                           hist.as<uint32_t>() + 256u * nb);
+        hipLaunchKernelGGL(k_rs_scan_digits, dim3(1), dim3(BLK), 0, st, hist.as<uint32_t>() + 256u * nb);
         hipLaunchKernelGGL(k_rs_scatter, dim3(nb), dim3(BLK), 0, st, keys[cur].as<uint64_t>(),
                            vals[cur].as<uint32_t>(), keys[cur ^ 1].as<uint64_t>(), vals[cur ^ 1].as<uint32_t>(),
-                           n, shift, nb, hist.as<uint32_t>());
+                           n, shift, nb, hist.as<uint32_t>(), hist.as<uint32_t>() + 256u * nb);
         cur ^= 1;
     }
     const uint64_t* sk = keys[cur].as<uint64_t>();
     const uint32_t* sv = vals[cur].as<uint32_t>();
 
     const uint32_t nn = 2 * n - 1;
-    DevBuf left, right, parent, rfirst, rlast, blo, bhi, flags;
+    DevBuf left, right, parent, rfirst, rlast, blo, bhi, seg_lo, seg_hi;
     left.alloc((size_t)std::max(n - 1, 1u) * 4);
     right.alloc((size_t)std::max(n - 1, 1u) * 4);
     rfirst.alloc((size_t)std::max(n - 1, 1u) * 4);
@@ -511,17 +545,37 @@ void build_bvh(const float* d_vertices, const uint32_t* d_indices, uint32_t n_tr
     parent.alloc((size_t)nn * 4);
     blo.alloc((size_t)nn * 16);
     bhi.alloc((size_t)nn * 16);
-    flags.alloc((size_t)std::max(n - 1, 1u) * 4);
     CRT_HIP_CHECK(hipMemsetAsync(parent.p, 0xFF, (size_t)nn * 4, st));   // root parent = -1
-    CRT_HIP_CHECK(hipMemsetAsync(flags.p, 0, flags.bytes, st));
     if (n > 1)
         hipLaunchKernelGGL(k_karras, dim3(cdiv(n - 1, BLK)), dim3(BLK), 0, st, sk, (int)n, left.as<int>(),
                            right.as<int>(), parent.as<int>(), rfirst.as<int>(), rlast.as<int>());
     hipLaunchKernelGGL(k_leaf_boxes, dim3(cdiv(n, BLK)), dim3(BLK), 0, st, (int)n, sv, tlo.as<float4>(),
                        thi.as<float4>(), blo.as<float4>(), bhi.as<float4>());
-    if (n > 1)
-        hipLaunchKernelGGL(k_refit, dim3(cdiv(n, BLK)), dim3(BLK), 0, st, (int)n, left.as<int>(), right.as<int>(),
-                           parent.as<int>(), blo.as<float4>(), bhi.as<float4>(), flags.as<uint32_t>());
+    if (n > 1) {
+        SegLevels L{};
+        uint32_t total = 0, sz = n;
+        L.count = 1;
+        while (sz > 1u) {
+            sz = (sz + 1u) / 2u;
+            CRT_REQUIRE(L.count < 40u, CRT_ERR_HIP, "segment tree too deep");
+            L.off[L.count++] = total;
+            total += sz;
+        }
+        seg_lo.alloc((size_t)total * 16);
+        seg_hi.alloc((size_t)total * 16);
+        sz = n;
+        for (uint32_t lv = 1; lv < L.count; ++lv) {
+            const uint32_t out_n = (sz + 1u) / 2u;
+            const float4* in_lo = lv == 1 ? blo.as<float4>() + (n - 1) : seg_lo.as<float4>() + L.off[lv - 1];
+            const float4* in_hi = lv == 1 ? bhi.as<float4>() + (n - 1) : seg_hi.as<float4>() + L.off[lv - 1];
+            hipLaunchKernelGGL(k_seg_level, dim3(cdiv(out_n, BLK)), dim3(BLK), 0, st, in_lo, in_hi, sz,
+                               seg_lo.as<float4>() + L.off[lv], seg_hi.as<float4>() + L.off[lv]);
+            sz = out_n;
+        }
+        hipLaunchKernelGGL(k_range_boxes, dim3(cdiv(n - 1, BLK)), dim3(BLK), 0, st, (int)n, rfirst.as<int>(),
+                           rlast.as<int>(), seg_lo.as<float4>(), seg_hi.as<float4>(), L, blo.as<float4>(),
+                           bhi.as<float4>());
+    }
 
     // ---- collapse to 8-wide, level by level ----
     const uint32_t node_cap = std::max(n, 2u);
