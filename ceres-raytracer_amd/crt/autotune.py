@@ -23,6 +23,8 @@ CANDIDATES = [
     dict(samples_per_pass=0, use_graph=False, persistent=False),
     dict(samples_per_pass=0, use_graph=True, persistent=False),
     dict(samples_per_pass=0, use_graph=True, persistent=False, extend_mode=2),
+    dict(samples_per_pass=0, use_graph=True, persistent=False, extend_mode=3),
+    dict(samples_per_pass=0, use_graph=True, persistent=True, blocks_per_cu=8, grab=4, extend_mode=3),
     dict(samples_per_pass=1, use_graph=True, persistent=False),
     dict(samples_per_pass=2, use_graph=True, persistent=False),
     dict(samples_per_pass=0, use_graph=True, persistent=True, blocks_per_cu=4, grab=4),

@@ -68,9 +68,19 @@ struct SceneView {
     uint32_t n_instances;
     uint32_t stack_cap;         // per-octet stack entries
     uint32_t lane_cap;          // per-lane group stack entries (max_depth + 1)
-    double center[3];           // bounding sphere for FP64 rebasing
+    double center[3];           // bounding sphere for FP64 rebasing (scene frame)
     double radius;
+    double origin[3];           // world position of the scene frame's origin
 };
+
+// Scene frame.  Triangles and nodes are stored relative to `origin` (the
+// scene's box centre rounded to the FP32 ulp of its largest coordinate),
+// so FP32 traversal resolves the scene at its own size however far it sits
+// from the world origin.  FP64 world points are moved in before rebasing and
+// hit points moved out after.
+__device__ inline void to_scene(const SceneView& s, const double w[3], double l[3]) {
+    l[0] = w[0] - s.origin[0]; l[1] = w[1] - s.origin[1]; l[2] = w[2] - s.origin[2];
+}
 
 // Closest-hit result of one traversal (local FP32 frame of the rebased ray).
 struct HitRec {
@@ -516,6 +526,28 @@ __device__ inline void camera_dir(const CamView& c, double u, double v, double d
     d[0] = c.rot[0] * x + c.rot[3] * y + c.rot[6] * z;
     d[1] = c.rot[1] * x + c.rot[4] * y + c.rot[7] * z;
     d[2] = c.rot[2] * x + c.rot[5] * y + c.rot[8] * z;
+}
+
+// FP64 re-evaluation of a hit on triangle `slot` (Moller-Trumbore on the FP32
+// vertices): t along the FP64 ray from o, u = weight of p1, v = weight of p2.
+// The FP32 watertight test decides *which* triangle is hit; this sharpens the
+// numbers, which FP32 resolves poorly on needle-thin triangles.  Returns
+// false (outputs untouched) if the triangle is degenerate in FP64 as well.
+__device__ inline bool refine_hit(const SceneView& s, int slot, const double o[3], const double d[3], double& t,
+                                  double& u, double& v) {
+    const float4 a = s.tris[3 * slot], b = s.tris[3 * slot + 1], c = s.tris[3 * slot + 2];
+    const double e1x = (double)b.x - a.x, e1y = (double)b.y - a.y, e1z = (double)b.z - a.z;
+    const double e2x = (double)c.x - a.x, e2y = (double)c.y - a.y, e2z = (double)c.z - a.z;
+    const double px = d[1] * e2z - d[2] * e2y, py = d[2] * e2x - d[0] * e2z, pz = d[0] * e2y - d[1] * e2x;
+    const double det = e1x * px + e1y * py + e1z * pz;
+    if (det == 0.0) return false;
+    const double inv = 1.0 / det;
+    const double tx = o[0] - a.x, ty = o[1] - a.y, tz = o[2] - a.z;
+    const double qx = ty * e1z - tz * e1y, qy = tz * e1x - tx * e1z, qz = tx * e1y - ty * e1x;
+    u = (tx * px + ty * py + tz * pz) * inv;
+    v = (d[0] * qx + d[1] * qy + d[2] * qz) * inv;
+    t = (e2x * qx + e2y * qy + e2z * qz) * inv;
+    return true;
 }
 
 // Distance along a unit FP64 ray to the entry of the scene's bounding sphere;

@@ -104,7 +104,8 @@ void build_bvh(const float* d_vertices, const uint32_t* d_indices, uint32_t n_tr
                uint32_t max_leaf, hipStream_t stream, BuildResult& out);
 
 // Shared helpers of the trace / render / sensor translation units.
-CamView make_cam_view(const crt_camera& c);
+// Camera in the scene frame of `s` (position moved by the scene origin).
+CamView make_cam_view(const crt_camera& c, const crt_scene* s);
 void check_camera_args(const crt_camera* c);
 size_t trace_lds_bytes(const SceneView& s);   // octet kernels
 size_t lane_lds_bytes(const SceneView& s);    // lane kernels

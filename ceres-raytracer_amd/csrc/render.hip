@@ -511,7 +511,7 @@ void launch_pass(const Ctx& C, uint64_t cap_paths, uint32_t cap_fold, const crt_
     for (int b = 0; b < C.nb; ++b) {
         const int cur = b & 1, nxt = cur ^ 1;
         hipLaunchKernelGGL(k_clear, dim3(1), dim3(32), 0, st, C.cnt, (1u << nxt) | (1u << 2) | (1u << 3) | (3u << 5));
-        const bool lane = B.extend_mode == 2;
+        const bool lane = B.extend_mode == 2 || (B.extend_mode == 3 && b == 0);
         if (lane)
             hipLaunchKernelGGL(k_extend_lane, dim3(std::max(1u, cdiv(cap_paths, LANE_BLOCK))), dim3(LANE_BLOCK),
                                lane_lds_bytes(C.s), st, C, cur);
@@ -595,7 +595,7 @@ int render_batch(crt_scene* scene, uint32_t n_cams, const crt_camera* cams, cons
     uint64_t total_pix = 0;
     for (uint32_t c = 0; c < n_cams; ++c) {
         check_camera_args(&cams[c]);
-        hc[c].v = make_cam_view(cams[c]);
+        hc[c].v = make_cam_view(cams[c], scene);
         spp_c[c] = max_samples ? max_samples[c] : opts->max_samples;
         CRT_REQUIRE(spp_c[c] >= 1, CRT_ERR_INVALID, "max_samples must be >= 1");
         hc[c].seed = seeds ? seeds[c] : opts->seed + c;
@@ -622,7 +622,7 @@ int render_batch(crt_scene* scene, uint32_t n_cams, const crt_camera* cams, cons
         LightDev& D = C.lights[l];
         D.type = L.type;
         D.intensity = (float)L.intensity;
-        for (int k = 0; k < 3; ++k) D.pos[k] = (float)L.position[k];
+        for (int k = 0; k < 3; ++k) D.pos[k] = (float)(L.position[k] - scene->view.origin[k]);   // scene frame
         for (int k = 0; k < 9; ++k) D.rot[k] = (float)L.rotation[k];
         D.size[0] = (float)L.size[0];
         D.size[1] = (float)L.size[1];

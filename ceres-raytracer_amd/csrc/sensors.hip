@@ -39,8 +39,10 @@ __device__ inline double range_f64(const SceneView& s, const double o[3], const 
                                   (float)d[0], (float)d[1], (float)d[2]);
     const HitRec h = trace_lane<false>(s, r, 0.0f, __int_as_float(0x7f800000), stk);
     if (h.slot < 0) return 0.0;
+    double t = t0 + (double)h.t, u = h.u, v = h.v;
+    refine_hit(s, h.slot, o, d, t, u, v);   // FP64 barycentrics of the chosen triangle
     const float4 a = s.tris[3 * h.slot], b = s.tris[3 * h.slot + 1], c = s.tris[3 * h.slot + 2];
-    const double u = h.u, v = h.v, w = 1.0 - u - v;
+    const double w = 1.0 - u - v;
     const double px = u * b.x + v * c.x + w * a.x - o[0];
     const double py = u * b.y + v * c.y + w * a.y - o[1];
     const double pz = u * b.z + v * c.z + w * a.z - o[2];
@@ -56,7 +58,9 @@ CRT_LANE_KERNEL void k_lidar_batch(SceneView s, const double* __restrict__ pos,
     const double* R = rot + 9 * i;
     // R^T (0, 0, +/-1) = +/- third row of R
     const double d[3] = {R[6] * zsign, R[7] * zsign, R[8] * zsign};
-    const double o[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+    const double ow[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+    double o[3];
+    to_scene(s, ow, o);
     range[i] = range_f64(s, o, d, stk);
 }
 
@@ -290,7 +294,7 @@ void flash_lidar_impl(crt_scene* scene, const crt_flash_opts* opts, double* sing
         CRT_REQUIRE(opts->sigma >= 0.0 && std::isfinite(opts->sigma), CRT_ERR_INVALID, "sigma must be >= 0");
         hipStream_t st = (hipStream_t)stream;
         FlashParams F;
-        F.cam = make_cam_view(opts->beams);
+        F.cam = make_cam_view(opts->beams, scene);
         F.sub_rays = opts->sub_rays;
         F.sigma = opts->sigma;
         F.seed = opts->seed;
@@ -353,14 +357,15 @@ void radiation_pressure_impl(crt_scene* scene, const crt_flux_opts* opts, double
         F.e2[0] = F.sun[1] * F.e1[2] - F.sun[2] * F.e1[1];
         F.e2[1] = F.sun[2] * F.e1[0] - F.sun[0] * F.e1[2];
         F.e2[2] = F.sun[0] * F.e1[1] - F.sun[1] * F.e1[0];
+        // grid centre and torque reference point in the scene frame
         if (opts->radius > 0) {
-            for (int k = 0; k < 3; ++k) F.center[k] = opts->center[k];
+            for (int k = 0; k < 3; ++k) F.center[k] = opts->center[k] - scene->view.origin[k];
             F.radius = opts->radius;
         } else {
             for (int k = 0; k < 3; ++k) F.center[k] = scene->view.center[k];
             F.radius = scene->view.radius;
         }
-        for (int k = 0; k < 3; ++k) F.ref[k] = opts->ref_point[k];
+        for (int k = 0; k < 3; ++k) F.ref[k] = opts->ref_point[k] - scene->view.origin[k];
         F.grid = opts->grid;
         F.h = 2.0 * F.radius / (double)F.grid;
         F.pressure = opts->pressure;

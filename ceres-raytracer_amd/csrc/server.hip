@@ -58,7 +58,7 @@ size_t request_bytes(const crt_request& r) {
             return (size_t)px * (r.sub_rays + 1) * 8 + px * 25;
         case CRT_REQ_FLUX: {
             const uint64_t n = (uint64_t)r.flux.grid * r.flux.grid;
-            return (size_t)(n / OCTETS_PER_BLOCK + 1) * 7 * 16 + 256;
+            return (size_t)(n / LANE_BLOCK + 1) * 7 * 16 + 256;
         }
         default:
             return 0;

@@ -39,9 +39,9 @@ int device_cus() {
     return cached[dev];
 }
 
-CamView make_cam_view(const crt_camera& c) {
+CamView make_cam_view(const crt_camera& c, const crt_scene* s) {
     CamView v;
-    for (int k = 0; k < 3; ++k) v.pos[k] = c.position[k];
+    for (int k = 0; k < 3; ++k) v.pos[k] = c.position[k] - s->view.origin[k];
     for (int k = 0; k < 9; ++k) v.rot[k] = c.rotation[k];
     // src/crt/cameras/simple_camera.hpp:23-26: centre = res/2, scale = res/sensor
     v.cx = c.resolution[0] / 2.0;
@@ -68,18 +68,18 @@ void check_camera_args(const crt_camera* c) {
 // extent so that it stays ~8 FP32 ulp above the coordinates' resolution.
 float self_offset(const crt_scene* s) {
     double ext = 0.0;
-    for (int k = 0; k < 6; ++k) ext = std::max(ext, std::fabs(s->info.bounds[k]));
+    for (int k = 0; k < 6; ++k) ext = std::max(ext, std::fabs((double)s->bvh.bounds[k]));   // scene frame
     return (float)std::max(1e-4, 1e-6 * ext);
 }
 
 namespace {
 
-// Closest/any hit of one FP64 ray (unit direction d) through the scene:
-// rebase in FP64, trace in FP32 (one ray per lane).  Returns false on miss;
-// otherwise t (FP64, from o), u, v (FP32) and the leaf-order slot.
+// Closest/any hit of one FP64 ray (unit direction d, origin in the scene
+// frame): rebase in FP64, trace in FP32 (one ray per lane), then t, u, v of
+// the hit triangle re-evaluated in FP64.  Returns false on a miss.
 template <bool ANY>
 __device__ inline bool trace_f64(const SceneView& s, const double o[3], const double d[3], double tmin,
-                                 double tmax, const LaneStack& stk, double& t_out, float& u, float& v, int& slot) {
+                                 double tmax, const LaneStack& stk, double& t_out, double& u, double& v, int& slot) {
     double t0;
     if (!rebase_t0(s, o, d, t0)) return false;
     if (t0 > tmax) return false;
@@ -93,11 +93,12 @@ __device__ inline bool trace_f64(const SceneView& s, const double o[3], const do
     u = h.u;
     v = h.v;
     slot = h.slot;
+    refine_hit(s, slot, o, d, t_out, u, v);
     return true;
 }
 
 // Hit point from barycentrics in FP64 (src/crt/passes.hpp:53-58).
-__device__ inline void bary_point(const SceneView& s, int slot, float u, float v, double p[3]) {
+__device__ inline void bary_point(const SceneView& s, int slot, double u, double v, double p[3]) {
     const float4 a = s.tris[3 * slot], b = s.tris[3 * slot + 1], c = s.tris[3 * slot + 2];
     const double du = u, dv = v, dw = 1.0 - du - dv;
     p[0] = du * b.x + dv * c.x + dw * a.x;
@@ -107,7 +108,7 @@ __device__ inline void bary_point(const SceneView& s, int slot, float u, float v
 
 // Normal pass value (src/crt/passes.hpp:176-181): flat = normalize((p0-p1)x(p2-p0))
 // (inward for CCW-outward meshes), smooth = normalize(u vn1 + v vn2 + w vn0).
-__device__ inline void hit_normal(const SceneView& s, int slot, float u, float v, double n[3]) {
+__device__ inline void hit_normal(const SceneView& s, int slot, double u, double v, double n[3]) {
     const float4 a = s.tris[3 * slot], b = s.tris[3 * slot + 1], c = s.tris[3 * slot + 2];
     const uint32_t orig = __float_as_uint(a.w);
     const uint32_t inst = s.tri_inst[orig];
@@ -156,7 +157,7 @@ CRT_LANE_KERNEL void k_camera_pass(SceneView s, CamView cam, PassOut out) {
     // integer pixel coordinates, like src/crt/passes.hpp:46
     camera_dir(cam, (double)px, (double)py, d);
     double t;
-    float u = 0.f, v = 0.f;
+    double u = 0.0, v = 0.0;
     int slot = -1;
     const bool hit = trace_f64<false>(s, cam.pos, d, 0.0, 1.0e300, stk, t, u, v, slot);
     const size_t pix = (size_t)py * cam.w + px;
@@ -166,9 +167,9 @@ CRT_LANE_KERNEL void k_camera_pass(SceneView s, CamView cam, PassOut out) {
         double p[3] = {0.0, 0.0, 0.0};
         if (hit) bary_point(s, slot, u, v, p);
         if (out.xyz) {
-            out.xyz[3 * pix] = p[0];
-            out.xyz[3 * pix + 1] = p[1];
-            out.xyz[3 * pix + 2] = p[2];
+            out.xyz[3 * pix] = hit ? p[0] + s.origin[0] : 0.0;
+            out.xyz[3 * pix + 1] = hit ? p[1] + s.origin[1] : 0.0;
+            out.xyz[3 * pix + 2] = hit ? p[2] + s.origin[2] : 0.0;
         }
         if (out.depth) {
             double dd = 0.0;
@@ -206,14 +207,16 @@ CRT_LANE_KERNEL void k_trace_rays(SceneView s, RayIn in, RayOut out, uint64_t n)
     const LaneStack stk = lane_stack(s);
     const uint64_t ray = (uint64_t)blockIdx.x * LANE_BLOCK + threadIdx.x;
     if (ray >= n) return;
-    double o[3] = {in.o[3 * ray], in.o[3 * ray + 1], in.o[3 * ray + 2]};
+    const double ow[3] = {in.o[3 * ray], in.o[3 * ray + 1], in.o[3 * ray + 2]};
+    double o[3];
+    to_scene(s, ow, o);
     double d[3] = {in.d[3 * ray], in.d[3 * ray + 1], in.d[3 * ray + 2]};
     const double len = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
     double tmin = in.tmin ? in.tmin[ray] : 0.0;
     double tmax = in.tmax ? in.tmax[ray] : 1.0e300;
     bool hit = false;
     double t = 0.0;
-    float u = 0.f, v = 0.f;
+    double u = 0.0, v = 0.0;
     int slot = -1;
     if (len > 0.0 && !(tmax < tmin)) {
         d[0] /= len; d[1] /= len; d[2] /= len;
@@ -225,8 +228,8 @@ CRT_LANE_KERNEL void k_trace_rays(SceneView s, RayIn in, RayOut out, uint64_t n)
     }
     if (out.t) out.t[ray] = hit ? t : -1.0;
     if (out.tri) out.tri[ray] = hit ? (int32_t)__float_as_uint(s.tris[3 * slot].w) : -1;
-    if (out.u) out.u[ray] = hit ? (double)u : 0.0;
-    if (out.v) out.v[ray] = hit ? (double)v : 0.0;
+    if (out.u) out.u[ray] = hit ? u : 0.0;
+    if (out.v) out.v[ray] = hit ? v : 0.0;
 }
 
 // ---- primary + sun shadow (wavefront: closest hit -> compacted shadow queue -> any hit)
@@ -249,7 +252,7 @@ CRT_LANE_KERNEL void k_primary_sun(SceneView s, CamView cam, double* depth, int3
         double d[3];
         camera_dir(cam, (double)px, (double)py, d);
         double t;
-        float u = 0.f, v = 0.f;
+        double u = 0.0, v = 0.0;
         int slot = -1;
         hit = trace_f64<false>(s, cam.pos, d, 0.0, 1.0e300, stk, t, u, v, slot);
         pix = (size_t)py * cam.w + px;
@@ -360,8 +363,9 @@ void crt::scene_make_view(crt_scene* s, const float* host_vertices) {
     // Bounding sphere for FP64 rebasing: centre of the box, radius = the
     // farthest vertex, padded so FP32 rounding of rebased origins stays
     // outside every triangle.
-    const double* bb = s->info.bounds;
-    const double cx = 0.5 * (bb[0] + bb[3]), cy = 0.5 * (bb[1] + bb[4]), cz = 0.5 * (bb[2] + bb[5]);
+    const float* bb = s->bvh.bounds;   // scene frame
+    const double cx = 0.5 * ((double)bb[0] + bb[3]), cy = 0.5 * ((double)bb[1] + bb[4]),
+                 cz = 0.5 * ((double)bb[2] + bb[5]);
     double r2 = 0.0;
     for (uint64_t i = 0; i < s->n_vertices; ++i) {
         const double dx = host_vertices[3 * i] - cx, dy = host_vertices[3 * i + 1] - cy,
@@ -372,7 +376,10 @@ void crt::scene_make_view(crt_scene* s, const float* host_vertices) {
     r = r * (1.0 + 1e-3) + 1e-6 * (std::fabs(cx) + std::fabs(cy) + std::fabs(cz) + r) + 1e-9;
     v.center[0] = cx; v.center[1] = cy; v.center[2] = cz;
     v.radius = r;
-    s->info.sphere[0] = cx; s->info.sphere[1] = cy; s->info.sphere[2] = cz; s->info.sphere[3] = r;
+    s->info.sphere[0] = cx + v.origin[0];   // world
+    s->info.sphere[1] = cy + v.origin[1];
+    s->info.sphere[2] = cz + v.origin[2];
+    s->info.sphere[3] = r;
     s->info.stack_capacity = v.stack_cap;
 }
 
@@ -417,13 +424,33 @@ int crt_scene_create(const float* vertices, uint32_t n_vertices, const uint32_t*
         s->n_tris = n_tris;
         s->n_vertices = n_vertices;
         s->n_instances = ni;
+        // scene frame: origin = box centre rounded to a multiple of the FP32
+        // ulp of the largest coordinate; vertex - origin is then exact for
+        // vertices of the origin's magnitude (a body far from the world
+        // origin) and never off by more than that ulp
+        double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        double amax = 0.0;
+        for (uint64_t i = 0; i < n_vertices; ++i)
+            for (int k = 0; k < 3; ++k) {
+                const double x = vertices[3 * i + k];
+                lo[k] = std::min(lo[k], x);
+                hi[k] = std::max(hi[k], x);
+                amax = std::max(amax, std::fabs(x));
+            }
+        int ex = 0;
+        std::frexp(amax, &ex);
+        const double q = std::ldexp(1.0, ex - 24);   // >= ulp of every vertex coordinate
+        for (int k = 0; k < 3; ++k) s->view.origin[k] = amax > 0.0 ? std::round(0.5 * (lo[k] + hi[k]) / q) * q : 0.0;
+        std::vector<float> local((size_t)n_vertices * 3);
+        for (uint64_t i = 0; i < 3ull * n_vertices; ++i)
+            local[i] = (float)((double)vertices[i] - s->view.origin[i % 3]);
         s->vertices.alloc((size_t)n_vertices * 12);
         s->indices.alloc((size_t)n_tris * 12);
         s->tri_vidx.alloc((size_t)n_tris * 12);
         s->tri_inst.alloc((size_t)n_tris * 4);
         s->inst_color.alloc((size_t)ni * 12);
         s->inst_smooth.alloc((size_t)ni);
-        CRT_HIP_CHECK(hipMemcpy(s->vertices.p, vertices, (size_t)n_vertices * 12, hipMemcpyHostToDevice));
+        CRT_HIP_CHECK(hipMemcpy(s->vertices.p, local.data(), (size_t)n_vertices * 12, hipMemcpyHostToDevice));
         CRT_HIP_CHECK(hipMemcpy(s->indices.p, indices, (size_t)n_tris * 12, hipMemcpyHostToDevice));
         CRT_HIP_CHECK(hipMemcpy(s->tri_vidx.p, indices, (size_t)n_tris * 12, hipMemcpyHostToDevice));
         if (tri_instance) {
@@ -452,11 +479,11 @@ int crt_scene_create(const float* vertices, uint32_t n_vertices, const uint32_t*
         inf.max_depth = s->bvh.max_depth;
         inf.build_ms = s->bvh.build_ms;
         inf.sah_cost = s->bvh.sah_cost;
-        for (int k = 0; k < 6; ++k) inf.bounds[k] = s->bvh.bounds[k];   // min xyz, max xyz
+        for (int k = 0; k < 6; ++k) inf.bounds[k] = s->bvh.bounds[k] + s->view.origin[k % 3];   // world, min/max xyz
         inf.device_bytes = s->vertices.bytes + s->indices.bytes + s->tri_vidx.bytes + s->tri_inst.bytes +
                            s->inst_color.bytes + s->inst_smooth.bytes + s->vnormals.bytes + s->bvh.nodes.bytes +
                            s->bvh.cnodes.bytes + s->bvh.tris.bytes;
-        scene_make_view(s.get(), vertices);
+        scene_make_view(s.get(), local.data());
         CRT_REQUIRE(s->bvh.n_nodes < (1u << 23), CRT_ERR_INVALID, "BVH exceeds 2^23 wide nodes");
         CRT_REQUIRE(trace_lds_bytes(s->view) <= 64u * 1024u && lane_lds_bytes(s->view) <= 64u * 1024u, CRT_ERR_INVALID,
                     "BVH too deep for the LDS traversal stack");
@@ -479,7 +506,7 @@ int crt_pass(crt_scene* scene, const crt_camera* camera, double* xyz, double* no
         CRT_REQUIRE(scene != nullptr, CRT_ERR_INVALID, "scene is NULL");
         check_camera_args(camera);
         hipStream_t st = (hipStream_t)stream;
-        const CamView cam = make_cam_view(*camera);
+        const CamView cam = make_cam_view(*camera, scene);
         const size_t npix = (size_t)cam.w * cam.h;
         const bool dev = (flags & CRT_OUT_DEVICE) != 0;
         Workspace& ws = scene->ws;
@@ -583,7 +610,7 @@ int crt_sun_pass(crt_scene* scene, const crt_camera* camera, const double sun_di
         CRT_REQUIRE(sl > 0 && std::isfinite(sl), CRT_ERR_INVALID, "sun_dir must be a finite non-zero vector");
         const float3 sun = make_float3((float)(sun_dir[0] / sl), (float)(sun_dir[1] / sl), (float)(sun_dir[2] / sl));
         hipStream_t st = (hipStream_t)stream;
-        const CamView cam = make_cam_view(*camera);
+        const CamView cam = make_cam_view(*camera, scene);
         const size_t npix = (size_t)cam.w * cam.h;
         const bool dev = (flags & CRT_OUT_DEVICE) != 0;
         Workspace& ws = scene->ws;

@@ -210,7 +210,8 @@ typedef struct {
     uint32_t blocks_per_cu;
     uint32_t grab;
     uint32_t extend_mode;     /* closest-hit extension rays: 0, 1 = an octet of lanes per ray,
-                                 2 = one lane per ray (coherent camera rays at high resolution) */
+                                 2 = one lane per ray, 3 = one lane per ray for camera rays and an
+                                 octet per ray after the first bounce */
 } crt_batch_opts;
 int crt_render_batch(crt_scene* scene, uint32_t n_cams, const crt_camera* cams, const int32_t* max_samples,
                      const uint64_t* seeds, const crt_light* lights, uint32_t n_lights,
