@@ -2,10 +2,13 @@
 
 Tunes crt_batch_opts -- samples per pass, hipGraph replay of a pass,
 persistent traversal kernels, persistent blocks per CU, rays claimed per
-atomic and the traversal form of extension rays -- by timing a representative subset of the batch (the real cameras,
-resolution-capped) and keeps the winner in a JSON cache keyed by device,
-scene size, bounces and light count.  Every candidate renders bit-identical
-images (the RNG is keyed per camera/pixel/sample), so only speed is compared.
+atomic and the traversal form of extension rays -- by timing a representative
+subset of the batch (the first cameras at their own resolution, capped only
+for very large ones: small images misjudge the traversal forms) and keeps the
+winner in a JSON cache keyed by device, scene size, bounces and light count.
+Every candidate renders the same images (the RNG is keyed per camera/pixel/
+sample; only the traversal form can flip an exact tie between two
+triangles), so only speed is compared.
 """
 from __future__ import annotations
 
@@ -24,12 +27,11 @@ CANDIDATES = [
     dict(samples_per_pass=0, use_graph=True, persistent=False),
     dict(samples_per_pass=0, use_graph=True, persistent=False, extend_mode=2),
     dict(samples_per_pass=0, use_graph=True, persistent=False, extend_mode=3),
-    dict(samples_per_pass=0, use_graph=True, persistent=True, blocks_per_cu=8, grab=4, extend_mode=3),
-    dict(samples_per_pass=1, use_graph=True, persistent=False),
     dict(samples_per_pass=2, use_graph=True, persistent=False),
-    dict(samples_per_pass=0, use_graph=True, persistent=True, blocks_per_cu=4, grab=4),
+    dict(samples_per_pass=2, use_graph=True, persistent=False, extend_mode=3),
+    dict(samples_per_pass=4, use_graph=True, persistent=False, extend_mode=3),
     dict(samples_per_pass=0, use_graph=True, persistent=True, blocks_per_cu=8, grab=4),
-    dict(samples_per_pass=0, use_graph=True, persistent=True, blocks_per_cu=8, grab=16),
+    dict(samples_per_pass=0, use_graph=True, persistent=True, blocks_per_cu=8, grab=4, extend_mode=3),
     dict(samples_per_pass=0, use_graph=True, persistent=True, blocks_per_cu=16, grab=1),
 ]
 
@@ -64,7 +66,7 @@ def save_cache(cache):
     os.replace(tmp, CACHE)
 
 
-def tune(scene: capi.Scene, cams, lights, spp, bounces, max_cams=64, max_res=256, repeats=2, use_cache=True,
+def tune(scene: capi.Scene, cams, lights, spp, bounces, max_cams=48, max_res=1024, repeats=1, use_cache=True,
          log=None):
     """Return the fastest crt_batch_opts (dict) for this kind of batch."""
     key = _key(scene, bounces, len(lights))
