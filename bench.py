@@ -135,9 +135,9 @@ def run_served(args, capi, scenes):
     """Mixed Poisson stream (60% renders 128^2-1024^2 x 1-16 spp, 30% flash
     LiDAR 64^2-512^2 x 8 sub-rays, 10% flux grids 0.25-4M rays) over 4
     resident geodesic spheres of 1.05M-10M triangles: continuous batching
-    server vs plain per-request launches (one at a time, and on three
-    streams), same requests (same per-pixel error), saturated throughput and
-    latency at a fixed arrival rate."""
+    server vs plain per-request launches (one at a time), same requests (same
+    per-pixel error), saturated throughput and latency at a fixed arrival
+    rate."""
     from crt import serving
     ks = [229, 400, 560, 707] if not args.small else [40, 64, 80, 100]
     meshes, sc = [], []
@@ -152,17 +152,21 @@ def run_served(args, capi, scenes):
            "config": {"workload": "C5 served: meshes %s tris, resident %.2f GB" % (
                [m.n_tris for m in meshes], resident / 1e9), "requests": args.requests}}
     # plain = the drop-in API called once per request, one request at a time
-    # (one worker, no batching); plain_concurrent = per-request launches on
-    # three worker streams without batching
-    modes = (("served", True, 3), ("plain", False, 1), ("plain_concurrent", False, 3))
+    # (the server's batching = 0 mode: one worker, no batching)
+    modes = (("served", True, 3), ("plain", False, 1))
+    # the same render launch parameters for every mode: lane traversal of
+    # camera rays; no hipGraph (a request batch renders a few passes, and a
+    # capture + instantiate per batch costs more than it saves)
+    ropts = dict(extend_mode=3)
+    out["render_launch_params"] = ropts
     for mode, batching, nw in modes:
-        srv = serving.Server(batching=batching, n_workers=nw)
+        srv = serving.Server(batching=batching, n_workers=nw, render_opts=ropts)
         for s in sc:
             srv.add_scene(s)
         warm = serving.mixed_workload(meshes, 8, 1e6, seed=99)
         serving.run_stream(srv, warm, realtime=False)
         srv.close()
-        srv = serving.Server(batching=batching, n_workers=nw)
+        srv = serving.Server(batching=batching, n_workers=nw, render_opts=ropts)
         for s in sc:
             srv.add_scene(s)
         reqs = serving.mixed_workload(meshes, args.requests, 1e6, seed=7)
@@ -173,7 +177,7 @@ def run_served(args, capi, scenes):
         srv.close()
     rate = args.rate
     for mode, batching, nw in modes:
-        srv = serving.Server(batching=batching, n_workers=nw)
+        srv = serving.Server(batching=batching, n_workers=nw, render_opts=ropts)
         for s in sc:
             srv.add_scene(s)
         reqs = serving.mixed_workload(meshes, args.requests, rate, seed=8)
@@ -184,7 +188,6 @@ def run_served(args, capi, scenes):
         srv.close()
     out["value"] = out["served"]["mrays_s"]
     out["speedup_vs_plain"] = out["served"]["mrays_s"] / max(out["plain"]["mrays_s"], 1e-9)
-    out["speedup_vs_plain_concurrent"] = out["served"]["mrays_s"] / max(out["plain_concurrent"]["mrays_s"], 1e-9)
     return out
 
 

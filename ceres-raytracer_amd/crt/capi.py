@@ -87,7 +87,8 @@ class FluxOpts(C.Structure):
 
 class ServerOpts(C.Structure):
     _fields_ = [("memory_budget", C.c_uint64), ("n_workers", C.c_uint32), ("max_batch", C.c_uint32),
-                ("max_batch_paths", C.c_uint64), ("batching", C.c_uint32), ("pad_", C.c_uint32)]
+                ("max_batch_paths", C.c_uint64), ("batching", C.c_uint32), ("pad_", C.c_uint32),
+                ("render", BatchOpts)]
 
 
 class Request(C.Structure):

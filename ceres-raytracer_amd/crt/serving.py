@@ -17,10 +17,16 @@ from crt import capi
 
 class Server:
     def __init__(self, batching=True, n_workers=3, memory_budget=16 << 30, max_batch=64,
-                 max_batch_paths=1 << 23):
+                 max_batch_paths=1 << 23, render_opts=None):
+        """render_opts: crt_batch_opts fields (dict, e.g. from crt.autotune)
+        used for every render launch, batched or plain."""
         capi.init()
+        ro = dict(render_opts or {})
+        bo = capi.BatchOpts(int(ro.get("samples_per_pass", 0)), int(bool(ro.get("use_graph", False))),
+                            int(bool(ro.get("persistent", False))), int(ro.get("blocks_per_cu", 0)),
+                            int(ro.get("grab", 0)), int(ro.get("extend_mode", 0)))
         o = capi.ServerOpts(int(memory_budget), int(n_workers), int(max_batch), int(max_batch_paths),
-                            int(bool(batching)), 0)
+                            int(bool(batching)), 0, bo)
         h = C.c_void_p()
         capi._check(capi.load().crt_server_create(C.byref(o), C.byref(h)))
         self.h = h

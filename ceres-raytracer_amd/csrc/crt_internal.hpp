@@ -117,10 +117,12 @@ struct RenderWorkspace;
 RenderWorkspace* new_render_workspace();
 void delete_render_workspace(RenderWorkspace* w);
 size_t render_bytes_estimate(uint64_t pixels, int spp, int n_lights);
+// `lights` holds n_lights lights shared by every camera, or, with
+// per_camera_lights, n_cams x n_lights (camera c's lights at c * n_lights).
 int render_batch(crt_scene* scene, uint32_t n_cams, const crt_camera* cams, const int32_t* max_samples,
-                 const uint64_t* seeds, const crt_light* lights, uint32_t n_lights, const crt_render_opts* opts,
-                 const crt_batch_opts* bopts, float* rgba, uint8_t* rgba_u8, int32_t* spp_used, uint32_t flags,
-                 void* stream, crt_timing* timing, RenderWorkspace* ws);
+                 const uint64_t* seeds, const crt_light* lights, uint32_t n_lights, bool per_camera_lights,
+                 const crt_render_opts* opts, const crt_batch_opts* bopts, float* rgba, uint8_t* rgba_u8,
+                 int32_t* spp_used, uint32_t flags, void* stream, crt_timing* timing, RenderWorkspace* ws);
 
 // Scratch buffers reused across calls on one scene (grow-only).
 struct Workspace {

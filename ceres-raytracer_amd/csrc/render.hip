@@ -66,6 +66,7 @@ struct CamDev {
     uint64_t seed;
     uint32_t pix_base;   // first global pixel of this camera
     uint32_t jitter;     // max_samples != 1
+    LightDev lights[MAX_LIGHTS];   // this camera's lights (scene frame); the count is batch-wide
 };
 
 struct Segment {
@@ -106,7 +107,6 @@ struct ShadowRec {
 // Everything a pass kernel needs, passed by value.
 struct Ctx {
     SceneView s;
-    LightDev lights[MAX_LIGHTS];
     int nl;
     int nb;
     int min_samples;
@@ -283,9 +283,9 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_persistent(Ctx C, int qse
 }
 
 // ---- shading -----------------------------------------------------------
-__device__ inline void light_sample(const Ctx& C, int l, float3 p, uint64_t seed, uint32_t pix, uint32_t smp,
-                                    int bounce, float3& dir, float& tmax, float& irr) {
-    const LightDev& L = C.lights[l];
+__device__ inline void light_sample(const Ctx& C, uint32_t cam, int l, float3 p, uint64_t seed, uint32_t pix,
+                                    uint32_t smp, int bounce, float3& dir, float& tmax, float& irr) {
+    const LightDev& L = C.cams[cam].lights[l];
     if (L.type == CRT_LIGHT_POINT) {
         // src/crt/lights/point_light.hpp:32-39
         const float3 dd = make_float3(L.pos[0] - p.x, L.pos[1] - p.y, L.pos[2] - p.z);
@@ -373,7 +373,7 @@ __global__ __launch_bounds__(TB) void k_shade(Ctx C, int bounce, int qsel) {
     for (int l = 0; l < C.nl; ++l) {
         float3 ld;
         float tmax, irr;
-        light_sample(C, l, p, seed, ps.pix, ps.sample, bounce, ld, tmax, irr);
+        light_sample(C, ps.cam, l, p, seed, ps.pix, ps.sample, bounce, ld, tmax, irr);
         // ColoredLambertianMaterial::compute: colour * (-L.n_s), no max(0, .)
         const float ldn = -dot3(ld, sn);
         ShadowRec r;
@@ -552,9 +552,9 @@ size_t render_bytes_estimate(uint64_t pixels, int spp, int n_lights) {
 }
 
 int render_batch(crt_scene* scene, uint32_t n_cams, const crt_camera* cams, const int32_t* max_samples,
-                 const uint64_t* seeds, const crt_light* lights, uint32_t n_lights, const crt_render_opts* opts,
-                 const crt_batch_opts* bopts, float* rgba, uint8_t* rgba_u8, int32_t* spp_used, uint32_t flags,
-                 void* stream, crt_timing* timing, RenderWorkspace* wsp) {
+                 const uint64_t* seeds, const crt_light* lights, uint32_t n_lights, bool per_camera_lights,
+                 const crt_render_opts* opts, const crt_batch_opts* bopts, float* rgba, uint8_t* rgba_u8,
+                 int32_t* spp_used, uint32_t flags, void* stream, crt_timing* timing, RenderWorkspace* wsp) {
     std::unique_ptr<RenderWorkspace> own;
     if (!wsp) {
         own.reset(new RenderWorkspace());
@@ -616,21 +616,23 @@ int render_batch(crt_scene* scene, uint32_t n_cams, const crt_camera* cams, cons
     Ctx C;
     std::memset(&C, 0, sizeof(C));
     C.s = scene->view;
-    for (uint32_t l = 0; l < n_lights; ++l) {
-        const crt_light& L = lights[l];
-        CRT_REQUIRE(L.type >= 0 && L.type <= 2, CRT_ERR_TYPE, "unknown light type");
-        LightDev& D = C.lights[l];
-        D.type = L.type;
-        D.intensity = (float)L.intensity;
-        for (int k = 0; k < 3; ++k) D.pos[k] = (float)(L.position[k] - scene->view.origin[k]);   // scene frame
-        for (int k = 0; k < 9; ++k) D.rot[k] = (float)L.rotation[k];
-        D.size[0] = (float)L.size[0];
-        D.size[1] = (float)L.size[1];
-        if (L.type == CRT_LIGHT_DIRECTIONAL) {
-            const double n = std::sqrt(L.direction[0] * L.direction[0] + L.direction[1] * L.direction[1] +
-                                       L.direction[2] * L.direction[2]);
-            CRT_REQUIRE(n > 0, CRT_ERR_INVALID, "directional light needs a non-zero direction");
-            for (int k = 0; k < 3; ++k) D.dir[k] = (float)(L.direction[k] / n);
+    for (uint32_t c = 0; c < n_cams; ++c) {
+        for (uint32_t l = 0; l < n_lights; ++l) {
+            const crt_light& L = lights[(per_camera_lights ? c * n_lights : 0u) + l];
+            CRT_REQUIRE(L.type >= 0 && L.type <= 2, CRT_ERR_TYPE, "unknown light type");
+            LightDev& D = hc[c].lights[l];
+            D.type = L.type;
+            D.intensity = (float)L.intensity;
+            for (int k = 0; k < 3; ++k) D.pos[k] = (float)(L.position[k] - scene->view.origin[k]);   // scene frame
+            for (int k = 0; k < 9; ++k) D.rot[k] = (float)L.rotation[k];
+            D.size[0] = (float)L.size[0];
+            D.size[1] = (float)L.size[1];
+            if (L.type == CRT_LIGHT_DIRECTIONAL) {
+                const double n = std::sqrt(L.direction[0] * L.direction[0] + L.direction[1] * L.direction[1] +
+                                           L.direction[2] * L.direction[2]);
+                CRT_REQUIRE(n > 0, CRT_ERR_INVALID, "directional light needs a non-zero direction");
+                for (int k = 0; k < 3; ++k) D.dir[k] = (float)(L.direction[k] / n);
+            }
         }
     }
     C.nl = (int)n_lights;
@@ -815,7 +817,7 @@ extern "C" int crt_render(crt_scene* scene, const crt_camera* camera, const crt_
     return guarded([&] {
         CRT_REQUIRE(opts != nullptr, CRT_ERR_INVALID, "render options are NULL");
         const uint64_t seed = opts->seed;
-        render_batch(scene, 1, camera, nullptr, &seed, lights, n_lights, opts, nullptr, rgba, rgba_u8, spp_used,
+        render_batch(scene, 1, camera, nullptr, &seed, lights, n_lights, false, opts, nullptr, rgba, rgba_u8, spp_used,
                      flags, stream, timing, nullptr);
     });
 }
@@ -826,7 +828,8 @@ extern "C" int crt_render_batch(crt_scene* scene, uint32_t n_cams, const crt_cam
                                 float* rgba, uint8_t* rgba_u8, int32_t* spp_used, uint32_t flags, void* stream,
                                 crt_timing* timing) {
     return guarded([&] {
-        render_batch(scene, n_cams, cams, max_samples, seeds, lights, n_lights, opts, bopts, rgba, rgba_u8, spp_used,
+        render_batch(scene, n_cams, cams, max_samples, seeds, lights, n_lights, false, opts, bopts, rgba, rgba_u8,
+                     spp_used,
                      flags, stream, timing, nullptr);
     });
 }

@@ -65,10 +65,11 @@ size_t request_bytes(const crt_request& r) {
     }
 }
 
+// Renders batch together when they share the scene, the bounce count and
+// the number of lights (every camera carries its own lights).
 bool same_render_setup(const crt_request& a, const crt_request& b) {
     if (a.kind != CRT_REQ_RENDER || b.kind != CRT_REQ_RENDER) return false;
-    if (a.scene != b.scene || a.bounces != b.bounces || a.n_lights != b.n_lights) return false;
-    return std::memcmp(a.lights, b.lights, sizeof(crt_light) * a.n_lights) == 0;
+    return a.scene == b.scene && a.bounces == b.bounces && a.n_lights == b.n_lights;
 }
 
 }  // namespace
@@ -104,11 +105,13 @@ void run_batch(crt_server* S, std::vector<std::shared_ptr<Job>>& batch, hipStrea
         std::vector<crt_camera> cams;
         std::vector<int32_t> spp;
         std::vector<uint64_t> seeds;
+        std::vector<crt_light> lights;
         uint64_t total = 0;
         for (auto& j : batch) {
             cams.push_back(j->req.camera);
             spp.push_back(j->req.spp);
             seeds.push_back(j->req.seed);
+            for (uint32_t l = 0; l < head.n_lights; ++l) lights.push_back(j->req.lights[l]);
             total += (uint64_t)std::floor(j->req.camera.resolution[0]) * (uint64_t)std::floor(j->req.camera.resolution[1]);
         }
         crt_render_opts o{};
@@ -116,11 +119,12 @@ void run_batch(crt_server* S, std::vector<std::shared_ptr<Job>>& batch, hipStrea
         o.max_samples = 1;
         o.noise_threshold = 0.0;
         o.num_bounces = head.bounces;
-        crt_batch_opts b{};
+        crt_batch_opts b = S->opts.render;
         std::vector<float> packed(total * 4);
         crt_timing tm{};
-        render_batch(scene, (uint32_t)batch.size(), cams.data(), spp.data(), seeds.data(), head.lights, head.n_lights,
-                     &o, &b, packed.data(), nullptr, nullptr, 0, st, &tm, rws);
+        render_batch(scene, (uint32_t)batch.size(), cams.data(), spp.data(), seeds.data(),
+                     lights.empty() ? nullptr : lights.data(), head.n_lights, true, &o, &b, packed.data(), nullptr,
+                     nullptr, 0, st, &tm, rws);
         uint64_t off = 0;
         for (auto& j : batch) {
             const uint64_t px = (uint64_t)std::floor(j->req.camera.resolution[0]) *

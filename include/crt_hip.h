@@ -285,6 +285,7 @@ typedef struct {
     uint64_t max_batch_paths;   /* camera paths (pixels x spp) per render batch */
     uint32_t batching;          /* 0: plain per-request launches                */
     uint32_t pad_;
+    crt_batch_opts render;      /* launch parameters of every render launch (zeros = defaults) */
 } crt_server_opts;
 
 typedef struct {

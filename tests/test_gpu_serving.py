@@ -14,16 +14,22 @@ def meshes():
     return ms, [capi.Scene(m.vertices, m.indices) for m in ms]
 
 
-def test_alone_vs_mix_identical(meshes):
+# render launch parameters: defaults, and the graph-captured / lane-camera-ray
+# form the batched benchmark picks (three workers capture graphs concurrently)
+RENDER_OPTS = [None, dict(samples_per_pass=4, use_graph=True, extend_mode=3)]
+
+
+@pytest.mark.parametrize("ropts", RENDER_OPTS)
+def test_alone_vs_mix_identical(meshes, ropts):
     ms, sc = meshes
     reqs = serving.mixed_workload(ms, 40, rate_hz=1e6, seed=3, max_render=256, max_lidar=128, max_flux=512)
-    srv = serving.Server(batching=True, n_workers=3)
+    srv = serving.Server(batching=True, n_workers=3, render_opts=ropts)
     for s in sc:
         srv.add_scene(s)
     mixed, lat, st = serving.run_stream(srv, reqs, realtime=False)
     assert st["completed"] == len(reqs)
     assert st["batches"] < len(reqs)          # requests were batched
-    alone_srv = serving.Server(batching=False)
+    alone_srv = serving.Server(batching=False, render_opts=ropts)
     for s in sc:
         alone_srv.add_scene(s)
     for i in range(0, len(reqs), 3):
@@ -36,7 +42,10 @@ def test_alone_vs_mix_identical(meshes):
         if r["kind"] == "render":
             rgba, _, _, _ = sc[r["scene"]].render(r["camera"], r["lights"], r["spp"], r["spp"], 0.0, r["bounces"],
                                                   seed=r["seed"])
-            assert np.array_equal(rgba, mixed[i]["rgba"])
+            if ropts is None:
+                assert np.array_equal(rgba, mixed[i]["rgba"])
+            else:   # another traversal form: only exact-tie hits may differ
+                assert np.mean(np.any(rgba != mixed[i]["rgba"], axis=-1)) < 1e-3
             break
     srv.close()
     alone_srv.close()
