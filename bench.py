@@ -131,6 +131,52 @@ def run_batched(args, capi, scenes):
             "speedup_vs_plain": tp / tb, "launch_params": best, "batch_equals_plain": bool(same)}
 
 
+def run_lidar_srp(args, capi, scenes):
+    """Same L8 body: 1024^2 flash-LiDAR beams x (1 boresight + 8 Gaussian
+    sub-rays, sigma 0.5 mrad) from a seeded FP64 pose, plus a 2048^2 (4.19M
+    ray) sun-direction flux grid (rho_s 0.3, rho_d 0.6, compensated FP64
+    reduction).  Outputs stay on the device; one step = one scan + one flux
+    integration."""
+    import torch
+    mesh = scenes.displaced_icosphere(8, 2000.0, 0.2, 6, 0)
+    scene = capi.Scene(mesh.vertices, mesh.indices)
+    beams = scenes.small_body_camera(mesh, resolution=1024, seed=3)
+    sun = scenes.sun_direction(4)
+    dev = torch.device("cuda:0")
+    nb = 1024 * 1024
+    outs = {"single": torch.empty(nb, dtype=torch.float64, device=dev).data_ptr(),
+            "first": torch.empty(nb, dtype=torch.float64, device=dev).data_ptr(),
+            "last": torch.empty(nb, dtype=torch.float64, device=dev).data_ptr(),
+            "n_returns": torch.empty(nb, dtype=torch.uint8, device=dev).data_ptr()}
+    stream = torch.cuda.current_stream().cuda_stream
+    for _ in range(args.warmup):
+        scene.flash_lidar(beams, 8, 5e-4, 4, outputs=outs, device=True, stream=stream)
+        scene.radiation_pressure(sun, 4.56e-6, 0.3, 0.6, 2048, stream=stream)
+    torch.cuda.synchronize()
+    lid_ms, flux_ms, lrays, frays = [], [], 0, 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        tl = scene.flash_lidar(beams, 8, 5e-4, 4, outputs=outs, device=True, stream=stream)
+        f, tq, nh, tf = scene.radiation_pressure(sun, 4.56e-6, 0.3, 0.6, 2048, stream=stream)
+        lid_ms.append(tl["trace_ms"])
+        flux_ms.append(tf["trace_ms"])
+        lrays += tl["rays_traced"]
+        frays += tf["rays_traced"]
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    return {"metric": "LiDAR + radiation-pressure Mrays/s (1024^2 x 9 flash LiDAR + 2048^2 flux grid, L8 body)",
+            "value": (lrays + frays) / el / 1e6, "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": el * 1e3 / args.steps, "higher_is_better": True, "dtype": "f32",
+            "data": "synthetic",
+            "config": {"workload": "C3: flash LiDAR 1024x1024 beams x (1+8) rays sigma 0.5 mrad seed 3/4 + SRP grid "
+                                   "2048x2048, rho_s 0.3 rho_d 0.6", "lidar_rays": lrays // args.steps,
+                       "flux_rays": frays // args.steps},
+            "lidar_mrays_s": lrays / args.steps / (np.mean(lid_ms) * 1e-3) / 1e6,
+            "flux_mrays_s": frays / args.steps / (np.mean(flux_ms) * 1e-3) / 1e6,
+            "kernel_ms": {"flash_lidar": float(np.mean(lid_ms)), "radiation_pressure": float(np.mean(flux_ms))},
+            "force_N": [float(x) for x in f], "torque_Nm": [float(x) for x in tq], "flux_hits": int(nh)}
+
+
 def run_served(args, capi, scenes):
     """Mixed Poisson stream (60% renders 128^2-1024^2 x 1-16 spp, 30% flash
     LiDAR 64^2-512^2 x 8 sub-rays, 10% flux grids 0.25-4M rays) over 4
@@ -199,7 +245,7 @@ def main():
     ap.add_argument("--resolution", type=int, default=2048)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--out", default=None, help="also write the JSON line to this file")
-    ap.add_argument("--workload", default="c2", choices=["c2", "batched", "served"],
+    ap.add_argument("--workload", default="c2", choices=["c2", "lidar_srp", "batched", "served"],
                     help="c2 (headline, default): closest hit + sun shadow on the 1.31M-tri body; "
                          "batched: 256-camera path tracing; served: mixed request stream")
     ap.add_argument("--cameras", type=int, default=256)
@@ -228,7 +274,8 @@ def main():
         return
 
     if args.workload != "c2":
-        line = run_batched(args, capi, scenes) if args.workload == "batched" else run_served(args, capi, scenes)
+        run = {"lidar_srp": run_lidar_srp, "batched": run_batched, "served": run_served}[args.workload]
+        line = run(args, capi, scenes)
         print(json.dumps(line), flush=True)
         if args.out:
             with open(args.out, "w") as fh:

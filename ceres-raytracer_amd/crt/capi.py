@@ -324,12 +324,22 @@ class Scene:
                                       None, C.byref(tm)))
         return out
 
-    def flash_lidar(self, beams: dict, sub_rays=8, sigma=5e-4, seed=0):
+    def flash_lidar(self, beams: dict, sub_rays=8, sigma=5e-4, seed=0, outputs=None, device=False, stream=None):
+        """Flash LiDAR over a beam grid.  With device=True, `outputs` holds
+        device addresses {single, first, last, n_returns} (ints) and only the
+        timing is returned."""
         o = FlashOpts()
         o.beams = make_camera(beams)
         o.sub_rays = int(sub_rays)
         o.sigma = float(sigma)
         o.seed = int(seed)
+        if device:
+            d = outputs or {}
+            tm = Timing()
+            _check(load().crt_flash_lidar(self.h, C.byref(o), ptr(d.get("single")), ptr(d.get("first")),
+                                          ptr(d.get("last")), ptr(d.get("n_returns")), CRT_OUT_DEVICE, stream,
+                                          C.byref(tm)))
+            return tm.as_dict()
         w, h = int(np.floor(beams["resolution"][0])), int(np.floor(beams["resolution"][1]))
         single, first, last = np.zeros((h, w)), np.zeros((h, w)), np.zeros((h, w))
         nret = np.zeros((h, w), dtype=np.uint8)
@@ -339,7 +349,7 @@ class Scene:
         return {"single": single, "first": first, "last": last, "n_returns": nret, "timing": tm.as_dict()}
 
     def radiation_pressure(self, sun_dir, pressure, rho_s, rho_d, grid, center=(0, 0, 0), radius=0.0,
-                           ref_point=(0, 0, 0)):
+                           ref_point=(0, 0, 0), stream=None):
         o = FluxOpts()
         o.sun_dir[:] = [float(x) for x in sun_dir]
         o.pressure, o.rho_s, o.rho_d, o.grid = float(pressure), float(rho_s), float(rho_d), int(grid)
@@ -350,7 +360,7 @@ class Scene:
         t = (C.c_double * 3)()
         nh = C.c_uint64(0)
         tm = Timing()
-        _check(load().crt_radiation_pressure(self.h, C.byref(o), f, t, C.byref(nh), None, C.byref(tm)))
+        _check(load().crt_radiation_pressure(self.h, C.byref(o), f, t, C.byref(nh), stream, C.byref(tm)))
         return np.array(f[:]), np.array(t[:]), nh.value, tm.as_dict()
 
     def render_batch(self, cams, lights, min_samples=1, max_samples=1, noise_threshold=0.0, num_bounces=1,
