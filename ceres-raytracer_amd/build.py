@@ -14,7 +14,7 @@ INCLUDE = os.path.join(ROOT, "include")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("CRT_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["bvh_build.hip", "trace.hip", "render.hip", "sensors.hip", "server.hip"]
+HIP_SOURCES = ["bvh_build.hip", "trace.hip", "render.hip", "sensors.hip", "server.hip", "scene_io.hip"]
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
              # packed FP32 instructions are disabled for every device code object
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -121,6 +121,8 @@ SIGNATURES = {
     "crt_scene_create": (C.c_int, [_p, C.c_uint32, _p, C.c_uint32, _p, _p, C.c_uint32, _p, _p, _p, _p]),
     "crt_scene_info": (C.c_int, [_p, _p]),
     "crt_scene_destroy": (None, [_p]),
+    "crt_scene_serialize": (C.c_int, [_p, _p, _p]),
+    "crt_scene_deserialize": (C.c_int, [_p, C.c_uint64, _p]),
     "crt_pass": (C.c_int, [_p, _p, _p, _p, _p, _p, _p, C.c_uint32, _p, _p]),
     "crt_trace_rays": (C.c_int, [_p, C.c_uint64, _p, _p, _p, _p, _p, _p, _p, _p, C.c_uint32, _p, _p]),
     "crt_sun_pass": (C.c_int, [_p, _p, _p, _p, _p, _p, C.c_uint32, _p, _p]),
@@ -240,6 +242,35 @@ class Scene:
             self.close()
         except Exception:
             pass
+
+    # ---- blobs: a built scene saved once, loaded without a rebuild --------
+    def to_bytes(self) -> bytes:
+        size = C.c_uint64(0)
+        _check(load().crt_scene_serialize(self.h, None, C.byref(size)))
+        buf = np.zeros(size.value, dtype=np.uint8)
+        _check(load().crt_scene_serialize(self.h, ptr(buf), C.byref(size)))
+        return buf[:size.value].tobytes()
+
+    def save(self, path):
+        with open(path, "wb") as fh:
+            fh.write(self.to_bytes())
+
+    @classmethod
+    def from_bytes(cls, blob) -> "Scene":
+        init()
+        buf = np.frombuffer(blob, dtype=np.uint8)
+        buf = np.ascontiguousarray(buf)
+        h = C.c_void_p()
+        _check(load().crt_scene_deserialize(ptr(buf), buf.size, C.byref(h)))
+        obj = cls.__new__(cls)
+        obj.h = h
+        obj.n_tris = obj.info()["n_tris"]
+        return obj
+
+    @classmethod
+    def load(cls, path) -> "Scene":
+        with open(path, "rb") as fh:
+            return cls.from_bytes(fh.read())
 
     def info(self) -> dict:
         inf = BvhInfo()

@@ -341,12 +341,15 @@ struct OutSlot {
 
 // ---- host helpers shared with other translation units ----
 void scene_make_view(crt_scene* s, const float* host_vertices);
+void scene_bind_view(crt_scene* s);
 
 }  // namespace crt
 
 using namespace crt;
 
-void crt::scene_make_view(crt_scene* s, const float* host_vertices) {
+// Pointers, counts and stack sizes of s->view; the frame origin, sphere
+// centre and radius must already be set.
+void crt::scene_bind_view(crt_scene* s) {
     SceneView& v = s->view;
     v.nodes = s->bvh.nodes.as<float4>();
     v.cnodes = s->bvh.cnodes.as<float4>();
@@ -360,6 +363,13 @@ void crt::scene_make_view(crt_scene* s, const float* host_vertices) {
     v.n_instances = s->n_instances;
     v.stack_cap = std::max<uint32_t>(8u, 7u * s->bvh.max_depth + 1u);
     v.lane_cap = s->bvh.max_depth + 1u;
+    for (int k = 0; k < 3; ++k) s->info.sphere[k] = v.center[k] + v.origin[k];   // world
+    s->info.sphere[3] = v.radius;
+    s->info.stack_capacity = v.stack_cap;
+}
+
+void crt::scene_make_view(crt_scene* s, const float* host_vertices) {
+    SceneView& v = s->view;
     // Bounding sphere for FP64 rebasing: centre of the box, radius = the
     // farthest vertex, padded so FP32 rounding of rebased origins stays
     // outside every triangle.
@@ -376,11 +386,7 @@ void crt::scene_make_view(crt_scene* s, const float* host_vertices) {
     r = r * (1.0 + 1e-3) + 1e-6 * (std::fabs(cx) + std::fabs(cy) + std::fabs(cz) + r) + 1e-9;
     v.center[0] = cx; v.center[1] = cy; v.center[2] = cz;
     v.radius = r;
-    s->info.sphere[0] = cx + v.origin[0];   // world
-    s->info.sphere[1] = cy + v.origin[1];
-    s->info.sphere[2] = cz + v.origin[2];
-    s->info.sphere[3] = r;
-    s->info.stack_capacity = v.stack_cap;
+    scene_bind_view(s);
 }
 
 extern "C" {

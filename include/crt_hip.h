@@ -152,6 +152,17 @@ int crt_scene_create(const float* vertices, uint32_t n_vertices,
 int  crt_scene_info(const crt_scene* scene, crt_bvh_info* info);
 void crt_scene_destroy(crt_scene* scene);
 
+/* ---- scene blobs (new; no reference counterpart: the reference keeps its
+ * BVH cache in memory only, src/crt/rendering_body_fixed/body_fixed_group.hpp:39)
+ * crt_scene_serialize: with buffer == NULL stores the blob size in *size;
+ * otherwise writes the blob (mesh in the scene frame, shading data, both BVH
+ * layouts, checksum) into buffer, which must hold *size bytes.
+ * crt_scene_deserialize: validates the blob (magic, version, sizes, checksum,
+ * node references, tree order and depth, index ranges) before any device
+ * memory is used, then makes the scene resident without a rebuild.        */
+int crt_scene_serialize(const crt_scene* scene, void* buffer, uint64_t* size);
+int crt_scene_deserialize(const void* buffer, uint64_t size, crt_scene** out_scene);
+
 /* ---- camera passes (src/crt/passes.hpp:13-198) --------------------------
  * One closest-hit ray per pixel at integer pixel coordinates.  Outputs are
  * row-major (H, W): xyz / normal f64 [H][W][3], instance u32 [H][W]; misses
