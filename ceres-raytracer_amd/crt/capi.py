@@ -157,6 +157,9 @@ def load(path: str = LIB_PATH):
     return _lib
 
 
+BUILD_PLOC = 1   # crt_build_opts.flags: PLOC clustering instead of the Morton-order LBVH
+
+
 def _check(code):
     if code != CRT_OK:
         msg = load().crt_last_error().decode(errors="replace")
@@ -214,7 +217,7 @@ class Scene:
     """A mesh + GPU-built BVH resident in HBM (crt_scene_create)."""
 
     def __init__(self, vertices, indices, tri_instance=None, vertex_normals=None, colors=None, smooth=None,
-                 max_leaf_size=8):
+                 max_leaf_size=8, ploc=False):
         init()
         v = np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1, 3)
         i = np.ascontiguousarray(indices, dtype=np.uint32).reshape(-1, 3)
@@ -225,7 +228,7 @@ class Scene:
         if col is not None:
             ni = max(ni, col.shape[0])
         sm = None if smooth is None else np.ascontiguousarray(smooth, dtype=np.uint8)
-        opts = BuildOpts(max_leaf_size, 0)
+        opts = BuildOpts(max_leaf_size, BUILD_PLOC if ploc else 0)
         h = C.c_void_p()
         _check(load().crt_scene_create(ptr(v), v.shape[0], ptr(i), i.shape[0], ptr(inst), ptr(vn), ni, ptr(col),
                                        ptr(sm), C.byref(opts), C.byref(h)))

@@ -93,6 +93,7 @@ struct BuildResult {
     uint32_t n_nodes = 0;
     uint32_t n_leaves = 0;
     uint32_t max_depth = 0;
+    uint32_t ploc_rounds = 0;   // 0: LBVH (Karras) tree
     double build_ms = 0.0;
     double sah_cost = 0.0;
     float bounds[6] = {0, 0, 0, 0, 0, 0};
@@ -101,7 +102,7 @@ struct BuildResult {
 // GPU BVH build: LBVH (63-bit Morton codes, LDS radix sort, Karras
 // hierarchy, bottom-up refit) collapsed into 8-wide nodes.
 void build_bvh(const float* d_vertices, const uint32_t* d_indices, uint32_t n_tris,
-               uint32_t max_leaf, hipStream_t stream, BuildResult& out);
+               uint32_t max_leaf, bool use_ploc, hipStream_t stream, BuildResult& out);
 
 // Shared helpers of the trace / render / sensor translation units.
 // Camera in the scene frame of `s` (position moved by the scene origin).

@@ -173,3 +173,34 @@ def test_single_triangle_scene():
     t, tri, u, vv = s.trace(np.array([[0.25, 0.25, 1.0]]), np.array([[0.0, 0.0, -1.0]]))
     assert tri[0] == 0 and abs(t[0] - 1.0) < 1e-6
     assert abs(u[0] - 0.25) < 1e-6 and abs(vv[0] - 0.25) < 1e-6
+
+
+def test_ploc_builder_matches(icosphere, cornell):
+    # the PLOC binary tree (CRT_BUILD_PLOC) changes only the BVH: hits agree
+    # with the default LBVH scene except at exact ties / edge grazes
+    gpu, cpu, m = icosphere
+    alt = capi.Scene(m.vertices, m.indices, ploc=True)
+    assert alt.info()["sah_cost"] < gpu.info()["sah_cost"]
+    cam = scenes.small_body_camera(m, resolution=512, seed=1)
+    a, b = gpu.camera_pass(cam), alt.camera_pass(cam)
+    mism = a["tri"] != b["tri"]
+    if mism.any():
+        o, d = oracle.camera_rays(cam)
+        sel = mism.ravel()
+        ok = cpu.tie_flags(o[sel], d[sel]) | cpu.edge_flags(o[sel], d[sel], b["tri"].ravel()[sel])
+        assert ok.all()
+    hit = (~mism) & (a["tri"] >= 0)
+    assert np.max(np.abs(a["depth"][hit] - b["depth"][hit]) / a["depth"][hit]) < REL
+    # multi-instance scene
+    g, c, v, i = cornell
+    sc = scenes.cornell_box(0)
+    _, _, inst = sc.merged()
+    alt = capi.Scene(v, i, inst, ploc=True)
+    cam = scenes.cornell_camera()
+    pa, pc = alt.camera_pass(cam), c.camera_pass(cam)
+    mism = pa["tri"] != pc["tri"]
+    if mism.any():
+        o, d = oracle.camera_rays(cam)
+        sel = mism.ravel()
+        ok = c.tie_flags(o[sel], d[sel]) | c.edge_flags(o[sel], d[sel], pa["tri"].ravel()[sel])
+        assert ok.all()
