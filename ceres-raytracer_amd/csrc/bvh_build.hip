@@ -646,7 +646,8 @@ __global__ __launch_bounds__(BLK) void k_compress(const float4* __restrict__ nod
         e[a] = ea;
         step[a] = ldexp(1.0, ea);
     }
-    uint32_t q[6][2] = {{0u, 0u}, {0u, 0u}, {0u, 0u}, {0u, 0u}, {0u, 0u}, {0u, 0u}};   // lo xyz, hi xyz
+    // lo xyz, hi xyz; empty slots keep the inverted box qlo = 255, qhi = 0
+    uint32_t q[6][2] = {{~0u, ~0u}, {~0u, ~0u}, {~0u, ~0u}, {0u, 0u}, {0u, 0u}, {0u, 0u}};
     for (int k = 0; k < 8; ++k) {
         if (refs[k] == REF_EMPTY) continue;
         const float4 l = nd[2 * k], h = nd[2 * k + 1];
@@ -656,8 +657,9 @@ __global__ __launch_bounds__(BLK) void k_compress(const float4* __restrict__ nod
             double qh = ceil(((double)ch[a] - (double)lo[a]) / step[a]);
             ql = fmin(fmax(ql, 0.0), 255.0);
             qh = fmin(fmax(qh, 0.0), 255.0);
-            q[a][k >> 2] |= (uint32_t)ql << (8 * (k & 3));
-            q[3 + a][k >> 2] |= (uint32_t)qh << (8 * (k & 3));
+            const uint32_t sh = 8u * (uint32_t)(k & 3);
+            q[a][k >> 2] = (q[a][k >> 2] & ~(0xFFu << sh)) | ((uint32_t)ql << sh);
+            q[3 + a][k >> 2] |= (uint32_t)qh << sh;
         }
     }
     const uint32_t ew = ((uint32_t)(uint8_t)(int8_t)e[0]) | ((uint32_t)(uint8_t)(int8_t)e[1] << 8) |

@@ -374,7 +374,8 @@ __device__ inline LaneStack lane_stack(const SceneView& s) {
 //   [3] qhi_y[0..7] | qhi_z[0..7]
 //   [4] ref[0..3]   [5] ref[4..7]      (same refs as the 256-B node)
 // Child k's box on axis a is [p_a + 2^e_a qlo_a[k], p_a + 2^e_a qhi_a[k]],
-// rounded outwards from the padded FP32 box, so it always contains it.
+// rounded outwards from the padded FP32 box, so it always contains it; an
+// empty slot has qlo = 255, qhi = 0 on every axis.
 // A lane reads the whole node with 6 loads instead of 16; the slab test per
 // plane stays one FMA, t = q * (2^e / d) + (p - o) / d, and the ray octant
 // picks near and far planes per axis for 4 children at a time.
@@ -422,13 +423,17 @@ __device__ inline uint32_t cnode_test(const float4* __restrict__ cn, const RayPr
         const float tfz = fmaf(cvt_u8(hi4 ? fz1 : fz0, byte), Az, Bz);
         const float tnear = fmaxf(max3f(tnx, tny, tnz), tmin);
         const float tfar = fminf(min3f(tfx, tfy, tfz), best);
-        const uint32_t ref = refs[k];
-        const bool hk = ((mask >> k) & 1u) && (ref != REF_EMPTY) && (tnear <= tfar * 1.00000024f);
-        if (hk) {
-            hits |= 1u << k;
-            if (tnear < tn0) { tn1 = tn0; tn0 = tnear; k0 = k; ref0 = ref; }
-            else tn1 = fminf(tn1, tnear);
-        }
+        // empty slots are stored inverted (qlo 255, qhi 0): with the octant's
+        // near/far planes their tnear exceeds tfar, so they never hit
+        const bool hk = ((mask >> k) & 1u) && (tnear <= tfar * 1.00000024f);
+        hits |= hk ? (1u << k) : 0u;
+        // branch-free nearest / second nearest
+        const float tk = hk ? tnear : __int_as_float(0x7f800000);
+        const bool lt = tk < tn0;
+        tn1 = lt ? tn0 : (tk < tn1 ? tk : tn1);
+        tn0 = lt ? tk : tn0;
+        k0 = lt ? k : k0;
+        ref0 = lt ? refs[k] : ref0;
     }
     return hits;
 }

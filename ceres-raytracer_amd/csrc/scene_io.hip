@@ -8,9 +8,9 @@
 // version, sizes against the counts, an FNV-1a 64 checksum of the sections,
 // and the structure itself -- every node reference in range, inner children
 // numbered above their parent (so traversal always terminates), leaves of
-// 1..8 triangles inside the triangle array, tree depth within the stored
-// maximum, triangle and vertex indices in range -- so a damaged blob is
-// rejected instead of being traced.
+// 1..8 triangles inside the triangle array, empty slots encoded so that they
+// can never be hit, tree depth within the stored maximum, triangle and vertex
+// indices in range -- so a damaged blob is rejected instead of being traced.
 #include "crt_internal.hpp"
 
 #include <cmath>
@@ -27,7 +27,7 @@ namespace {
 using namespace crt;
 
 constexpr char MAGIC[8] = {'C', 'R', 'T', 'S', 'C', 'N', '\0', '\1'};
-constexpr uint32_t VERSION = 1;
+constexpr uint32_t VERSION = 2;   // 2: empty compressed slots stored as inverted boxes
 
 enum Section { S_VERTICES, S_INDICES, S_VNORMALS, S_TRI_INST, S_INST_COLOR, S_INST_SMOOTH, S_NODES, S_CNODES,
                S_TRIS, N_SECTIONS };
@@ -110,7 +110,15 @@ void validate(const Header& h, const uint8_t* const sec[N_SECTIONS]) {
         for (uint32_t k = 0; k < 8; ++k) {
             const uint32_t ref = n256[(uint64_t)i * 64 + 8 * k + 3];
             CRT_REQUIRE(ref == n96[(uint64_t)i * 24 + 16 + k], CRT_ERR_INVALID, "scene blob: node layouts differ");
-            if (ref == REF_EMPTY) continue;
+            if (ref == REF_EMPTY) {
+                // traversal relies on empty slots never being hit: their
+                // quantized box must be the inverted one (qlo 255, qhi 0)
+                const uint8_t* q = reinterpret_cast<const uint8_t*>(n96 + (uint64_t)i * 24 + 4);
+                for (int a = 0; a < 3; ++a)
+                    CRT_REQUIRE(q[8 * a + k] == 255u && q[24 + 8 * a + k] == 0u, CRT_ERR_INVALID,
+                                "scene blob: empty BVH slot not encoded empty");
+                continue;
+            }
             if (ref & REF_LEAF) {
                 const uint64_t first = leaf_first(ref), count = leaf_count(ref);
                 CRT_REQUIRE(count <= 8 && first + count <= nt, CRT_ERR_INVALID, "scene blob: leaf out of range");
