@@ -15,9 +15,11 @@
 //   on a sun-facing triangle (flat outward normal n, cos = n.s > 0):
 //     dF = P A_ray [ (1 - rs) d - 2 (rs cos + rd / 3) n ],  d = -s,
 //     dT = (x - ref) x dF.
-//   Per-ray terms in FP32; block partials and the final sum in FP64 with
-//   TwoSum compensation (error-free transformation, Knuth), so the result is
-//   independent of summation order to well below 1e-12 relative.
+//   Per-ray force terms in FP32, torque arms in FP64; 64-ray wave sums in
+//   plain FP64 (a 6-level tree: exact for the FP32 force terms, a few ulp
+//   for torques), then block partials and the final sum in FP64 with TwoSum
+//   compensation (error-free transformation, Knuth), so the result does not
+//   depend on the summation order to well below 1e-12 relative.
 #include "crt_internal.hpp"
 #include "crt_rng.hpp"
 
@@ -147,8 +149,9 @@ struct FluxParams {
     uint32_t grid;
 };
 
-// Block partials: one wave of 64 rays per block; 7 compensated sums each
-// (Fx,Fy,Fz,Tx,Ty,Tz, hits), reduced across the wave with shuffles.
+// Block partials: one wave of 64 rays per block; 7 sums each (Fx,Fy,Fz,
+// Tx,Ty,Tz, hits), reduced across the wave with shuffles, then combined
+// across blocks with TwoSum compensation.
 CRT_LANE_KERNEL void k_flux(SceneView s, FluxParams F, Comp* __restrict__ partial,
                                                       uint64_t n) {
     const LaneStack stk = lane_stack(s);
@@ -195,13 +198,14 @@ CRT_LANE_KERNEL void k_flux(SceneView s, FluxParams F, Comp* __restrict__ partia
             }
         }
     }
-    Comp acc[7];
-    for (int k = 0; k < 7; ++k) acc[k] = Comp{term[k], 0.0};
+    // 64 terms in a 6-level FP64 tree: exact for the FP32 force terms, a
+    // few ulp for the FP64 torques; compensation starts at the block partials.
+    double acc[7];
+    for (int k = 0; k < 7; ++k) acc[k] = term[k];
     for (int w = 32; w > 0; w >>= 1)
-        for (int k = 0; k < 7; ++k)
-            acc[k] = comp_add(acc[k], Comp{__shfl_xor(acc[k].s, w, 64), __shfl_xor(acc[k].c, w, 64)});
+        for (int k = 0; k < 7; ++k) acc[k] += __shfl_xor(acc[k], w, 64);
     if (threadIdx.x == 0)
-        for (int k = 0; k < 7; ++k) partial[(size_t)blockIdx.x * 7 + k] = acc[k];
+        for (int k = 0; k < 7; ++k) partial[(size_t)blockIdx.x * 7 + k] = Comp{acc[k], 0.0};
 }
 
 __global__ __launch_bounds__(256) void k_flux_final(const Comp* __restrict__ partial, uint32_t nblocks,
