@@ -337,6 +337,81 @@ __device__ inline HitRec trace_octet(const SceneView& s, const RayPre& r, float 
 }
 
 // ------------------------------------------------------------------------
+// Quad any-hit traversal: one shadow ray per *quad* (4 lanes, 16 rays per
+// wave) over the 256-B nodes.  Lane j of the quad tests children j and j+4
+// and, in a leaf, triangles j and j+4, so the per-step bookkeeping (ballot,
+// next-node shuffle, stack pushes) is shared by twice as many rays as with
+// an octet.  Any hit never tightens the bound, so the stack holds refs only
+// and pops are unconditional.  Returns true when some triangle blocks
+// [tmin, tmax].  `stk` is the quad's stack of s.stack_cap refs.
+// ------------------------------------------------------------------------
+constexpr int QUAD = 4;
+
+__device__ inline bool child_hit(const float4 q0, const float4 q1, const RayPre& r, float tmin, float tmax) {
+    const float tx0 = fmaf(q0.x, r.ix, r.nx), tx1 = fmaf(q1.x, r.ix, r.nx);
+    const float ty0 = fmaf(q0.y, r.iy, r.ny), ty1 = fmaf(q1.y, r.iy, r.ny);
+    const float tz0 = fmaf(q0.z, r.iz, r.nz), tz1 = fmaf(q1.z, r.iz, r.nz);
+    const float tnear = max3f(fminf(tx0, tx1), fminf(ty0, ty1), fmaxf(fminf(tz0, tz1), tmin));
+    const float tfar = min3f(fmaxf(tx0, tx1), fmaxf(ty0, ty1), fminf(fmaxf(tz0, tz1), tmax));
+    return __float_as_uint(q0.w) != REF_EMPTY && tnear <= tfar * 1.00000024f;
+}
+
+__device__ inline bool trace_quad_any(const SceneView& s, const RayPre& r, float tmin, float tmax, uint32_t* stk,
+                                      int j, int lane_base) {
+    const float4* __restrict__ nodes = s.nodes;
+    const float4* __restrict__ tris = s.tris;
+    const uint32_t cap = s.stack_cap;
+    uint32_t cur = 0u;
+    uint32_t sp = 0u;
+    while (true) {
+        if (!(cur & REF_LEAF)) {
+            const float4* nd = nodes + (size_t)cur * 16u;
+            const float4 a0 = nd[2 * j], a1 = nd[2 * j + 1];
+            const float4 b0 = nd[2 * j + 8], b1 = nd[2 * j + 9];
+            const bool ha = child_hit(a0, a1, r, tmin, tmax);
+            const bool hb = child_hit(b0, b1, r, tmin, tmax);
+            // child mask of the quad: bit c for child c (lane c & 3, slot c >> 2)
+            const uint32_t m = (uint32_t)((__ballot(ha) >> lane_base) & 0xFull) |
+                               ((uint32_t)((__ballot(hb) >> lane_base) & 0xFull) << 4);
+            if (m != 0u) {
+                const int fc = __ffs(m) - 1;
+                const uint32_t sel = (fc >= 4) ? __float_as_uint(b0.w) : __float_as_uint(a0.w);
+                const uint32_t next = (uint32_t)__shfl((int)sel, lane_base + (fc & 3), 64);
+                const uint32_t rest = m & ~(1u << fc);
+                if (ha && ((rest >> j) & 1u)) {
+                    const uint32_t pos = sp + (uint32_t)__popc(rest & ((1u << j) - 1u));
+                    if (pos < cap) stk[pos] = __float_as_uint(a0.w);
+                }
+                if (hb && ((rest >> (j + 4)) & 1u)) {
+                    const uint32_t pos = sp + (uint32_t)__popc(rest & ((1u << (j + 4)) - 1u));
+                    if (pos < cap) stk[pos] = __float_as_uint(b0.w);
+                }
+                sp = min(sp + (uint32_t)__popc(rest), cap);   // cap >= 7 * depth + 1: never clamps
+                cur = next;
+                continue;
+            }
+        } else {
+            // leaf of at most 8 triangles: lane j tests j and j + 4
+            const uint32_t first = leaf_first(cur);
+            const uint32_t count = leaf_count(cur);
+            float t, u, v;
+            bool hit = false;
+            if ((uint32_t)j < count) {
+                const size_t ti = (size_t)(first + (uint32_t)j) * 3u;
+                hit = tri_test(r, tris[ti], tris[ti + 1], tris[ti + 2], tmin, tmax, t, u, v);
+            }
+            if (!hit && (uint32_t)j + 4u < count) {
+                const size_t ti = (size_t)(first + (uint32_t)j + 4u) * 3u;
+                hit = tri_test(r, tris[ti], tris[ti + 1], tris[ti + 2], tmin, tmax, t, u, v);
+            }
+            if (((__ballot(hit) >> lane_base) & 0xFull) != 0ull) return true;
+        }
+        if (sp == 0u) return false;
+        cur = stk[--sp];
+    }
+}
+
+// ------------------------------------------------------------------------
 // Lane traversal: one ray per lane, 64 independent rays per wave.
 //
 // A lane evaluates all 8 children of a (compressed) wide node itself.  The stack

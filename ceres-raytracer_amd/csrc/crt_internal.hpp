@@ -110,6 +110,7 @@ CamView make_cam_view(const crt_camera& c, const crt_scene* s);
 void check_camera_args(const crt_camera* c);
 size_t trace_lds_bytes(const SceneView& s);   // octet kernels
 size_t lane_lds_bytes(const SceneView& s);    // lane kernels
+size_t quad_lds_bytes(const SceneView& s);    // quad (any-hit) kernels
 int device_cus();                             // compute units of the current device
 float self_offset(const crt_scene* s);
 

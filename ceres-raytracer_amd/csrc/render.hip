@@ -18,9 +18,10 @@
 // (pixel, sample) of every segment is one path.  One pass runs
 //   k_raygen   FP64 camera ray (jittered), rebased, FP32 path state; paths
 //              whose ray misses the scene's bounding sphere are not queued
-//   per bounce: k_extend (octet per path, closest hit) -> k_shade (thread per
-//              path: light samples, shadow rays, next direction) -> k_shadow
-//              (octet per shadow ray, any hit) -> k_accumulate (thread per
+//   per bounce: k_trace / k_extend_lane (octet or lane per path, closest
+//              hit) -> k_shade (thread per path: light samples, shadow rays,
+//              next direction) -> k_shadow_quad (quad per shadow ray, any
+//              hit) -> k_accumulate (thread per
 //              path; surviving paths are compacted into the next queue with a
 //              wave ballot + one atomic per wave)
 //   k_fold     per pixel: folds the pass's samples into the running mean in
@@ -28,8 +29,9 @@
 // All kernels read their sizes from device memory, so a pass can be captured
 // once in a hipGraph and replayed (the host only rewrites the segment table
 // between passes).  Counters are cleared by a kernel, never by a memset node.
-// The traversal kernels exist in two forms: one octet per queue entry, or
-// persistent (a fixed grid whose octets claim `grab` entries per atomic).
+// The traversal kernels exist in two forms: one octet / quad per queue
+// entry, or persistent (a fixed grid whose octets / quads claim `grab`
+// entries per atomic).
 //
 // Randomness is counter-based: u01(seed_c, pixel, sample, 64*bounce + dim)
 // with a per-camera seed, so a camera rendered inside a batch is bit-for-bit
@@ -223,13 +225,6 @@ __device__ inline void extend_one(const Ctx& C, uint32_t k, uint32_t* sref, floa
     }
 }
 
-__device__ inline void shadow_one(const Ctx& C, uint32_t i, uint32_t* sref, float* sd, int sub, int lane_base) {
-    const float4 o = C.shadow[i].o, d = C.shadow[i].d;
-    const RayPre r = make_ray_pre(o.x, o.y, o.z, d.x, d.y, d.z);
-    const HitRec h = trace_octet<true>(C.s, r, 0.0f, o.w, sref, sd, sub, lane_base);
-    if (sub == 0) C.shadow[i].c.w = h.slot >= 0 ? 1.0f : 0.0f;
-}
-
 #define OCTET_SETUP(C)                                                                                      \
     uint32_t* smem = traversal_lds((C).s);                                                                  \
     const int lane = threadIdx.x & 63;                                                                      \
@@ -238,14 +233,48 @@ __device__ inline void shadow_one(const Ctx& C, uint32_t i, uint32_t* sref, floa
     float* sd = reinterpret_cast<float*>(smem + (size_t)OCTETS_PER_BLOCK * (C).s.stack_cap) +                \
                 (size_t)local * (C).s.stack_cap;
 
-template <bool SHADOW>
+// Extension rays: one per octet (closest hit).
 __global__ __launch_bounds__(TRACE_BLOCK) void k_trace(Ctx C, int qsel) {
     OCTET_SETUP(C)
     const uint32_t q = blockIdx.x * OCTETS_PER_BLOCK + local;
-    const uint32_t n = SHADOW ? C.cnt[2] : C.cnt[qsel];
-    if (q >= n) return;
-    if (SHADOW) shadow_one(C, q, sref, sd, sub, lane_base);
-    else extend_one(C, C.queue[qsel][q], sref, sd, sub, lane_base);
+    if (q >= C.cnt[qsel]) return;
+    extend_one(C, C.queue[qsel][q], sref, sd, sub, lane_base);
+}
+
+// Shadow rays: one per quad (any hit, see trace_quad_any).
+#define QUAD_SETUP(C)                                                                                       \
+    extern __shared__ uint32_t quad_stacks[];                                                               \
+    const int lane = threadIdx.x & 63;                                                                      \
+    const int qj = lane & (QUAD - 1), qbase = lane & ~(QUAD - 1), qlocal = threadIdx.x / QUAD;              \
+    uint32_t* qstk = quad_stacks + (size_t)qlocal * (C).s.stack_cap;
+
+__device__ inline void shadow_quad(const Ctx& C, uint32_t i, uint32_t* stk, int j, int lane_base) {
+    const float4 o = C.shadow[i].o, d = C.shadow[i].d;
+    const RayPre r = make_ray_pre(o.x, o.y, o.z, d.x, d.y, d.z);
+    const bool occ = trace_quad_any(C.s, r, 0.0f, o.w, stk, j, lane_base);
+    if (j == 0) C.shadow[i].c.w = occ ? 1.0f : 0.0f;
+}
+
+__global__ __launch_bounds__(TRACE_BLOCK) void k_shadow_quad(Ctx C) {
+    QUAD_SETUP(C)
+    const uint32_t q = blockIdx.x * (TRACE_BLOCK / QUAD) + qlocal;
+    if (q >= C.cnt[2]) return;   // quad-uniform
+    shadow_quad(C, q, qstk, qj, qbase);
+}
+
+// Persistent form: quads claim `grab` consecutive shadow rays per atomic.
+__global__ __launch_bounds__(TRACE_BLOCK) void k_shadow_quad_persistent(Ctx C) {
+    QUAD_SETUP(C)
+    uint32_t* work = C.cnt + 6;
+    const uint32_t n = C.cnt[2];
+    while (true) {
+        uint32_t base = 0;
+        if (qj == 0) base = atomicAdd(work, C.grab);
+        base = (uint32_t)__shfl((int)base, qbase, 64);
+        if (base >= n) break;
+        const uint32_t end = min(base + C.grab, n);
+        for (uint32_t q = base; q < end; ++q) shadow_quad(C, q, qstk, qj, qbase);
+    }
 }
 
 // One extension ray per lane (coherent rays: camera rays).
@@ -262,23 +291,19 @@ CRT_LANE_KERNEL void k_extend_lane(Ctx C, int qsel) {
     C.st[k].hv = h.v;
 }
 
-// Persistent form: octets claim `grab` consecutive entries per atomic.
-template <bool SHADOW>
+// Persistent form: octets claim `grab` consecutive extension rays per atomic.
 __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_persistent(Ctx C, int qsel) {
     OCTET_SETUP(C)
     (void)local;
-    uint32_t* work = C.cnt + (SHADOW ? 6 : 5);
-    const uint32_t n = SHADOW ? C.cnt[2] : C.cnt[qsel];
+    uint32_t* work = C.cnt + 5;
+    const uint32_t n = C.cnt[qsel];
     while (true) {
         uint32_t base = 0;
         if (sub == 0) base = atomicAdd(work, C.grab);
         base = (uint32_t)__shfl((int)base, lane_base, 64);
         if (base >= n) break;
         const uint32_t end = min(base + C.grab, n);
-        for (uint32_t q = base; q < end; ++q) {
-            if (SHADOW) shadow_one(C, q, sref, sd, sub, lane_base);
-            else extend_one(C, C.queue[qsel][q], sref, sd, sub, lane_base);
-        }
+        for (uint32_t q = base; q < end; ++q) extend_one(C, C.queue[qsel][q], sref, sd, sub, lane_base);
     }
 }
 
@@ -504,7 +529,8 @@ void launch_pass(const Ctx& C, uint64_t cap_paths, uint32_t cap_fold, const crt_
     const size_t lds = trace_lds_bytes(C.s);
     const uint32_t gp = std::max(1u, cdiv(cap_paths, TB));
     const uint32_t go = std::max(1u, cdiv(cap_paths, OCTETS_PER_BLOCK));
-    const uint32_t gs = std::max(1u, cdiv(cap_paths * std::max(1, C.nl), OCTETS_PER_BLOCK));
+    const uint32_t gsq = std::max(1u, cdiv(cap_paths * std::max(1, C.nl), TRACE_BLOCK / QUAD));
+    const size_t qlds = quad_lds_bytes(C.s);
     const uint32_t gpers = std::max(1u, B.blocks_per_cu) * (uint32_t)num_cus;
     hipLaunchKernelGGL(k_clear, dim3(1), dim3(32), 0, st, C.cnt, 0x7Fu);
     hipLaunchKernelGGL(k_raygen, dim3(gp), dim3(TB), 0, st, C);
@@ -516,16 +542,17 @@ void launch_pass(const Ctx& C, uint64_t cap_paths, uint32_t cap_fold, const crt_
             hipLaunchKernelGGL(k_extend_lane, dim3(std::max(1u, cdiv(cap_paths, LANE_BLOCK))), dim3(LANE_BLOCK),
                                lane_lds_bytes(C.s), st, C, cur);
         else if (B.persistent)
-            hipLaunchKernelGGL(k_trace_persistent<false>, dim3(std::min(gpers, go)), dim3(TRACE_BLOCK), lds, st, C, cur);
+            hipLaunchKernelGGL(k_trace_persistent, dim3(std::min(gpers, go)), dim3(TRACE_BLOCK), lds, st, C, cur);
         else
-            hipLaunchKernelGGL(k_trace<false>, dim3(go), dim3(TRACE_BLOCK), lds, st, C, cur);
+            hipLaunchKernelGGL(k_trace, dim3(go), dim3(TRACE_BLOCK), lds, st, C, cur);
         hipLaunchKernelGGL(k_shade, dim3(gp), dim3(TB), 0, st, C, b, cur);
         hipLaunchKernelGGL(k_tally, dim3(1), dim3(32), 0, st, C, cur);
         if (C.nl > 0) {
             if (B.persistent)
-                hipLaunchKernelGGL(k_trace_persistent<true>, dim3(std::min(gpers, gs)), dim3(TRACE_BLOCK), lds, st, C, 0);
+                hipLaunchKernelGGL(k_shadow_quad_persistent, dim3(std::min(gpers, gsq)), dim3(TRACE_BLOCK), qlds, st,
+                                   C);
             else
-                hipLaunchKernelGGL(k_trace<true>, dim3(gs), dim3(TRACE_BLOCK), lds, st, C, 0);
+                hipLaunchKernelGGL(k_shadow_quad, dim3(gsq), dim3(TRACE_BLOCK), qlds, st, C);
         }
         hipLaunchKernelGGL(k_accumulate, dim3(gp), dim3(TB), 0, st, C, b, nxt);
     }

@@ -253,7 +253,8 @@ int crt_scene_deserialize(const void* buffer, uint64_t size, crt_scene** out_sce
         s->info.device_bytes = s->vertices.bytes + s->indices.bytes + s->tri_vidx.bytes + s->tri_inst.bytes +
                                s->inst_color.bytes + s->inst_smooth.bytes + s->vnormals.bytes + s->bvh.nodes.bytes +
                                s->bvh.cnodes.bytes + s->bvh.tris.bytes;
-        CRT_REQUIRE(trace_lds_bytes(s->view) <= 64u * 1024u && lane_lds_bytes(s->view) <= 64u * 1024u,
+        CRT_REQUIRE(trace_lds_bytes(s->view) <= 64u * 1024u && lane_lds_bytes(s->view) <= 64u * 1024u &&
+                        quad_lds_bytes(s->view) <= 64u * 1024u,
                     CRT_ERR_INVALID, "scene blob: BVH too deep for the LDS traversal stack");
         *out_scene = s.release();
     });

@@ -23,6 +23,7 @@ inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b
 
 size_t trace_lds_bytes(const SceneView& s) { return (size_t)OCTETS_PER_BLOCK * s.stack_cap * 8u; }
 size_t lane_lds_bytes(const SceneView& s) { return (size_t)LANE_BLOCK * s.lane_cap * 8u; }
+size_t quad_lds_bytes(const SceneView& s) { return (size_t)(TRACE_BLOCK / QUAD) * s.stack_cap * 4u; }
 
 int device_cus() {
     static std::mutex mu;
@@ -289,23 +290,22 @@ CRT_LANE_KERNEL void k_primary_sun(SceneView s, CamView cam, double* depth, int3
     }
 }
 
-// Shadow rays are any-hit: octet traversal (8 rays per wave, children tested
-// in parallel) beats one ray per lane here, whose waves wait on their
-// longest unoccluded ray.
+// Shadow rays are any-hit: quad traversal (16 rays per wave, two children
+// per lane) beats both one ray per lane, whose waves wait on their longest
+// unoccluded ray, and one ray per octet.
 __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow_sun(SceneView s, const ShadowRay* queue,
                                                             const uint32_t* qcount, float3 sun, uint8_t* lit) {
-    uint32_t* smem = traversal_lds(s);
+    extern __shared__ uint32_t quad_stacks[];
     const int lane = threadIdx.x & 63;
-    const int sub = lane & 7, lane_base = lane & ~7, local = threadIdx.x >> 3;
-    uint32_t* sref = smem + (size_t)local * s.stack_cap;
-    float* st = reinterpret_cast<float*>(smem + (size_t)OCTETS_PER_BLOCK * s.stack_cap) + (size_t)local * s.stack_cap;
+    const int j = lane & (QUAD - 1), lane_base = lane & ~(QUAD - 1), local = threadIdx.x / QUAD;
+    uint32_t* stk = quad_stacks + (size_t)local * s.stack_cap;
     const uint32_t n = *qcount;
-    const uint32_t q = blockIdx.x * OCTETS_PER_BLOCK + local;
-    if (q >= n) return;
+    const uint32_t q = blockIdx.x * (TRACE_BLOCK / QUAD) + local;
+    if (q >= n) return;   // quad-uniform
     const float4 o = queue[q].o;
     const RayPre r = make_ray_pre(o.x, o.y, o.z, sun.x, sun.y, sun.z);
-    const HitRec h = trace_octet<true>(s, r, 0.0f, __int_as_float(0x7f800000), sref, st, sub, lane_base);
-    if (sub == 0 && lit) lit[__float_as_uint(o.w)] = h.slot < 0 ? 1 : 0;
+    const bool occluded = trace_quad_any(s, r, 0.0f, __int_as_float(0x7f800000), stk, j, lane_base);
+    if (j == 0 && lit) lit[__float_as_uint(o.w)] = occluded ? 0 : 1;
 }
 
 struct EventTimer {
@@ -492,7 +492,9 @@ int crt_scene_create(const float* vertices, uint32_t n_vertices, const uint32_t*
                            s->bvh.cnodes.bytes + s->bvh.tris.bytes;
         scene_make_view(s.get(), local.data());
         CRT_REQUIRE(s->bvh.n_nodes < (1u << 23), CRT_ERR_INVALID, "BVH exceeds 2^23 wide nodes");
-        CRT_REQUIRE(trace_lds_bytes(s->view) <= 64u * 1024u && lane_lds_bytes(s->view) <= 64u * 1024u, CRT_ERR_INVALID,
+        CRT_REQUIRE(trace_lds_bytes(s->view) <= 64u * 1024u && lane_lds_bytes(s->view) <= 64u * 1024u &&
+                        quad_lds_bytes(s->view) <= 64u * 1024u,
+                    CRT_ERR_INVALID,
                     "BVH too deep for the LDS traversal stack");
         *out_scene = s.release();
     });
@@ -640,8 +642,8 @@ int crt_sun_pass(crt_scene* scene, const crt_camera* camera, const double sun_di
         hipLaunchKernelGGL(k_primary_sun, dim3(blocks), dim3(LANE_BLOCK), lane_lds_bytes(scene->view), st,
                            scene->view, cam, d_depth, d_tri, d_lit, ws.h.as<ShadowRay>(), qcount, offset);
         CRT_HIP_CHECK(hipEventRecord(tm.b, st));
-        hipLaunchKernelGGL(k_shadow_sun, dim3(cdiv(npix, OCTETS_PER_BLOCK)), dim3(TRACE_BLOCK),
-                           trace_lds_bytes(scene->view), st, scene->view, ws.h.as<ShadowRay>(), qcount, sun, d_lit);
+        hipLaunchKernelGGL(k_shadow_sun, dim3(cdiv(npix, TRACE_BLOCK / QUAD)), dim3(TRACE_BLOCK),
+                           quad_lds_bytes(scene->view), st, scene->view, ws.h.as<ShadowRay>(), qcount, sun, d_lit);
         CRT_HIP_CHECK(hipGetLastError());
         CRT_HIP_CHECK(hipEventRecord(tm.c, st));
         uint32_t nshadow = 0;
