@@ -356,6 +356,19 @@ __device__ inline bool child_hit(const float4 q0, const float4 q1, const RayPre&
     return __float_as_uint(q0.w) != REF_EMPTY && tnear <= tfar * 1.00000024f;
 }
 
+// Any hit is an OR over all triangles, so testing one triangle before the
+// traversal cannot change the answer.  Shadow rays test the triangle their
+// origin lies on first: when the light is behind that surface the ray
+// re-enters it within a few offsets and ends after one test instead of a
+// traversal through the body.  Every lane of the quad/octet tests the same
+// triangle, so the result is uniform across the group.
+__device__ inline bool hint_blocks(const SceneView& s, const RayPre& r, uint32_t slot, float tmin, float tmax) {
+    if (slot >= s.n_tris) return false;
+    const size_t ti = (size_t)slot * 3u;
+    float t, u, v;
+    return tri_test(r, s.tris[ti], s.tris[ti + 1], s.tris[ti + 2], tmin, tmax, t, u, v);
+}
+
 __device__ inline bool trace_quad_any(const SceneView& s, const RayPre& r, float tmin, float tmax, uint32_t* stk,
                                       int j, int lane_base) {
     const float4* __restrict__ nodes = s.nodes;

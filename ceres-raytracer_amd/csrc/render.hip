@@ -102,7 +102,7 @@ struct PathState {
 
 struct ShadowRec {
     float4 o;      // origin, tmax
-    float4 d;      // direction
+    float4 d;      // direction, origin triangle (leaf order, bits; tested first)
     float4 c;      // colour * cos * irradiance, occluded flag (written by k_shadow)
 };
 
@@ -167,11 +167,12 @@ __global__ void k_clear(uint32_t* cnt, uint32_t mask) {
     if (i < 32 && ((mask >> i) & 1u)) cnt[i] = 0u;
 }
 
-__global__ __launch_bounds__(TB) void k_raygen(Ctx C) {
-    const uint64_t k = (uint64_t)blockIdx.x * TB + threadIdx.x;
-    const PassCtl ctl = *C.ctl;
+// Camera path k of the pass: FP64 camera ray (jittered), rebased to the
+// scene's bounding sphere, FP32 path state.  Returns false (and writes a
+// zero sample) when the ray misses the sphere.
+__device__ inline bool gen_camera_path(const Ctx& C, const PassCtl& ctl, uint64_t k, PathState& ps) {
     bool alive = false;
-    if (k < ctl.npath) {
+    {
         const uint32_t si = seg_search(ctl.nseg, k, [&](uint32_t i) { return C.segs[i].path_begin; });
         const Segment sg = C.segs[si];
         const uint64_t local = k - sg.path_begin;
@@ -188,7 +189,6 @@ __global__ __launch_bounds__(TB) void k_raygen(Ctx C) {
         double d[3];
         camera_dir(cd.v, u, v, d);
         double t0 = 0.0;
-        PathState ps;
         ps.cam = sg.cam;
         ps.pix = pix;
         ps.sample = smp;
@@ -207,6 +207,17 @@ __global__ __launch_bounds__(TB) void k_raygen(Ctx C) {
             ps.d = make_float4(0.f, 0.f, 1.f, 0.f);
             C.samples[k] = make_float4(0.f, 0.f, 0.f, 0.f);   // camera ray misses the scene
         }
+    }
+    return alive;
+}
+
+__global__ __launch_bounds__(TB) void k_raygen(Ctx C) {
+    const uint64_t k = (uint64_t)blockIdx.x * TB + threadIdx.x;
+    const PassCtl ctl = *C.ctl;
+    bool alive = false;
+    if (k < ctl.npath) {
+        PathState ps;
+        alive = gen_camera_path(C, ctl, k, ps);
         C.st[k] = ps;
     }
     const uint32_t slot = wave_append(alive, C.cnt + 0);
@@ -251,7 +262,8 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace(Ctx C, int qsel) {
 __device__ inline void shadow_quad(const Ctx& C, uint32_t i, uint32_t* stk, int j, int lane_base) {
     const float4 o = C.shadow[i].o, d = C.shadow[i].d;
     const RayPre r = make_ray_pre(o.x, o.y, o.z, d.x, d.y, d.z);
-    const bool occ = trace_quad_any(C.s, r, 0.0f, o.w, stk, j, lane_base);
+    const bool occ =
+        hint_blocks(C.s, r, __float_as_uint(d.w), 0.0f, o.w) || trace_quad_any(C.s, r, 0.0f, o.w, stk, j, lane_base);
     if (j == 0) C.shadow[i].c.w = occ ? 1.0f : 0.0f;
 }
 
@@ -360,6 +372,54 @@ __device__ inline float3 cosine_sample(float3 n, float r1, float r2) {
     return make_float3(nx.x * x + ny.x * y + n.x * z, nx.y * x + ny.y * y + n.y * z, nx.z * x + ny.z * y + n.z * z);
 }
 
+// Shading of a path that hit triangle `slot` (leaf order) with barycentrics
+// (hu, hv) at this bounce: one shadow record per light at live slot `ls`
+// (records ls*nl .. ls*nl+nl-1) and, unless this is the last bounce, the
+// next ray of path k with the pending bounce factor in w.w.
+__device__ inline void shade_hit(const Ctx& C, int bounce, uint32_t k, uint32_t cam, uint32_t pix, uint32_t smp,
+                                 int32_t slot, float hu, float hv, uint32_t ls) {
+    const SceneView& s = C.s;
+    const float4 a = s.tris[3 * slot], b = s.tris[3 * slot + 1], c = s.tris[3 * slot + 2];
+    const uint32_t orig = __float_as_uint(a.w);
+    const uint32_t inst = s.tri_inst[orig];
+    const float u = hu, v = hv, w = 1.0f - u - v;
+    // geometric normal (inward): normalize((p0 - p1) x (p2 - p0))
+    const float3 e1 = make_float3(a.x - b.x, a.y - b.y, a.z - b.z);
+    const float3 e2 = make_float3(c.x - a.x, c.y - a.y, c.z - a.z);
+    const float3 gn = norm3(cross3(e1, e2));
+    float3 sn = gn;
+    if (s.vnormals && inst >= 1 && inst <= s.n_instances && s.inst_smooth[inst - 1]) {
+        const uint32_t i0 = s.tri_vidx[3 * orig], i1 = s.tri_vidx[3 * orig + 1], i2 = s.tri_vidx[3 * orig + 2];
+        sn = norm3(make_float3(u * s.vnormals[3 * i1] + v * s.vnormals[3 * i2] + w * s.vnormals[3 * i0],
+                               u * s.vnormals[3 * i1 + 1] + v * s.vnormals[3 * i2 + 1] + w * s.vnormals[3 * i0 + 1],
+                               u * s.vnormals[3 * i1 + 2] + v * s.vnormals[3 * i2 + 2] + w * s.vnormals[3 * i0 + 2]));
+    }
+    float3 p = make_float3(u * b.x + v * c.x + w * a.x, u * b.y + v * c.y + w * a.y, u * b.z + v * c.z + w * a.z);
+    p = make_float3(p.x - C.offset * gn.x, p.y - C.offset * gn.y, p.z - C.offset * gn.z);
+    const float* col = s.inst_color + 3 * (inst >= 1 && inst <= s.n_instances ? inst - 1 : 0);
+    const uint64_t seed = C.cams[cam].seed;
+    for (int l = 0; l < C.nl; ++l) {
+        float3 ld;
+        float tmax, irr;
+        light_sample(C, cam, l, p, seed, pix, smp, bounce, ld, tmax, irr);
+        // ColoredLambertianMaterial::compute: colour * (-L.n_s), no max(0, .)
+        const float ldn = -dot3(ld, sn);
+        ShadowRec r;
+        r.o = make_float4(p.x, p.y, p.z, tmax);
+        r.d = make_float4(ld.x, ld.y, ld.z, __uint_as_float((uint32_t)slot));
+        r.c = make_float4(col[0] * ldn * irr, col[1] * ldn * irr, col[2] * ldn * irr, 0.f);
+        C.shadow[ls * (uint32_t)C.nl + (uint32_t)l] = r;
+    }
+    if (bounce < C.nb - 1) {
+        const float r1 = (float)rng_u01(seed, pix, smp, rng_dim(bounce, RNG_BRDF));
+        const float r2 = (float)rng_u01(seed, pix, smp, rng_dim(bounce, RNG_BRDF + 1));
+        const float3 nd = cosine_sample(sn, r1, r2);
+        C.st[k].o = make_float4(p.x, p.y, p.z, 0.f);
+        C.st[k].d = make_float4(nd.x, nd.y, nd.z, 0.f);
+        C.st[k].w.w = 1.0f - r1;
+    }
+}
+
 __global__ __launch_bounds__(TB) void k_shade(Ctx C, int bounce, int qsel) {
     const uint32_t q = blockIdx.x * TB + threadIdx.x;
     const uint32_t n = C.cnt[qsel];
@@ -375,46 +435,7 @@ __global__ __launch_bounds__(TB) void k_shade(Ctx C, int bounce, int qsel) {
     const uint32_t ls = wave_append(has_hit, C.cnt + 3);   // live slot = shadow slots ls*nl .. ls*nl+nl-1
     if (has_hit) C.live[ls] = k;
     if (!has_hit) return;
-    const SceneView& s = C.s;
-    const float4 a = s.tris[3 * ps.slot], b = s.tris[3 * ps.slot + 1], c = s.tris[3 * ps.slot + 2];
-    const uint32_t orig = __float_as_uint(a.w);
-    const uint32_t inst = s.tri_inst[orig];
-    const float u = ps.hu, v = ps.hv, w = 1.0f - u - v;
-    // geometric normal (inward): normalize((p0 - p1) x (p2 - p0))
-    const float3 e1 = make_float3(a.x - b.x, a.y - b.y, a.z - b.z);
-    const float3 e2 = make_float3(c.x - a.x, c.y - a.y, c.z - a.z);
-    const float3 gn = norm3(cross3(e1, e2));
-    float3 sn = gn;
-    if (s.vnormals && inst >= 1 && inst <= s.n_instances && s.inst_smooth[inst - 1]) {
-        const uint32_t i0 = s.tri_vidx[3 * orig], i1 = s.tri_vidx[3 * orig + 1], i2 = s.tri_vidx[3 * orig + 2];
-        sn = norm3(make_float3(u * s.vnormals[3 * i1] + v * s.vnormals[3 * i2] + w * s.vnormals[3 * i0],
-                               u * s.vnormals[3 * i1 + 1] + v * s.vnormals[3 * i2 + 1] + w * s.vnormals[3 * i0 + 1],
-                               u * s.vnormals[3 * i1 + 2] + v * s.vnormals[3 * i2 + 2] + w * s.vnormals[3 * i0 + 2]));
-    }
-    float3 p = make_float3(u * b.x + v * c.x + w * a.x, u * b.y + v * c.y + w * a.y, u * b.z + v * c.z + w * a.z);
-    p = make_float3(p.x - C.offset * gn.x, p.y - C.offset * gn.y, p.z - C.offset * gn.z);
-    const float* col = s.inst_color + 3 * (inst >= 1 && inst <= s.n_instances ? inst - 1 : 0);
-    const uint64_t seed = C.cams[ps.cam].seed;
-    for (int l = 0; l < C.nl; ++l) {
-        float3 ld;
-        float tmax, irr;
-        light_sample(C, ps.cam, l, p, seed, ps.pix, ps.sample, bounce, ld, tmax, irr);
-        // ColoredLambertianMaterial::compute: colour * (-L.n_s), no max(0, .)
-        const float ldn = -dot3(ld, sn);
-        ShadowRec r;
-        r.o = make_float4(p.x, p.y, p.z, tmax);
-        r.d = make_float4(ld.x, ld.y, ld.z, 0.f);
-        r.c = make_float4(col[0] * ldn * irr, col[1] * ldn * irr, col[2] * ldn * irr, 0.f);
-        C.shadow[ls * (uint32_t)C.nl + (uint32_t)l] = r;
-    }
-    if (bounce < C.nb - 1) {
-        const float r1 = (float)rng_u01(seed, ps.pix, ps.sample, rng_dim(bounce, RNG_BRDF));
-        const float r2 = (float)rng_u01(seed, ps.pix, ps.sample, rng_dim(bounce, RNG_BRDF + 1));
-        const float3 nd = cosine_sample(sn, r1, r2);
-        C.st[k].o = make_float4(p.x, p.y, p.z, 0.f);
-        C.st[k].d = make_float4(nd.x, nd.y, nd.z, 0.f);
-        C.st[k].w.w = 1.0f - r1;
-    }
+    shade_hit(C, bounce, k, ps.cam, ps.pix, ps.sample, ps.slot, ps.hu, ps.hv, ls);
 }
 
 // After k_shade: shadow rays = nl per live path; tally traced rays.

@@ -237,10 +237,13 @@ CRT_LANE_KERNEL void k_trace_rays(SceneView s, RayIn in, RayOut out, uint64_t n)
 struct ShadowRay {
     float4 o;       // origin xyz, pixel index (bits)
 };
+// The queue buffer holds n ShadowRay records followed by n uint32 origin
+// triangles (leaf order), tested before the traversal (hint_blocks).
+inline uint32_t* shadow_slots(ShadowRay* q, size_t n) { return reinterpret_cast<uint32_t*>(q + n); }
 
 CRT_LANE_KERNEL void k_primary_sun(SceneView s, CamView cam, double* depth, int32_t* tri,
-                                                             uint8_t* lit, ShadowRay* queue, uint32_t* qcount,
-                                                             float offset) {
+                                                             uint8_t* lit, ShadowRay* queue, uint32_t* qslot,
+                                                             uint32_t* qcount, float offset) {
     const LaneStack stk = lane_stack(s);
     int px, py;
     tile_pixel(blockIdx.x, threadIdx.x, cam.w, px, py);
@@ -249,12 +252,12 @@ CRT_LANE_KERNEL void k_primary_sun(SceneView s, CamView cam, double* depth, int3
     double p[3] = {0.0, 0.0, 0.0};
     float nx = 0.f, ny = 0.f, nz = 0.f;
     size_t pix = 0;
+    int slot = -1;
     if (active) {
         double d[3];
         camera_dir(cam, (double)px, (double)py, d);
         double t;
         double u = 0.0, v = 0.0;
-        int slot = -1;
         hit = trace_f64<false>(s, cam.pos, d, 0.0, 1.0e300, stk, t, u, v, slot);
         pix = (size_t)py * cam.w + px;
         if (hit) {
@@ -287,6 +290,7 @@ CRT_LANE_KERNEL void k_primary_sun(SceneView s, CamView cam, double* depth, int3
         // offset off the surface along the outward normal (the geometric normal is inward)
         const float ox = (float)p[0] - offset * nx, oy = (float)p[1] - offset * ny, oz = (float)p[2] - offset * nz;
         queue[slot_q].o = make_float4(ox, oy, oz, __uint_as_float((uint32_t)pix));
+        qslot[slot_q] = (uint32_t)slot;
     }
 }
 
@@ -294,7 +298,8 @@ CRT_LANE_KERNEL void k_primary_sun(SceneView s, CamView cam, double* depth, int3
 // per lane) beats both one ray per lane, whose waves wait on their longest
 // unoccluded ray, and one ray per octet.
 __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow_sun(SceneView s, const ShadowRay* queue,
-                                                            const uint32_t* qcount, float3 sun, uint8_t* lit) {
+                                                            const uint32_t* qslot, const uint32_t* qcount, float3 sun,
+                                                            uint8_t* lit) {
     extern __shared__ uint32_t quad_stacks[];
     const int lane = threadIdx.x & 63;
     const int j = lane & (QUAD - 1), lane_base = lane & ~(QUAD - 1), local = threadIdx.x / QUAD;
@@ -304,7 +309,8 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow_sun(SceneView s, const S
     if (q >= n) return;   // quad-uniform
     const float4 o = queue[q].o;
     const RayPre r = make_ray_pre(o.x, o.y, o.z, sun.x, sun.y, sun.z);
-    const bool occluded = trace_quad_any(s, r, 0.0f, __int_as_float(0x7f800000), stk, j, lane_base);
+    const float inf = __int_as_float(0x7f800000);
+    const bool occluded = hint_blocks(s, r, qslot[q], 0.0f, inf) || trace_quad_any(s, r, 0.0f, inf, stk, j, lane_base);
     if (j == 0 && lit) lit[__float_as_uint(o.w)] = occluded ? 0 : 1;
 }
 
@@ -631,7 +637,7 @@ int crt_sun_pass(crt_scene* scene, const crt_camera* camera, const double sun_di
             if (tri_id) { ws.e.reserve(npix * 4); d_tri = ws.e.as<int32_t>(); }
             if (lit) { ws.c.reserve(npix); d_lit = ws.c.as<uint8_t>(); }
         }
-        ws.h.reserve(npix * sizeof(ShadowRay));
+        ws.h.reserve(npix * (sizeof(ShadowRay) + sizeof(uint32_t)));
         ws.counters.reserve(256);
         uint32_t* qcount = ws.counters.as<uint32_t>();
         const float offset = self_offset(scene);
@@ -640,10 +646,12 @@ int crt_sun_pass(crt_scene* scene, const crt_camera* camera, const double sun_di
         CRT_HIP_CHECK(hipMemsetAsync(qcount, 0, 4, st));
         CRT_HIP_CHECK(hipEventRecord(tm.a, st));
         hipLaunchKernelGGL(k_primary_sun, dim3(blocks), dim3(LANE_BLOCK), lane_lds_bytes(scene->view), st,
-                           scene->view, cam, d_depth, d_tri, d_lit, ws.h.as<ShadowRay>(), qcount, offset);
+                           scene->view, cam, d_depth, d_tri, d_lit, ws.h.as<ShadowRay>(),
+                           shadow_slots(ws.h.as<ShadowRay>(), npix), qcount, offset);
         CRT_HIP_CHECK(hipEventRecord(tm.b, st));
         hipLaunchKernelGGL(k_shadow_sun, dim3(cdiv(npix, TRACE_BLOCK / QUAD)), dim3(TRACE_BLOCK),
-                           quad_lds_bytes(scene->view), st, scene->view, ws.h.as<ShadowRay>(), qcount, sun, d_lit);
+                           quad_lds_bytes(scene->view), st, scene->view, ws.h.as<ShadowRay>(),
+                           shadow_slots(ws.h.as<ShadowRay>(), npix), qcount, sun, d_lit);
         CRT_HIP_CHECK(hipGetLastError());
         CRT_HIP_CHECK(hipEventRecord(tm.c, st));
         uint32_t nshadow = 0;
