@@ -23,16 +23,20 @@ from crt import capi
 CACHE = os.environ.get("CRT_AUTOTUNE_CACHE", os.path.join(os.path.expanduser("~"), ".cache", "crt", "autotune.json"))
 
 CANDIDATES = [
-    dict(samples_per_pass=0, use_graph=False, persistent=False),
-    dict(samples_per_pass=0, use_graph=True, persistent=False),
+    dict(samples_per_pass=0, use_graph=False, persistent=False, extend_mode=1),
+    dict(samples_per_pass=0, use_graph=True, persistent=False, extend_mode=1),
+    dict(samples_per_pass=0, use_graph=False, persistent=False, extend_mode=3),
     dict(samples_per_pass=0, use_graph=True, persistent=False, extend_mode=2),
     dict(samples_per_pass=0, use_graph=True, persistent=False, extend_mode=3),
-    dict(samples_per_pass=2, use_graph=True, persistent=False),
+    dict(samples_per_pass=2, use_graph=True, persistent=False, extend_mode=1),
     dict(samples_per_pass=2, use_graph=True, persistent=False, extend_mode=3),
     dict(samples_per_pass=4, use_graph=True, persistent=False, extend_mode=3),
-    dict(samples_per_pass=0, use_graph=True, persistent=True, blocks_per_cu=8, grab=4),
+    dict(samples_per_pass=4, use_graph=True, persistent=False, extend_mode=2),
+    dict(samples_per_pass=4, use_graph=True, persistent=False, extend_mode=3, separate_kernels=2),
+    dict(samples_per_pass=4, use_graph=True, persistent=False, extend_mode=3, separate_kernels=1),
+    dict(samples_per_pass=0, use_graph=True, persistent=True, blocks_per_cu=8, grab=4, extend_mode=1),
     dict(samples_per_pass=0, use_graph=True, persistent=True, blocks_per_cu=8, grab=4, extend_mode=3),
-    dict(samples_per_pass=0, use_graph=True, persistent=True, blocks_per_cu=16, grab=1),
+    dict(samples_per_pass=0, use_graph=True, persistent=True, blocks_per_cu=16, grab=1, extend_mode=1),
 ]
 
 
@@ -47,7 +51,7 @@ def _device_name():
 def _key(scene: capi.Scene, bounces, n_lights):
     n = scene.n_tris
     bucket = int(np.round(np.log2(max(n, 1))))
-    return "%s|tris2^%d|b%d|l%d" % (_device_name(), bucket, bounces, n_lights)
+    return "v2|%s|tris2^%d|b%d|l%d" % (_device_name(), bucket, bounces, n_lights)
 
 
 def load_cache():

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <new>
@@ -146,6 +147,17 @@ struct crt_scene {
     crt::BuildResult bvh;
     crt::SceneView view{};
     crt_bvh_info info{};
+    // Grow-only device buffers of the scene's own entry points (passes, ray
+    // queries, sensors: ws; renders: rws).  They are reused from call to
+    // call, so those calls take `mu` and run one at a time per scene; the
+    // server's workers bring their own buffers and run concurrently.
     crt::Workspace ws;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    crt::RenderWorkspace* rws = nullptr;
+    std::mutex mu;
+    crt_scene() = default;
+    crt_scene(const crt_scene&) = delete;
+    crt_scene& operator=(const crt_scene&) = delete;
+    ~crt_scene() {
+        if (rws) crt::delete_render_workspace(rws);
+    }
 };

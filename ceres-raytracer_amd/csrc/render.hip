@@ -53,6 +53,9 @@ inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b
 constexpr int MAX_LIGHTS = 8;
 constexpr int TB = 256;
 constexpr uint32_t NO_LIST = 0xFFFFFFFFu;
+// default extend mode: one lane per camera ray from this many paths per pass
+// (2^20 lanes = 16 waves per SIMD); smaller passes trace camera rays by octets
+constexpr uint64_t LANE_PASS_MIN = (uint64_t)1 << 20;
 
 struct LightDev {
     int type;
@@ -447,6 +450,29 @@ __global__ void k_tally(Ctx C, int qsel) {
     }
 }
 
+// Direct light of this bounce (sum over the unoccluded lights, clamped to
+// [0,1] from bounce 1 on) times the path weight; then the weight takes the
+// pending bounce factor.  Returns true when the path goes on to the next
+// bounce; after the last bounce the path's sample is written.
+__device__ inline bool accumulate_path(const Ctx& C, int bounce, uint32_t k, float lr[3]) {
+    PathState& ps = C.st[k];
+    if (bounce >= 1)
+        for (int j = 0; j < 3; ++j) lr[j] = fminf(fmaxf(lr[j], 0.0f), 1.0f);
+    float4 L = ps.L;
+    float4 w = ps.w;
+    L.x += lr[0] * w.x;
+    L.y += lr[1] * w.y;
+    L.z += lr[2] * w.z;
+    ps.L = L;
+    if (bounce == C.nb - 1) {
+        C.samples[k] = L;
+        return false;
+    }
+    w.x *= w.w; w.y *= w.w; w.z *= w.w;
+    ps.w = w;
+    return true;
+}
+
 __global__ __launch_bounds__(TB) void k_accumulate(Ctx C, int bounce, int next) {
     const uint32_t q = blockIdx.x * TB + threadIdx.x;
     const uint32_t n = C.cnt[3];
@@ -454,27 +480,117 @@ __global__ __launch_bounds__(TB) void k_accumulate(Ctx C, int bounce, int next) 
     uint32_t k = 0;
     if (q < n) {
         k = C.live[q];
-        PathState& ps = C.st[k];
         float lr[3] = {0.f, 0.f, 0.f};
         for (int l = 0; l < C.nl; ++l) {
             const float4 c = C.shadow[q * (uint32_t)C.nl + (uint32_t)l].c;
             if (c.w == 0.0f) { lr[0] += c.x; lr[1] += c.y; lr[2] += c.z; }
         }
-        if (bounce >= 1)
-            for (int j = 0; j < 3; ++j) lr[j] = fminf(fmaxf(lr[j], 0.0f), 1.0f);
-        float4 L = ps.L;
-        float4 w = ps.w;
-        L.x += lr[0] * w.x;
-        L.y += lr[1] * w.y;
-        L.z += lr[2] * w.z;
-        ps.L = L;
-        if (bounce == C.nb - 1) {
-            C.samples[k] = L;
-        } else {
-            w.x *= w.w; w.y *= w.w; w.z *= w.w;
-            ps.w = w;
-            cont = true;
+        cont = accumulate_path(C, bounce, k, lr);
+    }
+    const uint32_t slot = wave_append(cont, C.cnt + next);
+    if (cont) C.queue[next][slot] = k;
+}
+
+// ---- fused bounce kernels ------------------------------------------------
+// k_extend_shade_lane: closest hit of a queued path and, in the same lane,
+// either its shading (shadow records, next ray) or the end of a path that
+// missed.  With gen != 0 it also makes the camera rays of bounce 0 itself
+// (one lane per path of the pass, no queue).  k_extend_shade_octet: the
+// same with an octet per path.
+// k_shadow_accum: one quad per live path tests the path's nl shadow rays
+// and accumulates its direct light, then queues the path for the next
+// bounce.  The arithmetic is that of k_raygen, k_extend_lane, k_shade,
+// k_shadow_quad and k_accumulate in the same order, so images are
+// bit-identical to the one-kernel-per-stage pipeline, with less path-state
+// traffic and fewer launches per bounce.
+__device__ inline void finish_extend(const Ctx& C, int bounce, uint32_t k, const HitRec& h, uint32_t ls) {
+    if (h.slot < 0) {
+        C.samples[k] = C.st[k].L;   // path ends without a hit
+        return;
+    }
+    C.live[ls] = k;
+    const uint4 m = *reinterpret_cast<const uint4*>(&C.st[k].cam);   // cam, pix, sample
+    shade_hit(C, bounce, k, m.x, m.y, m.z, h.slot, h.u, h.v, ls);
+}
+
+CRT_LANE_KERNEL void k_extend_shade_lane(Ctx C, int bounce, int qsel, int gen) {
+    const LaneStack stk = lane_stack(C.s);
+    const uint32_t q = blockIdx.x * LANE_BLOCK + threadIdx.x;
+    uint32_t k = 0;
+    bool alive = false;
+    float4 o = make_float4(0.f, 0.f, 0.f, 0.f), d = o;
+    if (gen) {
+        const PassCtl ctl = *C.ctl;
+        if (q < ctl.npath) {
+            k = q;
+            PathState ps;
+            alive = gen_camera_path(C, ctl, k, ps);
+            if (alive) {
+                // the ray itself stays in registers; the next bounce's ray is
+                // written by the shading
+                C.st[k].w = ps.w;
+                C.st[k].L = ps.L;
+                *reinterpret_cast<uint4*>(&C.st[k].cam) = make_uint4(ps.cam, ps.pix, ps.sample, 0u);
+                o = ps.o;
+                d = ps.d;
+            }
         }
+        (void)wave_append(alive, C.cnt + qsel);   // count of traced camera rays (tally)
+    } else if (q < C.cnt[qsel]) {
+        k = C.queue[qsel][q];
+        o = C.st[k].o;
+        d = C.st[k].d;
+        alive = true;
+    }
+    if (!alive) return;
+    const RayPre r = make_ray_pre(o.x, o.y, o.z, d.x, d.y, d.z);
+    const HitRec h = trace_lane<false>(C.s, r, 0.0f, __int_as_float(0x7f800000), stk);
+    const uint32_t ls = wave_append(h.slot >= 0, C.cnt + 3);   // live slot
+    finish_extend(C, bounce, k, h, ls);
+}
+
+// Octet form for the bounces after the first: lane 0 of the octet carries
+// the shading.  Those rays leave the body more often than camera rays hit
+// it, so saving the path-state round trip and a launch outweighs the
+// one-lane-in-eight shading; camera rays traced by octets use k_trace +
+// k_shade instead.
+__global__ __launch_bounds__(TRACE_BLOCK) void k_extend_shade_octet(Ctx C, int bounce, int qsel) {
+    OCTET_SETUP(C)
+    const uint32_t q = blockIdx.x * OCTETS_PER_BLOCK + local;
+    if (q >= C.cnt[qsel]) return;   // octet-uniform
+    const uint32_t k = C.queue[qsel][q];
+    const float4 o = C.st[k].o, d = C.st[k].d;
+    const RayPre r = make_ray_pre(o.x, o.y, o.z, d.x, d.y, d.z);
+    const HitRec h = trace_octet<false>(C.s, r, 0.0f, __int_as_float(0x7f800000), sref, sd, sub, lane_base);
+    const uint32_t ls = wave_append(h.slot >= 0 && sub == 0, C.cnt + 3);   // the hit is octet-uniform
+    if (sub == 0) finish_extend(C, bounce, k, h, ls);
+}
+
+__global__ __launch_bounds__(TRACE_BLOCK) void k_shadow_accum(Ctx C, int bounce, int qsel, int next) {
+    QUAD_SETUP(C)
+    const uint32_t q = blockIdx.x * (TRACE_BLOCK / QUAD) + qlocal;
+    const uint32_t n = C.cnt[3];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        C.tally[0] += C.cnt[qsel];
+        C.tally[1] += n * (uint32_t)C.nl;
+    }
+    bool cont = false;
+    uint32_t k = 0;
+    if (q < n) {   // quad-uniform
+        k = C.live[q];
+        float lr[3] = {0.f, 0.f, 0.f};
+        for (int l = 0; l < C.nl; ++l) {
+            const uint32_t i = q * (uint32_t)C.nl + (uint32_t)l;
+            const float4 o = C.shadow[i].o, d = C.shadow[i].d;
+            const RayPre r = make_ray_pre(o.x, o.y, o.z, d.x, d.y, d.z);
+            const bool occ = hint_blocks(C.s, r, __float_as_uint(d.w), 0.0f, o.w) ||
+                             trace_quad_any(C.s, r, 0.0f, o.w, qstk, qj, qbase);
+            if (!occ) {
+                const float4 c = C.shadow[i].c;
+                lr[0] += c.x; lr[1] += c.y; lr[2] += c.z;
+            }
+        }
+        if (qj == 0) cont = accumulate_path(C, bounce, k, lr);
     }
     const uint32_t slot = wave_append(cont, C.cnt + next);
     if (cont) C.queue[next][slot] = k;
@@ -554,12 +670,38 @@ void launch_pass(const Ctx& C, uint64_t cap_paths, uint32_t cap_fold, const crt_
     const size_t qlds = quad_lds_bytes(C.s);
     const uint32_t gpers = std::max(1u, B.blocks_per_cu) * (uint32_t)num_cus;
     hipLaunchKernelGGL(k_clear, dim3(1), dim3(32), 0, st, C.cnt, 0x7Fu);
+    // extend_mode 0 (default): lanes for the coherent camera rays of passes
+    // large enough to fill the GPU with one lane per ray, octets otherwise
+    const uint32_t em = B.extend_mode != 0 ? B.extend_mode : (cap_paths >= LANE_PASS_MIN ? 3u : 1u);
+    const auto lane_at = [&](int b) { return em == 2 || (em == 3 && b == 0); };
+    const bool fused = B.separate_kernels != 1 && !B.persistent;
+    if (fused) {
+        const bool gen = C.nb > 0 && lane_at(0);
+        if (!gen && C.nb > 0) hipLaunchKernelGGL(k_raygen, dim3(gp), dim3(TB), 0, st, C);
+        const uint32_t gq = std::max(1u, cdiv(cap_paths, TRACE_BLOCK / QUAD));
+        for (int b = 0; b < C.nb; ++b) {
+            const int cur = b & 1, nxt = cur ^ 1;
+            hipLaunchKernelGGL(k_clear, dim3(1), dim3(32), 0, st, C.cnt, (1u << nxt) | (1u << 3));
+            if (lane_at(b))
+                hipLaunchKernelGGL(k_extend_shade_lane, dim3(std::max(1u, cdiv(cap_paths, LANE_BLOCK))),
+                                   dim3(LANE_BLOCK), lane_lds_bytes(C.s), st, C, b, cur, (int)(gen && b == 0));
+            else if (b > 0 && B.separate_kernels != 2)
+                hipLaunchKernelGGL(k_extend_shade_octet, dim3(go), dim3(TRACE_BLOCK), lds, st, C, b, cur);
+            else {
+                hipLaunchKernelGGL(k_trace, dim3(go), dim3(TRACE_BLOCK), lds, st, C, cur);
+                hipLaunchKernelGGL(k_shade, dim3(gp), dim3(TB), 0, st, C, b, cur);
+            }
+            hipLaunchKernelGGL(k_shadow_accum, dim3(gq), dim3(TRACE_BLOCK), qlds, st, C, b, cur, nxt);
+        }
+        if (C.nb == 0) hipLaunchKernelGGL(k_zero_samples, dim3(gp), dim3(TB), 0, st, C);
+        hipLaunchKernelGGL(k_fold, dim3(std::max(1u, cdiv(cap_fold, TB))), dim3(TB), 0, st, C);
+        return;
+    }
     hipLaunchKernelGGL(k_raygen, dim3(gp), dim3(TB), 0, st, C);
     for (int b = 0; b < C.nb; ++b) {
         const int cur = b & 1, nxt = cur ^ 1;
         hipLaunchKernelGGL(k_clear, dim3(1), dim3(32), 0, st, C.cnt, (1u << nxt) | (1u << 2) | (1u << 3) | (3u << 5));
-        const bool lane = B.extend_mode == 2 || (B.extend_mode == 3 && b == 0);
-        if (lane)
+        if (lane_at(b))
             hipLaunchKernelGGL(k_extend_lane, dim3(std::max(1u, cdiv(cap_paths, LANE_BLOCK))), dim3(LANE_BLOCK),
                                lane_lds_bytes(C.s), st, C, cur);
         else if (B.persistent)
@@ -864,9 +1006,12 @@ extern "C" int crt_render(crt_scene* scene, const crt_camera* camera, const crt_
                           uint32_t flags, void* stream, crt_timing* timing) {
     return guarded([&] {
         CRT_REQUIRE(opts != nullptr, CRT_ERR_INVALID, "render options are NULL");
+        CRT_REQUIRE(scene != nullptr, CRT_ERR_INVALID, "scene is NULL");
         const uint64_t seed = opts->seed;
+        std::lock_guard<std::mutex> scene_lock(scene->mu);
+        if (!scene->rws) scene->rws = new_render_workspace();
         render_batch(scene, 1, camera, nullptr, &seed, lights, n_lights, false, opts, nullptr, rgba, rgba_u8, spp_used,
-                     flags, stream, timing, nullptr);
+                     flags, stream, timing, scene->rws);
     });
 }
 
@@ -876,8 +1021,10 @@ extern "C" int crt_render_batch(crt_scene* scene, uint32_t n_cams, const crt_cam
                                 float* rgba, uint8_t* rgba_u8, int32_t* spp_used, uint32_t flags, void* stream,
                                 crt_timing* timing) {
     return guarded([&] {
+        CRT_REQUIRE(scene != nullptr, CRT_ERR_INVALID, "scene is NULL");
+        std::lock_guard<std::mutex> scene_lock(scene->mu);
+        if (!scene->rws) scene->rws = new_render_workspace();
         render_batch(scene, n_cams, cams, max_samples, seeds, lights, n_lights, false, opts, bopts, rgba, rgba_u8,
-                     spp_used,
-                     flags, stream, timing, nullptr);
+                     spp_used, flags, stream, timing, scene->rws);
     });
 }

@@ -524,6 +524,7 @@ int crt_pass(crt_scene* scene, const crt_camera* camera, double* xyz, double* no
         const CamView cam = make_cam_view(*camera, scene);
         const size_t npix = (size_t)cam.w * cam.h;
         const bool dev = (flags & CRT_OUT_DEVICE) != 0;
+        std::lock_guard<std::mutex> scene_lock(scene->mu);
         Workspace& ws = scene->ws;
         PassOut o{nullptr, nullptr, nullptr, nullptr, nullptr};
         if (xyz) { if (dev) o.xyz = xyz; else { ws.a.reserve(npix * 24); o.xyz = ws.a.as<double>(); } }
@@ -566,6 +567,7 @@ int crt_trace_rays(crt_scene* scene, uint64_t n, const double* origins, const do
         hipStream_t st = (hipStream_t)stream;
         const bool dev = (flags & CRT_OUT_DEVICE) != 0;
         const bool any = (flags & CRT_ANY_HIT) != 0;
+        std::lock_guard<std::mutex> scene_lock(scene->mu);
         Workspace& ws = scene->ws;
         RayIn in{origins, dirs, tmin, tmax};
         RayOut out{t, tri, u, v};
@@ -628,6 +630,7 @@ int crt_sun_pass(crt_scene* scene, const crt_camera* camera, const double sun_di
         const CamView cam = make_cam_view(*camera, scene);
         const size_t npix = (size_t)cam.w * cam.h;
         const bool dev = (flags & CRT_OUT_DEVICE) != 0;
+        std::lock_guard<std::mutex> scene_lock(scene->mu);
         Workspace& ws = scene->ws;
         double* d_depth = depth;
         int32_t* d_tri = tri_id;

@@ -18,6 +18,12 @@
  * tensor) and the call only enqueues work on `stream` (hipStream_t, may be
  * NULL for the null stream).
  *
+ * Threads: calls on one scene run one at a time (a per-scene lock); they
+ * reuse the scene's grow-only device workspace, so with CRT_OUT_DEVICE
+ * successive calls on one scene must use the same stream or synchronise in
+ * between.  Different scenes, and the request server's workers (own
+ * workspaces and streams), run concurrently.
+ *
  * Reference interfaces replaced (ceres-raytracer v0.6a4 paths):
  *   crt_scene_create/build  <- BodyFixedGroup ctor: transform + SweepSAH + PRO + NLO
  *                              (src/crt/rendering_body_fixed/body_fixed_group.hpp:45-89)
@@ -223,9 +229,15 @@ typedef struct {
     uint32_t persistent;
     uint32_t blocks_per_cu;
     uint32_t grab;
-    uint32_t extend_mode;     /* closest-hit extension rays: 0, 1 = an octet of lanes per ray,
+    uint32_t extend_mode;     /* closest-hit extension rays: 1 = an octet of lanes per ray,
                                  2 = one lane per ray, 3 = one lane per ray for camera rays and an
-                                 octet per ray after the first bounce */
+                                 octet per ray after the first bounce; 0 (default) = 3 for passes
+                                 of at least 2^20 paths, 1 for smaller ones */
+    uint32_t separate_kernels; /* 0: fused bounce kernels (trace + shade, shadow + accumulate;
+                                 camera rays generated in the lane trace kernel); 1: one kernel
+                                 per stage; 2: fused, except octet-traced bounces, which use
+                                 separate trace and shade kernels.  Persistent traversal always
+                                 uses separate kernels.  All give bit-identical images. */
 } crt_batch_opts;
 int crt_render_batch(crt_scene* scene, uint32_t n_cams, const crt_camera* cams, const int32_t* max_samples,
                      const uint64_t* seeds, const crt_light* lights, uint32_t n_lights,

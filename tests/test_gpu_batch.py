@@ -50,14 +50,15 @@ def test_batch_variants_identical(body, variant):
         assert np.array_equal(a[0], b[0])
 
 
-@pytest.mark.parametrize("variant", [dict(extend_mode=2), dict(extend_mode=3, samples_per_pass=2, use_graph=True)])
+@pytest.mark.parametrize("variant", [dict(extend_mode=2), dict(extend_mode=3, samples_per_pass=2, use_graph=True),
+                                     dict()])
 def test_batch_traversal_forms_agree(body, variant):
     # lane and octet traversal visit nodes in different orders, so only rays
     # whose closest hit is an exact tie between two triangles may differ
     m, s = body
     cams = _cams(m, 5, seed=9)
     sun = {"type": "directional", "intensity": 1.0, "direction": scenes.sun_direction(2)}
-    base, _ = s.render_batch(cams, [sun], 3, 3, 0.0, 3, seeds=list(range(5)))
+    base, _ = s.render_batch(cams, [sun], 3, 3, 0.0, 3, seeds=list(range(5)), extend_mode=1)
     alt, _ = s.render_batch(cams, [sun], 3, 3, 0.0, 3, seeds=list(range(5)), **variant)
     diff = sum(int(np.any(a[0] != b[0], axis=-1).sum()) for a, b in zip(base, alt))
     total = sum(a[0].size // 4 for a in base)
@@ -81,3 +82,23 @@ def test_batch_per_camera_spp_and_adaptive(body):
         single, _, sp1, _ = s.render(cams[k], [sun], 2, 6, 0.02, 2, seed=k + 1)
         assert np.array_equal(out2[k][1], sp1)
         assert np.array_equal(out2[k][0], single)
+
+
+@pytest.mark.parametrize("extend_mode,fused", [(1, 0), (2, 0), (3, 0), (1, 2), (3, 2), (0, 0)])
+@pytest.mark.parametrize("n_lights,bounces", [(1, 3), (2, 2), (0, 2), (1, 1)])
+def test_fused_equals_separate_kernels(body, extend_mode, fused, n_lights, bounces):
+    # the fused bounce kernels (trace + shade, shadow + accumulate, camera
+    # rays made in the lane trace kernel) run the stage kernels' arithmetic
+    # in the same order: images must be bit-identical
+    m, s = body
+    cams = _cams(m, 5, seed=21)
+    lights = [{"type": "directional", "intensity": 1.0, "direction": scenes.sun_direction(2)},
+              {"type": "point", "intensity": 4.0e11, "position": [3.0e5, -1.0e5, 2.0e5]}][:n_lights]
+    kw = dict(seeds=list(range(5)), extend_mode=extend_mode)
+    sep, tms = s.render_batch(cams, lights, 2, 2, 0.0, bounces, separate_kernels=1, **kw)
+    fus, tmf = s.render_batch(cams, lights, 2, 2, 0.0, bounces, separate_kernels=fused, **kw)
+    for a, b in zip(sep, fus):
+        assert np.array_equal(a[0], b[0])
+        assert np.array_equal(a[1], b[1])
+    assert tms["rays_traced"] == tmf["rays_traced"]
+    assert tms["shadow_rays"] == tmf["shadow_rays"]
