@@ -600,6 +600,31 @@ __device__ inline HitRec trace_lane(const SceneView& s, const RayPre& r, float t
 }
 
 // ------------------------------------------------------------------------
+// Pixel order for coherent waves: the i-th pixel of a w x h image in 8x8
+// tiles (row-major over tiles, Z-order inside a tile), then the pixels that
+// no whole tile covers (the right strip row by row, then the bottom rows).
+// A bijection of [0, w*h): consecutive i stay in compact 2^k x 2^k blocks,
+// so a wave's rays share BVH nodes.
+// ------------------------------------------------------------------------
+__host__ __device__ inline uint32_t tile_order(uint32_t i, uint32_t w, uint32_t h) {
+    const uint32_t tw = w >> 3, th = h >> 3;
+    const uint32_t nfull = tw * th * 64u;
+    if (i < nfull) {
+        const uint32_t tile = i >> 6, z = i & 63u;
+        const uint32_t tx = tile % tw, ty = tile / tw;
+        const uint32_t x = (z & 1u) | ((z >> 1) & 2u) | ((z >> 2) & 4u);
+        const uint32_t y = ((z >> 1) & 1u) | ((z >> 2) & 2u) | ((z >> 3) & 4u);
+        return (ty * 8u + y) * w + tx * 8u + x;
+    }
+    i -= nfull;
+    const uint32_t rw = w - tw * 8u;
+    const uint32_t nright = th * 8u * rw;
+    if (i < nright) return (i / rw) * w + tw * 8u + i % rw;
+    i -= nright;
+    return (th * 8u + i / w) * w + i % w;
+}
+
+// ------------------------------------------------------------------------
 // FP64 helpers: camera rays and rebasing
 // ------------------------------------------------------------------------
 struct CamView {

@@ -181,8 +181,9 @@ __device__ inline bool gen_camera_path(const Ctx& C, const PassCtl& ctl, uint64_
         const uint64_t local = k - sg.path_begin;
         const uint32_t i = (uint32_t)(local / sg.s_count);
         const uint32_t smp = sg.s_first + (uint32_t)(local % sg.s_count);
-        const uint32_t pix = sg.plist_off == NO_LIST ? i : C.plist[sg.plist_off + i];
         const CamDev& cd = C.cams[sg.cam];
+        const uint32_t pix = sg.plist_off == NO_LIST ? tile_order(i, (uint32_t)cd.v.w, (uint32_t)cd.v.h)
+                                                     : C.plist[sg.plist_off + i];
         const int px = (int)(pix % (uint32_t)cd.v.w), py = (int)(pix / (uint32_t)cd.v.w);
         double u = px, v = py;
         if (cd.jitter) {
@@ -549,35 +550,55 @@ CRT_LANE_KERNEL void k_extend_shade_lane(Ctx C, int bounce, int qsel, int gen) {
     finish_extend(C, bounce, k, h, ls);
 }
 
-// Octet form for the bounces after the first: lane 0 of the octet carries
-// the shading.  Those rays leave the body more often than camera rays hit
-// it, so saving the path-state round trip and a launch outweighs the
-// one-lane-in-eight shading; camera rays traced by octets use k_trace +
-// k_shade instead.
-__global__ __launch_bounds__(TRACE_BLOCK) void k_extend_shade_octet(Ctx C, int bounce, int qsel) {
+// Octet form for the bounces after the first.  Each octet traces R paths
+// one after another (R = 1..8, by launch size) and lane p of the octet then
+// shades path p, so a wave carries 8R paths through the shading and one
+// atomic per wave appends them.  Those rays leave the body more often than
+// camera rays hit it; camera rays traced by octets use k_trace + k_shade.
+__global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8)))
+void k_extend_shade_octet(Ctx C, int bounce, int qsel, int R) {
     OCTET_SETUP(C)
-    const uint32_t q = blockIdx.x * OCTETS_PER_BLOCK + local;
-    if (q >= C.cnt[qsel]) return;   // octet-uniform
-    const uint32_t k = C.queue[qsel][q];
-    const float4 o = C.st[k].o, d = C.st[k].d;
-    const RayPre r = make_ray_pre(o.x, o.y, o.z, d.x, d.y, d.z);
-    const HitRec h = trace_octet<false>(C.s, r, 0.0f, __int_as_float(0x7f800000), sref, sd, sub, lane_base);
-    const uint32_t ls = wave_append(h.slot >= 0 && sub == 0, C.cnt + 3);   // the hit is octet-uniform
-    if (sub == 0) finish_extend(C, bounce, k, h, ls);
+    const uint32_t q0 = (blockIdx.x * OCTETS_PER_BLOCK + local) * (uint32_t)R;
+    const uint32_t n = C.cnt[qsel];
+    uint32_t k_mine = 0;
+    HitRec h_mine;
+    h_mine.slot = -1;
+    bool mine = false;
+#pragma unroll 1
+    for (int p = 0; p < R; ++p) {
+        const uint32_t q = q0 + (uint32_t)p;
+        if (q >= n) break;   // octet-uniform
+        const uint32_t k = C.queue[qsel][q];
+        const float4 o = C.st[k].o, d = C.st[k].d;
+        const RayPre r = make_ray_pre(o.x, o.y, o.z, d.x, d.y, d.z);
+        const HitRec h = trace_octet<false>(C.s, r, 0.0f, __int_as_float(0x7f800000), sref, sd, sub, lane_base);
+        if (sub == p) {   // the hit is octet-uniform
+            k_mine = k;
+            h_mine = h;
+            mine = true;
+        }
+    }
+    const uint32_t ls = wave_append(mine && h_mine.slot >= 0, C.cnt + 3);
+    if (mine) finish_extend(C, bounce, k_mine, h_mine, ls);
 }
 
-__global__ __launch_bounds__(TRACE_BLOCK) void k_shadow_accum(Ctx C, int bounce, int qsel, int next) {
+// One quad traces the shadow rays of R live paths (R = 1..4, by launch
+// size) one after another; lane p of the quad then accumulates path p.
+__global__ __launch_bounds__(TRACE_BLOCK) void k_shadow_accum(Ctx C, int bounce, int qsel, int next, int R) {
     QUAD_SETUP(C)
-    const uint32_t q = blockIdx.x * (TRACE_BLOCK / QUAD) + qlocal;
+    const uint32_t q0 = (blockIdx.x * (TRACE_BLOCK / QUAD) + qlocal) * (uint32_t)R;
     const uint32_t n = C.cnt[3];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         C.tally[0] += C.cnt[qsel];
         C.tally[1] += n * (uint32_t)C.nl;
     }
-    bool cont = false;
-    uint32_t k = 0;
-    if (q < n) {   // quad-uniform
-        k = C.live[q];
+    float lr_mine[3] = {0.f, 0.f, 0.f};
+    uint32_t k_mine = 0;
+    bool mine = false;
+#pragma unroll 1
+    for (int p = 0; p < R; ++p) {
+        const uint32_t q = q0 + (uint32_t)p;
+        if (q >= n) break;   // quad-uniform
         float lr[3] = {0.f, 0.f, 0.f};
         for (int l = 0; l < C.nl; ++l) {
             const uint32_t i = q * (uint32_t)C.nl + (uint32_t)l;
@@ -590,10 +611,15 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow_accum(Ctx C, int bounce,
                 lr[0] += c.x; lr[1] += c.y; lr[2] += c.z;
             }
         }
-        if (qj == 0) cont = accumulate_path(C, bounce, k, lr);
+        if (qj == p) {
+            k_mine = C.live[q];
+            lr_mine[0] = lr[0]; lr_mine[1] = lr[1]; lr_mine[2] = lr[2];
+            mine = true;
+        }
     }
+    const bool cont = mine && accumulate_path(C, bounce, k_mine, lr_mine);
     const uint32_t slot = wave_append(cont, C.cnt + next);
-    if (cont) C.queue[next][slot] = k;
+    if (cont) C.queue[next][slot] = k_mine;
 }
 
 __global__ __launch_bounds__(TB) void k_zero_samples(Ctx C) {
@@ -609,8 +635,10 @@ __global__ __launch_bounds__(TB) void k_fold(Ctx C) {
     const uint32_t si = seg_search(ctl.nseg, t, [&](uint32_t i) { return (uint64_t)C.segs[i].fold_begin; });
     const Segment sg = C.segs[si];
     const uint32_t i = t - sg.fold_begin;
-    const uint32_t pix = sg.plist_off == NO_LIST ? i : C.plist[sg.plist_off + i];
-    const uint32_t g = C.cams[sg.cam].pix_base + pix;
+    const CamDev& cd = C.cams[sg.cam];
+    const uint32_t pix = sg.plist_off == NO_LIST ? tile_order(i, (uint32_t)cd.v.w, (uint32_t)cd.v.h)
+                                                 : C.plist[sg.plist_off + i];
+    const uint32_t g = cd.pix_base + pix;
     float4 m = C.mean[g];
     int used = C.spp[g];
     bool act = true;
@@ -678,7 +706,9 @@ void launch_pass(const Ctx& C, uint64_t cap_paths, uint32_t cap_fold, const crt_
     if (fused) {
         const bool gen = C.nb > 0 && lane_at(0);
         if (!gen && C.nb > 0) hipLaunchKernelGGL(k_raygen, dim3(gp), dim3(TB), 0, st, C);
-        const uint32_t gq = std::max(1u, cdiv(cap_paths, TRACE_BLOCK / QUAD));
+        // paths per octet / quad: enough groups for ~32k waves, at most one path per lane
+        const int ro = cap_paths >= (1u << 21) ? 8 : cap_paths >= (1u << 20) ? 4 : cap_paths >= (1u << 19) ? 2 : 1;
+        const int rq = cap_paths >= (1u << 21) ? 4 : cap_paths >= (1u << 20) ? 2 : 1;
         for (int b = 0; b < C.nb; ++b) {
             const int cur = b & 1, nxt = cur ^ 1;
             hipLaunchKernelGGL(k_clear, dim3(1), dim3(32), 0, st, C.cnt, (1u << nxt) | (1u << 3));
@@ -686,12 +716,14 @@ void launch_pass(const Ctx& C, uint64_t cap_paths, uint32_t cap_fold, const crt_
                 hipLaunchKernelGGL(k_extend_shade_lane, dim3(std::max(1u, cdiv(cap_paths, LANE_BLOCK))),
                                    dim3(LANE_BLOCK), lane_lds_bytes(C.s), st, C, b, cur, (int)(gen && b == 0));
             else if (b > 0 && B.separate_kernels != 2)
-                hipLaunchKernelGGL(k_extend_shade_octet, dim3(go), dim3(TRACE_BLOCK), lds, st, C, b, cur);
+                hipLaunchKernelGGL(k_extend_shade_octet, dim3(std::max(1u, cdiv(cap_paths, OCTETS_PER_BLOCK * ro))),
+                                   dim3(TRACE_BLOCK), lds, st, C, b, cur, ro);
             else {
                 hipLaunchKernelGGL(k_trace, dim3(go), dim3(TRACE_BLOCK), lds, st, C, cur);
                 hipLaunchKernelGGL(k_shade, dim3(gp), dim3(TB), 0, st, C, b, cur);
             }
-            hipLaunchKernelGGL(k_shadow_accum, dim3(gq), dim3(TRACE_BLOCK), qlds, st, C, b, cur, nxt);
+            hipLaunchKernelGGL(k_shadow_accum, dim3(std::max(1u, cdiv(cap_paths, (TRACE_BLOCK / QUAD) * rq))),
+                               dim3(TRACE_BLOCK), qlds, st, C, b, cur, nxt, rq);
         }
         if (C.nb == 0) hipLaunchKernelGGL(k_zero_samples, dim3(gp), dim3(TB), 0, st, C);
         hipLaunchKernelGGL(k_fold, dim3(std::max(1u, cdiv(cap_fold, TB))), dim3(TB), 0, st, C);

@@ -152,13 +152,15 @@ struct FluxParams {
 // Block partials: one wave of 64 rays per block; 7 sums each (Fx,Fy,Fz,
 // Tx,Ty,Tz, hits), reduced across the wave with shuffles, then combined
 // across blocks with TwoSum compensation.
-CRT_LANE_KERNEL void k_flux(SceneView s, FluxParams F, Comp* __restrict__ partial,
-                                                      uint64_t n) {
+CRT_LANE_KERNEL void k_flux(SceneView s, FluxParams F, Comp* __restrict__ partial) {
     const LaneStack stk = lane_stack(s);
-    const uint64_t q = (uint64_t)blockIdx.x * LANE_BLOCK + threadIdx.x;
+    // one 8 x 8 tile of the grid per wave (tiles row-major): a wave's
+    // parallel rays cover a compact patch and share BVH nodes
+    const uint32_t tiles = (F.grid + 7u) / 8u;
+    const uint32_t i = (blockIdx.x % tiles) * 8u + (threadIdx.x & 7u);
+    const uint32_t j = (blockIdx.x / tiles) * 8u + (threadIdx.x >> 3);
     double term[7] = {0, 0, 0, 0, 0, 0, 0};
-    if (q < n) {
-        const uint32_t i = (uint32_t)(q % F.grid), j = (uint32_t)(q / F.grid);
+    if (i < F.grid && j < F.grid) {
         const double x = -F.radius + (i + 0.5) * F.h, y = -F.radius + (j + 0.5) * F.h;
         double o[3], d[3];
         for (int k = 0; k < 3; ++k) {
@@ -378,14 +380,14 @@ void radiation_pressure_impl(crt_scene* scene, const crt_flux_opts* opts, double
         F.rho_d = opts->rho_d;
         F.origin_back = 2.0 * F.radius;
         const uint64_t n = (uint64_t)F.grid * F.grid;
-        const uint32_t nblocks = cdiv(n, LANE_BLOCK);
+        const uint32_t nblocks = cdiv(F.grid, 8) * cdiv(F.grid, 8);   // one 8 x 8 tile per block
         ws.h.reserve((size_t)nblocks * 7 * sizeof(Comp));
         ws.counters.reserve(256);
         double* out = ws.counters.as<double>();
         Timer tm;
         CRT_HIP_CHECK(hipEventRecord(tm.a, st));
         hipLaunchKernelGGL(k_flux, dim3(nblocks), dim3(LANE_BLOCK), lane_lds_bytes(scene->view), st, scene->view, F,
-                           ws.h.as<Comp>(), n);
+                           ws.h.as<Comp>());
         hipLaunchKernelGGL(k_flux_final, dim3(7), dim3(256), 0, st, ws.h.as<Comp>(), nblocks, out);
         CRT_HIP_CHECK(hipGetLastError());
         CRT_HIP_CHECK(hipEventRecord(tm.b, st));
