@@ -49,11 +49,13 @@ def algorithmic_bytes(stats):
     return bc, ba
 
 
-def cpu_baseline(mesh, cam, sun, budget_s=12.0):
+def cpu_baseline(mesh, cam, sun, budget_s=12.0, gpu_depth=None, gpu_tri=None):
     """Time a bounded sample of the same workload on the host cores.
 
     Preferred: the reference tracer itself (oracle/_ref, built from the
-    reference sources); otherwise the FP64 oracle restatement."""
+    reference sources); otherwise the FP64 oracle restatement.  With the GPU
+    pass's depth / triangle outputs (host arrays, [H, W]) it also states the
+    GPU result's error against that FP64 CPU tracer on the first sample."""
     import oracle
     ref = None
     try:
@@ -67,24 +69,65 @@ def cpu_baseline(mesh, cam, sun, budget_s=12.0):
     if ref is not None:
         g = ref.Group.from_arrays(mesh.vertices, mesh.indices)
         kind, threads = "reference", ref.num_threads()
-        trace = lambda oo, dd: g.trace(oo, dd)
+        trace = lambda oo, dd: g.trace(oo, dd)[:2]          # (triangle, t)
     else:
         s = oracle.Scene(mesh.vertices, mesh.indices)
         kind, threads = "port", oracle.num_threads()
-        trace = lambda oo, dd: s.trace(oo, dd)
+        trace = lambda oo, dd: s.trace(oo, dd)[1::-1]       # (triangle, t)
     n = 1 << 16
     total_rays = 0
     t_sum = 0.0
+    err = None
     while t_sum < budget_s:
         sel = rng.integers(0, o.shape[0], size=n)
         t0 = time.perf_counter()
-        trace(o[sel], d[sel])
+        res = trace(o[sel], d[sel])
         t_sum += time.perf_counter() - t0
+        if err is None and gpu_depth is not None and gpu_tri is not None:
+            err = _error_vs_cpu(res, sel, d, gpu_depth.ravel(), gpu_tri.ravel())
         total_rays += n
         n = min(n * 2, 1 << 20)
-    return {"value": total_rays / t_sum / 1e6, "unit": "Mrays/s", "cores": int(threads), "kind": kind,
-            "sample": "%d primary rays drawn uniformly (with replacement) from the 2048^2 camera of the same "
-                      "workload, closest hit, cached BVH (build not timed), %.1f s of tracing" % (total_rays, t_sum)}
+    out = {"value": total_rays / t_sum / 1e6, "unit": "Mrays/s", "cores": int(threads), "kind": kind,
+           "sample": "%d primary rays drawn uniformly (with replacement) from the 2048^2 camera of the same "
+                     "workload, closest hit, cached BVH (build not timed), %.1f s of tracing" % (total_rays, t_sum)}
+    if err is not None:
+        out["gpu_error_vs_cpu"] = err
+    return out
+
+
+def _error_vs_cpu(res, sel, d, gpu_depth, gpu_tri):
+    """GPU closest hits (FP32 traversal, FP64 refinement) against the FP64 CPU
+    tracer on the same rays: triangle-id disagreement and the relative
+    hit-distance error where both report the same triangle."""
+    prim, t = np.asarray(res[0]), np.asarray(res[1])
+    cpu_tri = np.where(prim >= 0, prim, -1).astype(np.int64)
+    g_tri = gpu_tri[sel].astype(np.int64)
+    same = g_tri == cpu_tri
+    hit = same & (cpu_tri >= 0)
+    dist = t[hit] * np.linalg.norm(d[sel][hit], axis=1)
+    rel = np.abs(gpu_depth[sel][hit] - dist) / dist
+    return {"rays": int(sel.size), "hits": int(hit.sum()), "id_mismatch_frac": float(1.0 - same.mean()),
+            "depth_max_rel_err": float(rel.max()) if rel.size else 0.0,
+            "depth_mean_rel_err": float(rel.mean()) if rel.size else 0.0}
+
+
+def flux_cpu_baseline(mesh, scene, sun, grid=2048, P=4.56e-6, rho_s=0.3, rho_d=0.6):
+    """The FP64 oracle's radiation pressure over the same ray grid (its own
+    FP64 traversal on the host cores), timed, and the GPU result's force /
+    torque error against it (compensated FP64 sums on both sides)."""
+    import oracle
+    info = scene.info()
+    ctr, rad = info["sphere"][:3], info["sphere"][3]
+    F, T, nh, _ = scene.radiation_pressure(sun, P, rho_s, rho_d, grid, ctr, rad, (0, 0, 0))
+    cs = oracle.Scene(mesh.vertices, mesh.indices)
+    t0 = time.perf_counter()
+    Fc, Tc, nhc = cs.radiation_pressure(sun, P, rho_s, rho_d, grid, ctr, rad, (0, 0, 0))
+    dt = time.perf_counter() - t0
+    return {"value": grid * grid / dt / 1e6, "unit": "Mrays/s", "cores": int(oracle.num_threads()), "kind": "port",
+            "sample": "the full %d^2 flux grid of the workload (FP64 oracle traversal, BVH build not timed)" % grid,
+            "gpu_error_vs_cpu": {"force_rel_err": float(np.linalg.norm(F - Fc) / np.linalg.norm(Fc)),
+                                 "torque_rel_err": float(np.linalg.norm(T - Tc) / np.linalg.norm(Tc)),
+                                 "hits_gpu": int(nh), "hits_cpu": int(nhc)}}
 
 
 def run_batched(args, capi, scenes):
@@ -169,7 +212,12 @@ def run_lidar_srp(args, capi, scenes):
         frays += tf["rays_traced"]
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline: FP64 oracle flux over the same 2048^2 grid")
+        cpu = flux_cpu_baseline(mesh, scene, sun)
     return {"metric": "LiDAR + radiation-pressure Mrays/s (1024^2 x 9 flash LiDAR + 2048^2 flux grid, L8 body)",
+            "cpu_baseline": cpu,
             "value": (lrays + frays) / el / 1e6, "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": el * 1e3 / args.steps, "higher_is_better": True, "dtype": "f32",
             "data": "synthetic",
@@ -344,7 +392,7 @@ def main():
     if not args.no_cpu_baseline:
         log("cpu baseline")
         try:
-            cpu = cpu_baseline(mesh, cam, sun)
+            cpu = cpu_baseline(mesh, cam, sun, gpu_depth=depth.cpu().numpy(), gpu_tri=tri.cpu().numpy())
         except Exception as e:
             log("cpu baseline failed: %r" % (e,))
     line = {
