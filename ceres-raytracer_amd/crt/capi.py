@@ -316,18 +316,19 @@ class Scene:
                                      ptr(v), flags, None, C.byref(tm)))
         return t, tri, u, v
 
-    def sun_pass(self, cam: dict, sun_dir, outputs=None, device=False, stream=None):
+    def sun_pass(self, cam: dict, sun_dir, outputs=None, device=False, stream=None, timing=True):
         """Primary closest hit + one sun shadow ray per hit.  With device=True,
         `outputs` holds device addresses {depth, tri, lit} (ints) and nothing is
-        copied back."""
+        copied back; with timing=False as well the call only enqueues the work
+        on `stream` (no synchronisation) and returns None."""
         c = make_camera(cam)
         s = (C.c_double * 3)(*[float(x) for x in sun_dir])
         tm = Timing()
         if device:
             o = outputs or {}
             _check(load().crt_sun_pass(self.h, C.byref(c), s, ptr(o.get("depth")), ptr(o.get("tri")),
-                                       ptr(o.get("lit")), CRT_OUT_DEVICE, stream, C.byref(tm)))
-            return tm.as_dict()
+                                       ptr(o.get("lit")), CRT_OUT_DEVICE, stream, C.byref(tm) if timing else None))
+            return tm.as_dict() if timing else None
         w, h = int(np.floor(cam["resolution"][0])), int(np.floor(cam["resolution"][1]))
         depth = np.zeros((h, w))
         tri = np.zeros((h, w), dtype=np.int32)

@@ -225,15 +225,20 @@ __global__ __launch_bounds__(256) void k_flux_final(const Comp* __restrict__ par
     if (threadIdx.x == 0) out[k] = red[0].s + red[0].c;
 }
 
+// Events only when the caller asked for timing (see EventTimer in trace.hip).
 struct Timer {
     hipEvent_t a = nullptr, b = nullptr;
-    Timer() {
+    explicit Timer(bool on) {
+        if (!on) return;
         CRT_HIP_CHECK(hipEventCreate(&a));
         CRT_HIP_CHECK(hipEventCreate(&b));
     }
     ~Timer() {
-        (void)hipEventDestroy(a);
-        (void)hipEventDestroy(b);
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+    static void rec(hipEvent_t e, hipStream_t st) {
+        if (e) CRT_HIP_CHECK(hipEventRecord(e, st));
     }
     double ms() const {
         float f = 0.f;
@@ -273,14 +278,14 @@ int crt_lidar_batch(crt_scene* scene, uint64_t n, const double* positions, const
             ws.g.reserve(n * 8);
             dout = ws.g.as<double>();
         }
-        Timer tm;
-        CRT_HIP_CHECK(hipEventRecord(tm.a, st));
+        Timer tm(timing != nullptr);
+        Timer::rec(tm.a, st);
         hipLaunchKernelGGL(k_lidar_batch, dim3(cdiv(n, LANE_BLOCK)), dim3(LANE_BLOCK),
                            lane_lds_bytes(scene->view), st, scene->view, dp, dr, z_positive ? 1.0 : -1.0, dout, n);
         CRT_HIP_CHECK(hipGetLastError());
-        CRT_HIP_CHECK(hipEventRecord(tm.b, st));
+        Timer::rec(tm.b, st);
         if (!dev) CRT_HIP_CHECK(hipMemcpyAsync(ranges, dout, n * 8, hipMemcpyDeviceToHost, st));
-        CRT_HIP_CHECK(hipStreamSynchronize(st));
+        if (!dev || timing) CRT_HIP_CHECK(hipStreamSynchronize(st));
         if (timing) {
             std::memset(timing, 0, sizeof(*timing));
             timing->trace_ms = timing->total_ms = tm.ms();
@@ -322,21 +327,21 @@ void flash_lidar_impl(crt_scene* scene, const crt_flash_opts* opts, double* sing
             d_last = last ? reinterpret_cast<double*>(base + nb * 16) : nullptr;
             d_n = n_returns ? reinterpret_cast<uint8_t*>(base + nb * 24) : nullptr;
         }
-        Timer tm;
-        CRT_HIP_CHECK(hipEventRecord(tm.a, st));
+        Timer tm(timing != nullptr);
+        Timer::rec(tm.a, st);
         hipLaunchKernelGGL(k_flash, dim3(cdiv(nr, LANE_BLOCK)), dim3(LANE_BLOCK), lane_lds_bytes(scene->view),
                            st, scene->view, F, ws.f.as<double>(), nr);
         hipLaunchKernelGGL(k_flash_reduce, dim3(cdiv(nb, 256)), dim3(256), 0, st, ws.f.as<double>(), F.sub_rays, nb,
                            d_single, d_first, d_last, d_n);
         CRT_HIP_CHECK(hipGetLastError());
-        CRT_HIP_CHECK(hipEventRecord(tm.b, st));
+        Timer::rec(tm.b, st);
         if (!dev) {
             if (single) CRT_HIP_CHECK(hipMemcpyAsync(single, d_single, nb * 8, hipMemcpyDeviceToHost, st));
             if (first) CRT_HIP_CHECK(hipMemcpyAsync(first, d_first, nb * 8, hipMemcpyDeviceToHost, st));
             if (last) CRT_HIP_CHECK(hipMemcpyAsync(last, d_last, nb * 8, hipMemcpyDeviceToHost, st));
             if (n_returns) CRT_HIP_CHECK(hipMemcpyAsync(n_returns, d_n, nb, hipMemcpyDeviceToHost, st));
         }
-        CRT_HIP_CHECK(hipStreamSynchronize(st));
+        if (!dev || timing) CRT_HIP_CHECK(hipStreamSynchronize(st));
         if (timing) {
             std::memset(timing, 0, sizeof(*timing));
             timing->trace_ms = timing->total_ms = tm.ms();
@@ -384,13 +389,13 @@ void radiation_pressure_impl(crt_scene* scene, const crt_flux_opts* opts, double
         ws.h.reserve((size_t)nblocks * 7 * sizeof(Comp));
         ws.counters.reserve(256);
         double* out = ws.counters.as<double>();
-        Timer tm;
-        CRT_HIP_CHECK(hipEventRecord(tm.a, st));
+        Timer tm(timing != nullptr);
+        Timer::rec(tm.a, st);
         hipLaunchKernelGGL(k_flux, dim3(nblocks), dim3(LANE_BLOCK), lane_lds_bytes(scene->view), st, scene->view, F,
                            ws.h.as<Comp>());
         hipLaunchKernelGGL(k_flux_final, dim3(7), dim3(256), 0, st, ws.h.as<Comp>(), nblocks, out);
         CRT_HIP_CHECK(hipGetLastError());
-        CRT_HIP_CHECK(hipEventRecord(tm.b, st));
+        Timer::rec(tm.b, st);
         double h[7];
         CRT_HIP_CHECK(hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, st));
         CRT_HIP_CHECK(hipStreamSynchronize(st));

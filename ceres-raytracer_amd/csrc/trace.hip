@@ -314,19 +314,25 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow_sun(SceneView s, const S
     if (j == 0 && lit) lit[__float_as_uint(o.w)] = occluded ? 0 : 1;
 }
 
+// Events only when the caller asked for timing: a call with device outputs
+// and no timing only enqueues work (no events, no synchronisation).
 struct EventTimer {
     hipEvent_t a = nullptr, b = nullptr, c = nullptr, d = nullptr;
-    EventTimer() {
+    explicit EventTimer(bool on) {
+        if (!on) return;
         CRT_HIP_CHECK(hipEventCreate(&a));
         CRT_HIP_CHECK(hipEventCreate(&b));
         CRT_HIP_CHECK(hipEventCreate(&c));
         CRT_HIP_CHECK(hipEventCreate(&d));
     }
     ~EventTimer() {
-        (void)hipEventDestroy(a);
-        (void)hipEventDestroy(b);
-        (void)hipEventDestroy(c);
-        (void)hipEventDestroy(d);
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+        if (c) (void)hipEventDestroy(c);
+        if (d) (void)hipEventDestroy(d);
+    }
+    static void rec(hipEvent_t e, hipStream_t st) {
+        if (e) CRT_HIP_CHECK(hipEventRecord(e, st));
     }
     static double ms(hipEvent_t x, hipEvent_t y) {
         float f = 0.f;
@@ -532,13 +538,13 @@ int crt_pass(crt_scene* scene, const crt_camera* camera, double* xyz, double* no
         if (instance) { if (dev) o.instance = instance; else { ws.c.reserve(npix * 4); o.instance = ws.c.as<uint32_t>(); } }
         if (depth) { if (dev) o.depth = depth; else { ws.d.reserve(npix * 8); o.depth = ws.d.as<double>(); } }
         if (tri_id) { if (dev) o.tri = tri_id; else { ws.e.reserve(npix * 4); o.tri = ws.e.as<int32_t>(); } }
-        EventTimer tm;
+        EventTimer tm(timing != nullptr);
         const uint32_t blocks = cdiv((uint64_t)cam.w, PIX_TILE) * cdiv((uint64_t)cam.h, PIX_TILE);
-        CRT_HIP_CHECK(hipEventRecord(tm.a, st));
+        EventTimer::rec(tm.a, st);
         hipLaunchKernelGGL(k_camera_pass, dim3(blocks), dim3(LANE_BLOCK), lane_lds_bytes(scene->view), st,
                            scene->view, cam, o);
         CRT_HIP_CHECK(hipGetLastError());
-        CRT_HIP_CHECK(hipEventRecord(tm.b, st));
+        EventTimer::rec(tm.b, st);
         if (!dev) {
             if (xyz) CRT_HIP_CHECK(hipMemcpyAsync(xyz, o.xyz, npix * 24, hipMemcpyDeviceToHost, st));
             if (normal) CRT_HIP_CHECK(hipMemcpyAsync(normal, o.normal, npix * 24, hipMemcpyDeviceToHost, st));
@@ -546,7 +552,7 @@ int crt_pass(crt_scene* scene, const crt_camera* camera, double* xyz, double* no
             if (depth) CRT_HIP_CHECK(hipMemcpyAsync(depth, o.depth, npix * 8, hipMemcpyDeviceToHost, st));
             if (tri_id) CRT_HIP_CHECK(hipMemcpyAsync(tri_id, o.tri, npix * 4, hipMemcpyDeviceToHost, st));
         }
-        CRT_HIP_CHECK(hipStreamSynchronize(st));
+        if (!dev || timing) CRT_HIP_CHECK(hipStreamSynchronize(st));
         if (timing) {
             std::memset(timing, 0, sizeof(*timing));
             timing->trace_ms = EventTimer::ms(tm.a, tm.b);
@@ -589,9 +595,9 @@ int crt_trace_rays(crt_scene* scene, uint64_t n, const double* origins, const do
             out.v = v ? reinterpret_cast<double*>(ob + n * 16) : nullptr;
             out.tri = tri ? reinterpret_cast<int32_t*>(ob + n * 24) : nullptr;
         }
-        EventTimer tm;
+        EventTimer tm(timing != nullptr);
         const uint32_t blocks = cdiv(n, LANE_BLOCK);
-        CRT_HIP_CHECK(hipEventRecord(tm.a, st));
+        EventTimer::rec(tm.a, st);
         if (any)
             hipLaunchKernelGGL(k_trace_rays<true>, dim3(blocks), dim3(LANE_BLOCK), lane_lds_bytes(scene->view), st,
                                scene->view, in, out, n);
@@ -599,14 +605,14 @@ int crt_trace_rays(crt_scene* scene, uint64_t n, const double* origins, const do
             hipLaunchKernelGGL(k_trace_rays<false>, dim3(blocks), dim3(LANE_BLOCK), lane_lds_bytes(scene->view), st,
                                scene->view, in, out, n);
         CRT_HIP_CHECK(hipGetLastError());
-        CRT_HIP_CHECK(hipEventRecord(tm.b, st));
+        EventTimer::rec(tm.b, st);
         if (!dev) {
             if (t) CRT_HIP_CHECK(hipMemcpyAsync(t, out.t, n * 8, hipMemcpyDeviceToHost, st));
             if (u) CRT_HIP_CHECK(hipMemcpyAsync(u, out.u, n * 8, hipMemcpyDeviceToHost, st));
             if (v) CRT_HIP_CHECK(hipMemcpyAsync(v, out.v, n * 8, hipMemcpyDeviceToHost, st));
             if (tri) CRT_HIP_CHECK(hipMemcpyAsync(tri, out.tri, n * 4, hipMemcpyDeviceToHost, st));
         }
-        CRT_HIP_CHECK(hipStreamSynchronize(st));
+        if (!dev || timing) CRT_HIP_CHECK(hipStreamSynchronize(st));
         if (timing) {
             std::memset(timing, 0, sizeof(*timing));
             const double ms = EventTimer::ms(tm.a, tm.b);
@@ -644,27 +650,27 @@ int crt_sun_pass(crt_scene* scene, const crt_camera* camera, const double sun_di
         ws.counters.reserve(256);
         uint32_t* qcount = ws.counters.as<uint32_t>();
         const float offset = self_offset(scene);
-        EventTimer tm;
+        EventTimer tm(timing != nullptr);
         const uint32_t blocks = cdiv((uint64_t)cam.w, PIX_TILE) * cdiv((uint64_t)cam.h, PIX_TILE);
         CRT_HIP_CHECK(hipMemsetAsync(qcount, 0, 4, st));
-        CRT_HIP_CHECK(hipEventRecord(tm.a, st));
+        EventTimer::rec(tm.a, st);
         hipLaunchKernelGGL(k_primary_sun, dim3(blocks), dim3(LANE_BLOCK), lane_lds_bytes(scene->view), st,
                            scene->view, cam, d_depth, d_tri, d_lit, ws.h.as<ShadowRay>(),
                            shadow_slots(ws.h.as<ShadowRay>(), npix), qcount, offset);
-        CRT_HIP_CHECK(hipEventRecord(tm.b, st));
+        EventTimer::rec(tm.b, st);
         hipLaunchKernelGGL(k_shadow_sun, dim3(cdiv(npix, TRACE_BLOCK / QUAD)), dim3(TRACE_BLOCK),
                            quad_lds_bytes(scene->view), st, scene->view, ws.h.as<ShadowRay>(),
                            shadow_slots(ws.h.as<ShadowRay>(), npix), qcount, sun, d_lit);
         CRT_HIP_CHECK(hipGetLastError());
-        CRT_HIP_CHECK(hipEventRecord(tm.c, st));
+        EventTimer::rec(tm.c, st);
         uint32_t nshadow = 0;
-        CRT_HIP_CHECK(hipMemcpyAsync(&nshadow, qcount, 4, hipMemcpyDeviceToHost, st));
+        if (timing) CRT_HIP_CHECK(hipMemcpyAsync(&nshadow, qcount, 4, hipMemcpyDeviceToHost, st));
         if (!dev) {
             if (depth) CRT_HIP_CHECK(hipMemcpyAsync(depth, d_depth, npix * 8, hipMemcpyDeviceToHost, st));
             if (tri_id) CRT_HIP_CHECK(hipMemcpyAsync(tri_id, d_tri, npix * 4, hipMemcpyDeviceToHost, st));
             if (lit) CRT_HIP_CHECK(hipMemcpyAsync(lit, d_lit, npix, hipMemcpyDeviceToHost, st));
         }
-        CRT_HIP_CHECK(hipStreamSynchronize(st));
+        if (!dev || timing) CRT_HIP_CHECK(hipStreamSynchronize(st));
         if (timing) {
             std::memset(timing, 0, sizeof(*timing));
             timing->trace_ms = EventTimer::ms(tm.a, tm.b);

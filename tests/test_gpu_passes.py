@@ -156,6 +156,49 @@ def test_sun_pass_shadows(icosphere):
     assert np.all(r["lit"][~hit] == 0)
 
 
+def _hip():
+    """hipMalloc / hipMemcpy / hipFree / hipDeviceSynchronize of the HIP
+    runtime libcrt_hip.so is bound to (looked up through its handle)."""
+    import ctypes as C
+    L = capi.load()
+    L.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    L.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    L.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+    L.hipFree.argtypes = [C.c_void_p]
+    return L
+
+
+def test_sun_pass_device_outputs_enqueue_only(icosphere):
+    # device outputs without timing: the call only enqueues work on the
+    # stream; several passes back to back, one synchronisation, then the
+    # buffers must hold exactly the host-output result
+    import ctypes as C
+    gpu, _, m = icosphere
+    n = 192
+    cam = scenes.small_body_camera(m, resolution=n, seed=1)
+    sun = scenes.sun_direction(2)
+    ref = gpu.sun_pass(cam, sun)
+    L = _hip()
+    bufs = {}
+    for name, size in (("depth", 8), ("tri", 4), ("lit", 1)):
+        p = C.c_void_p()
+        assert L.hipMalloc(C.byref(p), n * n * size) == 0
+        assert L.hipMemset(p, 0x5A, n * n * size) == 0
+        bufs[name] = p
+    try:
+        outs = {k: v.value for k, v in bufs.items()}
+        for _ in range(4):
+            assert gpu.sun_pass(cam, sun, outs, device=True, stream=None, timing=False) is None
+        assert L.hipDeviceSynchronize() == 0
+        got = {"depth": np.zeros((n, n)), "tri": np.zeros((n, n), np.int32), "lit": np.zeros((n, n), np.uint8)}
+        for k, a in got.items():
+            assert L.hipMemcpy(a.ctypes.data, bufs[k], a.nbytes, 2) == 0   # device to host
+            assert np.array_equal(a, ref[k]), k
+    finally:
+        for p in bufs.values():
+            L.hipFree(p)
+
+
 def test_empty_scene_rejected():
     with pytest.raises(ValueError):
         capi.Scene(np.zeros((0, 3), np.float32), np.zeros((0, 3), np.uint32))
