@@ -185,22 +185,19 @@ def run_lidar_srp(args, capi, scenes):
     ray) sun-direction flux grid (rho_s 0.3, rho_d 0.6, compensated FP64
     reduction).  Outputs stay on the device; one step = one scan + one flux
     integration."""
-    import torch
     mesh = scenes.displaced_icosphere(8, 2000.0, 0.2, 6, 0)
     scene = capi.Scene(mesh.vertices, mesh.indices)
     beams = scenes.small_body_camera(mesh, resolution=1024, seed=3)
     sun = scenes.sun_direction(4)
-    dev = torch.device("cuda:0")
     nb = 1024 * 1024
-    outs = {"single": torch.empty(nb, dtype=torch.float64, device=dev).data_ptr(),
-            "first": torch.empty(nb, dtype=torch.float64, device=dev).data_ptr(),
-            "last": torch.empty(nb, dtype=torch.float64, device=dev).data_ptr(),
-            "n_returns": torch.empty(nb, dtype=torch.uint8, device=dev).data_ptr()}
-    stream = torch.cuda.current_stream().cuda_stream
+    bufs = {"single": capi.DeviceBuffer(nb * 8), "first": capi.DeviceBuffer(nb * 8), "last": capi.DeviceBuffer(nb * 8),
+            "n_returns": capi.DeviceBuffer(nb)}
+    outs = {k: b.ptr for k, b in bufs.items()}
+    stream = None   # the library runtime's null stream
     for _ in range(args.warmup):
         scene.flash_lidar(beams, 8, 5e-4, 4, outputs=outs, device=True, stream=stream)
         scene.radiation_pressure(sun, 4.56e-6, 0.3, 0.6, 2048, stream=stream)
-    torch.cuda.synchronize()
+    capi.device_synchronize()
     lid_ms, flux_ms, lrays, frays = [], [], 0, 0
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -210,7 +207,7 @@ def run_lidar_srp(args, capi, scenes):
         flux_ms.append(tf["trace_ms"])
         lrays += tl["rays_traced"]
         frays += tf["rays_traced"]
-    torch.cuda.synchronize()
+    capi.device_synchronize()
     el = time.perf_counter() - t0
     cpu = None
     if not args.no_cpu_baseline:
@@ -313,17 +310,16 @@ def main():
               "--gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
-    import torch
     from crt import capi, scenes
 
     if args.copy_calibration:
-        n = 1 << 28   # 2^28 float32 = 1 GiB read + 1 GiB written per copy
-        x = torch.ones(n, dtype=torch.float32, device="cuda:0")
-        y = torch.empty_like(x)
+        nbytes = 1 << 30   # 1 GiB read + 1 GiB written per device-to-device copy (a blit kernel)
+        x, y = capi.DeviceBuffer(nbytes), capi.DeviceBuffer(nbytes)
+        x.fill(1)
         for _ in range(5):
-            y.copy_(x)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "copies": 5}))
+            capi.device_copy(y, x)
+        capi.device_synchronize()
+        print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "copies": 5}))
         return
 
     if args.workload != "c2":
@@ -345,16 +341,14 @@ def main():
     log("BVH: %d wide nodes, %d leaves, depth %d, build %.2f ms, SAH %.2f" % (
         info["n_wide_nodes"], info["n_leaves"], info["max_depth"], info["build_ms"], info["sah_cost"]))
     H = W = args.resolution
-    dev = torch.device("cuda:0")
-    depth = torch.empty((H, W), dtype=torch.float64, device=dev)
-    tri = torch.empty((H, W), dtype=torch.int32, device=dev)
-    lit = torch.empty((H, W), dtype=torch.uint8, device=dev)
-    outs = {"depth": depth.data_ptr(), "tri": tri.data_ptr(), "lit": lit.data_ptr()}
-    stream = torch.cuda.current_stream().cuda_stream
+    # device outputs and stream of the HIP runtime the library uses
+    depth, tri, lit = capi.DeviceBuffer(H * W * 8), capi.DeviceBuffer(H * W * 4), capi.DeviceBuffer(H * W)
+    outs = {"depth": depth.ptr, "tri": tri.ptr, "lit": lit.ptr}
+    stream = None   # the library runtime's null stream
 
     for _ in range(args.warmup):
         scene.sun_pass(cam, sun, outs, device=True, stream=stream)
-    torch.cuda.synchronize()
+    capi.device_synchronize()
     # per-kernel times (hipEvents on the launch stream) and the ray counts of
     # one pass: the same kernels, timed call by call
     trace_ms, shadow_ms = [], []
@@ -363,14 +357,16 @@ def main():
         trace_ms.append(tm["trace_ms"])
         shadow_ms.append(tm["shadow_ms"])
     prim_rays, sh_rays = tm["rays_traced"], tm["shadow_rays"]
-    torch.cuda.synchronize()
+    capi.device_synchronize()
     # whole-job throughput: K passes enqueued back to back on the stream
-    # (device outputs, no per-call timing or synchronisation)
+    # (device outputs, no per-call timing or synchronisation), bracketed by
+    # device synchronisations of the library's runtime
     log("timing %d steps" % args.steps)
+    capi.device_synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         scene.sun_pass(cam, sun, outs, device=True, stream=stream, timing=False)
-    torch.cuda.synchronize()
+    capi.device_synchronize()
     elapsed = time.perf_counter() - t0
     ms_per_step = elapsed * 1e3 / args.steps
     value = (prim_rays + sh_rays) * args.steps / elapsed / 1e6
@@ -396,7 +392,8 @@ def main():
     if not args.no_cpu_baseline:
         log("cpu baseline")
         try:
-            cpu = cpu_baseline(mesh, cam, sun, gpu_depth=depth.cpu().numpy(), gpu_tri=tri.cpu().numpy())
+            cpu = cpu_baseline(mesh, cam, sun, gpu_depth=depth.to_host(np.empty((H, W))),
+                               gpu_tri=tri.to_host(np.empty((H, W), np.int32)))
         except Exception as e:
             log("cpu baseline failed: %r" % (e,))
     line = {

@@ -41,11 +41,17 @@ CANDIDATES = [
 
 
 def _device_name():
+    """Name of device 0 from the HIP runtime the library uses."""
+    import ctypes as C
     try:
-        import torch
-        return torch.cuda.get_device_name(0)
+        L = capi._hip()
+        buf = C.create_string_buffer(256)
+        L.hipDeviceGetName.argtypes = [C.c_char_p, C.c_int, C.c_int]
+        if L.hipDeviceGetName(buf, 256, 0) == 0:
+            return buf.value.decode(errors="replace")
     except Exception:
-        return "unknown"
+        pass
+    return "unknown"
 
 
 def _key(scene: capi.Scene, bounces, n_lights):

@@ -189,7 +189,72 @@ def ptr(a):
     if isinstance(a, np.ndarray):
         assert a.flags["C_CONTIGUOUS"]
         return a.ctypes.data_as(C.c_void_p)
-    return C.c_void_p(int(a))   # raw device address (e.g. torch tensor.data_ptr())
+    return C.c_void_p(int(a))   # raw device address (e.g. DeviceBuffer.ptr)
+
+
+def _hip():
+    """The HIP runtime libcrt_hip.so is bound to (symbols looked up through
+    the library's handle), so the device buffers and synchronisation below
+    are those of the runtime its entry points use."""
+    L = load()
+    if not getattr(L, "_hip_ready", False):
+        L.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        L.hipFree.argtypes = [C.c_void_p]
+        L.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        L.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+        L.hipDeviceSynchronize.argtypes = []
+        L._hip_ready = True
+    return L
+
+
+class DeviceBuffer:
+    """Device memory for CRT_OUT_DEVICE outputs (hipMalloc of the library's
+    runtime).  `.ptr` is the device address; `.to_host(array)` copies back."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        if _hip().hipMalloc(C.byref(p), C.c_size_t(max(self.nbytes, 1))) != 0:
+            raise MemoryError("hipMalloc of %d bytes failed" % self.nbytes)
+        self._p = p
+
+    @property
+    def ptr(self) -> int:
+        return self._p.value
+
+    def fill(self, byte: int):
+        if _hip().hipMemset(self._p, int(byte), C.c_size_t(self.nbytes)) != 0:
+            raise RuntimeError("hipMemset failed")
+
+    def to_host(self, out: np.ndarray) -> np.ndarray:
+        assert out.nbytes <= self.nbytes and out.flags["C_CONTIGUOUS"]
+        if _hip().hipMemcpy(out.ctypes.data, self._p, C.c_size_t(out.nbytes), 2) != 0:   # device to host
+            raise RuntimeError("hipMemcpy failed")
+        return out
+
+    def free(self):
+        if self._p.value:
+            _hip().hipFree(self._p)
+            self._p = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def device_copy(dst: DeviceBuffer, src: DeviceBuffer):
+    """Device-to-device copy of src into dst (the runtime's blit kernel)."""
+    n = min(dst.nbytes, src.nbytes)
+    if _hip().hipMemcpy(C.c_void_p(dst.ptr), C.c_void_p(src.ptr), C.c_size_t(n), 3) != 0:
+        raise RuntimeError("hipMemcpy failed")
+
+
+def device_synchronize():
+    """Wait for all work the library has enqueued on the device."""
+    if _hip().hipDeviceSynchronize() != 0:
+        raise RuntimeError("hipDeviceSynchronize failed")
 
 
 def make_camera(cam: dict) -> Camera:
