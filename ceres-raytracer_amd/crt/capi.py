@@ -159,6 +159,7 @@ def load(path: str = LIB_PATH):
 
 
 BUILD_PLOC = 1   # crt_build_opts.flags: PLOC clustering instead of the Morton-order LBVH
+BUILD_GREEDY_WIDE = 2   # crt_build_opts.flags: greedy collapse into 8-wide nodes instead of the SAH-optimal DP
 
 
 def _check(code):
@@ -283,7 +284,7 @@ class Scene:
     """A mesh + GPU-built BVH resident in HBM (crt_scene_create)."""
 
     def __init__(self, vertices, indices, tri_instance=None, vertex_normals=None, colors=None, smooth=None,
-                 max_leaf_size=8, ploc=False):
+                 max_leaf_size=8, ploc=False, greedy_wide=False):
         init()
         v = np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1, 3)
         i = np.ascontiguousarray(indices, dtype=np.uint32).reshape(-1, 3)
@@ -294,7 +295,7 @@ class Scene:
         if col is not None:
             ni = max(ni, col.shape[0])
         sm = None if smooth is None else np.ascontiguousarray(smooth, dtype=np.uint8)
-        opts = BuildOpts(max_leaf_size, BUILD_PLOC if ploc else 0)
+        opts = BuildOpts(max_leaf_size, (BUILD_PLOC if ploc else 0) | (BUILD_GREEDY_WIDE if greedy_wide else 0))
         h = C.c_void_p()
         _check(load().crt_scene_create(ptr(v), v.shape[0], ptr(i), i.shape[0], ptr(inst), ptr(vn), ni, ptr(col),
                                        ptr(sm), C.byref(opts), C.byref(h)))

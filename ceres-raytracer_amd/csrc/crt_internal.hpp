@@ -103,7 +103,7 @@ struct BuildResult {
 // GPU BVH build: LBVH (63-bit Morton codes, LDS radix sort, Karras
 // hierarchy, bottom-up refit) collapsed into 8-wide nodes.
 void build_bvh(const float* d_vertices, const uint32_t* d_indices, uint32_t n_tris,
-               uint32_t max_leaf, bool use_ploc, hipStream_t stream, BuildResult& out);
+               uint32_t max_leaf, uint32_t build_flags, hipStream_t stream, BuildResult& out);
 
 // Shared helpers of the trace / render / sensor translation units.
 // Camera in the scene frame of `s` (position moved by the scene origin).

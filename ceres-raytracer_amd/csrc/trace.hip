@@ -489,8 +489,9 @@ int crt_scene_create(const float* vertices, uint32_t n_vertices, const uint32_t*
         }
         const uint32_t max_leaf = opts && opts->max_leaf_size ? opts->max_leaf_size : 8u;
         CRT_REQUIRE(max_leaf <= 8u, CRT_ERR_INVALID, "max_leaf_size must be in 1..8");
-        const bool ploc = opts && (opts->flags & CRT_BUILD_PLOC);
-        build_bvh(s->vertices.as<float>(), s->indices.as<uint32_t>(), n_tris, max_leaf, ploc, nullptr, s->bvh);
+        const uint32_t bflags = opts ? opts->flags : 0u;
+        CRT_REQUIRE((bflags & ~(CRT_BUILD_PLOC | CRT_BUILD_GREEDY_WIDE)) == 0u, CRT_ERR_INVALID, "unknown build flags");
+        build_bvh(s->vertices.as<float>(), s->indices.as<uint32_t>(), n_tris, max_leaf, bflags, nullptr, s->bvh);
         crt_bvh_info& inf = s->info;
         inf.n_tris = n_tris;
         inf.n_wide_nodes = s->bvh.n_nodes;

@@ -14,9 +14,9 @@
  *
  * Memory: unless a flag says otherwise, output pointers are HOST pointers and
  * the call copies results back before returning.  With CRT_OUT_DEVICE the
- * output pointers are device pointers (e.g. from hipMalloc or a torch CUDA
- * tensor) and the call only enqueues work on `stream` (hipStream_t, may be
- * NULL for the null stream).
+ * output pointers are device pointers (hipMalloc of the HIP runtime this
+ * library is linked against) and, when `timing` is NULL, the call only
+ * enqueues work on `stream` (hipStream_t, may be NULL for the null stream).
  *
  * Threads: calls on one scene run one at a time (a per-scene lock); they
  * reuse the scene's grow-only device workspace, so with CRT_OUT_DEVICE
@@ -96,10 +96,12 @@ typedef struct {
 
 #define CRT_BUILD_PLOC 1u     /* binary tree by locally-ordered clustering (PLOC) instead of LBVH:
                                  lower SAH, ~3x the build time */
+#define CRT_BUILD_GREEDY_WIDE 2u /* collapse the binary tree into 8-wide nodes greedily (largest box
+                                    first) instead of by SAH-optimal dynamic programming (default) */
 
 typedef struct {
     uint32_t max_leaf_size;   /* triangles per leaf, 1..8 (default 8)  */
-    uint32_t flags;           /* CRT_BUILD_*; 0 = LBVH (Morton radix tree, default) */
+    uint32_t flags;           /* CRT_BUILD_*; 0 = LBVH (Morton radix tree) + SAH-optimal collapse */
 } crt_build_opts;
 
 typedef struct {

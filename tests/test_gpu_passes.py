@@ -223,7 +223,9 @@ def test_ploc_builder_matches(icosphere, cornell):
     # with the default LBVH scene except at exact ties / edge grazes
     gpu, cpu, m = icosphere
     alt = capi.Scene(m.vertices, m.indices, ploc=True)
-    assert alt.info()["sah_cost"] < gpu.info()["sah_cost"]
+    g_greedy = capi.Scene(m.vertices, m.indices, greedy_wide=True)
+    a_greedy = capi.Scene(m.vertices, m.indices, ploc=True, greedy_wide=True)
+    assert a_greedy.info()["sah_cost"] < g_greedy.info()["sah_cost"]
     cam = scenes.small_body_camera(m, resolution=512, seed=1)
     a, b = gpu.camera_pass(cam), alt.camera_pass(cam)
     mism = a["tri"] != b["tri"]
@@ -247,3 +249,67 @@ def test_ploc_builder_matches(icosphere, cornell):
         sel = mism.ravel()
         ok = c.tie_flags(o[sel], d[sel]) | c.edge_flags(o[sel], d[sel], pa["tri"].ravel()[sel])
         assert ok.all()
+
+
+@pytest.mark.parametrize("ploc", [False, True])
+def test_greedy_collapse_matches(icosphere, cornell, ploc):
+    # the greedy collapse (CRT_BUILD_GREEDY_WIDE) and the default SAH-optimal
+    # (DP) collapse change only the wide tree: closest hits, sun shadows and
+    # any-hit queries agree except at exact ties / edge grazes, and every
+    # triangle stays reachable
+    gpu, cpu, m = icosphere
+    alt = capi.Scene(m.vertices, m.indices, ploc=ploc, greedy_wide=True)
+    inf = alt.info()
+    assert inf["n_wide_nodes"] >= 1 and inf["max_depth"] >= 1
+    cam = scenes.small_body_camera(m, resolution=512, seed=1)
+    a, b = gpu.camera_pass(cam), alt.camera_pass(cam)
+    mism = a["tri"] != b["tri"]
+    if mism.any():
+        o, d = oracle.camera_rays(cam)
+        sel = mism.ravel()
+        ok = cpu.tie_flags(o[sel], d[sel]) | cpu.edge_flags(o[sel], d[sel], b["tri"].ravel()[sel])
+        assert ok.all()
+    hit = (~mism) & (a["tri"] >= 0)
+    assert np.max(np.abs(a["depth"][hit] - b["depth"][hit]) / a["depth"][hit]) < REL
+    sun = scenes.sun_direction(2)
+    sa, sb = gpu.sun_pass(cam, sun), alt.sun_pass(cam, sun)
+    same = (sa["tri"] == sb["tri"]) & (sa["tri"] >= 0)
+    assert np.mean(sa["lit"][same] != sb["lit"][same]) < 1e-3
+    # any-hit queries from inside the body outwards: every ray is blocked
+    rng = np.random.default_rng(3)
+    dirs = rng.normal(size=(4096, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    t, tri, _, _ = alt.trace(np.zeros((4096, 3)), dirs, any_hit=True)
+    assert np.all(tri >= 0)
+    # multi-instance scene
+    g, c, v, i = cornell
+    sc = scenes.cornell_box(0)
+    _, _, inst = sc.merged()
+    alt = capi.Scene(v, i, inst, ploc=ploc, greedy_wide=True)
+    cam = scenes.cornell_camera()
+    pa, pc = alt.camera_pass(cam), c.camera_pass(cam)
+    mism = pa["tri"] != pc["tri"]
+    if mism.any():
+        o, d = oracle.camera_rays(cam)
+        sel = mism.ravel()
+        ok = c.tie_flags(o[sel], d[sel]) | c.edge_flags(o[sel], d[sel], pa["tri"].ravel()[sel])
+        assert ok.all()
+
+
+def test_collapse_small_scenes():
+    # 1, 2, 5, 9 and 17 triangles: roots that are a binary leaf, a single
+    # leaf range, or need more than one wide level
+    rng = np.random.default_rng(11)
+    for nt in (1, 2, 5, 9, 17):
+        v = rng.uniform(-1, 1, size=(3 * nt, 3)).astype(np.float32)
+        idx = np.arange(3 * nt, dtype=np.uint32).reshape(nt, 3)
+        a = capi.Scene(v, idx, greedy_wide=True)
+        b = capi.Scene(v, idx)
+        o = rng.uniform(-3, 3, size=(2048, 3))
+        d = rng.normal(size=(2048, 3))
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        ta, ia, _, _ = a.trace(o, d)
+        tb, ib, _, _ = b.trace(o, d)
+        assert np.array_equal(ia >= 0, ib >= 0), nt
+        both = (ia >= 0) & (ib >= 0)
+        assert np.allclose(ta[both], tb[both], rtol=1e-6), nt
