@@ -146,6 +146,65 @@ def test_flash_lidar(icosphere):
     assert np.median(spread) > 0
 
 
+def _flash_subrays(cam, seed, sub_rays, sigma):
+    """Sub-ray directions of every beam restated in FP64 numpy: boresight d
+    of pixel (x, y), tangent frame e1 = d x a (a = x axis, or y when
+    |d.x| > 0.9), e2 = d x e1, Gaussian angles by Box-Muller from the
+    counter RNG u01(seed, beam, k, 0/1), direction d + tan(s g1) e1 +
+    tan(s g2) e2 normalised.  Returns (origins, dirs) of shape (beams, K, 3)."""
+    o, d = oracle.camera_rays(cam)
+    nb = d.shape[0]
+    ax = np.zeros_like(d)
+    ax[:, 0] = 1.0
+    big = np.abs(d[:, 0]) > 0.9
+    ax[big] = [0.0, 1.0, 0.0]
+    e1 = np.cross(d, ax)
+    e1 /= np.linalg.norm(e1, axis=1, keepdims=True)
+    e2 = np.cross(d, e1)
+    dirs = np.empty((nb, sub_rays, 3))
+    for b in range(nb):
+        for k in range(1, sub_rays + 1):
+            r1, r2 = oracle.u01(seed, b, k, 0), oracle.u01(seed, b, k, 1)
+            rad = math.sqrt(-2.0 * math.log(1.0 - r1))
+            a1 = math.tan(sigma * rad * math.cos(2.0 * math.pi * r2))
+            a2 = math.tan(sigma * rad * math.sin(2.0 * math.pi * r2))
+            v = d[b] + a1 * e1[b] + a2 * e2[b]
+            dirs[b, k - 1] = v / np.linalg.norm(v)
+    return np.repeat(o[:, None, :], sub_rays, axis=1), dirs
+
+
+def test_flash_lidar_returns_match_oracle(icosphere):
+    # every sub-ray traced by the FP64 oracle: first / last return = min /
+    # max range over the sub-rays that hit, n_returns = their count; beams
+    # whose sub-rays graze an edge or vertex (a barycentric within 1e-5 of 0)
+    # may legitimately differ and are counted apart
+    m, gpu, cpu = icosphere
+    K, sigma, seed = 8, 5e-4, 4
+    beams = scenes.small_body_camera(m, resolution=48, seed=3)
+    out = gpu.flash_lidar(beams, sub_rays=K, sigma=sigma, seed=seed)
+    o, d = _flash_subrays(beams, seed, K, sigma)
+    t, tri, u, v = cpu.trace(o.reshape(-1, 3), d.reshape(-1, 3))
+    t, tri = t.reshape(-1, K), tri.reshape(-1, K)
+    w = 1.0 - u - v
+    graze = ((np.minimum(np.minimum(u, v), w) < 1e-5) & (tri.ravel() >= 0)).reshape(-1, K).any(axis=1)
+    hit = tri >= 0
+    rng = np.where(hit, t, np.nan)   # unit directions: range = t
+    n_c = hit.sum(axis=1)
+    first_c = np.where(n_c > 0, np.nanmin(np.where(hit, rng, np.inf), axis=1), 0.0)
+    last_c = np.where(n_c > 0, np.nanmax(np.where(hit, rng, -np.inf), axis=1), 0.0)
+    n_g = out["n_returns"].ravel().astype(int)
+    fg, lg = out["first"].ravel(), out["last"].ravel()
+    ok = ~graze
+    print("flash beams %d, with returns %d, grazing %d" % (n_c.size, int((n_c > 0).sum()), int(graze.sum())))
+    bad_n = (n_g != n_c) & ok
+    assert bad_n.sum() <= max(1, 2e-3 * n_c.size), int(bad_n.sum())
+    both = ok & (n_c > 0) & (n_g == n_c)
+    assert both.sum() > 0.3 * n_c.size
+    assert np.max(np.abs(fg[both] - first_c[both]) / first_c[both]) < 1e-5
+    assert np.max(np.abs(lg[both] - last_c[both]) / last_c[both]) < 1e-5
+    assert np.all(fg[n_g == 0] == 0) and np.all(lg[n_g == 0] == 0)
+
+
 def test_radiation_pressure_matches_oracle(icosphere):
     m, gpu, cpu = icosphere
     sun = scenes.sun_direction(2)
