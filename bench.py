@@ -130,6 +130,67 @@ def flux_cpu_baseline(mesh, scene, sun, grid=2048, P=4.56e-6, rho_s=0.3, rho_d=0
                                  "hits_gpu": int(nh), "hits_cpu": int(nhc)}}
 
 
+def run_cornell(args, capi, scenes):
+    """The reference's CPU-runnable case: the 36-triangle Cornell box, 256x256
+    pinhole camera (60 deg), one 0.25 x 0.25 area light, 16 spp, 4 bounces.
+    One step = the plain render() (uint8 + float radiance to host memory) plus
+    the intersection / normal / instance passes (host outputs), as a user of
+    the drop-in API calls them.  CPU baseline: the reference renderer and
+    passes themselves (oracle/_ref) on the host cores, with the GPU passes'
+    id agreement and the radiance means beside it."""
+    sc = scenes.cornell_box(0)
+    v, i, inst = sc.merged()
+    scene = capi.Scene(v, i, inst, colors=np.array(sc.colors))
+    cam = scenes.cornell_camera()
+    light = dict(scenes.cornell_light(), type="area")
+    for k in range(args.warmup):
+        scene.render(cam, [light], 16, 16, 0.0, 4, seed=k)
+        scene.camera_pass(cam)
+    rays, rend_ms = 0, []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        rgba, u8, _, tm = scene.render(cam, [light], 16, 16, 0.0, 4, seed=1000 + k)
+        p = scene.camera_pass(cam)
+        rays += tm["rays_traced"] + tm["shadow_rays"] + 256 * 256
+        rend_ms.append(tm["total_ms"])
+    el = time.perf_counter() - t0
+    out = {"metric": "Cornell box render + passes Mrays/s (36 tris, 256x256, 16 spp, 4 bounces)",
+           "value": rays / el / 1e6, "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": el * 1e3 / args.steps, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "C1: Cornell box (seed 0), 256x256 camera, area light 0.25x0.25, render 16 spp "
+                                  "4 bounces (uint8 + radiance to host) + xyz/normal/instance passes (host)",
+                      "rays_per_step": rays // args.steps},
+           "render_device_ms": float(np.mean(rend_ms))}
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cornell_cpu_baseline(sc, cam, light, u8, p)
+    return out
+
+
+def cornell_cpu_baseline(sc, cam, light, gpu_u8, gpu_pass):
+    """The reference's own render() and passes (oracle/_ref, OpenMP on the
+    host cores) on the same scene: time per step, and the GPU result beside
+    it (instance-id agreement of the passes; image mean of the uint8 render,
+    which is random on both sides: the reference seeds from random_device)."""
+    from oracle import ref as refmod
+    if not refmod.available():
+        return None
+    g = refmod.Group.from_scene(sc)
+    runs, t_sum, img, rp = 0, 0.0, None, None
+    while t_sum < 10.0 and runs < 5:
+        t0 = time.perf_counter()
+        img = g.render_u8(cam, [light], 16, 16, 0.0, 4)
+        rp = g.passes(cam)
+        t_sum += time.perf_counter() - t0
+        runs += 1
+    gi = gpu_pass["instance"]
+    return {"value": runs / t_sum, "unit": "steps/s", "cores": int(refmod.num_threads()), "kind": "reference",
+            "sample": "%d full steps (render 256x256 16 spp 4 bounces + passes), %.1f s" % (runs, t_sum),
+            "ms_per_step": t_sum * 1e3 / runs,
+            "gpu_vs_reference": {"instance_mismatch_frac": float(np.mean(gi != rp["instance"])),
+                                 "u8_image_mean_gpu": float(gpu_u8[..., :3].mean()),
+                                 "u8_image_mean_reference": float(img[..., :3].mean())}}
+
+
 def run_batched(args, capi, scenes):
     """256 cameras on the L8 body, sides 2^U(6,10) px, FP64 poses at 20-500 km,
     8 spp, 3 bounces, directional sun: one batched render (autotuned launch
@@ -295,8 +356,9 @@ def main():
     ap.add_argument("--resolution", type=int, default=2048)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--out", default=None, help="also write the JSON line to this file")
-    ap.add_argument("--workload", default="c2", choices=["c2", "lidar_srp", "batched", "served"],
+    ap.add_argument("--workload", default="c2", choices=["c2", "cornell", "lidar_srp", "batched", "served"],
                     help="c2 (headline, default): closest hit + sun shadow on the 1.31M-tri body; "
+                         "cornell: the reference's CPU-runnable Cornell case; lidar_srp: flash LiDAR + flux; "
                          "batched: 256-camera path tracing; served: mixed request stream")
     ap.add_argument("--cameras", type=int, default=256)
     ap.add_argument("--requests", type=int, default=300)
@@ -323,7 +385,8 @@ def main():
         return
 
     if args.workload != "c2":
-        run = {"lidar_srp": run_lidar_srp, "batched": run_batched, "served": run_served}[args.workload]
+        run = {"cornell": run_cornell, "lidar_srp": run_lidar_srp, "batched": run_batched,
+               "served": run_served}[args.workload]
         line = run(args, capi, scenes)
         print(json.dumps(line), flush=True)
         if args.out:
