@@ -203,10 +203,10 @@ __device__ inline bool tri_test(const RayPre& r, float4 a, float4 b, float4 c,
 }
 
 // ------------------------------------------------------------------------
-// Octet traversal.  All 8 lanes of an octet call this with the same ray and
-// stay converged inside (every branch below is octet-uniform).  `stk_ref` /
-// `stk_t` point at this octet's LDS stack of `cap` entries.
-// ANY_HIT: returns at the first accepted hit (shadow rays), hit.slot >= 0.
+// Octet traversal (closest hit).  All 8 lanes of an octet call this with the
+// same ray and stay converged inside (every branch below is octet-uniform).
+// `stk_ref` / `stk_t` point at this octet's LDS stack of `cap` entries.
+// (Any-hit rays use trace_quad_any.)
 // ------------------------------------------------------------------------
 // Dynamic LDS of a traversal block: 32 stacks of refs, then 32 stacks of
 // tnear (stack_cap entries each).
@@ -215,7 +215,6 @@ __device__ inline uint32_t* traversal_lds(const SceneView&) {
     return crt_dyn_lds;
 }
 
-template <bool ANY_HIT>
 __device__ inline HitRec trace_octet(const SceneView& s, const RayPre& r, float tmin, float tmax,
                                      uint32_t* stk_ref, float* stk_t, int sub, int lane_base) {
     HitRec hit;
@@ -241,30 +240,23 @@ __device__ inline HitRec trace_octet(const SceneView& s, const RayPre& r, float 
             const uint32_t m = (uint32_t)((__ballot(h) >> lane_base) & 0xFFull);
             if (m != 0u) {
                 const int nh = __popc(m);
-                int rank, first_lane;
-                if (ANY_HIT) {
-                    // any hit: order does not matter, rank by lane
-                    rank = __popc(m & ((1u << sub) - 1u));
-                    first_lane = __ffs(m) - 1;
-                } else {
-                    // rank among the hit children by tnear (>= 0, so its bits
-                    // without the sign order like the value; the low 3 bits
-                    // carry the lane)
-                    const uint32_t key = h ? ((__float_as_uint(tnear) & 0x7FFFFFF8u) | (uint32_t)sub) : 0xFFFFFFFFu;
-                    const uint32_t k7 = dpp<DPP_HALF_MIRROR>(key);
-                    const uint32_t k1 = dpp<DPP_XOR1>(key), k2 = dpp<DPP_XOR2>(key), k3 = dpp<DPP_XOR3>(key);
-                    const uint32_t k4 = dpp<DPP_XOR3>(k7), k5 = dpp<DPP_XOR2>(k7), k6 = dpp<DPP_XOR1>(k7);
-                    rank = (int)(k1 < key) + (int)(k2 < key) + (int)(k3 < key) + (int)(k4 < key) +
-                           (int)(k5 < key) + (int)(k6 < key) + (int)(k7 < key);
-                    const uint32_t kmin = min(min(min(key, k1), min(k2, k3)), min(min(k4, k5), min(k6, k7)));
-                    first_lane = (int)(kmin & 7u);
-                }
+                // rank among the hit children by tnear (>= 0, so its bits
+                // without the sign order like the value; the low 3 bits
+                // carry the lane)
+                const uint32_t key = h ? ((__float_as_uint(tnear) & 0x7FFFFFF8u) | (uint32_t)sub) : 0xFFFFFFFFu;
+                const uint32_t k7 = dpp<DPP_HALF_MIRROR>(key);
+                const uint32_t k1 = dpp<DPP_XOR1>(key), k2 = dpp<DPP_XOR2>(key), k3 = dpp<DPP_XOR3>(key);
+                const uint32_t k4 = dpp<DPP_XOR3>(k7), k5 = dpp<DPP_XOR2>(k7), k6 = dpp<DPP_XOR1>(k7);
+                const int rank = (int)(k1 < key) + (int)(k2 < key) + (int)(k3 < key) + (int)(k4 < key) +
+                                 (int)(k5 < key) + (int)(k6 < key) + (int)(k7 < key);
+                const uint32_t kmin = min(min(min(key, k1), min(k2, k3)), min(min(k4, k5), min(k6, k7)));
+                const int first_lane = (int)(kmin & 7u);
                 const uint32_t next = (uint32_t)__shfl((int)ref, lane_base + first_lane, 64);
                 if (h && rank > 0) {
                     const uint32_t pos = sp + (uint32_t)(nh - 1 - rank);
                     if (pos < cap) {
                         stk_ref[pos] = ref;
-                        if (!ANY_HIT) stk_t[pos] = tnear;   // any hit never tightens the bound
+                        stk_t[pos] = tnear;
                     }
                 }
                 sp += (uint32_t)(nh - 1);
@@ -286,14 +278,7 @@ __device__ inline HitRec trace_octet(const SceneView& s, const RayPre& r, float 
                 if (!ok) t = __int_as_float(0x7f800000);
             }
             const uint32_t mh = (uint32_t)((__ballot(ok) >> lane_base) & 0xFFull);
-            if (ANY_HIT) {
-                if (mh != 0u) {
-                    const int l = __ffs(mh) - 1;
-                    hit.t = __shfl(t, lane_base + l, 64);
-                    hit.slot = (int)first + l;
-                    return hit;
-                }
-            } else if (mh != 0u) {
+            if (mh != 0u) {
                 // octet min-reduce of (t, lane); exact ties go to the later
                 // triangle, like the "last tested wins" rule of
                 // lib/bvh/single_ray_traverser.hpp:55-60
@@ -319,11 +304,6 @@ __device__ inline HitRec trace_octet(const SceneView& s, const RayPre& r, float 
             }
         }
         // ---- pop the next entry that can still hold a closer hit ----
-        if (ANY_HIT) {
-            if (sp == 0u) break;
-            cur = stk_ref[--sp];
-            continue;
-        }
         bool found = false;
         while (sp > 0u) {
             --sp;

@@ -15,23 +15,27 @@
 //
 // MI355X structure.  Work is organised in *passes*; a pass is a list of
 // segments (camera, pixels, samples [s_first, s_first + s_count)) and every
-// (pixel, sample) of every segment is one path.  One pass runs
-//   k_raygen   FP64 camera ray (jittered), rebased, FP32 path state; paths
-//              whose ray misses the scene's bounding sphere are not queued
-//   per bounce: k_trace / k_extend_lane (octet or lane per path, closest
-//              hit) -> k_shade (thread per path: light samples, shadow rays,
-//              next direction) -> k_shadow_quad (quad per shadow ray, any
-//              hit) -> k_accumulate (thread per
-//              path; surviving paths are compacted into the next queue with a
-//              wave ballot + one atomic per wave)
-//   k_fold     per pixel: folds the pass's samples into the running mean in
-//              sample order and applies the adaptive stop.
-// All kernels read their sizes from device memory, so a pass can be captured
-// once in a hipGraph and replayed (the host only rewrites the segment table
-// between passes).  Counters are cleared by a kernel, never by a memset node.
-// The traversal kernels exist in two forms: one octet / quad per queue
-// entry, or persistent (a fixed grid whose octets / quads claim `grab`
-// entries per atomic).
+// (pixel, sample) of every segment is one path; a camera's paths follow the
+// 8x8-tile pixel order of tile_order with a pixel's samples adjacent.  The
+// default (fused) pass runs, per bounce:
+//   k_extend_shade_lane   camera rays of bounce 0 made in the kernel, one
+//                         lane per path: closest hit + shading (light
+//                         samples -> shadow records, next direction)
+//   k_extend_shade_octet  later bounces, an octet traces R paths in turn,
+//                         then lane p of the octet shades path p
+//   k_shadow_accum        a quad tests the shadow rays of R live paths in
+//                         turn (origin triangle first), then lane p adds path
+//                         p's clamped direct light and queues it on
+// plus k_fold per pixel at the end of the pass (running mean in sample order,
+// adaptive stop).  Small passes trace camera rays by octets (k_raygen, k_trace,
+// k_shade).  The stage pipeline (separate_kernels, and the persistent forms
+// whose octets / quads claim `grab` entries per atomic) runs k_raygen, then
+// per bounce k_trace / k_extend_lane -> k_shade -> k_tally -> k_shadow_quad ->
+// k_accumulate with the same arithmetic, so both give bit-identical images.
+// Queues are compacted with a wave ballot + one atomic per wave.  All kernels
+// read their sizes from device memory, so a pass can be captured once in a
+// hipGraph and replayed (the host only rewrites the segment table between
+// passes).  Counters are cleared by a kernel, never by a memset node.
 //
 // Randomness is counter-based: u01(seed_c, pixel, sample, 64*bounce + dim)
 // with a per-camera seed, so a camera rendered inside a batch is bit-for-bit
@@ -232,7 +236,7 @@ __global__ __launch_bounds__(TB) void k_raygen(Ctx C) {
 __device__ inline void extend_one(const Ctx& C, uint32_t k, uint32_t* sref, float* sd, int sub, int lane_base) {
     const float4 o = C.st[k].o, d = C.st[k].d;
     const RayPre r = make_ray_pre(o.x, o.y, o.z, d.x, d.y, d.z);
-    const HitRec h = trace_octet<false>(C.s, r, 0.0f, __int_as_float(0x7f800000), sref, sd, sub, lane_base);
+    const HitRec h = trace_octet(C.s, r, 0.0f, __int_as_float(0x7f800000), sref, sd, sub, lane_base);
     if (sub == 0) {
         C.st[k].slot = h.slot;
         C.st[k].hu = h.u;
@@ -571,7 +575,7 @@ void k_extend_shade_octet(Ctx C, int bounce, int qsel, int R) {
         const uint32_t k = C.queue[qsel][q];
         const float4 o = C.st[k].o, d = C.st[k].d;
         const RayPre r = make_ray_pre(o.x, o.y, o.z, d.x, d.y, d.z);
-        const HitRec h = trace_octet<false>(C.s, r, 0.0f, __int_as_float(0x7f800000), sref, sd, sub, lane_base);
+        const HitRec h = trace_octet(C.s, r, 0.0f, __int_as_float(0x7f800000), sref, sd, sub, lane_base);
         if (sub == p) {   // the hit is octet-uniform
             k_mine = k;
             h_mine = h;

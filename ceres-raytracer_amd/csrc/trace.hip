@@ -1,10 +1,10 @@
 // trace.hip -- scene lifetime, camera passes, general ray queries and the
 // primary + sun-shadow pass, exported through the C ABI of crt_hip.h.
 //
-// Every kernel here is "octet per ray": 8 lanes of a wave trace one ray
-// through the 8-wide BVH (crt_device.hpp).  A 256-thread block holds 32
-// octets; each octet owns a slice of the block's dynamic LDS for its
-// traversal stack (stack_cap refs + stack_cap distances).
+// Closest-hit kernels here trace one ray per lane over the compressed wide
+// nodes (trace_lane, one 64-lane wave per block with its group stacks in
+// LDS); the sun pass's shadow rays use one quad per ray over the 256-B nodes
+// (trace_quad_any, 16 rays per wave, 64 quad stacks per 256-thread block).
 #include "crt_internal.hpp"
 
 #include <algorithm>
