@@ -232,12 +232,60 @@ def run_batched(args, capi, scenes):
     same = all(np.array_equal(outb[j][0], scene.render_batch([cams[i]], [sun], spp, spp, 0.0, nb, seeds=[seeds[i]])[0][0][0])
                for j, i in enumerate(sel))
     pix = sum(int(c["resolution"][0]) * int(c["resolution"][1]) for c in cams)
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline: reference renderer on a sample of the cameras; oracle at the same RNG keys")
+        cpu = batched_cpu_baseline(scene, mesh, cams, sun, seeds, spp, nb)
     return {"metric": "batched render Mrays/s (256 cameras, 8 spp, 3 bounces, L8 body)", "value": rays / tb / 1e6,
+            "cpu_baseline": cpu,
             "unit": "Mrays/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
             "config": {"workload": "C4 batched: %d cameras, %d pixels, spp 8, bounces 3, sun" % (len(cams), pix)},
             "batched_ms": tb * 1e3, "batched_ms_runs": [t * 1e3 for t in times], "plain_ms": tp * 1e3,
             "plain_mrays_s": prays / tp / 1e6,
             "speedup_vs_plain": tp / tb, "launch_params": best, "batch_equals_plain": bool(same)}
+
+
+def batched_cpu_baseline(scene, mesh, cams, sun, seeds, spp, nb, budget_s=10.0):
+    """The reference renderer itself (oracle/_ref: do_render's per-pixel loop
+    and unidirectional() over the reference BVH, OpenMP on the host cores) on
+    the batch's cameras in batch order until ~budget_s, with the sun as a
+    PointLight 1e11 m away (the reference has no directional light; the
+    direction then differs by < 1e-7 rad); its rays per path are taken from
+    the GPU render of the same cameras.  Beside it, the GPU render's
+    agreement with the FP64 oracle at identical RNG keys on the batch's
+    smallest camera."""
+    import oracle
+    from oracle import ref as refmod
+    out = None
+    if refmod.available():
+        g = refmod.Group.from_arrays(mesh.vertices, mesh.indices, build_threads=refmod.num_threads())
+        pos = (np.asarray(sun["direction"], float) / np.linalg.norm(sun["direction"]) * 1e11).tolist()
+        light = {"type": "point", "intensity": 1e22, "position": pos}   # irradiance 1 at the body
+        n, paths, t_sum = 0, 0, 0.0
+        while t_sum < budget_s and n < len(cams):
+            t0 = time.perf_counter()
+            g.render_float(cams[n], [light], spp, nb)
+            t_sum += time.perf_counter() - t0
+            paths += int(cams[n]["resolution"][0]) * int(cams[n]["resolution"][1]) * spp
+            n += 1
+        _, tm = scene.render_batch(cams[:n], [sun], spp, spp, 0.0, nb, seeds=seeds[:n], outputs=False)
+        rpp = (tm["rays_traced"] + tm["shadow_rays"]) / max(paths, 1)
+        out = {"value": paths * rpp / t_sum / 1e6, "unit": "Mrays/s", "cores": int(refmod.num_threads()),
+               "kind": "reference",
+               "sample": "the first %d cameras of the batch (%d paths, %d spp, %d bounces), %.1f s of rendering; "
+                         "%.2f rays per path from the GPU render of the same cameras" % (n, paths, spp, nb, t_sum, rpp)}
+    k = int(np.argmin([int(c["resolution"][0]) * int(c["resolution"][1]) for c in cams]))
+    gpu_img = scene.render_batch([cams[k]], [sun], spp, spp, 0.0, nb, seeds=[seeds[k]])[0][0][0][..., :3]
+    cpu_img, _ = oracle.Scene(mesh.vertices, mesh.indices).render(cams[k], [sun], spp, spp, 0.0, nb, seed=seeds[k])
+    diff = np.abs(gpu_img - cpu_img)
+    close = np.all(diff <= 1e-3 * np.maximum(np.abs(cpu_img), 1e-3) + 1e-5, axis=-1)
+    err = {"camera": k, "pixels": int(close.size), "same_key_agreement_frac": float(close.mean()),
+           "image_mean_rel_diff": float(abs(gpu_img.mean() - cpu_img.mean()) / max(cpu_img.mean(), 1e-30))}
+    if out is None:
+        out = {"value": None, "unit": "Mrays/s", "cores": int(oracle.num_threads()), "kind": "port",
+               "sample": "reference tracer not built"}
+    out["gpu_error_vs_cpu"] = err
+    return out
 
 
 def run_lidar_srp(args, capi, scenes):
