@@ -302,20 +302,28 @@ def run_lidar_srp(args, capi, scenes):
     bufs = {"single": capi.DeviceBuffer(nb * 8), "first": capi.DeviceBuffer(nb * 8), "last": capi.DeviceBuffer(nb * 8),
             "n_returns": capi.DeviceBuffer(nb)}
     outs = {k: b.ptr for k, b in bufs.items()}
-    stream = None   # the library runtime's null stream
+    # the scan and the flux integration of a step run on two streams of the
+    # library's runtime and overlap (the flux call returns host results and
+    # waits for its own stream only)
+    s_lid, s_flux = capi.Stream(), capi.Stream()
     for _ in range(args.warmup):
-        scene.flash_lidar(beams, 8, 5e-4, 4, outputs=outs, device=True, stream=stream)
-        scene.radiation_pressure(sun, 4.56e-6, 0.3, 0.6, 2048, stream=stream)
+        scene.flash_lidar(beams, 8, 5e-4, 4, outputs=outs, device=True, stream=s_lid.handle)
+        scene.radiation_pressure(sun, 4.56e-6, 0.3, 0.6, 2048, stream=s_flux.handle)
     capi.device_synchronize()
-    lid_ms, flux_ms, lrays, frays = [], [], 0, 0
-    t0 = time.perf_counter()
+    # per-kernel times (hipEvents on each launch stream), call by call
+    lid_ms, flux_ms = [], []
     for _ in range(args.steps):
-        tl = scene.flash_lidar(beams, 8, 5e-4, 4, outputs=outs, device=True, stream=stream)
-        f, tq, nh, tf = scene.radiation_pressure(sun, 4.56e-6, 0.3, 0.6, 2048, stream=stream)
+        tl = scene.flash_lidar(beams, 8, 5e-4, 4, outputs=outs, device=True, stream=s_lid.handle)
+        f, tq, nh, tf = scene.radiation_pressure(sun, 4.56e-6, 0.3, 0.6, 2048, stream=s_flux.handle)
         lid_ms.append(tl["trace_ms"])
         flux_ms.append(tf["trace_ms"])
-        lrays += tl["rays_traced"]
-        frays += tf["rays_traced"]
+    capi.device_synchronize()
+    lrays, frays = tl["rays_traced"] * args.steps, tf["rays_traced"] * args.steps
+    # whole-job throughput: K steps, scan enqueued without timing
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        scene.flash_lidar(beams, 8, 5e-4, 4, outputs=outs, device=True, stream=s_lid.handle, timing=False)
+        f, tq, nh, _ = scene.radiation_pressure(sun, 4.56e-6, 0.3, 0.6, 2048, stream=s_flux.handle)
     capi.device_synchronize()
     el = time.perf_counter() - t0
     cpu = None
@@ -452,10 +460,15 @@ def main():
     log("BVH: %d wide nodes, %d leaves, depth %d, build %.2f ms, SAH %.2f" % (
         info["n_wide_nodes"], info["n_leaves"], info["max_depth"], info["build_ms"], info["sah_cost"]))
     H = W = args.resolution
-    # device outputs and stream of the HIP runtime the library uses
-    depth, tri, lit = capi.DeviceBuffer(H * W * 8), capi.DeviceBuffer(H * W * 4), capi.DeviceBuffer(H * W)
-    outs = {"depth": depth.ptr, "tri": tri.ptr, "lit": lit.ptr}
-    stream = None   # the library runtime's null stream
+    # device outputs and streams of the HIP runtime the library uses: two
+    # streams, each with its own output buffers; consecutive passes alternate
+    # between them, so one pass's kernels fill the other's tails (the scene
+    # keeps a workspace per stream)
+    streams = [capi.Stream() for _ in range(2)]
+    obufs = [(capi.DeviceBuffer(H * W * 8), capi.DeviceBuffer(H * W * 4), capi.DeviceBuffer(H * W)) for _ in streams]
+    outs_s = [{"depth": d.ptr, "tri": t.ptr, "lit": l.ptr} for d, t, l in obufs]
+    depth, tri, _ = obufs[0]
+    outs, stream = outs_s[0], streams[0].handle
 
     for _ in range(args.warmup):
         scene.sun_pass(cam, sun, outs, device=True, stream=stream)
@@ -469,14 +482,17 @@ def main():
         shadow_ms.append(tm["shadow_ms"])
     prim_rays, sh_rays = tm["rays_traced"], tm["shadow_rays"]
     capi.device_synchronize()
-    # whole-job throughput: K passes enqueued back to back on the stream
-    # (device outputs, no per-call timing or synchronisation), bracketed by
-    # device synchronisations of the library's runtime
+    # whole-job throughput: K passes enqueued back to back, alternating
+    # between the two streams (device outputs, no per-call timing or
+    # synchronisation), bracketed by device synchronisations of the
+    # library's runtime
     log("timing %d steps" % args.steps)
+    for k in range(2):
+        scene.sun_pass(cam, sun, outs_s[k], device=True, stream=streams[k].handle, timing=False)
     capi.device_synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        scene.sun_pass(cam, sun, outs, device=True, stream=stream, timing=False)
+    for k in range(args.steps):
+        scene.sun_pass(cam, sun, outs_s[k & 1], device=True, stream=streams[k & 1].handle, timing=False)
     capi.device_synchronize()
     elapsed = time.perf_counter() - t0
     ms_per_step = elapsed * 1e3 / args.steps

@@ -204,8 +204,43 @@ def _hip():
         L.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
         L.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
         L.hipDeviceSynchronize.argtypes = []
+        L.hipStreamCreate.argtypes = [C.POINTER(C.c_void_p)]
+        L.hipStreamDestroy.argtypes = [C.c_void_p]
+        L.hipStreamSynchronize.argtypes = [C.c_void_p]
         L._hip_ready = True
     return L
+
+
+class Stream:
+    """A HIP stream (default priority) of the library's runtime; `.handle`
+    is the hipStream_t to pass as `stream`.  Calls on one scene with
+    different streams use separate workspaces, so their device work may
+    overlap."""
+
+    def __init__(self):
+        p = C.c_void_p()
+        if _hip().hipStreamCreate(C.byref(p)) != 0:
+            raise RuntimeError("hipStreamCreate failed")
+        self._s = p
+
+    @property
+    def handle(self) -> int:
+        return self._s.value
+
+    def synchronize(self):
+        if _hip().hipStreamSynchronize(self._s) != 0:
+            raise RuntimeError("hipStreamSynchronize failed")
+
+    def destroy(self):
+        if self._s.value:
+            _hip().hipStreamDestroy(self._s)
+            self._s = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
 
 
 class DeviceBuffer:
@@ -426,10 +461,12 @@ class Scene:
                                       None, C.byref(tm)))
         return out
 
-    def flash_lidar(self, beams: dict, sub_rays=8, sigma=5e-4, seed=0, outputs=None, device=False, stream=None):
+    def flash_lidar(self, beams: dict, sub_rays=8, sigma=5e-4, seed=0, outputs=None, device=False, stream=None,
+                    timing=True):
         """Flash LiDAR over a beam grid.  With device=True, `outputs` holds
         device addresses {single, first, last, n_returns} (ints) and only the
-        timing is returned."""
+        timing is returned; with timing=False as well the call only enqueues
+        the work on `stream` and returns None."""
         o = FlashOpts()
         o.beams = make_camera(beams)
         o.sub_rays = int(sub_rays)
@@ -440,8 +477,8 @@ class Scene:
             tm = Timing()
             _check(load().crt_flash_lidar(self.h, C.byref(o), ptr(d.get("single")), ptr(d.get("first")),
                                           ptr(d.get("last")), ptr(d.get("n_returns")), CRT_OUT_DEVICE, stream,
-                                          C.byref(tm)))
-            return tm.as_dict()
+                                          C.byref(tm) if timing else None))
+            return tm.as_dict() if timing else None
         w, h = int(np.floor(beams["resolution"][0])), int(np.floor(beams["resolution"][1]))
         single, first, last = np.zeros((h, w)), np.zeros((h, w)), np.zeros((h, w))
         nret = np.zeros((h, w), dtype=np.uint8)

@@ -2,10 +2,12 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <memory>
 #include <mutex>
 #include <stdexcept>
 #include <string>
 #include <new>
+#include <utility>
 #include <vector>
 
 #include "crt_device.hpp"
@@ -147,13 +149,22 @@ struct crt_scene {
     crt::BuildResult bvh;
     crt::SceneView view{};
     crt_bvh_info info{};
-    // Grow-only device buffers of the scene's own entry points (passes, ray
-    // queries, sensors: ws; renders: rws).  They are reused from call to
-    // call, so those calls take `mu` and run one at a time per scene; the
-    // server's workers bring their own buffers and run concurrently.
-    crt::Workspace ws;
+    // Grow-only device buffers of the scene's own entry points: one
+    // workspace per stream for passes, ray queries and sensors (workspace()),
+    // one render workspace (rws).  Calls take `mu` while they enqueue, so
+    // the host side runs one call at a time per scene; device work of calls
+    // on different streams uses different buffers and may overlap.  Renders
+    // synchronise before they return, so they do not overlap.  The server's
+    // workers bring their own buffers and run concurrently.
+    std::vector<std::pair<void*, std::unique_ptr<crt::Workspace>>> stream_ws;
     crt::RenderWorkspace* rws = nullptr;
     std::mutex mu;
+    crt::Workspace& workspace(void* stream) {   // call with mu held
+        for (auto& e : stream_ws)
+            if (e.first == stream) return *e.second;
+        stream_ws.emplace_back(stream, std::unique_ptr<crt::Workspace>(new crt::Workspace()));
+        return *stream_ws.back().second;
+    }
     crt_scene() = default;
     crt_scene(const crt_scene&) = delete;
     crt_scene& operator=(const crt_scene&) = delete;

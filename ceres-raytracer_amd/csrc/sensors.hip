@@ -264,7 +264,7 @@ int crt_lidar_batch(crt_scene* scene, uint64_t n, const double* positions, const
         hipStream_t st = (hipStream_t)stream;
         const bool dev = (flags & CRT_OUT_DEVICE) != 0;
         std::lock_guard<std::mutex> scene_lock(scene->mu);
-        Workspace& ws = scene->ws;
+        Workspace& ws = scene->workspace(stream);
         const double* dp = positions;
         const double* dr = rotations;
         double* dout = ranges;
@@ -417,7 +417,8 @@ int crt_flash_lidar(crt_scene* scene, const crt_flash_opts* opts, double* single
     return guarded([&] {
         CRT_REQUIRE(scene != nullptr, CRT_ERR_INVALID, "scene is NULL");
         std::lock_guard<std::mutex> scene_lock(scene->mu);
-        flash_lidar_impl(scene, opts, single, first, last, n_returns, flags, stream, timing, scene->ws);
+        flash_lidar_impl(scene, opts, single, first, last, n_returns, flags, stream, timing,
+                         scene->workspace(stream));
     });
 }
 
@@ -426,7 +427,8 @@ int crt_radiation_pressure(crt_scene* scene, const crt_flux_opts* opts, double f
     return guarded([&] {
         CRT_REQUIRE(scene != nullptr, CRT_ERR_INVALID, "scene is NULL");
         std::lock_guard<std::mutex> scene_lock(scene->mu);
-        radiation_pressure_impl(scene, opts, force, torque, n_hits, stream, timing, scene->ws);
+        radiation_pressure_impl(scene, opts, force, torque, n_hits, stream, timing,
+                                scene->workspace(stream));
     });
 }
 

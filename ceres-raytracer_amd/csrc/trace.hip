@@ -532,7 +532,7 @@ int crt_pass(crt_scene* scene, const crt_camera* camera, double* xyz, double* no
         const size_t npix = (size_t)cam.w * cam.h;
         const bool dev = (flags & CRT_OUT_DEVICE) != 0;
         std::lock_guard<std::mutex> scene_lock(scene->mu);
-        Workspace& ws = scene->ws;
+        Workspace& ws = scene->workspace(stream);
         PassOut o{nullptr, nullptr, nullptr, nullptr, nullptr};
         if (xyz) { if (dev) o.xyz = xyz; else { ws.a.reserve(npix * 24); o.xyz = ws.a.as<double>(); } }
         if (normal) { if (dev) o.normal = normal; else { ws.b.reserve(npix * 24); o.normal = ws.b.as<double>(); } }
@@ -575,7 +575,7 @@ int crt_trace_rays(crt_scene* scene, uint64_t n, const double* origins, const do
         const bool dev = (flags & CRT_OUT_DEVICE) != 0;
         const bool any = (flags & CRT_ANY_HIT) != 0;
         std::lock_guard<std::mutex> scene_lock(scene->mu);
-        Workspace& ws = scene->ws;
+        Workspace& ws = scene->workspace(stream);
         RayIn in{origins, dirs, tmin, tmax};
         RayOut out{t, tri, u, v};
         if (!dev) {
@@ -638,7 +638,7 @@ int crt_sun_pass(crt_scene* scene, const crt_camera* camera, const double sun_di
         const size_t npix = (size_t)cam.w * cam.h;
         const bool dev = (flags & CRT_OUT_DEVICE) != 0;
         std::lock_guard<std::mutex> scene_lock(scene->mu);
-        Workspace& ws = scene->ws;
+        Workspace& ws = scene->workspace(stream);
         double* d_depth = depth;
         int32_t* d_tri = tri_id;
         uint8_t* d_lit = lit;

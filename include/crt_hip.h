@@ -18,10 +18,12 @@
  * library is linked against) and, when `timing` is NULL, the call only
  * enqueues work on `stream` (hipStream_t, may be NULL for the null stream).
  *
- * Threads: calls on one scene run one at a time (a per-scene lock); they
- * reuse the scene's grow-only device workspace, so with CRT_OUT_DEVICE
- * successive calls on one scene must use the same stream or synchronise in
- * between.  Different scenes, and the request server's workers (own
+ * Threads and streams: calls on one scene enqueue one at a time (a
+ * per-scene lock).  Each scene keeps one grow-only device workspace per
+ * stream for passes, ray queries and sensors, so device work of calls on
+ * different streams may overlap (e.g. consecutive frames on two streams),
+ * while calls on one stream are ordered by it.  Renders synchronise before
+ * they return.  Different scenes, and the request server's workers (own
  * workspaces and streams), run concurrently.
  *
  * Reference interfaces replaced (ceres-raytracer v0.6a4 paths):

@@ -199,6 +199,31 @@ def test_sun_pass_device_outputs_enqueue_only(icosphere):
             L.hipFree(p)
 
 
+def test_sun_passes_on_two_streams(icosphere):
+    # one scene, passes alternating between two streams with their own
+    # output buffers and no synchronisation in between: the scene keeps a
+    # workspace per stream, so the overlapping passes must not disturb each
+    # other; both buffer sets end with the host-output result
+    gpu, _, m = icosphere
+    n = 160
+    cam = scenes.small_body_camera(m, resolution=n, seed=2)
+    sun = scenes.sun_direction(3)
+    ref = gpu.sun_pass(cam, sun)
+    streams = [capi.Stream(), capi.Stream()]
+    bufs = [(capi.DeviceBuffer(n * n * 8), capi.DeviceBuffer(n * n * 4), capi.DeviceBuffer(n * n)) for _ in streams]
+    for b in bufs:
+        for x in b:
+            x.fill(0x33)
+    outs = [{"depth": d.ptr, "tri": t.ptr, "lit": l.ptr} for d, t, l in bufs]
+    for k in range(8):
+        assert gpu.sun_pass(cam, sun, outs[k & 1], device=True, stream=streams[k & 1].handle, timing=False) is None
+    capi.device_synchronize()
+    for d, t, l in bufs:
+        assert np.array_equal(d.to_host(np.empty((n, n))), ref["depth"])
+        assert np.array_equal(t.to_host(np.empty((n, n), np.int32)), ref["tri"])
+        assert np.array_equal(l.to_host(np.empty((n, n), np.uint8)), ref["lit"])
+
+
 def test_empty_scene_rejected():
     with pytest.raises(ValueError):
         capi.Scene(np.zeros((0, 3), np.float32), np.zeros((0, 3), np.uint32))
