@@ -504,7 +504,7 @@ class Scene:
 
     def render_batch(self, cams, lights, min_samples=1, max_samples=1, noise_threshold=0.0, num_bounces=1,
                      seeds=None, spp=None, samples_per_pass=0, use_graph=False, persistent=False, blocks_per_cu=0,
-                     grab=0, extend_mode=0, separate_kernels=0, outputs=True):
+                     grab=0, extend_mode=0, separate_kernels=0, outputs=True, stream=None):
         """Render many cameras in one set of launches.  Returns a list of
         per-camera (rgba f32, spp_used) and the timing dict."""
         n = len(cams)
@@ -521,7 +521,7 @@ class Scene:
         sppo = np.zeros(total, dtype=np.int32) if outputs else None
         tm = Timing()
         _check(load().crt_render_batch(self.h, n, arr, ptr(ms), ptr(sd), larr, len(lights), C.byref(opts),
-                                       C.byref(bo), ptr(rgba), None, ptr(sppo), 0, None, C.byref(tm)))
+                                       C.byref(bo), ptr(rgba), None, ptr(sppo), 0, stream, C.byref(tm)))
         out = []
         if outputs:
             off = 0

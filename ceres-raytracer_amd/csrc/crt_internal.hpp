@@ -149,26 +149,34 @@ struct crt_scene {
     crt::BuildResult bvh;
     crt::SceneView view{};
     crt_bvh_info info{};
-    // Grow-only device buffers of the scene's own entry points: one
-    // workspace per stream for passes, ray queries and sensors (workspace()),
-    // one render workspace (rws).  Calls take `mu` while they enqueue, so
-    // the host side runs one call at a time per scene; device work of calls
-    // on different streams uses different buffers and may overlap.  Renders
-    // synchronise before they return, so they do not overlap.  The server's
-    // workers bring their own buffers and run concurrently.
-    std::vector<std::pair<void*, std::unique_ptr<crt::Workspace>>> stream_ws;
-    crt::RenderWorkspace* rws = nullptr;
+    // Grow-only device buffers of the scene's own entry points, one set per
+    // stream: `ws` for passes, ray queries and sensors, `rws` for renders.
+    // A call holds its stream's slot lock for its whole duration, so calls
+    // on one stream of a scene run one at a time, while calls on different
+    // streams use different buffers and run concurrently (host and device).
+    // `mu` guards only the slot table.  The server's workers bring their
+    // own buffers.
+    struct StreamSlot {
+        void* stream = nullptr;
+        crt::Workspace ws;
+        crt::RenderWorkspace* rws = nullptr;
+        std::mutex mu;
+    };
+    std::vector<std::unique_ptr<StreamSlot>> slots;
     std::mutex mu;
-    crt::Workspace& workspace(void* stream) {   // call with mu held
-        for (auto& e : stream_ws)
-            if (e.first == stream) return *e.second;
-        stream_ws.emplace_back(stream, std::unique_ptr<crt::Workspace>(new crt::Workspace()));
-        return *stream_ws.back().second;
+    StreamSlot& slot(void* stream) {
+        std::lock_guard<std::mutex> lk(mu);
+        for (auto& e : slots)
+            if (e->stream == stream) return *e;
+        slots.emplace_back(new StreamSlot());
+        slots.back()->stream = stream;
+        return *slots.back();
     }
     crt_scene() = default;
     crt_scene(const crt_scene&) = delete;
     crt_scene& operator=(const crt_scene&) = delete;
     ~crt_scene() {
-        if (rws) crt::delete_render_workspace(rws);
+        for (auto& e : slots)
+            if (e->rws) crt::delete_render_workspace(e->rws);
     }
 };

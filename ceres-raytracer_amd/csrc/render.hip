@@ -1044,10 +1044,11 @@ extern "C" int crt_render(crt_scene* scene, const crt_camera* camera, const crt_
         CRT_REQUIRE(opts != nullptr, CRT_ERR_INVALID, "render options are NULL");
         CRT_REQUIRE(scene != nullptr, CRT_ERR_INVALID, "scene is NULL");
         const uint64_t seed = opts->seed;
-        std::lock_guard<std::mutex> scene_lock(scene->mu);
-        if (!scene->rws) scene->rws = new_render_workspace();
+        crt_scene::StreamSlot& slot = scene->slot(stream);
+        std::lock_guard<std::mutex> slot_lock(slot.mu);
+        if (!slot.rws) slot.rws = new_render_workspace();
         render_batch(scene, 1, camera, nullptr, &seed, lights, n_lights, false, opts, nullptr, rgba, rgba_u8, spp_used,
-                     flags, stream, timing, scene->rws);
+                     flags, stream, timing, slot.rws);
     });
 }
 
@@ -1058,9 +1059,10 @@ extern "C" int crt_render_batch(crt_scene* scene, uint32_t n_cams, const crt_cam
                                 crt_timing* timing) {
     return guarded([&] {
         CRT_REQUIRE(scene != nullptr, CRT_ERR_INVALID, "scene is NULL");
-        std::lock_guard<std::mutex> scene_lock(scene->mu);
-        if (!scene->rws) scene->rws = new_render_workspace();
+        crt_scene::StreamSlot& slot = scene->slot(stream);
+        std::lock_guard<std::mutex> slot_lock(slot.mu);
+        if (!slot.rws) slot.rws = new_render_workspace();
         render_batch(scene, n_cams, cams, max_samples, seeds, lights, n_lights, false, opts, bopts, rgba, rgba_u8,
-                     spp_used, flags, stream, timing, scene->rws);
+                     spp_used, flags, stream, timing, slot.rws);
     });
 }

@@ -263,8 +263,9 @@ int crt_lidar_batch(crt_scene* scene, uint64_t n, const double* positions, const
         CRT_REQUIRE(n <= (1ull << 31), CRT_ERR_INVALID, "too many poses in one call");
         hipStream_t st = (hipStream_t)stream;
         const bool dev = (flags & CRT_OUT_DEVICE) != 0;
-        std::lock_guard<std::mutex> scene_lock(scene->mu);
-        Workspace& ws = scene->workspace(stream);
+        crt_scene::StreamSlot& slot = scene->slot(stream);
+        std::lock_guard<std::mutex> slot_lock(slot.mu);
+        Workspace& ws = slot.ws;
         const double* dp = positions;
         const double* dr = rotations;
         double* dout = ranges;
@@ -416,9 +417,9 @@ int crt_flash_lidar(crt_scene* scene, const crt_flash_opts* opts, double* single
                     uint8_t* n_returns, uint32_t flags, void* stream, crt_timing* timing) {
     return guarded([&] {
         CRT_REQUIRE(scene != nullptr, CRT_ERR_INVALID, "scene is NULL");
-        std::lock_guard<std::mutex> scene_lock(scene->mu);
-        flash_lidar_impl(scene, opts, single, first, last, n_returns, flags, stream, timing,
-                         scene->workspace(stream));
+        crt_scene::StreamSlot& slot = scene->slot(stream);
+        std::lock_guard<std::mutex> slot_lock(slot.mu);
+        flash_lidar_impl(scene, opts, single, first, last, n_returns, flags, stream, timing, slot.ws);
     });
 }
 
@@ -426,9 +427,9 @@ int crt_radiation_pressure(crt_scene* scene, const crt_flux_opts* opts, double f
                            uint64_t* n_hits, void* stream, crt_timing* timing) {
     return guarded([&] {
         CRT_REQUIRE(scene != nullptr, CRT_ERR_INVALID, "scene is NULL");
-        std::lock_guard<std::mutex> scene_lock(scene->mu);
-        radiation_pressure_impl(scene, opts, force, torque, n_hits, stream, timing,
-                                scene->workspace(stream));
+        crt_scene::StreamSlot& slot = scene->slot(stream);
+        std::lock_guard<std::mutex> slot_lock(slot.mu);
+        radiation_pressure_impl(scene, opts, force, torque, n_hits, stream, timing, slot.ws);
     });
 }
 

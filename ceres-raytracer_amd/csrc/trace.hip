@@ -531,8 +531,9 @@ int crt_pass(crt_scene* scene, const crt_camera* camera, double* xyz, double* no
         const CamView cam = make_cam_view(*camera, scene);
         const size_t npix = (size_t)cam.w * cam.h;
         const bool dev = (flags & CRT_OUT_DEVICE) != 0;
-        std::lock_guard<std::mutex> scene_lock(scene->mu);
-        Workspace& ws = scene->workspace(stream);
+        crt_scene::StreamSlot& slot = scene->slot(stream);
+        std::lock_guard<std::mutex> slot_lock(slot.mu);
+        Workspace& ws = slot.ws;
         PassOut o{nullptr, nullptr, nullptr, nullptr, nullptr};
         if (xyz) { if (dev) o.xyz = xyz; else { ws.a.reserve(npix * 24); o.xyz = ws.a.as<double>(); } }
         if (normal) { if (dev) o.normal = normal; else { ws.b.reserve(npix * 24); o.normal = ws.b.as<double>(); } }
@@ -574,8 +575,9 @@ int crt_trace_rays(crt_scene* scene, uint64_t n, const double* origins, const do
         hipStream_t st = (hipStream_t)stream;
         const bool dev = (flags & CRT_OUT_DEVICE) != 0;
         const bool any = (flags & CRT_ANY_HIT) != 0;
-        std::lock_guard<std::mutex> scene_lock(scene->mu);
-        Workspace& ws = scene->workspace(stream);
+        crt_scene::StreamSlot& slot = scene->slot(stream);
+        std::lock_guard<std::mutex> slot_lock(slot.mu);
+        Workspace& ws = slot.ws;
         RayIn in{origins, dirs, tmin, tmax};
         RayOut out{t, tri, u, v};
         if (!dev) {
@@ -637,8 +639,9 @@ int crt_sun_pass(crt_scene* scene, const crt_camera* camera, const double sun_di
         const CamView cam = make_cam_view(*camera, scene);
         const size_t npix = (size_t)cam.w * cam.h;
         const bool dev = (flags & CRT_OUT_DEVICE) != 0;
-        std::lock_guard<std::mutex> scene_lock(scene->mu);
-        Workspace& ws = scene->workspace(stream);
+        crt_scene::StreamSlot& slot = scene->slot(stream);
+        std::lock_guard<std::mutex> slot_lock(slot.mu);
+        Workspace& ws = slot.ws;
         double* d_depth = depth;
         int32_t* d_tri = tri_id;
         uint8_t* d_lit = lit;

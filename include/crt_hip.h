@@ -18,13 +18,13 @@
  * library is linked against) and, when `timing` is NULL, the call only
  * enqueues work on `stream` (hipStream_t, may be NULL for the null stream).
  *
- * Threads and streams: calls on one scene enqueue one at a time (a
- * per-scene lock).  Each scene keeps one grow-only device workspace per
- * stream for passes, ray queries and sensors, so device work of calls on
- * different streams may overlap (e.g. consecutive frames on two streams),
- * while calls on one stream are ordered by it.  Renders synchronise before
- * they return.  Different scenes, and the request server's workers (own
- * workspaces and streams), run concurrently.
+ * Threads and streams: each scene keeps one grow-only device workspace per
+ * stream.  Calls on one stream of a scene run one at a time (that stream's
+ * lock) and are ordered by the stream; calls on different streams use
+ * different workspaces and run concurrently on the host and overlap on the
+ * device (e.g. consecutive frames on two streams, or two render batches
+ * from two threads).  Different scenes, and the request server's workers
+ * (own workspaces and streams), run concurrently too.
  *
  * Reference interfaces replaced (ceres-raytracer v0.6a4 paths):
  *   crt_scene_create/build  <- BodyFixedGroup ctor: transform + SweepSAH + PRO + NLO
