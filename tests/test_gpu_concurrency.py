@@ -65,3 +65,27 @@ def test_concurrent_calls_on_two_scenes(bodies):
     got = _run_threads(f1 + f2, repeat=2)
     for i, g in enumerate(got):
         assert np.array_equal(g, ref[i % len(f1)]), i
+
+
+def test_renders_on_separate_streams_match_sequential(bodies):
+    # renders of one scene from three threads, each on its own stream (its
+    # own workspace, so they run concurrently): every image equals the same
+    # render made alone
+    m, s, _ = bodies
+    sun = {"type": "directional", "intensity": 1.0, "direction": scenes.sun_direction(2)}
+    cams = [scenes.small_body_camera(m, resolution=r, seed=40 + k) for k, r in enumerate((96, 160, 128))]
+    ref = [s.render_batch([c], [sun], 4, 4, 0.0, 3, seeds=[7 + k])[0][0][0] for k, c in enumerate(cams)]
+    streams = [capi.Stream() for _ in cams]
+    got = [None] * len(cams)
+
+    def work(k):
+        got[k] = s.render_batch([cams[k]], [sun], 4, 4, 0.0, 3, seeds=[7 + k], stream=streams[k].handle)[0][0][0]
+
+    for _ in range(2):
+        ths = [threading.Thread(target=work, args=(k,)) for k in range(len(cams))]
+        for t in ths:
+            t.start()
+        for t in ths:
+            t.join()
+        for k in range(len(cams)):
+            assert np.array_equal(got[k], ref[k]), k
