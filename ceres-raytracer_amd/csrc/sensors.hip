@@ -152,7 +152,7 @@ struct FluxParams {
 // Block partials: one wave of 64 rays per block; 7 sums each (Fx,Fy,Fz,
 // Tx,Ty,Tz, hits), reduced across the wave with shuffles, then combined
 // across blocks with TwoSum compensation.
-CRT_LANE_KERNEL void k_flux(SceneView s, FluxParams F, Comp* __restrict__ partial) {
+CRT_LANE_KERNEL void k_flux(SceneView s, FluxParams F, double* __restrict__ partial, uint32_t nblocks) {
     const LaneStack stk = lane_stack(s);
     // one 8 x 8 tile of the grid per wave (tiles row-major): a wave's
     // parallel rays cover a compact patch and share BVH nodes
@@ -207,22 +207,43 @@ CRT_LANE_KERNEL void k_flux(SceneView s, FluxParams F, Comp* __restrict__ partia
     for (int w = 32; w > 0; w >>= 1)
         for (int k = 0; k < 7; ++k) acc[k] += __shfl_xor(acc[k], w, 64);
     if (threadIdx.x == 0)
-        for (int k = 0; k < 7; ++k) partial[(size_t)blockIdx.x * 7 + k] = Comp{acc[k], 0.0};
+        for (int k = 0; k < 7; ++k) partial[(size_t)k * nblocks + blockIdx.x] = acc[k];   // component-major
 }
 
-__global__ __launch_bounds__(256) void k_flux_final(const Comp* __restrict__ partial, uint32_t nblocks,
-                                                    double* __restrict__ out) {
-    __shared__ Comp red[256];
-    const int k = blockIdx.x;   // component 0..6
-    Comp acc{0.0, 0.0};
-    for (uint32_t b = threadIdx.x; b < nblocks; b += 256) acc = comp_add(acc, partial[(size_t)b * 7 + k]);
+// Cross-block reduction in two compensated (TwoSum) stages: FLUX_SLICES
+// blocks per component each sum a contiguous slice of the block partials,
+// then one block per component combines the slices.
+constexpr int FLUX_SLICES = 64;
+
+__device__ inline Comp block_comp_sum(Comp acc, Comp* red) {
     red[threadIdx.x] = acc;
     __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
+    for (int w = (int)blockDim.x / 2; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) red[threadIdx.x] = comp_add(red[threadIdx.x], red[threadIdx.x + w]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[k] = red[0].s + red[0].c;
+    return red[0];
+}
+
+__global__ __launch_bounds__(256) void k_flux_slices(const double* __restrict__ partial, uint32_t nblocks,
+                                                     Comp* __restrict__ slices) {
+    __shared__ Comp red[256];
+    const int k = blockIdx.y;   // component 0..6
+    const uint32_t per = (nblocks + FLUX_SLICES - 1) / FLUX_SLICES;
+    const uint32_t lo = blockIdx.x * per, hi = min(lo + per, nblocks);
+    Comp acc{0.0, 0.0};
+    for (uint32_t b = lo + threadIdx.x; b < hi; b += 256)
+        acc = comp_add(acc, Comp{partial[(size_t)k * nblocks + b], 0.0});
+    const Comp r = block_comp_sum(acc, red);
+    if (threadIdx.x == 0) slices[k * FLUX_SLICES + blockIdx.x] = r;
+}
+
+__global__ __launch_bounds__(FLUX_SLICES) void k_flux_final(const Comp* __restrict__ slices,
+                                                            double* __restrict__ out) {
+    __shared__ Comp red[FLUX_SLICES];
+    const int k = blockIdx.x;   // component 0..6
+    const Comp r = block_comp_sum(slices[k * FLUX_SLICES + threadIdx.x], red);
+    if (threadIdx.x == 0) out[k] = r.s + r.c;
 }
 
 // Events only when the caller asked for timing (see EventTimer in trace.hip).
@@ -387,14 +408,17 @@ void radiation_pressure_impl(crt_scene* scene, const crt_flux_opts* opts, double
         F.origin_back = 2.0 * F.radius;
         const uint64_t n = (uint64_t)F.grid * F.grid;
         const uint32_t nblocks = cdiv(F.grid, 8) * cdiv(F.grid, 8);   // one 8 x 8 tile per block
-        ws.h.reserve((size_t)nblocks * 7 * sizeof(Comp));
+        ws.h.reserve((size_t)nblocks * 7 * sizeof(double) + (size_t)7 * FLUX_SLICES * sizeof(Comp));
         ws.counters.reserve(256);
         double* out = ws.counters.as<double>();
         Timer tm(timing != nullptr);
         Timer::rec(tm.a, st);
+        double* part = ws.h.as<double>();
+        Comp* slices = reinterpret_cast<Comp*>(part + (size_t)nblocks * 7);
         hipLaunchKernelGGL(k_flux, dim3(nblocks), dim3(LANE_BLOCK), lane_lds_bytes(scene->view), st, scene->view, F,
-                           ws.h.as<Comp>());
-        hipLaunchKernelGGL(k_flux_final, dim3(7), dim3(256), 0, st, ws.h.as<Comp>(), nblocks, out);
+                           part, nblocks);
+        hipLaunchKernelGGL(k_flux_slices, dim3(FLUX_SLICES, 7), dim3(256), 0, st, part, nblocks, slices);
+        hipLaunchKernelGGL(k_flux_final, dim3(7), dim3(FLUX_SLICES), 0, st, slices, out);
         CRT_HIP_CHECK(hipGetLastError());
         Timer::rec(tm.b, st);
         double h[7];
