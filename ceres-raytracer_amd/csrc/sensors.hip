@@ -73,58 +73,70 @@ struct FlashParams {
     uint64_t seed;
 };
 
-// One octet per (beam, ray) with ray 0 = boresight, 1..sub_rays perturbed.
-CRT_LANE_KERNEL void k_flash(SceneView s, FlashParams F, double* __restrict__ ranges,
-                                                       uint64_t n) {
-    const LaneStack stk = lane_stack(s);
-    const uint64_t q = (uint64_t)blockIdx.x * LANE_BLOCK + threadIdx.x;
-    if (q >= n) return;
-    const uint32_t per = F.sub_rays + 1;
-    const uint64_t beam = q / per;
-    const uint32_t k = (uint32_t)(q % per);
-    const int px = (int)(beam % (uint64_t)F.cam.w), py = (int)(beam / (uint64_t)F.cam.w);
-    double d[3];
+// Direction of ray k of a beam (k = 0: the boresight; 1..sub_rays: the
+// boresight tilted by two N(0, sigma^2) angles in the beam's tangent plane).
+// The Gaussian angles (sigma ~ mrad) are formed in FP32: their rounding moves
+// a sub-ray by ~1e-10 rad; the beam direction and the tilted direction stay FP64.
+__device__ inline void flash_dir(const FlashParams& F, uint32_t beam, uint32_t k, double d[3]) {
+    const int px = (int)(beam % (uint32_t)F.cam.w), py = (int)(beam / (uint32_t)F.cam.w);
     camera_dir(F.cam, (double)px, (double)py, d);
-    if (k > 0) {
-        // tangent frame of the beam
-        double ax[3] = {1.0, 0.0, 0.0};
-        if (fabs(d[0]) > 0.9) { ax[0] = 0.0; ax[1] = 1.0; }
-        double e1[3] = {d[1] * ax[2] - d[2] * ax[1], d[2] * ax[0] - d[0] * ax[2], d[0] * ax[1] - d[1] * ax[0]};
-        const double l1 = sqrt(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]);
-        e1[0] /= l1; e1[1] /= l1; e1[2] /= l1;
-        const double e2[3] = {d[1] * e1[2] - d[2] * e1[1], d[2] * e1[0] - d[0] * e1[2], d[0] * e1[1] - d[1] * e1[0]};
-        const double r1 = rng_u01(F.seed, beam, k, 0), r2 = rng_u01(F.seed, beam, k, 1);
-        const double rad = sqrt(-2.0 * log(1.0 - r1));
-        const double g1 = rad * cos(2.0 * M_PI * r2), g2 = rad * sin(2.0 * M_PI * r2);
-        const double a1 = tan(F.sigma * g1), a2 = tan(F.sigma * g2);
-        double nd[3] = {d[0] + a1 * e1[0] + a2 * e2[0], d[1] + a1 * e1[1] + a2 * e2[1], d[2] + a1 * e1[2] + a2 * e2[2]};
-        const double ln = sqrt(nd[0] * nd[0] + nd[1] * nd[1] + nd[2] * nd[2]);
-        d[0] = nd[0] / ln; d[1] = nd[1] / ln; d[2] = nd[2] / ln;
-    }
-    ranges[q] = range_f64(s, F.cam.pos, d, stk);
+    if (k == 0) return;
+    // tangent frame of the beam
+    double ax[3] = {1.0, 0.0, 0.0};
+    if (fabs(d[0]) > 0.9) { ax[0] = 0.0; ax[1] = 1.0; }
+    double e1[3] = {d[1] * ax[2] - d[2] * ax[1], d[2] * ax[0] - d[0] * ax[2], d[0] * ax[1] - d[1] * ax[0]};
+    const double l1 = sqrt(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]);
+    e1[0] /= l1; e1[1] /= l1; e1[2] /= l1;
+    const double e2[3] = {d[1] * e1[2] - d[2] * e1[1], d[2] * e1[0] - d[0] * e1[2], d[0] * e1[1] - d[1] * e1[0]};
+    const double r1 = rng_u01(F.seed, beam, k, 0), r2 = rng_u01(F.seed, beam, k, 1);
+    // Box-Muller (1 - r1 in (0, 1]: its FP32 rounding stays > 0)
+    const float rad = sqrtf(-2.0f * logf((float)(1.0 - r1)));
+    float sn, cs;
+    sincosf(6.2831853071795865f * (float)r2, &sn, &cs);
+    const float sg = (float)F.sigma;
+    const double a1 = (double)tanf(sg * rad * cs), a2 = (double)tanf(sg * rad * sn);
+    double nd[3] = {d[0] + a1 * e1[0] + a2 * e2[0], d[1] + a1 * e1[1] + a2 * e2[1], d[2] + a1 * e1[2] + a2 * e2[2]};
+    const double ln = sqrt(nd[0] * nd[0] + nd[1] * nd[1] + nd[2] * nd[2]);
+    d[0] = nd[0] / ln; d[1] = nd[1] / ln; d[2] = nd[2] / ln;
 }
 
-__global__ __launch_bounds__(256) void k_flash_reduce(const double* __restrict__ ranges, uint32_t sub_rays,
-                                                      uint64_t n_beams, double* __restrict__ single,
-                                                      double* __restrict__ first, double* __restrict__ last,
-                                                      uint8_t* __restrict__ n_returns) {
-    const uint64_t b = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-    if (b >= n_beams) return;
-    const double* r = ranges + b * (sub_rays + 1);
+// One lane per (beam, ray).  The per = sub_rays + 1 <= 64 rays of a beam sit
+// in consecutive lanes of one wave (64 / per beams per wave, the remaining
+// lanes idle), so the returns are reduced across lanes and the per-ray ranges
+// never leave registers.  No early exit: every lane takes part in the shuffles.
+CRT_LANE_KERNEL void k_flash(SceneView s, FlashParams F, uint32_t n_beams, double* __restrict__ single,
+                                                       double* __restrict__ first, double* __restrict__ last,
+                                                       uint8_t* __restrict__ n_returns) {
+    const LaneStack stk = lane_stack(s);
+    const uint32_t per = F.sub_rays + 1, groups = LANE_BLOCK / per;
+    const uint32_t lane = threadIdx.x, g = lane / per, k = lane - g * per;
+    const uint64_t beam64 = (uint64_t)blockIdx.x * groups + g;
+    const bool live = g < groups && beam64 < n_beams;
+    const uint32_t beam = (uint32_t)beam64;
+    double rg = 0.0;
+    if (live) {
+        double d[3];
+        flash_dir(F, beam, k, d);
+        rg = range_f64(s, F.cam.pos, d, stk);
+    }
+    // lane k = 0 of each group gathers lanes +1 .. +sub_rays (all inside the
+    // wave: g * per + per - 1 <= 63)
     double mn = 0.0, mx = 0.0;
     uint32_t cnt = 0;
-    for (uint32_t k = 1; k <= sub_rays; ++k) {
-        const double x = r[k];
+    for (uint32_t j = 1; j < per; ++j) {
+        const double x = __shfl(rg, (int)min(lane + j, (uint32_t)LANE_BLOCK - 1u), LANE_BLOCK);
         if (x > 0.0) {
             mn = cnt == 0 ? x : fmin(mn, x);
             mx = cnt == 0 ? x : fmax(mx, x);
             ++cnt;
         }
     }
-    if (single) single[b] = r[0];
-    if (first) first[b] = mn;
-    if (last) last[b] = mx;
-    if (n_returns) n_returns[b] = (uint8_t)min(cnt, 255u);
+    if (live && k == 0) {
+        if (single) single[beam] = rg;
+        if (first) first[beam] = mn;
+        if (last) last[beam] = mx;
+        if (n_returns) n_returns[beam] = (uint8_t)cnt;
+    }
 }
 
 // ---- radiation pressure ----
@@ -212,8 +224,12 @@ CRT_LANE_KERNEL void k_flux(SceneView s, FluxParams F, double* __restrict__ part
 
 // Cross-block reduction in two compensated (TwoSum) stages: FLUX_SLICES
 // blocks per component each sum a contiguous slice of the block partials,
-// then one block per component combines the slices.
+// then one block per component combines the slices.  Both stages run
+// one-wave blocks: while another stream's traversal kernel holds the CUs, a
+// one-wave block takes any free wave slot, where a multi-wave block waits for
+// several slots to free up on one CU at once.
 constexpr int FLUX_SLICES = 64;
+constexpr int FLUX_SLICE_THREADS = 64;
 
 __device__ inline Comp block_comp_sum(Comp acc, Comp* red) {
     red[threadIdx.x] = acc;
@@ -225,14 +241,14 @@ __device__ inline Comp block_comp_sum(Comp acc, Comp* red) {
     return red[0];
 }
 
-__global__ __launch_bounds__(256) void k_flux_slices(const double* __restrict__ partial, uint32_t nblocks,
-                                                     Comp* __restrict__ slices) {
-    __shared__ Comp red[256];
+__global__ __launch_bounds__(FLUX_SLICE_THREADS) void k_flux_slices(const double* __restrict__ partial,
+                                                                    uint32_t nblocks, Comp* __restrict__ slices) {
+    __shared__ Comp red[FLUX_SLICE_THREADS];
     const int k = blockIdx.y;   // component 0..6
     const uint32_t per = (nblocks + FLUX_SLICES - 1) / FLUX_SLICES;
     const uint32_t lo = blockIdx.x * per, hi = min(lo + per, nblocks);
     Comp acc{0.0, 0.0};
-    for (uint32_t b = lo + threadIdx.x; b < hi; b += 256)
+    for (uint32_t b = lo + threadIdx.x; b < hi; b += FLUX_SLICE_THREADS)
         acc = comp_add(acc, Comp{partial[(size_t)k * nblocks + b], 0.0});
     const Comp r = block_comp_sum(acc, red);
     if (threadIdx.x == 0) slices[k * FLUX_SLICES + blockIdx.x] = r;
@@ -324,7 +340,7 @@ void flash_lidar_impl(crt_scene* scene, const crt_flash_opts* opts, double* sing
     {
         CRT_REQUIRE(scene != nullptr && opts != nullptr, CRT_ERR_INVALID, "NULL argument");
         check_camera_args(&opts->beams);
-        CRT_REQUIRE(opts->sub_rays <= 64, CRT_ERR_INVALID, "at most 64 sub-rays per beam");
+        CRT_REQUIRE(opts->sub_rays < (uint32_t)LANE_BLOCK, CRT_ERR_INVALID, "at most 63 sub-rays per beam");
         CRT_REQUIRE(opts->sigma >= 0.0 && std::isfinite(opts->sigma), CRT_ERR_INVALID, "sigma must be >= 0");
         hipStream_t st = (hipStream_t)stream;
         FlashParams F;
@@ -336,7 +352,6 @@ void flash_lidar_impl(crt_scene* scene, const crt_flash_opts* opts, double* sing
         const uint64_t nr = nb * (F.sub_rays + 1);
         CRT_REQUIRE(nr < (1ull << 32), CRT_ERR_INVALID, "flash LiDAR too large");
         const bool dev = (flags & CRT_OUT_DEVICE) != 0;
-        ws.f.reserve(nr * 8);
         double* d_single = single;
         double* d_first = first;
         double* d_last = last;
@@ -351,10 +366,9 @@ void flash_lidar_impl(crt_scene* scene, const crt_flash_opts* opts, double* sing
         }
         Timer tm(timing != nullptr);
         Timer::rec(tm.a, st);
-        hipLaunchKernelGGL(k_flash, dim3(cdiv(nr, LANE_BLOCK)), dim3(LANE_BLOCK), lane_lds_bytes(scene->view),
-                           st, scene->view, F, ws.f.as<double>(), nr);
-        hipLaunchKernelGGL(k_flash_reduce, dim3(cdiv(nb, 256)), dim3(256), 0, st, ws.f.as<double>(), F.sub_rays, nb,
-                           d_single, d_first, d_last, d_n);
+        const uint32_t groups = LANE_BLOCK / (F.sub_rays + 1);   // beams per wave
+        hipLaunchKernelGGL(k_flash, dim3(cdiv(nb, groups)), dim3(LANE_BLOCK), lane_lds_bytes(scene->view), st,
+                           scene->view, F, (uint32_t)nb, d_single, d_first, d_last, d_n);
         CRT_HIP_CHECK(hipGetLastError());
         Timer::rec(tm.b, st);
         if (!dev) {
@@ -417,7 +431,8 @@ void radiation_pressure_impl(crt_scene* scene, const crt_flux_opts* opts, double
         Comp* slices = reinterpret_cast<Comp*>(part + (size_t)nblocks * 7);
         hipLaunchKernelGGL(k_flux, dim3(nblocks), dim3(LANE_BLOCK), lane_lds_bytes(scene->view), st, scene->view, F,
                            part, nblocks);
-        hipLaunchKernelGGL(k_flux_slices, dim3(FLUX_SLICES, 7), dim3(256), 0, st, part, nblocks, slices);
+        hipLaunchKernelGGL(k_flux_slices, dim3(FLUX_SLICES, 7), dim3(FLUX_SLICE_THREADS), 0, st, part, nblocks,
+                           slices);
         hipLaunchKernelGGL(k_flux_final, dim3(7), dim3(FLUX_SLICES), 0, st, slices, out);
         CRT_HIP_CHECK(hipGetLastError());
         Timer::rec(tm.b, st);

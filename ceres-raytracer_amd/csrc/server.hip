@@ -55,7 +55,7 @@ size_t request_bytes(const crt_request& r) {
         case CRT_REQ_RENDER:
             return render_bytes_estimate(px, r.spp, (int)r.n_lights) + px * 16;
         case CRT_REQ_LIDAR:
-            return (size_t)px * (r.sub_rays + 1) * 8 + px * 25;
+            return (size_t)px * 25;   // per-beam outputs; sub-ray ranges stay in registers
         case CRT_REQ_FLUX: {
             const uint64_t n = (uint64_t)r.flux.grid * r.flux.grid;
             return (size_t)(n / LANE_BLOCK + 1) * 7 * 16 + 256;

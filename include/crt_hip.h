@@ -264,7 +264,7 @@ int crt_lidar_batch(crt_scene* scene, uint64_t n, const double* positions, const
  * the sub-rays that hit (0 if none), n_returns = number of sub-ray hits.     */
 typedef struct {
     crt_camera beams;
-    uint32_t sub_rays;        /* <= 64 */
+    uint32_t sub_rays;        /* <= 63 (a beam's rays share one wavefront) */
     uint32_t pad_;
     double sigma;
     uint64_t seed;

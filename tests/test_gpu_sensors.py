@@ -199,15 +199,24 @@ def _flash_subrays(cam, seed, sub_rays, sigma):
     return np.repeat(o[:, None, :], sub_rays, axis=1), dirs
 
 
-def test_flash_lidar_returns_match_oracle(icosphere):
+@pytest.mark.parametrize("K,res", [(1, 48), (8, 48), (23, 47), (63, 40)])
+def test_flash_lidar_returns_match_oracle(icosphere, K, res):
     # every sub-ray traced by the FP64 oracle: first / last return = min /
     # max range over the sub-rays that hit, n_returns = their count; beams
     # whose sub-rays graze an edge or vertex (a barycentric within 1e-5 of 0)
-    # may legitimately differ and are counted apart
+    # may legitimately differ and are counted apart.  K sets how many beams
+    # share a wave (64 // (K + 1): 32, 7, 2, 1) and res the partial last wave.
     m, gpu, cpu = icosphere
-    K, sigma, seed = 8, 5e-4, 4
-    beams = scenes.small_body_camera(m, resolution=48, seed=3)
+    sigma, seed = 5e-4, 4
+    beams = scenes.small_body_camera(m, resolution=res, seed=3)
     out = gpu.flash_lidar(beams, sub_rays=K, sigma=sigma, seed=seed)
+    # single return = the boresight ray's closest hit
+    c = cpu.camera_pass(beams)
+    hit0 = c["tri"] >= 0
+    assert ((out["single"] > 0) != hit0).sum() <= 2   # silhouette grazes
+    hit0 &= out["single"] > 0
+    rel0 = np.abs(out["single"][hit0] - c["depth"][hit0]) / c["depth"][hit0]
+    assert rel0.max() < 1e-5
     o, d = _flash_subrays(beams, seed, K, sigma)
     t, tri, u, v = cpu.trace(o.reshape(-1, 3), d.reshape(-1, 3))
     t, tri = t.reshape(-1, K), tri.reshape(-1, K)
@@ -229,6 +238,14 @@ def test_flash_lidar_returns_match_oracle(icosphere):
     assert np.max(np.abs(fg[both] - first_c[both]) / first_c[both]) < 1e-5
     assert np.max(np.abs(lg[both] - last_c[both]) / last_c[both]) < 1e-5
     assert np.all(fg[n_g == 0] == 0) and np.all(lg[n_g == 0] == 0)
+
+
+def test_flash_lidar_sub_ray_limit(icosphere):
+    # a beam's rays share one wavefront: at most 63 sub-rays
+    m, gpu, cpu = icosphere
+    beams = scenes.small_body_camera(m, resolution=8, seed=3)
+    with pytest.raises(ValueError):
+        gpu.flash_lidar(beams, sub_rays=64)
 
 
 def test_radiation_pressure_matches_oracle(icosphere):
