@@ -34,9 +34,9 @@ CANDIDATES = [
     dict(samples_per_pass=4, use_graph=True, persistent=False, extend_mode=2),
     dict(samples_per_pass=4, use_graph=True, persistent=False, extend_mode=3, separate_kernels=2),
     dict(samples_per_pass=4, use_graph=True, persistent=False, extend_mode=3, separate_kernels=1),
-    dict(samples_per_pass=0, use_graph=True, persistent=True, blocks_per_cu=8, grab=4, extend_mode=1),
-    dict(samples_per_pass=0, use_graph=True, persistent=True, blocks_per_cu=8, grab=4, extend_mode=3),
-    dict(samples_per_pass=0, use_graph=True, persistent=True, blocks_per_cu=16, grab=1, extend_mode=1),
+    dict(samples_per_pass=0, use_graph=True, persistent=True, blocks_per_cu=32, grab=4, extend_mode=1),
+    dict(samples_per_pass=0, use_graph=True, persistent=True, blocks_per_cu=32, grab=4, extend_mode=3),
+    dict(samples_per_pass=0, use_graph=True, persistent=True, blocks_per_cu=64, grab=1, extend_mode=1),
 ]
 
 
@@ -57,7 +57,7 @@ def _device_name():
 def _key(scene: capi.Scene, bounces, n_lights):
     n = scene.n_tris
     bucket = int(np.round(np.log2(max(n, 1))))
-    return "v2|%s|tris2^%d|b%d|l%d" % (_device_name(), bucket, bounces, n_lights)
+    return "v3|%s|tris2^%d|b%d|l%d" % (_device_name(), bucket, bounces, n_lights)
 
 
 def load_cache():

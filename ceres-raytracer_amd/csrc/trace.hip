@@ -296,7 +296,7 @@ CRT_LANE_KERNEL void k_primary_sun(SceneView s, CamView cam, double* depth, int3
 
 // Shadow rays are any-hit: quad traversal (16 rays per wave, two children
 // per lane) beats both one ray per lane, whose waves wait on their longest
-// unoccluded ray, and one ray per octet.
+// unoccluded ray, and one ray per octet.  One wave (16 rays) per block.
 __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow_sun(SceneView s, const ShadowRay* queue,
                                                             const uint32_t* qslot, const uint32_t* qcount, float3 sun,
                                                             uint8_t* lit) {

@@ -226,7 +226,7 @@ int crt_render(crt_scene* scene, const crt_camera* camera, const crt_light* ligh
  * row-major [H][W][4] (rgba, rgba_u8) or [H][W] (spp_used).
  * Knobs (crt_batch_opts, zeros = defaults): samples per pass, hipGraph replay
  * of a pass (non-adaptive renders only), persistent traversal kernels with
- * blocks_per_cu x CUs blocks claiming `grab` rays per atomic.              */
+ * blocks_per_cu x CUs one-wave blocks claiming `grab` rays per atomic.     */
 typedef struct {
     uint32_t samples_per_pass;
     uint32_t use_graph;
