@@ -49,13 +49,42 @@ struct Job {
     uint64_t rays = 0;
 };
 
+// Page-locked host staging, grow-only (powers of two), one per worker.
+// Results go device -> staging by DMA, then into the request's own (pageable)
+// buffer by a plain memcpy on the worker's thread: first-touch page faults
+// of fresh output arrays are then taken outside the HIP runtime's pageable
+// copy path, which the workers would otherwise share.
+struct PinnedBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    PinnedBuf() = default;
+    PinnedBuf(const PinnedBuf&) = delete;
+    PinnedBuf& operator=(const PinnedBuf&) = delete;
+    ~PinnedBuf() {
+        if (p) (void)hipHostFree(p);
+    }
+    template <typename T>
+    T* get(size_t bytes) {
+        if (bytes > cap) {
+            size_t c = (size_t)1 << 20;
+            while (c < bytes) c <<= 1;
+            if (p) (void)hipHostFree(p);
+            p = nullptr;
+            cap = 0;
+            CRT_HIP_CHECK(hipHostMalloc(&p, c, hipHostMallocDefault));
+            cap = c;
+        }
+        return static_cast<T*>(p);
+    }
+};
+
 size_t request_bytes(const crt_request& r) {
     const uint64_t px = (uint64_t)std::floor(r.camera.resolution[0]) * (uint64_t)std::floor(r.camera.resolution[1]);
     switch (r.kind) {
         case CRT_REQ_RENDER:
             return render_bytes_estimate(px, r.spp, (int)r.n_lights) + px * 16;
         case CRT_REQ_LIDAR:
-            return (size_t)px * 25;   // per-beam outputs; sub-ray ranges stay in registers
+            return (size_t)px * 24;   // per-beam outputs; sub-ray ranges stay in registers
         case CRT_REQ_FLUX: {
             const uint64_t n = (uint64_t)r.flux.grid * r.flux.grid;
             return (size_t)(n / LANE_BLOCK + 1) * 7 * 16 + 256;
@@ -98,9 +127,12 @@ namespace crt {
 namespace {
 
 void run_batch(crt_server* S, std::vector<std::shared_ptr<Job>>& batch, hipStream_t st, Workspace& ws,
-               RenderWorkspace* rws) {
+               RenderWorkspace* rws, PinnedBuf& pin) {
     const crt_request& head = batch[0]->req;
     crt_scene* scene = S->scenes[head.scene];
+    const auto pixels = [](const crt_request& r) {
+        return (uint64_t)std::floor(r.camera.resolution[0]) * (uint64_t)std::floor(r.camera.resolution[1]);
+    };
     if (head.kind == CRT_REQ_RENDER) {
         std::vector<crt_camera> cams;
         std::vector<int32_t> spp;
@@ -112,7 +144,7 @@ void run_batch(crt_server* S, std::vector<std::shared_ptr<Job>>& batch, hipStrea
             spp.push_back(j->req.spp);
             seeds.push_back(j->req.seed);
             for (uint32_t l = 0; l < head.n_lights; ++l) lights.push_back(j->req.lights[l]);
-            total += (uint64_t)std::floor(j->req.camera.resolution[0]) * (uint64_t)std::floor(j->req.camera.resolution[1]);
+            total += pixels(j->req);
         }
         crt_render_opts o{};
         o.min_samples = 1;
@@ -120,40 +152,63 @@ void run_batch(crt_server* S, std::vector<std::shared_ptr<Job>>& batch, hipStrea
         o.noise_threshold = 0.0;
         o.num_bounces = head.bounces;
         crt_batch_opts b = S->opts.render;
-        std::vector<float> packed(total * 4);
+        // the batch's images (packed in camera order) stay on the device,
+        // then come back through the pinned staging in one copy
+        ws.a.reserve((size_t)total * 16);
+        float* d_rgba = ws.a.as<float>();
         crt_timing tm{};
         render_batch(scene, (uint32_t)batch.size(), cams.data(), spp.data(), seeds.data(),
-                     lights.empty() ? nullptr : lights.data(), head.n_lights, true, &o, &b, packed.data(), nullptr,
-                     nullptr, 0, st, &tm, rws);
+                     lights.empty() ? nullptr : lights.data(), head.n_lights, true, &o, &b, d_rgba, nullptr,
+                     nullptr, CRT_OUT_DEVICE, st, &tm, rws);
+        float* h = pin.get<float>((size_t)total * 16);
+        CRT_HIP_CHECK(hipMemcpyAsync(h, d_rgba, (size_t)total * 16, hipMemcpyDeviceToHost, st));
+        CRT_HIP_CHECK(hipStreamSynchronize(st));
         uint64_t off = 0;
         for (auto& j : batch) {
-            const uint64_t px = (uint64_t)std::floor(j->req.camera.resolution[0]) *
-                                (uint64_t)std::floor(j->req.camera.resolution[1]);
-            if (j->req.rgba) std::memcpy(j->req.rgba, packed.data() + off * 4, px * 16);
+            const uint64_t px = pixels(j->req);
+            if (j->req.rgba) std::memcpy(j->req.rgba, h + off * 4, px * 16);
             j->rays = (tm.rays_traced + tm.shadow_rays) * px / std::max<uint64_t>(total, 1);
+            off += px;
+        }
+    } else if (head.kind == CRT_REQ_LIDAR) {
+        // every scan of the batch is enqueued (device outputs single / first
+        // / last, 3 x px doubles per request, back to back), one copy back
+        uint64_t total = 0;
+        for (auto& j : batch) total += pixels(j->req);
+        ws.b.reserve((size_t)total * 24);
+        double* d = ws.b.as<double>();
+        uint64_t off = 0;
+        for (auto& j : batch) {
+            const crt_request& r = j->req;
+            crt_flash_opts fo{};
+            fo.beams = r.camera;
+            fo.sub_rays = r.sub_rays;
+            fo.sigma = r.sigma;
+            fo.seed = r.seed;
+            const uint64_t px = pixels(r);
+            double* dr = d + 3 * off;
+            flash_lidar_impl(scene, &fo, dr, dr + px, dr + 2 * px, nullptr, CRT_OUT_DEVICE, st, nullptr, ws);
+            j->rays = px * (uint64_t)(r.sub_rays + 1);
+            off += px;
+        }
+        double* h = pin.get<double>((size_t)total * 24);
+        CRT_HIP_CHECK(hipMemcpyAsync(h, d, (size_t)total * 24, hipMemcpyDeviceToHost, st));
+        CRT_HIP_CHECK(hipStreamSynchronize(st));
+        off = 0;
+        for (auto& j : batch) {
+            const uint64_t px = pixels(j->req);
+            if (j->req.lidar) std::memcpy(j->req.lidar, h + 3 * off, px * 24);
             off += px;
         }
     } else {
         for (auto& j : batch) {
             crt_request& r = j->req;
             crt_timing tm{};
-            if (r.kind == CRT_REQ_LIDAR) {
-                crt_flash_opts fo{};
-                fo.beams = r.camera;
-                fo.sub_rays = r.sub_rays;
-                fo.sigma = r.sigma;
-                fo.seed = r.seed;
-                const uint64_t px = (uint64_t)std::floor(r.camera.resolution[0]) *
-                                    (uint64_t)std::floor(r.camera.resolution[1]);
-                flash_lidar_impl(scene, &fo, r.lidar, r.lidar ? r.lidar + px : nullptr,
-                                 r.lidar ? r.lidar + 2 * px : nullptr, nullptr, 0, st, &tm, ws);
-            } else {
-                double f[3], t[3];
-                radiation_pressure_impl(scene, &r.flux, f, t, nullptr, st, &tm, ws);
-                if (r.force_torque) {
-                    std::memcpy(r.force_torque, f, sizeof(f));
-                    std::memcpy(r.force_torque + 3, t, sizeof(t));
-                }
+            double f[3], t[3];
+            radiation_pressure_impl(scene, &r.flux, f, t, nullptr, st, &tm, ws);
+            if (r.force_torque) {
+                std::memcpy(r.force_torque, f, sizeof(f));
+                std::memcpy(r.force_torque + 3, t, sizeof(t));
             }
             j->rays = tm.rays_traced + tm.shadow_rays;
         }
@@ -167,6 +222,7 @@ void worker_main(crt_server* S, int index) {
     (void)hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
     Workspace ws;
     std::unique_ptr<RenderWorkspace, void (*)(RenderWorkspace*)> rws(new_render_workspace(), delete_render_workspace);
+    PinnedBuf pin;
     const bool batching = S->opts.batching != 0;
     while (true) {
         std::vector<std::shared_ptr<Job>> batch;
@@ -219,7 +275,7 @@ void worker_main(crt_server* S, int index) {
         int status = CRT_OK;
         std::string err;
         try {
-            run_batch(S, batch, st, ws, rws.get());
+            run_batch(S, batch, st, ws, rws.get(), pin);
         } catch (const Error& e) {
             status = e.code;
             err = e.what();
