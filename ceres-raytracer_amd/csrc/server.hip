@@ -14,8 +14,9 @@
 //     join the next batch as soon as a worker is free, while the other
 //     workers' batches are still on the device: continuous batching.
 //   * renders of a batch go through render_batch (one set of launches for all
-//     cameras); flash-LiDAR and flux requests run back to back on the
-//     worker's stream.
+//     cameras); the flash-LiDAR scans of a batch are enqueued back to back
+//     with one synchronisation; flux requests run back to back.  Images and
+//     scans return through the worker's pinned staging buffer.
 // Every result is a pure function of its request (counter-based RNG keyed by
 // the request seed), so it is the same alone and in any mix.
 // `batching = 0` gives the plain baseline: one worker, one request per
