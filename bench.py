@@ -157,7 +157,7 @@ def run_cornell(args, capi, scenes):
     out = {"metric": "Cornell box render + passes Mrays/s (36 tris, 256x256, 16 spp, 4 bounces)",
            "value": rays / el / 1e6, "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "ms_per_step": el * 1e3 / args.steps, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
-           "config": {"workload": "C1: Cornell box (seed 0), 256x256 camera, area light 0.25x0.25, render 16 spp "
+           "config": {"workload": "Cornell box (seed 0), 256x256 camera, area light 0.25x0.25, render 16 spp "
                                   "4 bounces (uint8 + radiance to host) + xyz/normal/instance passes (host)",
                       "rays_per_step": rays // args.steps},
            "render_device_ms": float(np.mean(rend_ms))}
@@ -273,7 +273,7 @@ def run_batched(args, capi, scenes):
             "cpu_baseline": cpu, "one_stream_mrays_s": rays / t1 / 1e6, "one_stream_ms": t1 * 1e3,
             "streams": ng,
             "unit": "Mrays/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
-            "config": {"workload": "C4 batched: %d cameras, %d pixels, spp 8, bounces 3, sun; rendered as %d "
+            "config": {"workload": "batched path tracing: %d cameras, %d pixels, spp 8, bounces 3, sun; rendered as %d "
                                    "pixel-balanced batches on %d streams at once" % (len(cams), pix, ng, ng)},
             "batched_ms": tb * 1e3, "batched_ms_runs": [t * 1e3 for t in times], "plain_ms": tp * 1e3,
             "rays": rays3,
@@ -371,7 +371,7 @@ def run_lidar_srp(args, capi, scenes):
             "value": (lrays + frays) / el / 1e6, "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": el * 1e3 / args.steps, "higher_is_better": True, "dtype": "f32",
             "data": "synthetic",
-            "config": {"workload": "C3: flash LiDAR 1024x1024 beams x (1+8) rays sigma 0.5 mrad seed 3/4 + SRP grid "
+            "config": {"workload": "flash LiDAR 1024x1024 beams x (1+8) rays sigma 0.5 mrad seed 3/4 + SRP grid "
                                    "2048x2048, rho_s 0.3 rho_d 0.6", "lidar_rays": lrays // args.steps,
                        "flux_rays": frays // args.steps},
             "lidar_mrays_s": lrays / args.steps / (np.mean(lid_ms) * 1e-3) / 1e6,
@@ -398,7 +398,7 @@ def run_served(args, capi, scenes):
     resident = sum(s.info()["device_bytes"] for s in sc)
     out = {"metric": "served Mrays/s (mixed render/LiDAR/flux stream, 4 resident meshes)", "unit": "Mrays/s",
            "n_gpus": 1, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
-           "config": {"workload": "C5 served: meshes %s tris, resident %.2f GB" % (
+           "config": {"workload": "served mixed stream: meshes %s tris, resident %.2f GB" % (
                [m.n_tris for m in meshes], resident / 1e9), "requests": args.requests}}
     # plain = the drop-in API called once per request, one request at a time
     # (the server's batching = 0 mode: one worker, no batching)

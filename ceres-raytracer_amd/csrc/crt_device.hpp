@@ -34,8 +34,9 @@ constexpr int OCTET = 8;
 // block: a block's LDS stacks are released as soon as its wave ends, so
 // short waves (rays that miss) never hold a long-running neighbour's slot,
 // and a one-wave block takes any free wave slot while another stream's
-// kernels fill the CUs (octet/quad render kernels: C4 +8.5 % over 4-wave
-// blocks; the sun pass's quad shadow kernel 0.35 vs 0.39 ms).
+// kernels fill the CUs (octet/quad render kernels of a 256-camera batch
+// +8.5 % over 4-wave blocks; the sun pass's quad shadow kernel on 1.8M rays
+// 0.35 vs 0.39 ms).
 constexpr int TRACE_BLOCK = 64;                  // threads per octet / quad traversal block
 constexpr int OCTETS_PER_BLOCK = TRACE_BLOCK / OCTET;
 constexpr int LANE_BLOCK = 64;

@@ -4,7 +4,7 @@
 // Closest-hit kernels here trace one ray per lane over the compressed wide
 // nodes (trace_lane, one 64-lane wave per block with its group stacks in
 // LDS); the sun pass's shadow rays use one quad per ray over the 256-B nodes
-// (trace_quad_any, 16 rays per wave, 64 quad stacks per 256-thread block).
+// (trace_quad_any, 16 rays per wave and per one-wave block).
 #include "crt_internal.hpp"
 
 #include <algorithm>
