@@ -257,11 +257,22 @@ void worker_main(crt_server* S, int index) {
                     }
                 }
             } else if (batching) {
-                // LiDAR / flux requests of the same scene and kind ride along
+                // LiDAR / flux requests of the same scene and kind ride along;
+                // a LiDAR batch holds at most max_batch_paths rays (it shares
+                // one device buffer and one pinned staging copy)
+                const auto lidar_rays = [](const crt_request& r) {
+                    return r.kind == CRT_REQ_LIDAR ? (uint64_t)std::floor(r.camera.resolution[0]) *
+                                                         (uint64_t)std::floor(r.camera.resolution[1]) *
+                                                         (uint64_t)(r.sub_rays + 1)
+                                                   : 0u;
+                };
+                uint64_t rays = lidar_rays(head->req);
                 for (auto it = S->queue.begin(); it != S->queue.end() && batch.size() < S->opts.max_batch;) {
                     const crt_request& r = (*it)->req;
                     if (r.kind == head->req.kind && r.scene == head->req.scene &&
+                        (r.kind != CRT_REQ_LIDAR || rays + lidar_rays(r) <= S->opts.max_batch_paths) &&
                         S->reserved + mem + (*it)->mem <= S->opts.memory_budget) {
+                        rays += lidar_rays(r);
                         mem += (*it)->mem;
                         batch.push_back(*it);
                         it = S->queue.erase(it);

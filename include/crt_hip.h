@@ -312,7 +312,7 @@ typedef struct {
     uint64_t memory_budget;     /* bytes of working memory admitted requests may hold (0: 16 GiB) */
     uint32_t n_workers;         /* concurrent batches / streams (0: 3)          */
     uint32_t max_batch;         /* requests per batch (0: 64)                   */
-    uint64_t max_batch_paths;   /* camera paths (pixels x spp) per render batch */
+    uint64_t max_batch_paths;   /* camera paths (pixels x spp) per render batch; LiDAR rays per scan batch */
     uint32_t batching;          /* 0: plain per-request launches                */
     uint32_t pad_;
     crt_batch_opts render;      /* launch parameters of every render launch (zeros = defaults) */

@@ -47,6 +47,22 @@ def test_alone_vs_mix_identical(meshes, ropts):
             else:   # another traversal form: only exact-tie hits may differ
                 assert np.mean(np.any(rgba != mixed[i]["rgba"], axis=-1)) < 1e-3
             break
+    # and so do plain flash-LiDAR and flux calls (the server stacks single /
+    # first / last; its LiDAR scans of one batch share a device buffer and
+    # come back through the worker's pinned staging)
+    seen = set()
+    for i, (_, r) in enumerate(reqs):
+        if r["kind"] in seen or r["kind"] == "render":
+            continue
+        seen.add(r["kind"])
+        if r["kind"] == "lidar":
+            o = sc[r["scene"]].flash_lidar(r["camera"], r["sub_rays"], r["sigma"], r["seed"])
+            assert np.array_equal(np.stack([o["single"], o["first"], o["last"]]), mixed[i]["lidar"])
+        else:
+            f, t, _, _ = sc[r["scene"]].radiation_pressure(r["sun_dir"], r["pressure"], r["rho_s"], r["rho_d"],
+                                                          r["grid"])
+            assert np.array_equal(np.concatenate([np.asarray(f), np.asarray(t)]), mixed[i]["force_torque"])
+    assert seen == {"lidar", "flux"}
     srv.close()
     alone_srv.close()
 
