@@ -23,12 +23,23 @@ __host__ __device__ inline uint64_t rng_mix64(uint64_t z) {
     return z ^ (z >> 31);
 }
 
-__host__ __device__ inline double rng_u01(uint64_t seed, uint64_t a, uint64_t b, uint64_t c) {
+// The first three rounds depend on (seed, a, b) only: a caller drawing
+// several numbers for one pixel and sample computes that key once and pays
+// one round per number (a 64-bit multiply is several VALU instructions).
+// rng_u01(seed, a, b, c) == rng_u01k(rng_key(seed, a, b), c), bit for bit.
+__host__ __device__ inline uint64_t rng_key(uint64_t seed, uint64_t a, uint64_t b) {
     uint64_t h = rng_mix64(seed + 0x9E3779B97F4A7C15ull);
     h = rng_mix64(h ^ a);
-    h = rng_mix64(h ^ (b + 0x632BE59BD9B4E019ull));
-    h = rng_mix64(h ^ (c * 0x8CB92BA72F3D8DD7ull + 1ull));
+    return rng_mix64(h ^ (b + 0x632BE59BD9B4E019ull));
+}
+
+__host__ __device__ inline double rng_u01k(uint64_t key, uint64_t c) {
+    const uint64_t h = rng_mix64(key ^ (c * 0x8CB92BA72F3D8DD7ull + 1ull));
     return (double)(h >> 11) * (1.0 / 9007199254740992.0);
+}
+
+__host__ __device__ inline double rng_u01(uint64_t seed, uint64_t a, uint64_t b, uint64_t c) {
+    return rng_u01k(rng_key(seed, a, b), c);
 }
 
 __host__ __device__ inline uint64_t rng_dim(int bounce, unsigned dim) { return (uint64_t)bounce * 64u + dim; }

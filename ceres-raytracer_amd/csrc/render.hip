@@ -191,8 +191,9 @@ __device__ inline bool gen_camera_path(const Ctx& C, const PassCtl& ctl, uint64_
         const int px = (int)(pix % (uint32_t)cd.v.w), py = (int)(pix / (uint32_t)cd.v.w);
         double u = px, v = py;
         if (cd.jitter) {
-            u += rng_u01(cd.seed, pix, smp, rng_dim(0, RNG_JITTER)) - 0.5;
-            v += rng_u01(cd.seed, pix, smp, rng_dim(0, RNG_JITTER + 1)) - 0.5;
+            const uint64_t key = rng_key(cd.seed, pix, smp);
+            u += rng_u01k(key, rng_dim(0, RNG_JITTER)) - 0.5;
+            v += rng_u01k(key, rng_dim(0, RNG_JITTER + 1)) - 0.5;
         }
         double d[3];
         camera_dir(cd.v, u, v, d);
@@ -328,8 +329,9 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_persistent(Ctx C, int qse
 }
 
 // ---- shading -----------------------------------------------------------
-__device__ inline void light_sample(const Ctx& C, uint32_t cam, int l, float3 p, uint64_t seed, uint32_t pix,
-                                    uint32_t smp, int bounce, float3& dir, float& tmax, float& irr) {
+// `key` = rng_key(camera seed, pixel, sample)
+__device__ inline void light_sample(const Ctx& C, uint32_t cam, int l, float3 p, uint64_t key, int bounce,
+                                    float3& dir, float& tmax, float& irr) {
     const LightDev& L = C.cams[cam].lights[l];
     if (L.type == CRT_LIGHT_POINT) {
         // src/crt/lights/point_light.hpp:32-39
@@ -342,8 +344,8 @@ __device__ inline void light_sample(const Ctx& C, uint32_t cam, int l, float3 p,
     } else if (L.type == CRT_LIGHT_AREA) {
         // src/crt/lights/area_light.hpp:54-72: uniform point of the local XY
         // rectangle, placed with R (transform.hpp:31-40); I / d^2, no cosine
-        const float r1 = (float)rng_u01(seed, pix, smp, rng_dim(bounce, RNG_LIGHT + 2 * l));
-        const float r2 = (float)rng_u01(seed, pix, smp, rng_dim(bounce, RNG_LIGHT + 2 * l + 1));
+        const float r1 = (float)rng_u01k(key, rng_dim(bounce, RNG_LIGHT + 2 * l));
+        const float r2 = (float)rng_u01k(key, rng_dim(bounce, RNG_LIGHT + 2 * l + 1));
         const float x = L.size[0] * r1 - L.size[0] / 2, y = L.size[1] * r2 - L.size[1] / 2;
         const float3 sp = make_float3(L.rot[0] * x + L.rot[1] * y + L.pos[0], L.rot[3] * x + L.rot[4] * y + L.pos[1],
                                       L.rot[6] * x + L.rot[7] * y + L.pos[2]);
@@ -405,11 +407,11 @@ __device__ inline void shade_hit(const Ctx& C, int bounce, uint32_t k, uint32_t 
     float3 p = make_float3(u * b.x + v * c.x + w * a.x, u * b.y + v * c.y + w * a.y, u * b.z + v * c.z + w * a.z);
     p = make_float3(p.x - C.offset * gn.x, p.y - C.offset * gn.y, p.z - C.offset * gn.z);
     const float* col = s.inst_color + 3 * (inst >= 1 && inst <= s.n_instances ? inst - 1 : 0);
-    const uint64_t seed = C.cams[cam].seed;
+    const uint64_t key = rng_key(C.cams[cam].seed, pix, smp);
     for (int l = 0; l < C.nl; ++l) {
         float3 ld;
         float tmax, irr;
-        light_sample(C, cam, l, p, seed, pix, smp, bounce, ld, tmax, irr);
+        light_sample(C, cam, l, p, key, bounce, ld, tmax, irr);
         // ColoredLambertianMaterial::compute: colour * (-L.n_s), no max(0, .)
         const float ldn = -dot3(ld, sn);
         ShadowRec r;
@@ -419,8 +421,8 @@ __device__ inline void shade_hit(const Ctx& C, int bounce, uint32_t k, uint32_t 
         C.shadow[ls * (uint32_t)C.nl + (uint32_t)l] = r;
     }
     if (bounce < C.nb - 1) {
-        const float r1 = (float)rng_u01(seed, pix, smp, rng_dim(bounce, RNG_BRDF));
-        const float r2 = (float)rng_u01(seed, pix, smp, rng_dim(bounce, RNG_BRDF + 1));
+        const float r1 = (float)rng_u01k(key, rng_dim(bounce, RNG_BRDF));
+        const float r2 = (float)rng_u01k(key, rng_dim(bounce, RNG_BRDF + 1));
         const float3 nd = cosine_sample(sn, r1, r2);
         C.st[k].o = make_float4(p.x, p.y, p.z, 0.f);
         C.st[k].d = make_float4(nd.x, nd.y, nd.z, 0.f);

@@ -88,7 +88,8 @@ __device__ inline void flash_dir(const FlashParams& F, uint32_t beam, uint32_t k
     const double l1 = sqrt(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]);
     e1[0] /= l1; e1[1] /= l1; e1[2] /= l1;
     const double e2[3] = {d[1] * e1[2] - d[2] * e1[1], d[2] * e1[0] - d[0] * e1[2], d[0] * e1[1] - d[1] * e1[0]};
-    const double r1 = rng_u01(F.seed, beam, k, 0), r2 = rng_u01(F.seed, beam, k, 1);
+    const uint64_t key = rng_key(F.seed, beam, k);
+    const double r1 = rng_u01k(key, 0), r2 = rng_u01k(key, 1);
     // Box-Muller (1 - r1 in (0, 1]: its FP32 rounding stays > 0)
     const float rad = sqrtf(-2.0f * logf((float)(1.0 - r1)));
     float sn, cs;
