@@ -17,6 +17,7 @@ import pytest
 
 import oracle
 from crt import capi, scenes
+from flash_ref import flash_subrays
 
 pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -172,33 +173,6 @@ def test_flash_lidar(icosphere):
     assert np.median(spread) > 0
 
 
-def _flash_subrays(cam, seed, sub_rays, sigma):
-    """Sub-ray directions of every beam restated in FP64 numpy: boresight d
-    of pixel (x, y), tangent frame e1 = d x a (a = x axis, or y when
-    |d.x| > 0.9), e2 = d x e1, Gaussian angles by Box-Muller from the
-    counter RNG u01(seed, beam, k, 0/1), direction d + tan(s g1) e1 +
-    tan(s g2) e2 normalised.  Returns (origins, dirs) of shape (beams, K, 3)."""
-    o, d = oracle.camera_rays(cam)
-    nb = d.shape[0]
-    ax = np.zeros_like(d)
-    ax[:, 0] = 1.0
-    big = np.abs(d[:, 0]) > 0.9
-    ax[big] = [0.0, 1.0, 0.0]
-    e1 = np.cross(d, ax)
-    e1 /= np.linalg.norm(e1, axis=1, keepdims=True)
-    e2 = np.cross(d, e1)
-    dirs = np.empty((nb, sub_rays, 3))
-    for b in range(nb):
-        for k in range(1, sub_rays + 1):
-            r1, r2 = oracle.u01(seed, b, k, 0), oracle.u01(seed, b, k, 1)
-            rad = math.sqrt(-2.0 * math.log(1.0 - r1))
-            a1 = math.tan(sigma * rad * math.cos(2.0 * math.pi * r2))
-            a2 = math.tan(sigma * rad * math.sin(2.0 * math.pi * r2))
-            v = d[b] + a1 * e1[b] + a2 * e2[b]
-            dirs[b, k - 1] = v / np.linalg.norm(v)
-    return np.repeat(o[:, None, :], sub_rays, axis=1), dirs
-
-
 @pytest.mark.parametrize("K,res", [(1, 48), (8, 48), (23, 47), (63, 40)])
 def test_flash_lidar_returns_match_oracle(icosphere, K, res):
     # every sub-ray traced by the FP64 oracle: first / last return = min /
@@ -217,7 +191,7 @@ def test_flash_lidar_returns_match_oracle(icosphere, K, res):
     hit0 &= out["single"] > 0
     rel0 = np.abs(out["single"][hit0] - c["depth"][hit0]) / c["depth"][hit0]
     assert rel0.max() < 1e-5
-    o, d = _flash_subrays(beams, seed, K, sigma)
+    o, d = flash_subrays(beams, seed, K, sigma)
     t, tri, u, v = cpu.trace(o.reshape(-1, 3), d.reshape(-1, 3))
     t, tri = t.reshape(-1, K), tri.reshape(-1, K)
     w = 1.0 - u - v

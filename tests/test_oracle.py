@@ -179,6 +179,38 @@ def test_counter_rng_uniform_and_decorrelated():
     assert oracle.u01(5, 1, 2, 3) == oracle.u01(5, 1, 2, 3)
 
 
+def test_flash_subray_angles_are_gaussian():
+    # the restated flash sub-ray directions: each sub-ray's two angles in its
+    # beam's tangent frame, recovered as atan(v.e / v.d), must be independent
+    # N(0, sigma^2) draws -- a dropped factor, a swapped sin / cos, a reused
+    # uniform or a frame that is not orthonormal fails one of these checks
+    from flash_ref import flash_subrays, tangent_frame
+    R = scenes.look_at_rotation([3.0e4, -2.0e4, 1.0e5])
+    cam = {"position": np.array([3.0e4, -2.0e4, 1.0e5]), "rotation": R, "focal_length": 1.0,
+           "resolution": [32, 32], "sensor_size": [0.05, 0.05], "z_positive": False}
+    sigma, K = 5e-4, 31
+    _, dirs = flash_subrays(cam, 4, K, sigma)
+    _, d = oracle.camera_rays(cam)
+    e1, e2 = tangent_frame(d)
+    assert np.allclose(np.einsum("ij,ij->i", e1, d), 0.0, atol=1e-14)
+    assert np.allclose(np.einsum("ij,ij->i", e1, e2), 0.0, atol=1e-14)
+    vd = np.einsum("bkj,bj->bk", dirs, d)
+    g1 = (np.arctan(np.einsum("bkj,bj->bk", dirs, e1) / vd) / sigma).ravel()
+    g2 = (np.arctan(np.einsum("bkj,bj->bk", dirs, e2) / vd) / sigma).ravel()
+    n = g1.size
+    for g in (g1, g2):
+        assert abs(g.mean()) < 4.0 / math.sqrt(n)
+        assert abs(g.std() - 1.0) < 0.03
+        assert abs(np.mean(g ** 4) / np.mean(g ** 2) ** 2 - 3.0) < 0.2      # Gaussian kurtosis
+        assert 0.0015 < np.mean(np.abs(g) > 3.0) < 0.0040                  # 2.7e-3 beyond 3 sigma
+    assert abs(np.corrcoef(g1, g2)[0, 1]) < 0.03
+    # the radial angle is Rayleigh: E[g1^2 + g2^2] = 2
+    assert abs(np.mean(g1 ** 2 + g2 ** 2) - 2.0) < 0.06
+    # another seed draws other angles
+    _, dirs_b = flash_subrays(cam, 5, 2, sigma)
+    assert not np.allclose(dirs_b, dirs[:, :2])
+
+
 # ---------------------------------------------------------------- radiation pressure
 def test_oracle_srp_sphere_closed_form():
     m = scenes.displaced_icosphere(6, radius=1000.0, amplitude=0.0)
