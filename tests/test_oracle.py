@@ -179,6 +179,31 @@ def test_counter_rng_uniform_and_decorrelated():
     assert oracle.u01(5, 1, 2, 3) == oracle.u01(5, 1, 2, 3)
 
 
+def test_oracle_directional_light_is_the_reference_far_point_light():
+    # The directional sun has no reference counterpart; the reference models
+    # the sun as a PointLight (src/crt/lights/point_light.hpp:32-39) far
+    # away.  At identical RNG keys the oracle's directional light of
+    # intensity I along s must render like its point light at 1e11 s with
+    # intensity I * 1e22 (direction difference < 1e-10 rad; the point light's
+    # 1e4 irradiance cap is not reached), and the point light is the one the
+    # reference radiance statistics pin
+    sc = scenes.cornell_box(0)
+    v, i, inst = sc.merged()
+    s = oracle.Scene(v, i, inst, oracle.vertex_normals(v, i), np.array(sc.colors))
+    cam = scenes.cornell_camera()
+    cam["resolution"] = [48, 48]
+    sd = np.array([0.3, 0.2, 1.0])
+    sd /= np.linalg.norm(sd)
+    area = dict(scenes.cornell_light(), type="area")
+    a, _ = s.render(cam, [area, {"type": "directional", "intensity": 0.2, "direction": sd}], 2, 2, 0.0, 3, seed=7)
+    b, _ = s.render(cam, [area, {"type": "point", "intensity": 0.2e22, "position": sd * 1e11}], 2, 2, 0.0, 3, seed=7)
+    assert np.abs(a).max() > 0
+    assert np.allclose(a, b, rtol=1e-5, atol=1e-7)
+    # and the sun actually lights part of the scene (the box is open at the front)
+    c, _ = s.render(cam, [area], 2, 2, 0.0, 3, seed=7)
+    assert np.mean(np.any(np.abs(a - c) > 1e-6, axis=-1)) > 0.01
+
+
 def test_flash_subray_angles_are_gaussian():
     # the restated flash sub-ray directions: each sub-ray's two angles in its
     # beam's tangent frame, recovered as atan(v.e / v.d), must be independent
