@@ -107,6 +107,19 @@ struct PathState {
     uint32_t pad[2];
 };
 
+// Path state in HBM as planes (structure of arrays): a wave's paths are
+// nearly consecutive ids, so each plane's 16-B loads of a wave touch a few
+// contiguous cache lines instead of one line per lane for a 96-B struct.
+struct PathPlanes {
+    float4* o;     // origin
+    float4* d;     // direction
+    float4* w;     // weight rgb, pending bounce factor (1 - r1)
+    float4* L;     // radiance rgb
+    uint4* m;      // camera, pixel, sample (1-based), hit slot (int, -1 = none)
+    float2* h;     // hit barycentrics u, v
+};
+constexpr size_t PATH_BYTES = 5 * sizeof(float4) + sizeof(float2);   // per path over all planes
+
 struct ShadowRec {
     float4 o;      // origin, tmax
     float4 d;      // direction, origin triangle (leaf order, bits; tested first)
@@ -126,7 +139,7 @@ struct Ctx {
     const Segment* segs;
     const PassCtl* ctl;
     const uint32_t* plist;
-    PathState* st;
+    PathPlanes st;
     uint32_t* queue[2];
     uint32_t* live;
     ShadowRec* shadow;
@@ -227,7 +240,12 @@ __global__ __launch_bounds__(TB) void k_raygen(Ctx C) {
     if (k < ctl.npath) {
         PathState ps;
         alive = gen_camera_path(C, ctl, k, ps);
-        C.st[k] = ps;
+        C.st.o[k] = ps.o;
+        C.st.d[k] = ps.d;
+        C.st.w[k] = ps.w;
+        C.st.L[k] = ps.L;
+        C.st.m[k] = make_uint4(ps.cam, ps.pix, ps.sample, (uint32_t)ps.slot);
+        C.st.h[k] = make_float2(ps.hu, ps.hv);
     }
     const uint32_t slot = wave_append(alive, C.cnt + 0);
     if (alive) C.queue[0][slot] = (uint32_t)k;
@@ -235,13 +253,12 @@ __global__ __launch_bounds__(TB) void k_raygen(Ctx C) {
 
 // ---- traversal kernels -------------------------------------------------
 __device__ inline void extend_one(const Ctx& C, uint32_t k, uint32_t* sref, float* sd, int sub, int lane_base) {
-    const float4 o = C.st[k].o, d = C.st[k].d;
+    const float4 o = C.st.o[k], d = C.st.d[k];
     const RayPre r = make_ray_pre(o.x, o.y, o.z, d.x, d.y, d.z);
     const HitRec h = trace_octet(C.s, r, 0.0f, __int_as_float(0x7f800000), sref, sd, sub, lane_base);
     if (sub == 0) {
-        C.st[k].slot = h.slot;
-        C.st[k].hu = h.u;
-        C.st[k].hv = h.v;
+        C.st.m[k].w = (uint32_t)h.slot;
+        C.st.h[k] = make_float2(h.u, h.v);
     }
 }
 
@@ -304,12 +321,11 @@ CRT_LANE_KERNEL void k_extend_lane(Ctx C, int qsel) {
     const uint32_t q = blockIdx.x * LANE_BLOCK + threadIdx.x;
     if (q >= C.cnt[qsel]) return;
     const uint32_t k = C.queue[qsel][q];
-    const float4 o = C.st[k].o, d = C.st[k].d;
+    const float4 o = C.st.o[k], d = C.st.d[k];
     const RayPre r = make_ray_pre(o.x, o.y, o.z, d.x, d.y, d.z);
     const HitRec h = trace_lane<false>(C.s, r, 0.0f, __int_as_float(0x7f800000), stk);
-    C.st[k].slot = h.slot;
-    C.st[k].hu = h.u;
-    C.st[k].hv = h.v;
+    C.st.m[k].w = (uint32_t)h.slot;
+    C.st.h[k] = make_float2(h.u, h.v);
 }
 
 // Persistent form: octets claim `grab` consecutive extension rays per atomic.
@@ -424,9 +440,9 @@ __device__ inline void shade_hit(const Ctx& C, int bounce, uint32_t k, uint32_t 
         const float r1 = (float)rng_u01k(key, rng_dim(bounce, RNG_BRDF));
         const float r2 = (float)rng_u01k(key, rng_dim(bounce, RNG_BRDF + 1));
         const float3 nd = cosine_sample(sn, r1, r2);
-        C.st[k].o = make_float4(p.x, p.y, p.z, 0.f);
-        C.st[k].d = make_float4(nd.x, nd.y, nd.z, 0.f);
-        C.st[k].w.w = 1.0f - r1;
+        C.st.o[k] = make_float4(p.x, p.y, p.z, 0.f);
+        C.st.d[k] = make_float4(nd.x, nd.y, nd.z, 0.f);
+        C.st.w[k].w = 1.0f - r1;
     }
 }
 
@@ -435,17 +451,18 @@ __global__ __launch_bounds__(TB) void k_shade(Ctx C, int bounce, int qsel) {
     const uint32_t n = C.cnt[qsel];
     bool has_hit = false;
     uint32_t k = 0;
-    PathState ps;
+    uint4 m = make_uint4(0u, 0u, 0u, 0u);
     if (q < n) {
         k = C.queue[qsel][q];
-        ps = C.st[k];
-        if (ps.slot < 0) C.samples[k] = ps.L;   // path ends without a hit
+        m = C.st.m[k];
+        if ((int32_t)m.w < 0) C.samples[k] = C.st.L[k];   // path ends without a hit
         else has_hit = true;
     }
     const uint32_t ls = wave_append(has_hit, C.cnt + 3);   // live slot = shadow slots ls*nl .. ls*nl+nl-1
     if (has_hit) C.live[ls] = k;
     if (!has_hit) return;
-    shade_hit(C, bounce, k, ps.cam, ps.pix, ps.sample, ps.slot, ps.hu, ps.hv, ls);
+    const float2 hb = C.st.h[k];
+    shade_hit(C, bounce, k, m.x, m.y, m.z, (int32_t)m.w, hb.x, hb.y, ls);
 }
 
 // After k_shade: shadow rays = nl per live path; tally traced rays.
@@ -462,21 +479,20 @@ __global__ void k_tally(Ctx C, int qsel) {
 // pending bounce factor.  Returns true when the path goes on to the next
 // bounce; after the last bounce the path's sample is written.
 __device__ inline bool accumulate_path(const Ctx& C, int bounce, uint32_t k, float lr[3]) {
-    PathState& ps = C.st[k];
     if (bounce >= 1)
         for (int j = 0; j < 3; ++j) lr[j] = fminf(fmaxf(lr[j], 0.0f), 1.0f);
-    float4 L = ps.L;
-    float4 w = ps.w;
+    float4 L = C.st.L[k];
+    float4 w = C.st.w[k];
     L.x += lr[0] * w.x;
     L.y += lr[1] * w.y;
     L.z += lr[2] * w.z;
-    ps.L = L;
+    C.st.L[k] = L;
     if (bounce == C.nb - 1) {
         C.samples[k] = L;
         return false;
     }
     w.x *= w.w; w.y *= w.w; w.z *= w.w;
-    ps.w = w;
+    C.st.w[k] = w;
     return true;
 }
 
@@ -512,11 +528,11 @@ __global__ __launch_bounds__(TB) void k_accumulate(Ctx C, int bounce, int next) 
 // traffic and fewer launches per bounce.
 __device__ inline void finish_extend(const Ctx& C, int bounce, uint32_t k, const HitRec& h, uint32_t ls) {
     if (h.slot < 0) {
-        C.samples[k] = C.st[k].L;   // path ends without a hit
+        C.samples[k] = C.st.L[k];   // path ends without a hit
         return;
     }
     C.live[ls] = k;
-    const uint4 m = *reinterpret_cast<const uint4*>(&C.st[k].cam);   // cam, pix, sample
+    const uint4 m = C.st.m[k];   // cam, pix, sample
     shade_hit(C, bounce, k, m.x, m.y, m.z, h.slot, h.u, h.v, ls);
 }
 
@@ -535,9 +551,9 @@ CRT_LANE_KERNEL void k_extend_shade_lane(Ctx C, int bounce, int qsel, int gen) {
             if (alive) {
                 // the ray itself stays in registers; the next bounce's ray is
                 // written by the shading
-                C.st[k].w = ps.w;
-                C.st[k].L = ps.L;
-                *reinterpret_cast<uint4*>(&C.st[k].cam) = make_uint4(ps.cam, ps.pix, ps.sample, 0u);
+                C.st.w[k] = ps.w;
+                C.st.L[k] = ps.L;
+                C.st.m[k] = make_uint4(ps.cam, ps.pix, ps.sample, 0u);
                 o = ps.o;
                 d = ps.d;
             }
@@ -545,8 +561,8 @@ CRT_LANE_KERNEL void k_extend_shade_lane(Ctx C, int bounce, int qsel, int gen) {
         (void)wave_append(alive, C.cnt + qsel);   // count of traced camera rays (tally)
     } else if (q < C.cnt[qsel]) {
         k = C.queue[qsel][q];
-        o = C.st[k].o;
-        d = C.st[k].d;
+        o = C.st.o[k];
+        d = C.st.d[k];
         alive = true;
     }
     if (!alive) return;
@@ -575,7 +591,7 @@ void k_extend_shade_octet(Ctx C, int bounce, int qsel, int R) {
         const uint32_t q = q0 + (uint32_t)p;
         if (q >= n) break;   // octet-uniform
         const uint32_t k = C.queue[qsel][q];
-        const float4 o = C.st[k].o, d = C.st[k].d;
+        const float4 o = C.st.o[k], d = C.st.d[k];
         const RayPre r = make_ray_pre(o.x, o.y, o.z, d.x, d.y, d.z);
         const HitRec h = trace_octet(C.s, r, 0.0f, __int_as_float(0x7f800000), sref, sd, sub, lane_base);
         if (sub == p) {   // the hit is octet-uniform
@@ -775,7 +791,7 @@ void delete_render_workspace(RenderWorkspace* w) { delete w; }
 
 size_t render_bytes_estimate(uint64_t pixels, int spp, int n_lights) {
     const uint64_t paths = std::min<uint64_t>(pixels * (uint64_t)std::max(spp, 1), (uint64_t)1 << 23);
-    return (size_t)(paths * (sizeof(PathState) + 12 + 16 + std::max(n_lights, 1) * sizeof(ShadowRec)) +
+    return (size_t)(paths * (PATH_BYTES + 12 + 16 + std::max(n_lights, 1) * sizeof(ShadowRec)) +
                     pixels * (16 + 4 + 1 + 4 + 16 + 4));
 }
 
@@ -892,7 +908,7 @@ int render_batch(crt_scene* scene, uint32_t n_cams, const crt_camera* cams, cons
     spp.reserve((size_t)P * 4);
     active.reserve(P);
     counters.reserve(32 * 4);
-    pstate.reserve(cap_paths * sizeof(PathState));
+    pstate.reserve(cap_paths * PATH_BYTES);
     q0.reserve(cap_paths * 4);
     q1.reserve(cap_paths * 4);
     live.reserve(cap_paths * 4);
@@ -908,7 +924,15 @@ int render_batch(crt_scene* scene, uint32_t n_cams, const crt_camera* cams, cons
     C.segs = d_segs.as<Segment>();
     C.ctl = d_ctl.as<PassCtl>();
     C.plist = d_plist.as<uint32_t>();
-    C.st = pstate.as<PathState>();
+    {
+        float4* base = pstate.as<float4>();
+        C.st.o = base;
+        C.st.d = base + cap_paths;
+        C.st.w = base + 2 * cap_paths;
+        C.st.L = base + 3 * cap_paths;
+        C.st.m = reinterpret_cast<uint4*>(base + 4 * cap_paths);
+        C.st.h = reinterpret_cast<float2*>(base + 5 * cap_paths);
+    }
     C.queue[0] = q0.as<uint32_t>();
     C.queue[1] = q1.as<uint32_t>();
     C.live = live.as<uint32_t>();
