@@ -182,6 +182,20 @@ __device__ inline uint32_t seg_search(uint32_t nseg, uint64_t x, F key) {
     return lo;
 }
 
+// The same for callers whose lanes hold ascending x (consecutive slots of a
+// pass): one wave-uniform search from the first active lane, whose loads
+// are scalar, then each lane steps over the few segment starts inside its
+// wave; a lane below the first lane's segment (not expected) searches alone.
+template <typename F>
+__device__ inline uint32_t seg_search_wave(uint32_t nseg, uint64_t x, F key) {
+    const uint64_t x0 = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(x >> 32)) << 32) |
+                        (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)x);
+    uint32_t lo = seg_search(nseg, x0, key);
+    if (key(lo) > x) return seg_search(nseg, x, key);
+    while (lo + 1 < nseg && key(lo + 1) <= x) ++lo;
+    return lo;
+}
+
 __global__ void k_clear(uint32_t* cnt, uint32_t mask) {
     const uint32_t i = threadIdx.x;
     if (i < 32 && ((mask >> i) & 1u)) cnt[i] = 0u;
@@ -193,7 +207,7 @@ __global__ void k_clear(uint32_t* cnt, uint32_t mask) {
 __device__ inline bool gen_camera_path(const Ctx& C, const PassCtl& ctl, uint64_t k, PathState& ps) {
     bool alive = false;
     {
-        const uint32_t si = seg_search(ctl.nseg, k, [&](uint32_t i) { return C.segs[i].path_begin; });
+        const uint32_t si = seg_search_wave(ctl.nseg, k, [&](uint32_t i) { return C.segs[i].path_begin; });
         const Segment sg = C.segs[si];
         const uint64_t local = k - sg.path_begin;
         const uint32_t i = (uint32_t)(local / sg.s_count);
@@ -654,7 +668,7 @@ __global__ __launch_bounds__(TB) void k_fold(Ctx C) {
     const uint32_t t = blockIdx.x * TB + threadIdx.x;
     const PassCtl ctl = *C.ctl;
     if (t >= ctl.nfold) return;
-    const uint32_t si = seg_search(ctl.nseg, t, [&](uint32_t i) { return (uint64_t)C.segs[i].fold_begin; });
+    const uint32_t si = seg_search_wave(ctl.nseg, t, [&](uint32_t i) { return (uint64_t)C.segs[i].fold_begin; });
     const Segment sg = C.segs[si];
     const uint32_t i = t - sg.fold_begin;
     const CamDev& cd = C.cams[sg.cam];
