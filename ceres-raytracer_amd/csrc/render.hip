@@ -209,9 +209,9 @@ __device__ inline bool gen_camera_path(const Ctx& C, const PassCtl& ctl, uint64_
     {
         const uint32_t si = seg_search_wave(ctl.nseg, k, [&](uint32_t i) { return C.segs[i].path_begin; });
         const Segment sg = C.segs[si];
-        const uint64_t local = k - sg.path_begin;
-        const uint32_t i = (uint32_t)(local / sg.s_count);
-        const uint32_t smp = sg.s_first + (uint32_t)(local % sg.s_count);
+        const uint32_t local = (uint32_t)(k - sg.path_begin);   // < paths per pass < 2^31
+        const uint32_t i = local / sg.s_count;
+        const uint32_t smp = sg.s_first + local % sg.s_count;
         const CamDev& cd = C.cams[sg.cam];
         const uint32_t pix = sg.plist_off == NO_LIST ? tile_order(i, (uint32_t)cd.v.w, (uint32_t)cd.v.h)
                                                      : C.plist[sg.plist_off + i];
