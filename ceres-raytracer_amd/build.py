@@ -29,12 +29,15 @@ def _run(cmd):
 
 
 def build_hip():
-    objs = []
+    from concurrent.futures import ThreadPoolExecutor
     os.makedirs(os.path.join(HERE, "_build"), exist_ok=True)
-    for src in HIP_SOURCES:
-        obj = os.path.join(HERE, "_build", src.replace(".hip", ".o"))
-        _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
-        objs.append(obj)
+    objs = [os.path.join(HERE, "_build", src.replace(".hip", ".o")) for src in HIP_SOURCES]
+    jobs = max(1, min(len(HIP_SOURCES), int(os.environ.get("MAX_JOBS", "0") or 0) or (os.cpu_count() or 1)))
+    # translation units compile independently; the link waits for all of them
+    with ThreadPoolExecutor(max_workers=jobs) as pool:
+        for f in [pool.submit(_run, [HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+                  for src, obj in zip(HIP_SOURCES, objs)]:
+            f.result()
     tmp = LIB_HIP + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, *objs, "-lpthread", "-o", tmp])
     os.replace(tmp, LIB_HIP)

@@ -120,6 +120,7 @@ SIGNATURES = {
     "crt_last_error": (C.c_char_p, []),
     "crt_version": (C.c_int, []),
     "crt_scene_create": (C.c_int, [_p, C.c_uint32, _p, C.c_uint32, _p, _p, C.c_uint32, _p, _p, _p, _p]),
+    "crt_scene_create_f64": (C.c_int, [_p, C.c_uint32, _p, C.c_uint32, _p, _p, C.c_uint32, _p, _p, _p, _p]),
     "crt_scene_info": (C.c_int, [_p, _p]),
     "crt_scene_destroy": (None, [_p]),
     "crt_scene_serialize": (C.c_int, [_p, _p, _p]),
@@ -319,9 +320,11 @@ class Scene:
     """A mesh + GPU-built BVH resident in HBM (crt_scene_create)."""
 
     def __init__(self, vertices, indices, tri_instance=None, vertex_normals=None, colors=None, smooth=None,
-                 max_leaf_size=8, ploc=False, greedy_wide=False):
+                 max_leaf_size=8, ploc=False, greedy_wide=False, f64=False):
+        """f64=True hands the vertices over in FP64 (crt_scene_create_f64):
+        the frame origin is subtracted before the rounding to FP32."""
         init()
-        v = np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1, 3)
+        v = np.ascontiguousarray(vertices, dtype=np.float64 if f64 else np.float32).reshape(-1, 3)
         i = np.ascontiguousarray(indices, dtype=np.uint32).reshape(-1, 3)
         inst = None if tri_instance is None else np.ascontiguousarray(tri_instance, dtype=np.uint32)
         vn = None if vertex_normals is None else np.ascontiguousarray(vertex_normals, dtype=np.float32)
@@ -332,8 +335,9 @@ class Scene:
         sm = None if smooth is None else np.ascontiguousarray(smooth, dtype=np.uint8)
         opts = BuildOpts(max_leaf_size, (BUILD_PLOC if ploc else 0) | (BUILD_GREEDY_WIDE if greedy_wide else 0))
         h = C.c_void_p()
-        _check(load().crt_scene_create(ptr(v), v.shape[0], ptr(i), i.shape[0], ptr(inst), ptr(vn), ni, ptr(col),
-                                       ptr(sm), C.byref(opts), C.byref(h)))
+        create = load().crt_scene_create_f64 if f64 else load().crt_scene_create
+        _check(create(ptr(v), v.shape[0], ptr(i), i.shape[0], ptr(inst), ptr(vn), ni, ptr(col), ptr(sm),
+                      C.byref(opts), C.byref(h)))
         self.h = h
         self.n_tris = i.shape[0]
 

@@ -833,8 +833,13 @@ __global__ __launch_bounds__(BLK) void k_compress(const float4* __restrict__ nod
             q[3 + a][k >> 2] |= (uint32_t)qh << sh;
         }
     }
+    // byte 3: mask of the non-empty slots, so traversal never depends on the
+    // inverted box of an empty slot failing the slab test (a node of extent
+    // below the slab test's relative slack could otherwise let one through)
+    uint32_t valid = 0u;
+    for (int k = 0; k < 8; ++k) valid |= (refs[k] != REF_EMPTY) ? (1u << k) : 0u;
     const uint32_t ew = ((uint32_t)(uint8_t)(int8_t)e[0]) | ((uint32_t)(uint8_t)(int8_t)e[1] << 8) |
-                        ((uint32_t)(uint8_t)(int8_t)e[2] << 16);
+                        ((uint32_t)(uint8_t)(int8_t)e[2] << 16) | (valid << 24);
     float4* o = cnodes + (size_t)i * 6u;
     o[0] = make_float4(lo[0], lo[1], lo[2], __uint_as_float(ew));
     o[1] = make_float4(__uint_as_float(q[0][0]), __uint_as_float(q[0][1]), __uint_as_float(q[1][0]),

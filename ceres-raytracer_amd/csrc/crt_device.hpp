@@ -173,17 +173,27 @@ __device__ inline bool tri_test(const RayPre& r, float4 a, float4 b, float4 c,
     const float Akx = sel3(Ax0, Ay0, Az0, r.kx), Aky = sel3(Ax0, Ay0, Az0, r.ky), Akz = sel3(Ax0, Ay0, Az0, r.kz);
     const float Bkx = sel3(Bx0, By0, Bz0, r.kx), Bky = sel3(Bx0, By0, Bz0, r.ky), Bkz = sel3(Bx0, By0, Bz0, r.kz);
     const float Ckx = sel3(Cx0, Cy0, Cz0, r.kx), Cky = sel3(Cx0, Cy0, Cz0, r.ky), Ckz = sel3(Cx0, Cy0, Cz0, r.kz);
-    // shear (per vertex: identical bits for a vertex shared by two triangles)
-    const float Ax = __fsub_rn(Akx, __fmul_rn(r.sx, Akz));
-    const float Ay = __fsub_rn(Aky, __fmul_rn(r.sy, Akz));
-    const float Bx = __fsub_rn(Bkx, __fmul_rn(r.sx, Bkz));
-    const float By = __fsub_rn(Bky, __fmul_rn(r.sy, Bkz));
-    const float Cx = __fsub_rn(Ckx, __fmul_rn(r.sx, Ckz));
-    const float Cy = __fsub_rn(Cky, __fmul_rn(r.sy, Ckz));
-    // edge functions, never contracted into FMA (exact antisymmetry on shared edges)
-    float U = __fsub_rn(__fmul_rn(Cx, By), __fmul_rn(Cy, Bx));
-    float V = __fsub_rn(__fmul_rn(Ax, Cy), __fmul_rn(Ay, Cx));
-    float W = __fsub_rn(__fmul_rn(Bx, Ay), __fmul_rn(By, Ax));
+    // shear: one explicit FMA per coordinate, so a vertex shared by two
+    // triangles gets identical bits in both
+    const float Ax = fmaf(-r.sx, Akz, Akx);
+    const float Ay = fmaf(-r.sy, Akz, Aky);
+    const float Bx = fmaf(-r.sx, Bkz, Bkx);
+    const float By = fmaf(-r.sy, Bkz, Bky);
+    const float Cx = fmaf(-r.sx, Ckz, Ckx);
+    const float Cy = fmaf(-r.sy, Ckz, Cky);
+    // Edge functions: both products rounded, then subtracted.  HIP's
+    // __fmul_rn/__fsub_rn are plain operators that hipcc's default
+    // -ffp-contract=fast-honor-pragmas fuses into fma(a, b, -round(c d)),
+    // which breaks the exact antisymmetry on shared edges (cracks) and gives
+    // a zero-area triangle nonzero edge functions of one sign (a false hit);
+    // the pragma keeps this block unfused.
+    float U, V, W;
+    {
+#pragma clang fp contract(off)
+        U = Cx * By - Cy * Bx;
+        V = Ax * Cy - Ay * Cx;
+        W = Bx * Ay - By * Ax;
+    }
     if (U == 0.0f || V == 0.0f || W == 0.0f) {
         const double CxBy = (double)Cx * (double)By, CyBx = (double)Cy * (double)Bx;
         const double AxCy = (double)Ax * (double)Cy, AyCx = (double)Ay * (double)Cx;
@@ -441,6 +451,7 @@ __device__ inline LaneStack lane_stack(const SceneView& s) {
 // Compressed wide node (lane traversal), 96 B = 6 x 16 B, written by the
 // build from the 256-B node with the same child order:
 //   [0] origin p.xyz (float), w = exponents ex | ey << 8 | ez << 16 (int8)
+//       | valid << 24 (bit k set: slot k holds a child)
 //   [1] qlo_x[0..7] | qlo_y[0..7]      (u8, child k in byte k)
 //   [2] qlo_z[0..7] | qhi_x[0..7]
 //   [3] qhi_y[0..7] | qhi_z[0..7]
@@ -467,6 +478,7 @@ __device__ inline uint32_t cnode_test(const float4* __restrict__ cn, const RayPr
     const uint4 r0 = __builtin_bit_cast(uint4, cn[4]);
     const uint4 r1 = __builtin_bit_cast(uint4, cn[5]);
     const uint32_t ew = __float_as_uint(h.w);
+    mask &= ew >> 24;   // empty slots are never tested
     const float sx = __int_as_float(((int)(int8_t)(ew & 0xFFu) + 127) << 23);
     const float sy = __int_as_float(((int)(int8_t)((ew >> 8) & 0xFFu) + 127) << 23);
     const float sz = __int_as_float(((int)(int8_t)((ew >> 16) & 0xFFu) + 127) << 23);
@@ -495,8 +507,7 @@ __device__ inline uint32_t cnode_test(const float4* __restrict__ cn, const RayPr
         const float tfz = fmaf(cvt_u8(hi4 ? fz1 : fz0, byte), Az, Bz);
         const float tnear = fmaxf(max3f(tnx, tny, tnz), tmin);
         const float tfar = fminf(min3f(tfx, tfy, tfz), best);
-        // empty slots are stored inverted (qlo 255, qhi 0): with the octant's
-        // near/far planes their tnear exceeds tfar, so they never hit
+        // empty slots are out of `mask` (and stored inverted, qlo 255, qhi 0)
         const bool hk = ((mask >> k) & 1u) && (tnear <= tfar * 1.00000024f);
         hits |= hk ? (1u << k) : 0u;
         // branch-free nearest / second nearest

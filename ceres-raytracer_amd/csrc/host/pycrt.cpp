@@ -191,7 +191,11 @@ struct GpuScene {
 
 std::shared_ptr<GpuScene> build_scene(const std::vector<const Entity*>& ents, uint32_t max_leaf) {
     ensure_device();
-    std::vector<float> V, VN, col;
+    // vertices stay FP64 up to the library, which rounds them to FP32 only
+    // after moving them into the scene frame (an entity 1e7 m out keeps its
+    // own resolution, as the reference's double geometry does)
+    std::vector<double> V;
+    std::vector<float> VN, col;
     std::vector<uint32_t> I, inst;
     std::vector<uint8_t> sm;
     uint32_t base = 0;
@@ -202,9 +206,9 @@ std::shared_ptr<GpuScene> build_scene(const std::vector<const Entity*>& ents, ui
         const double* R = en.rotation;
         for (size_t i = 0; i < m.n_vertices(); ++i) {
             const double x = en.scale * m.v[3 * i], y = en.scale * m.v[3 * i + 1], z = en.scale * m.v[3 * i + 2];
-            V.push_back((float)(R[0] * x + R[1] * y + R[2] * z + en.position[0]));
-            V.push_back((float)(R[3] * x + R[4] * y + R[5] * z + en.position[1]));
-            V.push_back((float)(R[6] * x + R[7] * y + R[8] * z + en.position[2]));
+            V.push_back(R[0] * x + R[1] * y + R[2] * z + en.position[0]);
+            V.push_back(R[3] * x + R[4] * y + R[5] * z + en.position[1]);
+            V.push_back(R[6] * x + R[7] * y + R[8] * z + en.position[2]);
             const double a = m.vn[3 * i], b = m.vn[3 * i + 1], c = m.vn[3 * i + 2];
             VN.push_back((float)(R[0] * a + R[1] * b + R[2] * c));
             VN.push_back((float)(R[3] * a + R[4] * b + R[5] * c));
@@ -223,8 +227,8 @@ std::shared_ptr<GpuScene> build_scene(const std::vector<const Entity*>& ents, ui
     crt_scene* h = nullptr;
     {
         py::gil_scoped_release nogil;
-        check(crt_scene_create(V.data(), (uint32_t)(V.size() / 3), I.data(), (uint32_t)(I.size() / 3), inst.data(),
-                               VN.data(), (uint32_t)ents.size(), col.data(), sm.data(), &opts, &h));
+        check(crt_scene_create_f64(V.data(), (uint32_t)(V.size() / 3), I.data(), (uint32_t)(I.size() / 3),
+                                   inst.data(), VN.data(), (uint32_t)ents.size(), col.data(), sm.data(), &opts, &h));
     }
     g->h = h;
     return g;
@@ -571,7 +575,7 @@ PYBIND11_MODULE(_crt, m) {
                  return py::make_tuple(py::array_t<double>(3, f), py::array_t<double>(3, t), hits);
              },
              py::arg("sun_dir"), py::arg("pressure"), py::arg("rho_s"), py::arg("rho_d"), py::arg("grid") = 2048u,
-             py::arg("ref_point") = darr(3))
+             py::arg("ref_point") = std::vector<double>{0.0, 0.0, 0.0})
         .def("bvh_info", [](BodyFixedGroup& g) {
             crt_bvh_info i{};
             check(crt_scene_info(g.scene->h, &i));

@@ -27,7 +27,7 @@ namespace {
 using namespace crt;
 
 constexpr char MAGIC[8] = {'C', 'R', 'T', 'S', 'C', 'N', '\0', '\1'};
-constexpr uint32_t VERSION = 2;   // 2: empty compressed slots stored as inverted boxes
+constexpr uint32_t VERSION = 3;   // 3: valid-slot mask in byte 3 of the compressed node header
 
 enum Section { S_VERTICES, S_INDICES, S_VNORMALS, S_TRI_INST, S_INST_COLOR, S_INST_SMOOTH, S_NODES, S_CNODES,
                S_TRIS, N_SECTIONS };
@@ -107,7 +107,10 @@ void validate(const Header& h, const uint8_t* const sec[N_SECTIONS]) {
     uint32_t max_depth = nn ? 1u : 0u;
     for (uint32_t i = 0; i < nn; ++i) {
         CRT_REQUIRE(i == 0 || depth[i] != 0, CRT_ERR_INVALID, "scene blob: unreachable BVH node");
+        const uint32_t valid = n96[(uint64_t)i * 24 + 3] >> 24;
         for (uint32_t k = 0; k < 8; ++k) {
+            CRT_REQUIRE(((valid >> k) & 1u) == (n256[(uint64_t)i * 64 + 8 * k + 3] != REF_EMPTY ? 1u : 0u),
+                        CRT_ERR_INVALID, "scene blob: valid-slot mask disagrees with the node's refs");
             const uint32_t ref = n256[(uint64_t)i * 64 + 8 * k + 3];
             CRT_REQUIRE(ref == n96[(uint64_t)i * 24 + 16 + k], CRT_ERR_INVALID, "scene blob: node layouts differ");
             if (ref == REF_EMPTY) {

@@ -417,10 +417,19 @@ int crt_device_init(int device) {
     });
 }
 
-int crt_scene_create(const float* vertices, uint32_t n_vertices, const uint32_t* indices, uint32_t n_tris,
-                     const uint32_t* tri_instance, const float* vertex_normals, uint32_t n_instances,
-                     const float* instance_color, const uint8_t* instance_smooth, const crt_build_opts* opts,
-                     crt_scene** out_scene) {
+}  // extern "C"
+
+namespace {
+
+// Shared by crt_scene_create (FP32 vertices) and crt_scene_create_f64 (FP64
+// vertices): the frame origin is chosen from the vertices at their own
+// precision and subtracted before the one rounding to FP32, so an FP64 mesh
+// far from the world origin keeps the FP32 resolution of its own size.
+template <typename VT>
+int scene_create_impl(const VT* vertices, uint32_t n_vertices, const uint32_t* indices, uint32_t n_tris,
+                      const uint32_t* tri_instance, const float* vertex_normals, uint32_t n_instances,
+                      const float* instance_color, const uint8_t* instance_smooth, const crt_build_opts* opts,
+                      crt_scene** out_scene) {
     return guarded([&] {
         CRT_REQUIRE(out_scene != nullptr, CRT_ERR_INVALID, "out_scene is NULL");
         *out_scene = nullptr;
@@ -430,7 +439,7 @@ int crt_scene_create(const float* vertices, uint32_t n_vertices, const uint32_t*
         for (uint64_t i = 0; i < 3ull * n_tris; ++i)
             CRT_REQUIRE(indices[i] < n_vertices, CRT_ERR_INVALID, "triangle index out of range");
         for (uint64_t i = 0; i < 3ull * n_vertices; ++i)
-            CRT_REQUIRE(std::isfinite(vertices[i]) && std::fabs(vertices[i]) <= 1e18f, CRT_ERR_INVALID,
+            CRT_REQUIRE(std::isfinite((double)vertices[i]) && std::fabs((double)vertices[i]) <= 1e18, CRT_ERR_INVALID,
                         "vertex coordinate is not finite or exceeds 1e18");
         const uint32_t ni = std::max(n_instances, 1u);
         if (tri_instance)
@@ -443,9 +452,10 @@ int crt_scene_create(const float* vertices, uint32_t n_vertices, const uint32_t*
         s->n_vertices = n_vertices;
         s->n_instances = ni;
         // scene frame: origin = box centre rounded to a multiple of the FP32
-        // ulp of the largest coordinate; vertex - origin is then exact for
-        // vertices of the origin's magnitude (a body far from the world
-        // origin) and never off by more than that ulp
+        // ulp of the largest coordinate; for FP32 input vertex - origin is
+        // then exact for vertices of the origin's magnitude (a body far from
+        // the world origin) and never off by more than that ulp; FP64 input is
+        // rounded once, after the subtraction
         double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
         double amax = 0.0;
         for (uint64_t i = 0; i < n_vertices; ++i)
@@ -511,6 +521,26 @@ int crt_scene_create(const float* vertices, uint32_t n_vertices, const uint32_t*
                     "BVH too deep for the LDS traversal stack");
         *out_scene = s.release();
     });
+}
+
+}  // namespace
+
+extern "C" {
+
+int crt_scene_create(const float* vertices, uint32_t n_vertices, const uint32_t* indices, uint32_t n_tris,
+                     const uint32_t* tri_instance, const float* vertex_normals, uint32_t n_instances,
+                     const float* instance_color, const uint8_t* instance_smooth, const crt_build_opts* opts,
+                     crt_scene** out_scene) {
+    return scene_create_impl(vertices, n_vertices, indices, n_tris, tri_instance, vertex_normals, n_instances,
+                             instance_color, instance_smooth, opts, out_scene);
+}
+
+int crt_scene_create_f64(const double* vertices, uint32_t n_vertices, const uint32_t* indices, uint32_t n_tris,
+                         const uint32_t* tri_instance, const float* vertex_normals, uint32_t n_instances,
+                         const float* instance_color, const uint8_t* instance_smooth, const crt_build_opts* opts,
+                         crt_scene** out_scene) {
+    return scene_create_impl(vertices, n_vertices, indices, n_tris, tri_instance, vertex_normals, n_instances,
+                             instance_color, instance_smooth, opts, out_scene);
 }
 
 int crt_scene_info(const crt_scene* scene, crt_bvh_info* info) {

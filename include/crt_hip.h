@@ -162,6 +162,20 @@ int crt_scene_create(const float* vertices, uint32_t n_vertices,
                      const uint8_t* instance_smooth,
                      const crt_build_opts* opts,
                      crt_scene** out_scene);
+/* The same with FP64 vertices (n_vertices x 3 double), as the reference keeps
+ * its transformed geometry (Scalar = double, src/pybind.cpp:30;
+ * src/crt/transform.hpp:42-85).  The scene-frame origin is chosen in FP64
+ * and subtracted before the single rounding to FP32, so an entity posed far
+ * from the world origin keeps the FP32 resolution of its own extent. */
+int crt_scene_create_f64(const double* vertices, uint32_t n_vertices,
+                         const uint32_t* indices, uint32_t n_tris,
+                         const uint32_t* tri_instance,
+                         const float* vertex_normals,
+                         uint32_t n_instances,
+                         const float* instance_color,
+                         const uint8_t* instance_smooth,
+                         const crt_build_opts* opts,
+                         crt_scene** out_scene);
 int  crt_scene_info(const crt_scene* scene, crt_bvh_info* info);
 void crt_scene_destroy(crt_scene* scene);
 

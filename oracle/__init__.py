@@ -47,6 +47,8 @@ _p = C.c_void_p
 def _declare(L):
     L.crt_oracle_scene_create.restype = _p
     L.crt_oracle_scene_create.argtypes = [_p, C.c_int64, _p, C.c_int64, _p, _p, C.c_int64, _p, _p]
+    L.crt_oracle_scene_create_f64.restype = _p
+    L.crt_oracle_scene_create_f64.argtypes = [_p, C.c_int64, _p, C.c_int64, _p, _p, C.c_int64, _p, _p]
     L.crt_oracle_scene_destroy.argtypes = [_p]
     L.crt_oracle_tri_intersect.restype = C.c_int
     L.crt_oracle_tri_intersect.argtypes = [_p, _p, _p, _p, _p, C.c_double, C.c_double, _p]
@@ -98,8 +100,8 @@ def make_light(light: dict) -> Light:
 class Scene:
     """FP64 oracle scene (median-split BVH over the same triangles)."""
 
-    def __init__(self, vertices, indices, tri_inst=None, vnormals=None, colors=None, smooth=None):
-        self.v = np.ascontiguousarray(vertices, dtype=np.float32)
+    def __init__(self, vertices, indices, tri_inst=None, vnormals=None, colors=None, smooth=None, f64=False):
+        self.v = np.ascontiguousarray(vertices, dtype=np.float64 if f64 else np.float32)
         self.i = np.ascontiguousarray(indices, dtype=np.uint32)
         self.inst = None if tri_inst is None else np.ascontiguousarray(tri_inst, dtype=np.uint32)
         self.vn = None if vnormals is None else np.ascontiguousarray(vnormals, dtype=np.float32)
@@ -109,9 +111,9 @@ class Scene:
             ni = max(ni, self.colors.shape[0])
         self.smooth = None if smooth is None else np.ascontiguousarray(smooth, dtype=np.uint8)
         self.ni = ni
-        self.h = lib().crt_oracle_scene_create(_ptr(self.v), self.v.shape[0], _ptr(self.i), self.i.shape[0],
-                                               _ptr(self.inst), _ptr(self.vn), ni, _ptr(self.colors),
-                                               _ptr(self.smooth))
+        create = lib().crt_oracle_scene_create_f64 if f64 else lib().crt_oracle_scene_create
+        self.h = create(_ptr(self.v), self.v.shape[0], _ptr(self.i), self.i.shape[0], _ptr(self.inst),
+                        _ptr(self.vn), ni, _ptr(self.colors), _ptr(self.smooth))
         if not self.h:
             raise ValueError("oracle scene creation failed (empty mesh?)")
 

@@ -111,9 +111,29 @@ static int32_t build_rec(orc_scene* s, int32_t first, int32_t count) {
     return id;
 }
 
+/* Vertices come as FP32 (vf) or FP64 (vd); the reference holds its
+ * transformed geometry in double (src/pybind.cpp:30, src/crt/transform.hpp:42-85). */
+static orc_scene* scene_create(const float* vf, const double* vd, int64_t nv, const uint32_t* indices, int64_t nt,
+                               const uint32_t* tri_inst, const float* vnormals, int64_t ni, const float* colors,
+                               const uint8_t* smooth);
+
 orc_scene* crt_oracle_scene_create(const float* vertices, int64_t nv, const uint32_t* indices, int64_t nt,
                                    const uint32_t* tri_inst, const float* vnormals, int64_t ni,
                                    const float* colors, const uint8_t* smooth) {
+    return scene_create(vertices, NULL, nv, indices, nt, tri_inst, vnormals, ni, colors, smooth);
+}
+
+orc_scene* crt_oracle_scene_create_f64(const double* vertices, int64_t nv, const uint32_t* indices, int64_t nt,
+                                       const uint32_t* tri_inst, const float* vnormals, int64_t ni,
+                                       const float* colors, const uint8_t* smooth) {
+    return scene_create(NULL, vertices, nv, indices, nt, tri_inst, vnormals, ni, colors, smooth);
+}
+
+#define VTX(i) (vd ? vd[(i)] : (double)vf[(i)])
+
+static orc_scene* scene_create(const float* vf, const double* vd, int64_t nv, const uint32_t* indices, int64_t nt,
+                               const uint32_t* tri_inst, const float* vnormals, int64_t ni, const float* colors,
+                               const uint8_t* smooth) {
     if (nt <= 0 || nv <= 0) return NULL;
     orc_scene* s = (orc_scene*)calloc(1, sizeof(orc_scene));
     s->nt = nt; s->nv = nv; s->ni = ni > 0 ? ni : 1;
@@ -124,9 +144,9 @@ orc_scene* crt_oracle_scene_create(const float* vertices, int64_t nv, const uint
     for (int64_t t = 0; t < nt; ++t) {
         const uint32_t a = indices[3 * t], b = indices[3 * t + 1], c = indices[3 * t + 2];
         s->vidx[3 * t] = a; s->vidx[3 * t + 1] = b; s->vidx[3 * t + 2] = c;
-        s->p0[t] = mk(vertices[3 * a], vertices[3 * a + 1], vertices[3 * a + 2]);
-        s->p1[t] = mk(vertices[3 * b], vertices[3 * b + 1], vertices[3 * b + 2]);
-        s->p2[t] = mk(vertices[3 * c], vertices[3 * c + 1], vertices[3 * c + 2]);
+        s->p0[t] = mk(VTX(3 * a), VTX(3 * a + 1), VTX(3 * a + 2));
+        s->p1[t] = mk(VTX(3 * b), VTX(3 * b + 1), VTX(3 * b + 2));
+        s->p2[t] = mk(VTX(3 * c), VTX(3 * c + 1), VTX(3 * c + 2));
         s->e1[t] = sub(s->p0[t], s->p1[t]);
         s->e2[t] = sub(s->p2[t], s->p0[t]);
         s->n[t] = cross(s->e1[t], s->e2[t]);

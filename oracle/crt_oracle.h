@@ -34,6 +34,9 @@ typedef struct {
 orc_scene* crt_oracle_scene_create(const float* vertices, int64_t nv, const uint32_t* indices, int64_t nt,
                                    const uint32_t* tri_inst, const float* vnormals, int64_t ni,
                                    const float* colors, const uint8_t* smooth);
+orc_scene* crt_oracle_scene_create_f64(const double* vertices, int64_t nv, const uint32_t* indices, int64_t nt,
+                                       const uint32_t* tri_inst, const float* vnormals, int64_t ni,
+                                       const float* colors, const uint8_t* smooth);
 void crt_oracle_scene_destroy(orc_scene* s);
 
 int crt_oracle_tri_intersect(const double p0[3], const double p1[3], const double p2[3], const double o[3],

@@ -100,3 +100,50 @@ def test_unsupported_geometry_type_rejected(tmp_path):
     p.write_text("v 0 0 0\nv 1 0 0\nv 0 1 0\nf 1 2 3\n")
     with pytest.raises(ValueError):
         Entity(str(p), geometry_type="ply")
+
+
+def _rot(axis, ang):
+    c, s = np.cos(ang), np.sin(ang)
+    if axis == 0:
+        return np.array([[1, 0, 0], [0, c, -s], [0, s, c]])
+    if axis == 1:
+        return np.array([[c, 0, s], [0, 1, 0], [-s, 0, c]])
+    return np.array([[c, -s, 0], [s, c, 0], [0, 0, 1]])
+
+
+def test_transform_to_body_closed_form():
+    """BodyFixedGroup.transform_to_body (crt/body_fixed.py:87-101 of the
+    reference): p' = scale * R_g (p - p_g), R' = (R_g R^T)^T, pinned by a
+    hand-computed case and by round trips through the inverse map on rotated,
+    translated and scaled group poses.  A transposed R_g, a dropped scale or
+    a missing final transpose fails one of them."""
+    from types import SimpleNamespace
+    from crt.body_fixed import BodyFixedGroup
+    f = BodyFixedGroup.transform_to_body
+    # hand-computed: group turned 90 deg about z, at (10, 0, 0), scale 2
+    g = SimpleNamespace(position=np.array([10.0, 0.0, 0.0]), rotation=_rot(2, np.pi / 2), scale=2.0)
+    p, R = f(g, np.array([10.0, 5.0, 0.0]), np.eye(3))
+    assert np.allclose(p, [-10.0, 0.0, 0.0], atol=1e-12)
+    assert np.allclose(R, [[0, 1, 0], [-1, 0, 0], [0, 0, 1]], atol=1e-15)
+    # a camera rotated 30 deg about x in the world, group turned about y
+    g = SimpleNamespace(position=np.array([0.0, 0.0, 0.0]), rotation=_rot(1, 0.25), scale=1.0)
+    Rw = _rot(0, np.pi / 6)
+    _, R = f(g, np.zeros(3), Rw)
+    assert np.allclose(R, Rw @ _rot(1, 0.25).T, atol=1e-15)
+    # a body-frame ray direction is R_g times the world one: R'^T e = R_g R_w^T e
+    e = np.array([0.0, 0.0, -1.0])
+    assert np.allclose(R.T @ e, _rot(1, 0.25) @ (Rw.T @ e), atol=1e-15)
+    # round trips: the world pose of a known body-frame pose (q, Rc) maps back
+    rng = np.random.default_rng(0)
+    for scale in (1.0, 2.5, 1e-3):
+        for _ in range(20):
+            Rg, _r = np.linalg.qr(rng.normal(size=(3, 3)))
+            Rg = Rg * np.sign(np.linalg.det(Rg))
+            Rc, _r = np.linalg.qr(rng.normal(size=(3, 3)))
+            Rc = Rc * np.sign(np.linalg.det(Rc))
+            pg = rng.normal(size=3) * 1e5
+            q = rng.normal(size=3) * 1e3
+            g = SimpleNamespace(position=pg, rotation=Rg, scale=scale)
+            p, R = f(g, pg + Rg.T @ (q / scale), Rc @ Rg)
+            assert np.allclose(p, q, rtol=1e-9, atol=1e-9 * np.linalg.norm(pg) * scale)
+            assert np.allclose(R, Rc, atol=1e-14)

@@ -127,3 +127,45 @@ def test_unknown_light_type_raises(cornell_group):
     sc, group, _ = cornell_group
     with pytest.raises(TypeError):
         group._cpp.render(_camera()._cpp, [object()], 1, 1, 1.0, 1, None)
+
+
+def test_rotated_translated_group_matches_reference_golden(cornell_group):
+    """A group posed away from the identity (rotated about two axes and
+    translated 1e5 m) seen by a camera whose world pose is the closed-form
+    image of the golden camera pose, p_w = p_g + R_g^T q and R_w = R_c R_g:
+    transform_to_body maps it back to (q, R_c), so the passes equal the
+    reference's own identity-pose golden passes (crt/body_fixed.py:87-101)."""
+    from crt.cameras import SimpleCamera
+    sc, group, _ = cornell_group
+    g = np.load(os.path.join(GOLD, "cornell_passes.npz"))
+    c0 = scenes.cornell_camera()
+    a, b = 0.7, -0.4
+    Rg = np.array([[np.cos(a), -np.sin(a), 0.0], [np.sin(a), np.cos(a), 0.0], [0.0, 0.0, 1.0]]) @ \
+        np.array([[1.0, 0.0, 0.0], [0.0, np.cos(b), -np.sin(b)], [0.0, np.sin(b), np.cos(b)]])
+    pg = np.array([1.0e5, -3.0e4, 2.0e4])
+    group.set_pose(pg, Rg)
+    try:
+        Rc = np.asarray(c0["rotation"], dtype=np.float64)
+        pw = pg + Rg.T @ np.asarray(c0["position"], dtype=np.float64)
+        def cam():
+            return SimpleCamera(c0["focal_length"], c0["resolution"], c0["sensor_size"], c0["z_positive"],
+                                position=pw, rotation=Rc @ Rg)
+        c = cam()
+        tri = group.triangle_pass(c)
+        # the body-frame pose is written back into the caller's camera, like the reference
+        assert np.allclose(c.position, c0["position"], atol=1e-9)
+        assert np.allclose(c.rotation, Rc, atol=1e-14)
+        xyz = group.intersection_pass(cam())
+    finally:
+        group.set_pose(np.zeros(3), np.eye(3))
+    mism = (tri != g["tri"]).ravel()
+    v, i, ids = sc.merged()
+    cpu = oracle.Scene(v, i, ids)
+    o, d = oracle.camera_rays(c0)
+    expl = cpu.tie_flags(o[mism], d[mism]) | cpu.edge_flags(o[mism], d[mism], g["tri"].ravel()[mism]) | \
+        cpu.edge_flags(o[mism], d[mism], tri.ravel()[mism])
+    print("posed group vs reference: %d id mismatches, %d explained as ties/edges" % (mism.sum(), expl.sum()))
+    assert np.all(expl)
+    hit = (~mism.reshape(tri.shape)) & (g["tri"] >= 0)
+    err = np.linalg.norm(xyz[hit] - g["xyz"][hit], axis=1)
+    assert err.max() < 1e-5

@@ -118,3 +118,72 @@ def test_tiny_mesh_at_large_offset_single_leaf():
     tgt = 1e6 + rng.uniform(0.0, 0.6, size=(n, 3))
     trg, trc = _compare(v, i, o, tgt - o)
     assert (trc >= 0).sum() > n // 2
+
+
+def test_point_degenerate_cluster_at_frame_origin():
+    # A floor and a ceiling symmetric about the world origin put the scene
+    # frame's origin at (0, 0, 0); 20 triangles collapsed to that point form
+    # a wide node of zero extent with empty slots.  Its quantized boxes sit
+    # far inside the slab test's relative slack for rays passing through the
+    # point, so only the node's valid-slot mask keeps traversal off the
+    # empty slots.
+    v, i = _grid(16, 120.0)
+    v[:, :2] -= 60.0
+    floor = v.copy()
+    floor[:, 2] = -50.0
+    ceil = v.copy()
+    ceil[:, 2] = 50.0
+    nv = floor.shape[0]
+    pts = np.zeros((60, 3), dtype=np.float32)
+    verts = np.concatenate([floor, ceil, pts]).astype(np.float32)
+    idx = np.concatenate([i, i + nv, (np.arange(60, dtype=np.uint32) + 2 * nv).reshape(20, 3)])
+    gpu = capi.Scene(verts, idx)
+    assert np.allclose(gpu.info()["sphere"][:3], 0.0)
+    # the library's own blob validation checks every node's valid-slot mask
+    # against its child refs on the host, before anything is traced
+    capi.Scene.from_bytes(gpu.to_bytes())
+    rng = np.random.default_rng(5)
+    n = 8192
+    o = np.concatenate([rng.uniform(-40, 40, size=(n, 2)), rng.uniform(10, 45, size=(n, 1))], axis=1)
+    d = -o   # every ray passes exactly through the degenerate point
+    _compare(verts, idx, o, d)
+
+
+def test_fp64_entity_far_from_origin_dynamic_api(tmp_path):
+    # An entity posed 2.3e7 m out: its FP64-transformed vertices reach the
+    # library as FP64 and are rounded to FP32 only in the scene frame, so the
+    # passes resolve the 10-m body to ~1e-6 m instead of snapping it to the
+    # 2-m FP32 grid of its world coordinates.
+    from crt import Entity, rendering
+    from crt.cameras import SimpleCamera
+    import _crt
+    v0, f0 = scenes.icosphere(3)
+    mesh = scenes.Mesh(np.asarray(v0, dtype=np.float64) * 10.0, np.asarray(f0, dtype=np.uint32))
+    p = str(tmp_path / "ico.obj")
+    scenes.write_obj(p, mesh)
+    shift = np.array([1.0e7, -2.0e7, 5.0e6])
+    ang = 0.3
+    R = np.array([[np.cos(ang), -np.sin(ang), 0.0], [np.sin(ang), np.cos(ang), 0.0], [0.0, 0.0, 1.0]])
+    ent = Entity(p)
+    ent.set_position(shift)
+    ent.set_rotation(R)
+    cam_pos = shift + np.array([0.0, 0.0, 200.0])
+    cam = SimpleCamera(1.0, [96, 96], [0.12, 0.12], False, position=cam_pos, rotation=np.eye(3))
+    xyz = rendering.intersection_pass(cam, [ent])
+    # oracle: the OBJ's parsed (FP32) vertices transformed in FP64, like the reference
+    vp, fp, _ = _crt.load_obj(p)
+    vw = vp @ R.T + shift
+    cpu = oracle.Scene(vw, fp, f64=True)
+    ref = cpu.camera_pass({"position": cam_pos, "rotation": np.eye(3), "focal_length": 1.0,
+                           "resolution": [96, 96], "sensor_size": [0.12, 0.12], "z_positive": False})
+    hit = np.linalg.norm(ref["xyz"], axis=2) > 0
+    ghit = np.linalg.norm(xyz, axis=2) > 0
+    assert hit.sum() > 2000
+    assert (hit != ghit).sum() <= 2   # at most a silhouette graze
+    hit &= ghit
+    err = np.linalg.norm(xyz[hit] - ref["xyz"][hit], axis=1) / np.linalg.norm(ref["xyz"][hit] - cam_pos, axis=1)
+    print("far entity: %d hits, %d silhouette differences, max rel. hit-point error %.2e"
+          % (hit.sum(), (np.linalg.norm(ref["xyz"], axis=2) > 0).sum() - hit.sum(), err.max()))
+    # FP32 vertices in the scene frame (ulp 1e-6 m at 10 m); the old FP32
+    # world-coordinate path snapped the body to a 2-m grid (errors ~1e-2)
+    assert err.max() < 1e-6
