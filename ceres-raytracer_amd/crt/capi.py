@@ -141,6 +141,7 @@ SIGNATURES = {
     "crt_lidar_batch": (C.c_int, [_p, C.c_uint64, _p, _p, C.c_int32, _p, C.c_uint32, _p, _p]),
     "crt_flash_lidar": (C.c_int, [_p, _p, _p, _p, _p, _p, C.c_uint32, _p, _p]),
     "crt_radiation_pressure": (C.c_int, [_p, _p, _p, _p, _p, _p, _p]),
+    "crt_radiation_pressure_noalloc": (C.c_int, [_p, _p, _p, _p]),
 }
 
 

@@ -26,6 +26,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 namespace crt {
 namespace {
@@ -286,7 +287,43 @@ struct Timer {
     }
 };
 
+// ---- allocation-free radiation pressure (crt_radiation_pressure_noalloc) ----
+// Its scratch (the block partials, the 7 x 64 TwoSum slices) lives in one
+// file-scope device array instead of a workspace.  In host code `scratch`
+// names a host-side stand-in, so kernels receive the device address that
+// hipGetSymbolAddress returns for the current device, kept per device.
+constexpr uint32_t NOALLOC_MAX_GRID = 2048;
+constexpr size_t NOALLOC_PARTIAL_BYTES = (size_t)(NOALLOC_MAX_GRID / 8) * (NOALLOC_MAX_GRID / 8) * 7 * sizeof(double);
+constexpr size_t NOALLOC_SCRATCH_BYTES = NOALLOC_PARTIAL_BYTES + (size_t)7 * FLUX_SLICES * sizeof(Comp);
+static_assert(NOALLOC_PARTIAL_BYTES % 256 == 0, "slices stay 256-B aligned");
+
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// Device address of `scratch` on the current device (looked up once per
+// device, with that device current).
+void* scratch_ptr() {
+    int dev = 0;
+    CRT_HIP_CHECK(hipGetDevice(&dev));
+    CRT_REQUIRE(dev >= 0 && dev < NOALLOC_MAX_DEVICES, CRT_ERR_INVALID, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        CRT_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        CRT_REQUIRE(p != nullptr, CRT_ERR_HIP, "hipGetSymbolAddress returned NULL for the scratch array");
+        g_scratch_ptr[dev] = p;
+    }
+    return g_scratch_ptr[dev];
+}
+
 }  // namespace
+
+// Called by crt_device_init with the device current.
+void noalloc_setup_current_device() { (void)scratch_ptr(); }
+
 }  // namespace crt
 
 using namespace crt;
@@ -388,55 +425,75 @@ void flash_lidar_impl(crt_scene* scene, const crt_flash_opts* opts, double* sing
     }
 }
 
+namespace {
+
+// Ray lattice of a flux request in the scene frame (see crt_flux_opts).
+FluxParams flux_params(const crt_scene* scene, const crt_flux_opts* opts) {
+    CRT_REQUIRE(scene != nullptr && opts != nullptr, CRT_ERR_INVALID, "NULL argument");
+    CRT_REQUIRE(opts->grid >= 1 && opts->grid <= 65536, CRT_ERR_INVALID, "grid must be in 1..65536");
+    const double sl = std::sqrt(opts->sun_dir[0] * opts->sun_dir[0] + opts->sun_dir[1] * opts->sun_dir[1] +
+                                opts->sun_dir[2] * opts->sun_dir[2]);
+    CRT_REQUIRE(sl > 0 && std::isfinite(sl), CRT_ERR_INVALID, "sun_dir must be a finite non-zero vector");
+    FluxParams F;
+    for (int k = 0; k < 3; ++k) F.sun[k] = opts->sun_dir[k] / sl;
+    const double ax[3] = {std::fabs(F.sun[0]) < 0.9 ? 1.0 : 0.0, std::fabs(F.sun[0]) < 0.9 ? 0.0 : 1.0, 0.0};
+    double e1[3] = {F.sun[1] * ax[2] - F.sun[2] * ax[1], F.sun[2] * ax[0] - F.sun[0] * ax[2],
+                    F.sun[0] * ax[1] - F.sun[1] * ax[0]};
+    const double l1 = std::sqrt(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]);
+    for (int k = 0; k < 3; ++k) F.e1[k] = e1[k] / l1;
+    F.e2[0] = F.sun[1] * F.e1[2] - F.sun[2] * F.e1[1];
+    F.e2[1] = F.sun[2] * F.e1[0] - F.sun[0] * F.e1[2];
+    F.e2[2] = F.sun[0] * F.e1[1] - F.sun[1] * F.e1[0];
+    // grid centre and torque reference point in the scene frame
+    if (opts->radius > 0) {
+        for (int k = 0; k < 3; ++k) F.center[k] = opts->center[k] - scene->view.origin[k];
+        F.radius = opts->radius;
+    } else {
+        for (int k = 0; k < 3; ++k) F.center[k] = scene->view.center[k];
+        F.radius = scene->view.radius;
+    }
+    for (int k = 0; k < 3; ++k) F.ref[k] = opts->ref_point[k] - scene->view.origin[k];
+    F.grid = opts->grid;
+    F.h = 2.0 * F.radius / (double)F.grid;
+    F.pressure = opts->pressure;
+    F.rho_s = opts->rho_s;
+    F.rho_d = opts->rho_d;
+    F.origin_back = 2.0 * F.radius;
+    return F;
+}
+
+inline uint32_t flux_blocks(const FluxParams& F) { return cdiv(F.grid, 8) * cdiv(F.grid, 8); }   // 8 x 8 tiles
+
+// Enqueue the flux kernels: `part` holds 7 x flux_blocks doubles, `slices`
+// 7 x FLUX_SLICES Comp; out[0..6] = force xyz, torque xyz, hits.
+void flux_launch(const crt_scene* scene, const FluxParams& F, double* part, Comp* slices, double* out,
+                 hipStream_t st) {
+    const uint32_t nblocks = flux_blocks(F);
+    hipLaunchKernelGGL(k_flux, dim3(nblocks), dim3(LANE_BLOCK), lane_lds_bytes(scene->view), st, scene->view, F,
+                       part, nblocks);
+    hipLaunchKernelGGL(k_flux_slices, dim3(FLUX_SLICES, 7), dim3(FLUX_SLICE_THREADS), 0, st, part, nblocks,
+                       slices);
+    hipLaunchKernelGGL(k_flux_final, dim3(7), dim3(FLUX_SLICES), 0, st, slices, out);
+    CRT_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace
+
 void radiation_pressure_impl(crt_scene* scene, const crt_flux_opts* opts, double force[3], double torque[3],
                              uint64_t* n_hits, void* stream, crt_timing* timing, Workspace& ws) {
     {
-        CRT_REQUIRE(scene != nullptr && opts != nullptr && force && torque, CRT_ERR_INVALID, "NULL argument");
-        CRT_REQUIRE(opts->grid >= 1 && opts->grid <= 65536, CRT_ERR_INVALID, "grid must be in 1..65536");
-        const double sl = std::sqrt(opts->sun_dir[0] * opts->sun_dir[0] + opts->sun_dir[1] * opts->sun_dir[1] +
-                                    opts->sun_dir[2] * opts->sun_dir[2]);
-        CRT_REQUIRE(sl > 0 && std::isfinite(sl), CRT_ERR_INVALID, "sun_dir must be a finite non-zero vector");
+        CRT_REQUIRE(force && torque, CRT_ERR_INVALID, "NULL argument");
+        const FluxParams F = flux_params(scene, opts);
         hipStream_t st = (hipStream_t)stream;
-        FluxParams F;
-        for (int k = 0; k < 3; ++k) F.sun[k] = opts->sun_dir[k] / sl;
-        const double ax[3] = {std::fabs(F.sun[0]) < 0.9 ? 1.0 : 0.0, std::fabs(F.sun[0]) < 0.9 ? 0.0 : 1.0, 0.0};
-        double e1[3] = {F.sun[1] * ax[2] - F.sun[2] * ax[1], F.sun[2] * ax[0] - F.sun[0] * ax[2],
-                        F.sun[0] * ax[1] - F.sun[1] * ax[0]};
-        const double l1 = std::sqrt(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]);
-        for (int k = 0; k < 3; ++k) F.e1[k] = e1[k] / l1;
-        F.e2[0] = F.sun[1] * F.e1[2] - F.sun[2] * F.e1[1];
-        F.e2[1] = F.sun[2] * F.e1[0] - F.sun[0] * F.e1[2];
-        F.e2[2] = F.sun[0] * F.e1[1] - F.sun[1] * F.e1[0];
-        // grid centre and torque reference point in the scene frame
-        if (opts->radius > 0) {
-            for (int k = 0; k < 3; ++k) F.center[k] = opts->center[k] - scene->view.origin[k];
-            F.radius = opts->radius;
-        } else {
-            for (int k = 0; k < 3; ++k) F.center[k] = scene->view.center[k];
-            F.radius = scene->view.radius;
-        }
-        for (int k = 0; k < 3; ++k) F.ref[k] = opts->ref_point[k] - scene->view.origin[k];
-        F.grid = opts->grid;
-        F.h = 2.0 * F.radius / (double)F.grid;
-        F.pressure = opts->pressure;
-        F.rho_s = opts->rho_s;
-        F.rho_d = opts->rho_d;
-        F.origin_back = 2.0 * F.radius;
         const uint64_t n = (uint64_t)F.grid * F.grid;
-        const uint32_t nblocks = cdiv(F.grid, 8) * cdiv(F.grid, 8);   // one 8 x 8 tile per block
+        const uint32_t nblocks = flux_blocks(F);
         ws.h.reserve((size_t)nblocks * 7 * sizeof(double) + (size_t)7 * FLUX_SLICES * sizeof(Comp));
         ws.counters.reserve(256);
         double* out = ws.counters.as<double>();
         Timer tm(timing != nullptr);
         Timer::rec(tm.a, st);
         double* part = ws.h.as<double>();
-        Comp* slices = reinterpret_cast<Comp*>(part + (size_t)nblocks * 7);
-        hipLaunchKernelGGL(k_flux, dim3(nblocks), dim3(LANE_BLOCK), lane_lds_bytes(scene->view), st, scene->view, F,
-                           part, nblocks);
-        hipLaunchKernelGGL(k_flux_slices, dim3(FLUX_SLICES, 7), dim3(FLUX_SLICE_THREADS), 0, st, part, nblocks,
-                           slices);
-        hipLaunchKernelGGL(k_flux_final, dim3(7), dim3(FLUX_SLICES), 0, st, slices, out);
-        CRT_HIP_CHECK(hipGetLastError());
+        flux_launch(scene, F, part, reinterpret_cast<Comp*>(part + (size_t)nblocks * 7), out, st);
         Timer::rec(tm.b, st);
         double h[7];
         CRT_HIP_CHECK(hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, st));
@@ -471,6 +528,21 @@ int crt_radiation_pressure(crt_scene* scene, const crt_flux_opts* opts, double f
         crt_scene::StreamSlot& slot = scene->slot(stream);
         std::lock_guard<std::mutex> slot_lock(slot.mu);
         radiation_pressure_impl(scene, opts, force, torque, n_hits, stream, timing, slot.ws);
+    });
+}
+
+int crt_radiation_pressure_noalloc(crt_scene* scene, const crt_flux_opts* opts, double* out7, void* stream) {
+    return guarded([&] {
+        CRT_REQUIRE(scene != nullptr && opts != nullptr && out7 != nullptr, CRT_ERR_INVALID, "NULL argument");
+        CRT_REQUIRE(opts->grid <= NOALLOC_MAX_GRID, CRT_ERR_INVALID,
+                    "grid exceeds the allocation-free maximum of 2048 (use crt_radiation_pressure)");
+        int dev = 0;
+        CRT_HIP_CHECK(hipGetDevice(&dev));
+        CRT_REQUIRE(dev == scene->device, CRT_ERR_INVALID, "the scene's device must be current");
+        const FluxParams F = flux_params(scene, opts);
+        unsigned char* base = static_cast<unsigned char*>(scratch_ptr());
+        flux_launch(scene, F, reinterpret_cast<double*>(base), reinterpret_cast<Comp*>(base + NOALLOC_PARTIAL_BYTES),
+                    out7, (hipStream_t)stream);
     });
 }
 

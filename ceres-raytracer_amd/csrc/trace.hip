@@ -354,6 +354,7 @@ struct OutSlot {
 // ---- host helpers shared with other translation units ----
 void scene_make_view(crt_scene* s, const float* host_vertices);
 void scene_bind_view(crt_scene* s);
+void noalloc_setup_current_device();   // sensors.hip
 
 }  // namespace crt
 
@@ -414,6 +415,7 @@ int crt_device_init(int device) {
         CRT_REQUIRE(n > 0, CRT_ERR_HIP, "no HIP device available");
         CRT_REQUIRE(device >= 0 && device < n, CRT_ERR_INVALID, "device index out of range");
         CRT_HIP_CHECK(hipSetDevice(device));
+        noalloc_setup_current_device();
     });
 }
 

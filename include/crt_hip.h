@@ -309,6 +309,20 @@ typedef struct {
 int crt_radiation_pressure(crt_scene* scene, const crt_flux_opts* opts, double force[3], double torque[3],
                            uint64_t* n_hits, void* stream, crt_timing* timing);
 
+/* Allocation-free variant of crt_radiation_pressure.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * It calls no hipMalloc/hipFree or allocating routine: its scratch (block
+ * partials and compensated slices, 3.7 MB) is a static device array of the
+ * library, one per GPU, held until the process ends.  grid must be in
+ * 1..2048 (larger grids: CRT_ERR_INVALID).  out7 is a DEVICE pointer to 7
+ * doubles: force xyz, torque xyz, number of hit rays; the call only enqueues
+ * work on `stream` (no synchronisation) and needs the scene's device current.
+ * Results are bit-identical to crt_radiation_pressure.  Every call on one GPU
+ * shares that GPU's scratch: calls that overlap on one GPU (different
+ * streams or threads before the previous call's kernels finished) race. */
+int crt_radiation_pressure_noalloc(crt_scene* scene, const crt_flux_opts* opts, double* out7, void* stream);
+
 /* ---- serving (new) ---------------------------------------------------------
  * A server keeps scenes resident, admits requests by device-memory budget and
  * runs them on n_workers streams; each worker batches every queued request
