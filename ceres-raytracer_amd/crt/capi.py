@@ -53,7 +53,7 @@ class BvhInfo(C.Structure):
     _fields_ = [("n_tris", C.c_uint32), ("n_wide_nodes", C.c_uint32), ("n_leaves", C.c_uint32),
                 ("max_depth", C.c_uint32), ("stack_capacity", C.c_uint32), ("pad_", C.c_uint32),
                 ("build_ms", C.c_double), ("sah_cost", C.c_double), ("bounds", C.c_double * 6),
-                ("sphere", C.c_double * 4), ("device_bytes", C.c_uint64)]
+                ("sphere", C.c_double * 4), ("device_bytes", C.c_uint64), ("self_offset", C.c_double)]
 
 
 class Timing(C.Structure):

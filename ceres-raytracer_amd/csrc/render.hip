@@ -393,13 +393,17 @@ __device__ inline void light_sample(const Ctx& C, uint32_t cam, int l, float3 p,
 }
 
 // Malley cosine sample around the (inward) shading normal, brdfs.hpp:17-42.
-__device__ inline float3 cosine_sample(float3 n, float r1, float r2) {
+// The reference's z = -sqrt(1 - x^2 - y^2) equals -sqrt(1 - r1) (x^2 + y^2 =
+// r1); it is taken from omr = 1 - r1 formed in FP64, because in FP32 the
+// subtraction cancels for grazing samples (x^2 + y^2 -> 1) and would tilt
+// the bounce ray by up to ~1e-4 rad.
+__device__ inline float3 cosine_sample(float3 n, float r1, float r2, float omr) {
     const float r = sqrtf(r1);
     const float th = r2 * 2.0f * 3.1415926f;
     float sn, cs;
     sincosf(th, &sn, &cs);
     const float x = -r * cs, y = -r * sn;
-    const float z = -sqrtf(fmaxf(0.0f, 1.0f - x * x - y * y));
+    const float z = -sqrtf(omr);
     float3 nx;
     if (fabsf(n.x) > fabsf(n.y)) {
         const float inv = 1.0f / sqrtf(n.x * n.x + n.z * n.z);
@@ -451,12 +455,13 @@ __device__ inline void shade_hit(const Ctx& C, int bounce, uint32_t k, uint32_t 
         C.shadow[ls * (uint32_t)C.nl + (uint32_t)l] = r;
     }
     if (bounce < C.nb - 1) {
-        const float r1 = (float)rng_u01k(key, rng_dim(bounce, RNG_BRDF));
+        const double r1 = rng_u01k(key, rng_dim(bounce, RNG_BRDF));
         const float r2 = (float)rng_u01k(key, rng_dim(bounce, RNG_BRDF + 1));
-        const float3 nd = cosine_sample(sn, r1, r2);
+        const float omr = (float)(1.0 - r1);   // the bounce weight (1 - r1), as the reference's float(1 - r1)
+        const float3 nd = cosine_sample(sn, (float)r1, r2, omr);
         C.st.o[k] = make_float4(p.x, p.y, p.z, 0.f);
         C.st.d[k] = make_float4(nd.x, nd.y, nd.z, 0.f);
-        C.st.w[k].w = 1.0f - r1;
+        C.st.w[k].w = omr;
     }
 }
 

@@ -453,11 +453,14 @@ int scene_create_impl(const VT* vertices, uint32_t n_vertices, const uint32_t* i
         s->n_tris = n_tris;
         s->n_vertices = n_vertices;
         s->n_instances = ni;
-        // scene frame: origin = box centre rounded to a multiple of the FP32
-        // ulp of the largest coordinate; for FP32 input vertex - origin is
-        // then exact for vertices of the origin's magnitude (a body far from
-        // the world origin) and never off by more than that ulp; FP64 input is
-        // rounded once, after the subtraction
+        // scene frame: a body whose box contains the world origin keeps
+        // origin 0 (its FP32 vertices stay exact; moving them by the box
+        // centre could push coordinates across a binade and round them,
+        // tilting small triangles' normals by ulp / edge); any other body
+        // takes its box centre rounded to a multiple of the FP32 ulp of its
+        // largest coordinate, which makes vertex - origin exact for FP32
+        // input vertices of the origin's magnitude (a body far from the world
+        // origin).  FP64 input is rounded once, after the subtraction.
         double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
         double amax = 0.0;
         for (uint64_t i = 0; i < n_vertices; ++i)
@@ -470,7 +473,10 @@ int scene_create_impl(const VT* vertices, uint32_t n_vertices, const uint32_t* i
         int ex = 0;
         std::frexp(amax, &ex);
         const double q = std::ldexp(1.0, ex - 24);   // >= ulp of every vertex coordinate
-        for (int k = 0; k < 3; ++k) s->view.origin[k] = amax > 0.0 ? std::round(0.5 * (lo[k] + hi[k]) / q) * q : 0.0;
+        const bool around_zero = lo[0] <= 0.0 && hi[0] >= 0.0 && lo[1] <= 0.0 && hi[1] >= 0.0 && lo[2] <= 0.0 &&
+                                 hi[2] >= 0.0;
+        for (int k = 0; k < 3; ++k)
+            s->view.origin[k] = (amax > 0.0 && !around_zero) ? std::round(0.5 * (lo[k] + hi[k]) / q) * q : 0.0;
         std::vector<float> local((size_t)n_vertices * 3);
         for (uint64_t i = 0; i < 3ull * n_vertices; ++i)
             local[i] = (float)((double)vertices[i] - s->view.origin[i % 3]);
@@ -516,6 +522,7 @@ int scene_create_impl(const VT* vertices, uint32_t n_vertices, const uint32_t* i
                            s->inst_color.bytes + s->inst_smooth.bytes + s->vnormals.bytes + s->bvh.nodes.bytes +
                            s->bvh.cnodes.bytes + s->bvh.tris.bytes;
         scene_make_view(s.get(), local.data());
+        inf.self_offset = self_offset(s.get());
         CRT_REQUIRE(s->bvh.n_nodes < (1u << 23), CRT_ERR_INVALID, "BVH exceeds 2^23 wide nodes");
         CRT_REQUIRE(trace_lds_bytes(s->view) <= 64u * 1024u && lane_lds_bytes(s->view) <= 64u * 1024u &&
                         quad_lds_bytes(s->view) <= 64u * 1024u,

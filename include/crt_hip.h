@@ -118,6 +118,9 @@ typedef struct {
     double   bounds[6];       /* xmin,ymin,zmin,xmax,ymax,zmax */
     double   sphere[4];       /* bounding sphere centre xyz + radius used for rebasing */
     uint64_t device_bytes;    /* HBM held by the scene (mesh + BVH) */
+    double   self_offset;     /* secondary-ray origin offset along the geometric normal:
+                                 max(1e-4, 1e-6 x the scene's half extent), the reference's
+                                 1e-4 (unidirectional.hpp:61) at unit scale */
 } crt_bvh_info;
 
 /* Kernel timings of the last call, measured with hipEvents on the launch stream. */

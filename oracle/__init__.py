@@ -36,6 +36,31 @@ class Camera(C.Structure):
                 ("pad_", C.c_int32)]
 
 
+class Material(C.Structure):
+    # src/crt/materials/material.hpp:90-184; type 0 ColoredLambertian,
+    # 1 TexturedLambertian, 2 TexturedBlinnPhong, 3 Mirror
+    _fields_ = [("type", C.c_uint32), ("texture", C.c_int32), ("spec_texture", C.c_int32), ("alpha", C.c_float),
+                ("color", C.c_float * 3), ("coeffs", C.c_float * 3)]
+
+
+class Texture(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("rgba", C.c_void_p)]
+
+
+MATERIAL_TYPES = {"lambertian": 0, "textured_lambertian": 1, "blinn_phong": 2, "mirror": 3}
+
+
+def make_material(m: dict) -> Material:
+    x = Material()
+    x.type = MATERIAL_TYPES[m.get("type", "lambertian")]
+    x.texture = int(m.get("texture", -1))
+    x.spec_texture = int(m.get("spec_texture", -1))
+    x.alpha = float(m.get("alpha", 24.0))
+    x.color[:] = [float(c) for c in m.get("color", (1.0, 1.0, 1.0))]
+    x.coeffs[:] = [float(c) for c in m.get("coeffs", (0.0, 1.0, 0.0))]
+    return x
+
+
 class Light(C.Structure):
     _fields_ = [("type", C.c_int32), ("pad_", C.c_int32), ("intensity", C.c_double), ("position", C.c_double * 3),
                 ("rotation", C.c_double * 9), ("size", C.c_double * 2), ("direction", C.c_double * 3)]
@@ -64,6 +89,9 @@ def _declare(L):
     L.crt_oracle_u01.argtypes = [C.c_uint64] * 4
     L.crt_oracle_render.argtypes = [_p, _p, _p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_uint64,
                                     C.c_uint64, _p, _p]
+    L.crt_oracle_render_ex.argtypes = [_p, _p, _p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_uint64,
+                                       C.c_uint64, C.c_double, C.c_double, _p, _p, _p]
+    L.crt_oracle_scene_set_materials.argtypes = [_p, C.c_int64, _p, _p, C.c_int64, C.c_int, _p]
     L.crt_oracle_radiation_pressure.argtypes = [_p, _p, C.c_double, C.c_double, C.c_double, C.c_int64, _p,
                                                 C.c_double, _p, _p, _p, _p]
     L.crt_oracle_vertex_normals.argtypes = [_p, C.c_int64, _p, C.c_int64, _p]
@@ -178,16 +206,35 @@ class Scene:
         lib().crt_oracle_lidar_batch(self.h, pos.shape[0], _ptr(pos), _ptr(rot), int(z_positive), _ptr(out))
         return out
 
+    def set_materials(self, materials, uv=None, textures=()):
+        """Per-instance materials (dicts for make_material), per-vertex UVs
+        (V, 2) and RGBA8 textures (H, W, 4) referenced by index."""
+        mats = (Material * max(len(materials), 1))(*[make_material(m) for m in materials])
+        self._tex_keep = [np.ascontiguousarray(t, dtype=np.uint8) for t in textures]
+        tex = (Texture * max(len(self._tex_keep), 1))(
+            *[Texture(t.shape[1], t.shape[0], t.ctypes.data) for t in self._tex_keep])
+        uvs = None if uv is None else np.ascontiguousarray(uv, dtype=np.float32).reshape(-1, 2)
+        lib().crt_oracle_scene_set_materials(self.h, len(materials), mats, _ptr(uvs), 0 if uvs is None else
+                                             uvs.shape[0], len(self._tex_keep), tex)
+
     def render(self, cam: dict, lights, min_samples, max_samples, noise_threshold, num_bounces, seed=0,
-               key_offset=0):
+               key_offset=0, offset=1e-4, eps=None):
+        """Radiance (H, W, 3) f32 and samples used (H, W).  `offset` is the
+        self-intersection offset (the reference's 1e-4 by default); with `eps`
+        also the per-pixel fragile-decision counts (H, W, 3): closest hits,
+        shadow rays, texel lookups decided within eps (see crt_oracle.h)."""
         c = make_camera(cam)
         arr = (Light * max(len(lights), 1))(*[make_light(l) for l in lights])
         w, h = int(np.floor(cam["resolution"][0])), int(np.floor(cam["resolution"][1]))
         rgb = np.zeros((h, w, 3), dtype=np.float32)
         spp = np.zeros((h, w), dtype=np.int32)
-        lib().crt_oracle_render(self.h, C.byref(c), arr, len(lights), min_samples, max_samples, noise_threshold,
-                                num_bounces, seed, key_offset, _ptr(rgb), _ptr(spp))
-        return rgb, spp
+        frag = None if eps is None else np.zeros((h, w, 3), dtype=np.int32)
+        lib().crt_oracle_render_ex(self.h, C.byref(c), arr, len(lights), min_samples, max_samples, noise_threshold,
+                                   num_bounces, seed, key_offset, float(offset), float(eps or 0.0), _ptr(rgb),
+                                   _ptr(spp), _ptr(frag))
+        if eps is None:
+            return rgb, spp
+        return rgb, spp, frag
 
     def radiation_pressure(self, sun, pressure, rho_s, rho_d, grid, center, radius, ref=(0.0, 0.0, 0.0)):
         sun = np.ascontiguousarray(sun, dtype=np.float64)

@@ -54,6 +54,10 @@ struct orc_scene {
     bnode* nodes;
     int32_t* prim;
     int64_t nnodes;
+    crt_oracle_material* mat; /* per instance, NULL: ColoredLambertian with `color` */
+    float* uv;                /* per vertex (u, v), NULL: zero */
+    int ntex;
+    crt_oracle_texture* tex;  /* pixel data owned by the scene */
 };
 
 static void bbox_of(const orc_scene* s, const int32_t* idx, int64_t cnt, double lo[3], double hi[3]) {
@@ -170,12 +174,44 @@ static orc_scene* scene_create(const float* vf, const double* vd, int64_t nv, co
     return s;
 }
 
+static void free_materials(orc_scene* s) {
+    for (int k = 0; k < s->ntex; ++k) free((void*)s->tex[k].rgba);
+    free(s->tex); free(s->mat); free(s->uv);
+    s->tex = NULL; s->mat = NULL; s->uv = NULL; s->ntex = 0;
+}
+
 void crt_oracle_scene_destroy(orc_scene* s) {
     if (!s) return;
     free(s->p0); free(s->p1); free(s->p2); free(s->e1); free(s->e2); free(s->n);
     free(s->inst); free(s->vidx); free(s->vn); free(s->color); free(s->smooth);
     free(s->prim); free(s->nodes);
+    free_materials(s);
     free(s);
+}
+
+void crt_oracle_scene_set_materials(orc_scene* s, int64_t n, const crt_oracle_material* mats, const float* uv,
+                                    int64_t nv, int ntex, const crt_oracle_texture* tex) {
+    free_materials(s);
+    s->mat = (crt_oracle_material*)calloc((size_t)s->ni, sizeof(crt_oracle_material));
+    for (int64_t i = 0; i < s->ni; ++i) {
+        if (i < n) s->mat[i] = mats[i];
+        else { s->mat[i].type = 0; s->mat[i].texture = s->mat[i].spec_texture = -1;
+               for (int k = 0; k < 3; ++k) s->mat[i].color[k] = s->color[3 * i + k]; }
+    }
+    if (uv && nv == s->nv) {
+        s->uv = (float*)malloc(sizeof(float) * 2 * nv);
+        memcpy(s->uv, uv, sizeof(float) * 2 * nv);
+    }
+    s->ntex = ntex;
+    s->tex = (crt_oracle_texture*)calloc((size_t)(ntex > 0 ? ntex : 1), sizeof(crt_oracle_texture));
+    for (int k = 0; k < ntex; ++k) {
+        const size_t b = (size_t)tex[k].width * tex[k].height * 4;
+        uint8_t* px = (uint8_t*)malloc(b > 0 ? b : 1);
+        memcpy(px, tex[k].rgba, b);
+        s->tex[k].width = tex[k].width;
+        s->tex[k].height = tex[k].height;
+        s->tex[k].rgba = px;
+    }
 }
 
 /* ------------------------------------------------------ triangle test */
@@ -284,6 +320,102 @@ void crt_oracle_trace(const orc_scene* s, int64_t n, const double* o, const doub
         if (u) u[i] = h >= 0 ? uu : 0.0;
         if (v) v[i] = h >= 0 ? vv : 0.0;
     }
+}
+
+/* ------------------------------------------------------ fragility */
+/* Triangle test with every edge moved outwards by `slack` / |cos| (negative:
+ * inwards), cos = angle between the unit ray and the triangle's normal: an
+ * uncertainty of `slack` across the ray moves a hit by slack / |cos| within
+ * the triangle's plane.  The barycentric coordinate of a vertex times the
+ * triangle's doubled area |n| over the opposite edge's length is the
+ * distance of the point from that edge. */
+static inline int tri_isect_slack(const orc_scene* s, int64_t t_i, v3 o, v3 d, double tmin, double tmax,
+                                  double slack, double* t) {
+    const v3 c = sub(s->p0[t_i], o);
+    const v3 r = cross(d, c);
+    const double nd = dot(s->n[t_i], d);
+    if (nd == 0.0) return 0;
+    const double inv_det = 1.0 / nd;
+    const double uu = dot(r, s->e2[t_i]) * inv_det;
+    const double vv = dot(r, s->e1[t_i]) * inv_det;
+    const double ww = 1.0 - uu - vv;
+    const double an = len(s->n[t_i]);
+    if (an == 0.0) return 0;
+    double cs = fabs(nd) / (an * len(d));
+    if (cs < 1e-3) cs = 1e-3;
+    const double sl = slack / cs;
+    const double l0 = len(s->e2[t_i]), l1 = len(s->e1[t_i]), l2 = len(sub(s->p2[t_i], s->p1[t_i]));
+    if (uu * an < -sl * l0 || vv * an < -sl * l1 || ww * an < -sl * l2) return 0;
+    const double tt = dot(s->n[t_i], c) * inv_det;
+    if (!(tt >= tmin && tt <= tmax)) return 0;
+    *t = tt;
+    return 1;
+}
+
+/* Smallest distance from a hit point (barycentrics u, v) to an edge of t_i,
+ * times |cos| of the ray (unit d) with the triangle's normal (see above). */
+static double edge_margin(const orc_scene* s, int64_t t_i, v3 d, double u, double v) {
+    const double an = len(s->n[t_i]);
+    double cs = fabs(dot(s->n[t_i], d)) / (an * len(d));
+    if (cs < 1e-3) cs = 1e-3;
+    const double w = 1.0 - u - v;
+    const double d0 = u * an / len(s->e2[t_i]), d1 = v * an / len(s->e1[t_i]);
+    const double d2 = w * an / len(sub(s->p2[t_i], s->p1[t_i]));
+    const double m = d0 < d1 ? d0 : d1;
+    return (d2 < m ? d2 : m) * cs;
+}
+
+/* Closest / any hit over triangles grown (slack > 0) or shrunk (slack < 0);
+ * node boxes are padded by |slack|. */
+static int64_t trace_slack(const orc_scene* s, v3 o, v3 d, double tmin, double tmax, int any_hit, double slack,
+                           double* tb) {
+    const v3 inv = mk(safe_inv(d.x), safe_inv(d.y), safe_inv(d.z));
+    const double pad = fabs(slack);
+    int32_t stack[128];
+    int sp = 0;
+    int64_t best = -1;
+    double bt = tmax;
+    stack[sp++] = 0;
+    while (sp > 0) {
+        bnode nd = s->nodes[stack[--sp]];
+        for (int k = 0; k < 3; ++k) { nd.lo[k] -= pad; nd.hi[k] += pad; }
+        double te;
+        if (!box_hit(&nd, o, inv, tmin, bt, &te)) continue;
+        if (nd.count > 0) {
+            for (int32_t i = 0; i < nd.count; ++i) {
+                const int32_t t = s->prim[nd.first + i];
+                double tt;
+                if (tri_isect_slack(s, t, o, d, tmin, bt, slack, &tt)) {
+                    best = t; bt = tt; *tb = tt;
+                    if (any_hit) return best;
+                }
+            }
+        } else {
+            stack[sp++] = nd.right;
+            stack[sp++] = nd.left;
+        }
+    }
+    return best;
+}
+
+/* A closest-hit decision is fragile when the hit lies within eps of an edge
+ * of its triangle, or when triangles grown by eps give another closest hit
+ * (a near miss in front, or a near miss where the strict ray misses). */
+static int closest_fragile(const orc_scene* s, v3 o, v3 d, double tmin, int64_t h, double u, double v, double eps) {
+    if (h >= 0 && edge_margin(s, h, d, u, v) < eps) return 1;
+    double tl;
+    const int64_t hl = trace_slack(s, o, d, tmin, INFINITY, 0, eps, &tl);
+    return hl != h;
+}
+
+/* A shadow ray (unit direction) is fragile when its occlusion differs between
+ * triangles grown by eps over [tmin, tmax + eps] and shrunk by eps over
+ * [tmin + eps, tmax - eps]. */
+static int shadow_fragile(const orc_scene* s, v3 o, v3 d, double tmin, double tmax, double eps) {
+    double t;
+    const int loose = trace_slack(s, o, d, tmin, tmax + eps, 1, eps, &t) >= 0;
+    const int tight = (tmin + eps <= tmax - eps) && trace_slack(s, o, d, tmin + eps, tmax - eps, 1, -eps, &t) >= 0;
+    return loose != tight;
 }
 
 /* Tie / edge classification on the brute-force hit list: a ray is flagged
@@ -508,11 +640,91 @@ static void light_sample(const crt_oracle_light* L, v3 p, uint64_t seed, uint64_
     }
 }
 
-/* src/crt/path_tracing/unidirectional.hpp:24-96 with float Color arithmetic. */
+/* ImageUVMap::operator() (src/crt/materials/material.hpp:50-60): nearest
+ * texel x = (size_t)(u W + 0.5), y = (size_t)(v H + 0.5), channels / 255.
+ * The indices are clamped to the image, where the reference reads past the
+ * end of its pixel array for u or v near 1. */
+static void tex_lookup(const orc_scene* s, int idx, float u, float v, float out[3]) {
+    const crt_oracle_texture* T = &s->tex[idx];
+    const double fx = (double)(u * (float)T->width) + 0.5, fy = (double)(v * (float)T->height) + 0.5;
+    int64_t x = fx > 0.0 ? (int64_t)fx : 0, y = fy > 0.0 ? (int64_t)fy : 0;
+    if (x > (int64_t)T->width - 1) x = (int64_t)T->width - 1;
+    if (y > (int64_t)T->height - 1) y = (int64_t)T->height - 1;
+    const uint8_t* p = T->rgba + 4 * ((size_t)T->width * (size_t)y + (size_t)x);
+    for (int k = 0; k < 3; ++k) out[k] = p[k] / 255.f;
+}
+
+/* Texel coordinate within 2e-3 of a texel boundary: FP32 barycentrics may
+ * pick the neighbouring texel. */
+static int texel_fragile(const orc_scene* s, int idx, float u, float v) {
+    const crt_oracle_texture* T = &s->tex[idx];
+    const double fx = (double)(u * (float)T->width) + 0.5, fy = (double)(v * (float)T->height) + 0.5;
+    const double ex = fabs(fx - floor(fx + 0.5)), ey = fabs(fy - floor(fy + 0.5));
+    return (ex < 2e-3 && fx > 0.5 && fx < T->width - 0.5) || (ey < 2e-3 && fy > 0.5 && fy < T->height - 0.5);
+}
+
+static crt_oracle_material material_of(const orc_scene* s, int64_t h) {
+    const uint32_t in = s->inst[h];
+    crt_oracle_material m;
+    if (s->mat && in >= 1 && in <= (uint32_t)s->ni) return s->mat[in - 1];
+    memset(&m, 0, sizeof(m));
+    m.texture = m.spec_texture = -1;
+    for (int k = 0; k < 3; ++k) m.color[k] = (in >= 1 && in <= (uint32_t)s->ni) ? s->color[3 * (in - 1) + k] : 1.f;
+    return m;
+}
+
+/* Per-vertex UVs interpolated like unidirectional.hpp:55 (float arithmetic,
+ * u * uv1 + v * uv2 first, then + w * uv0). */
+static void interp_uv(const orc_scene* s, int64_t h, double u, double v, float* iu, float* iv) {
+    *iu = 0.f; *iv = 0.f;
+    if (!s->uv) return;
+    const float* a = s->uv + 2 * s->vidx[3 * h];
+    const float* b = s->uv + 2 * s->vidx[3 * h + 1];
+    const float* c = s->uv + 2 * s->vidx[3 * h + 2];
+    const float fu = (float)u, fv = (float)v, fw = (float)(1.0 - u - v);
+    *iu = fu * b[0] + fv * c[0] + fw * a[0];
+    *iv = fu * b[1] + fv * c[1] + fw * a[1];
+}
+
+/* Material::compute for the light direction ld (unit, towards the light),
+ * view direction vd (the incoming path ray) and shading normal sn:
+ * ColoredLambertian / TexturedLambertian (material.hpp:100-103, 123-126):
+ * map(u,v) * (float)(-L.n); TexturedBlinnPhong (material.hpp:152-160):
+ * clamp(map (ka + kd diffuse) + ks spec), spec = pow(n^.normalize(L - vd),
+ * alpha); Mirror (material.hpp:176-178): 0. */
+static void mat_compute(const orc_scene* s, const crt_oracle_material* m, float iu, float iv, v3 ld, v3 vd, v3 sn,
+                        float c[3]) {
+    float col[3];
+    if (m->type == 3) { c[0] = c[1] = c[2] = 0.f; return; }
+    if (m->type == 0 || m->texture < 0) for (int k = 0; k < 3; ++k) col[k] = m->color[k];
+    else tex_lookup(s, m->texture, iu, iv, col);
+    if (m->type != 2) {
+        const float ldn = (float)(-dot(ld, sn));
+        for (int k = 0; k < 3; ++k) c[k] = col[k] * ldn;
+        return;
+    }
+    float co[3];
+    if (m->spec_texture < 0) for (int k = 0; k < 3; ++k) co[k] = m->coeffs[k];
+    else tex_lookup(s, m->spec_texture, iu, iv, co);
+    const float diffuse = -(float)dot(ld, sn);
+    const float spec = (float)pow(dot(nrm(sn), nrm(add(scl(vd, -1.0), ld))), (double)m->alpha);
+    for (int k = 0; k < 3; ++k) {
+        const float x = col[k] * (co[0] + co[1] * diffuse) + co[2] * spec * 1.0f;
+        c[k] = x < 0.f ? 0.f : (1.f < x ? 1.f : x);   /* std::clamp keeps NaN */
+    }
+}
+
+/* src/crt/path_tracing/unidirectional.hpp:24-96 with float Color arithmetic;
+ * the material of the hit triangle's instance computes the direct light and
+ * samples the next direction (Material::compute / sample).  `frag` (may be
+ * NULL) counts fragile decisions: [0] closest hits, [1] shadow rays, [2]
+ * texel choices. */
 static void unidirectional(const orc_scene* s, const crt_oracle_light* lights, int nl, v3 o, v3 d, int nb,
-                           uint64_t seed, uint64_t key_a, uint64_t sample, float out[3]) {
+                           uint64_t seed, uint64_t key_a, uint64_t sample, double offset, double eps, float out[3],
+                           int32_t* frag) {
     double tt, uu, vv;
     int64_t h = trace(s, o, d, 0.0, INFINITY, 0, &tt, &uu, &vv);
+    if (frag && closest_fragile(s, o, d, 0.0, h, uu, vv, eps)) frag[0]++;
     float path[3] = {0.f, 0.f, 0.f};
     float weight[3];
     weight[0] = weight[1] = weight[2] = (float)(2 * M_PI);
@@ -520,44 +732,57 @@ static void unidirectional(const orc_scene* s, const crt_oracle_light* lights, i
         if (h < 0) break;
         const v3 gn = nrm(s->n[h]);
         const v3 sn = hit_normal(s, h, uu, vv);
+        const crt_oracle_material m = material_of(s, h);
+        float iu, iv;
+        interp_uv(s, h, uu, vv, &iu, &iv);
+        if (frag && m.type != 0 && m.type != 3 &&
+            ((m.texture >= 0 && texel_fragile(s, m.texture, iu, iv)) ||
+             (m.type == 2 && m.spec_texture >= 0 && texel_fragile(s, m.spec_texture, iu, iv))))
+            frag[2]++;
         v3 p = hit_point(s, h, uu, vv);
-        p = sub(p, scl(gn, 0.0001));
-        const float* col = s->color + 3 * (s->inst[h] - 1);
+        p = sub(p, scl(gn, offset));
         float lr[3] = {0.f, 0.f, 0.f};
-        for (int l = 0; l < nl; ++l) {
-            v3 ld;
-            double ltmax, irr;
-            light_sample(&lights[l], p, seed, key_a, sample, b, l, &ld, &ltmax, &irr);
-            double t2, u2, v2;
-            const int64_t occ = trace(s, p, ld, 0.0, ltmax, 1, &t2, &u2, &v2);
-            float c[3] = {0.f, 0.f, 0.f};
-            if (occ < 0) {
-                /* ColoredLambertianMaterial::compute (material.hpp:100-103) */
-                const float ldn = (float)(-dot(ld, sn));
-                for (int k = 0; k < 3; ++k) c[k] = col[k] * ldn;
+        if (m.type != 3) {   /* a mirror's compute() is 0 whatever the shadow ray finds */
+            for (int l = 0; l < nl; ++l) {
+                v3 ld;
+                double ltmax, irr;
+                light_sample(&lights[l], p, seed, key_a, sample, b, l, &ld, &ltmax, &irr);
+                double t2, u2, v2;
+                const int64_t occ = trace(s, p, ld, 0.0, ltmax, 1, &t2, &u2, &v2);
+                if (frag && shadow_fragile(s, p, ld, 0.0, ltmax, eps)) frag[1]++;
+                float c[3] = {0.f, 0.f, 0.f};
+                if (occ < 0) mat_compute(s, &m, iu, iv, ld, d, sn, c);
+                for (int k = 0; k < 3; ++k) lr[k] += c[k] * (float)irr;
             }
-            for (int k = 0; k < 3; ++k) lr[k] += c[k] * (float)irr;
         }
         if (b >= 1)
             for (int k = 0; k < 3; ++k) lr[k] = lr[k] < 0.f ? 0.f : (lr[k] > 1.f ? 1.f : lr[k]);
         for (int k = 0; k < 3; ++k) path[k] += lr[k] * weight[k];
         if (b == nb - 1) break;
-        const double r1 = crt_oracle_u01(seed, key_a, sample, rng_c(b, RNG_BRDF));
-        const double r2 = crt_oracle_u01(seed, key_a, sample, rng_c(b, RNG_BRDF + 1));
-        const v3 nd = cosine_importance(sn, r1, r2);
-        const float bc = (float)(1 - r1);
-        for (int k = 0; k < 3; ++k) weight[k] *= bc;
+        v3 nd;
+        if (m.type == 3) {
+            /* MirrorMaterial::sample (material.hpp:179-182): d - 2 n (d.n), weight x 1 */
+            nd = sub(d, scl(scl(sn, 2.0), dot(d, sn)));
+        } else {
+            const double r1 = crt_oracle_u01(seed, key_a, sample, rng_c(b, RNG_BRDF));
+            const double r2 = crt_oracle_u01(seed, key_a, sample, rng_c(b, RNG_BRDF + 1));
+            nd = cosine_importance(sn, r1, r2);
+            const float bc = (float)(1 - r1);
+            for (int k = 0; k < 3; ++k) weight[k] *= bc;
+        }
         o = p;
         d = nd;
         h = trace(s, o, d, 0.0, INFINITY, 0, &tt, &uu, &vv);
+        if (frag && closest_fragile(s, o, d, 0.0, h, uu, vv, eps)) frag[0]++;
     }
     out[0] = path[0]; out[1] = path[1]; out[2] = path[2];
 }
 
 /* src/crt/do_render.hpp:65-116: per pixel running mean with adaptive stop. */
-void crt_oracle_render(const orc_scene* s, const crt_oracle_camera* c, const crt_oracle_light* lights, int nl,
-                       int min_samples, int max_samples, double noise_threshold, int num_bounces, uint64_t seed,
-                       uint64_t key_offset, float* rgb, int32_t* spp_used) {
+void crt_oracle_render_ex(const orc_scene* s, const crt_oracle_camera* c, const crt_oracle_light* lights, int nl,
+                          int min_samples, int max_samples, double noise_threshold, int num_bounces, uint64_t seed,
+                          uint64_t key_offset, double offset, double eps, float* rgb, int32_t* spp_used,
+                          int32_t* fragile) {
     const int64_t w = (int64_t)floor(c->resolution[0]), h = (int64_t)floor(c->resolution[1]);
 #pragma omp parallel for schedule(dynamic, 2)
     for (int64_t j = 0; j < h; ++j)
@@ -565,6 +790,7 @@ void crt_oracle_render(const orc_scene* s, const crt_oracle_camera* c, const crt
             const int64_t k = j * w + i;
             const uint64_t key_a = key_offset + (uint64_t)k;
             float px[3] = {0.f, 0.f, 0.f};
+            int32_t fr[3] = {0, 0, 0};
             int used = 0;
             for (int smp = 1; smp < max_samples + 1; ++smp) {
                 double ju = (double)i, jv = (double)j;
@@ -575,7 +801,8 @@ void crt_oracle_render(const orc_scene* s, const crt_oracle_camera* c, const crt
                 v3 O, D;
                 cam_ray(c, ju, jv, &O, &D);
                 float pr[3];
-                unidirectional(s, lights, nl, O, D, num_bounces, seed, key_a, (uint64_t)smp, pr);
+                unidirectional(s, lights, nl, O, D, num_bounces, seed, key_a, (uint64_t)smp, offset, eps, pr,
+                               fragile ? fr : NULL);
                 float rc[3];
                 const float inv = 1.0f / (float)smp;
                 for (int q = 0; q < 3; ++q) { rc[q] = (pr[q] - px[q]) * inv; px[q] += rc[q]; }
@@ -587,7 +814,16 @@ void crt_oracle_render(const orc_scene* s, const crt_oracle_camera* c, const crt
             }
             rgb[3 * k] = px[0]; rgb[3 * k + 1] = px[1]; rgb[3 * k + 2] = px[2];
             if (spp_used) spp_used[k] = used;
+            if (fragile) { fragile[3 * k] = fr[0]; fragile[3 * k + 1] = fr[1]; fragile[3 * k + 2] = fr[2]; }
         }
+}
+
+void crt_oracle_render(const orc_scene* s, const crt_oracle_camera* c, const crt_oracle_light* lights, int nl,
+                       int min_samples, int max_samples, double noise_threshold, int num_bounces, uint64_t seed,
+                       uint64_t key_offset, float* rgb, int32_t* spp_used) {
+    /* the reference's self-intersection offset, src/crt/path_tracing/unidirectional.hpp:61 */
+    crt_oracle_render_ex(s, c, lights, nl, min_samples, max_samples, noise_threshold, num_bounces, seed, key_offset,
+                         1e-4, 0.0, rgb, spp_used, NULL);
 }
 
 /* ------------------------------------------------------ radiation pressure */

@@ -298,3 +298,15 @@ def test_oracle_render_statistics_match_reference():
         d = (mu[m] - mr[m]).mean()
         sek = np.sqrt((var[m] / M + vr[m] / K).sum()) / m.sum()
         assert abs(d) < 4 * sek + 1e-9, (k, d, sek)
+
+
+def test_vectorised_counter_rng_matches_oracle():
+    # the numpy restatement used to build flash sub-rays for large scans is
+    # the oracle's crt_oracle_u01 bit for bit
+    import flash_ref
+    rng = np.random.default_rng(0)
+    a = rng.integers(0, 2 ** 40, size=64, dtype=np.uint64)
+    for seed, b, c in ((0, 0, 0), (7, 3, 1), (2 ** 40 + 5, 17, 129)):
+        got = flash_ref.u01(seed, a, b, c)
+        want = np.array([oracle.u01(seed, int(x), b, c) for x in a])
+        assert np.array_equal(got, want)
