@@ -91,6 +91,8 @@ def _declare(L):
                                     C.c_uint64, _p, _p]
     L.crt_oracle_render_ex.argtypes = [_p, _p, _p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_uint64,
                                        C.c_uint64, C.c_double, C.c_double, _p, _p, _p]
+    L.crt_oracle_closest_fragile.argtypes = [_p, C.c_int64, _p, _p, C.c_double, _p]
+    L.crt_oracle_shadow_fragile.argtypes = [_p, C.c_int64, _p, _p, _p, C.c_double, _p]
     L.crt_oracle_scene_set_materials.argtypes = [_p, C.c_int64, _p, _p, C.c_int64, C.c_int, _p]
     L.crt_oracle_radiation_pressure.argtypes = [_p, _p, C.c_double, C.c_double, C.c_double, C.c_int64, _p,
                                                 C.c_double, _p, _p, _p, _p]
@@ -186,6 +188,25 @@ class Scene:
         other = np.ascontiguousarray(other_tri, dtype=np.int32)
         f = np.zeros(o.shape[0], dtype=np.uint8)
         lib().crt_oracle_edge_flags(self.h, o.shape[0], _ptr(o), _ptr(d), _ptr(other), rel_t, bary_eps, _ptr(f))
+        return f.astype(bool)
+
+    def closest_fragile(self, o, d, eps):
+        """Per ray: the closest-hit decision lies within eps (distance, after
+        the 1/|cos| grazing stretch) of a triangle edge or of another triangle."""
+        o = np.ascontiguousarray(o, dtype=np.float64).reshape(-1, 3)
+        d = np.ascontiguousarray(d, dtype=np.float64).reshape(-1, 3)
+        f = np.zeros(o.shape[0], dtype=np.uint8)
+        lib().crt_oracle_closest_fragile(self.h, o.shape[0], _ptr(o), _ptr(d), float(eps), _ptr(f))
+        return f.astype(bool)
+
+    def shadow_fragile(self, o, d, eps, tmax=None):
+        """Per shadow ray over [0, tmax]: occlusion differs between triangles
+        grown and shrunk by eps (a terminator or an edge within eps)."""
+        o = np.ascontiguousarray(o, dtype=np.float64).reshape(-1, 3)
+        d = np.ascontiguousarray(d, dtype=np.float64).reshape(-1, 3)
+        tm = None if tmax is None else np.ascontiguousarray(tmax, dtype=np.float64)
+        f = np.zeros(o.shape[0], dtype=np.uint8)
+        lib().crt_oracle_shadow_fragile(self.h, o.shape[0], _ptr(o), _ptr(d), _ptr(tm), float(eps), _ptr(f))
         return f.astype(bool)
 
     def camera_pass(self, cam: dict):

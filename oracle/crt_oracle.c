@@ -418,6 +418,26 @@ static int shadow_fragile(const orc_scene* s, v3 o, v3 d, double tmin, double tm
     return loose != tight;
 }
 
+void crt_oracle_closest_fragile(const orc_scene* s, int64_t n, const double* o, const double* d, double eps,
+                                uint8_t* flags) {
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < n; ++i) {
+        const v3 O = mk(o[3 * i], o[3 * i + 1], o[3 * i + 2]), D = nrm(mk(d[3 * i], d[3 * i + 1], d[3 * i + 2]));
+        double tt = 0, uu = 0, vv = 0;
+        const int64_t h = trace(s, O, D, 0.0, INFINITY, 0, &tt, &uu, &vv);
+        flags[i] = (uint8_t)closest_fragile(s, O, D, 0.0, h, uu, vv, eps);
+    }
+}
+
+void crt_oracle_shadow_fragile(const orc_scene* s, int64_t n, const double* o, const double* d, const double* tmax,
+                               double eps, uint8_t* flags) {
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < n; ++i) {
+        const v3 O = mk(o[3 * i], o[3 * i + 1], o[3 * i + 2]), D = nrm(mk(d[3 * i], d[3 * i + 1], d[3 * i + 2]));
+        flags[i] = (uint8_t)shadow_fragile(s, O, D, 0.0, tmax ? tmax[i] : INFINITY, eps);
+    }
+}
+
 /* Tie / edge classification on the brute-force hit list: a ray is flagged
  * when a second triangle is hit within rel_t of the closest t, or when the
  * closest hit's smallest barycentric is below bary_eps. */

@@ -73,6 +73,15 @@ void crt_oracle_tie_flags(const orc_scene* s, int64_t n, const double* o, const 
                           double bary_eps, uint8_t* flags);
 void crt_oracle_edge_flags(const orc_scene* s, int64_t n, const double* o, const double* d, const int32_t* other,
                            double rel_t, double bary_eps, uint8_t* flags);
+/* Per ray (directions normalised here): 1 where the closest-hit decision is
+ * fragile at distance eps (hit within eps / |cos| of an edge of its triangle,
+ * or triangles grown by eps give another closest hit), resp. where a shadow
+ * ray over [0, tmax] (NULL: infinite) is fragile (occlusion differs between
+ * triangles grown and shrunk by eps). */
+void crt_oracle_closest_fragile(const orc_scene* s, int64_t n, const double* o, const double* d, double eps,
+                                uint8_t* flags);
+void crt_oracle_shadow_fragile(const orc_scene* s, int64_t n, const double* o, const double* d, const double* tmax,
+                               double eps, uint8_t* flags);
 void crt_oracle_camera_rays(const crt_oracle_camera* c, double* o, double* d);
 void crt_oracle_pass(const orc_scene* s, const crt_oracle_camera* c, double* xyz, double* normal, uint32_t* instance,
                      double* depth, int32_t* tri);

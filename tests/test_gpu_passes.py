@@ -9,6 +9,7 @@ import pytest
 
 import oracle
 from crt import capi, scenes
+from render_parity import fp32_eps
 
 pytestmark = pytest.mark.gpu
 
@@ -38,17 +39,24 @@ def icosphere():
     return gpu, cpu, m
 
 
-def _compare_pass(gpu_out, cpu_out, cpu, cam, max_mismatch_frac):
+def _compare_pass(gpu_out, cpu_out, cpu, cam, eps):
+    # every id that differs must be an exact tie, an edge/vertex graze
+    # (barycentric within 1e-5), or a closest-hit decision within eps of an
+    # edge or of another triangle (FP32 resolution; crt_oracle_closest_fragile)
     o, d = oracle.camera_rays(cam)
     tri_g = gpu_out["tri"].ravel()
     tri_c = cpu_out["tri"].ravel()
     mism = tri_g != tri_c
-    ties = (cpu.tie_flags(o[mism], d[mism]) | cpu.edge_flags(o[mism], d[mism], tri_g[mism])) \
-        if mism.any() else np.zeros(0, bool)
-    hard = int(mism.sum() - ties.sum())
+    ties = np.zeros(0, bool)
+    fr = np.zeros(0, bool)
+    if mism.any():
+        ties = cpu.tie_flags(o[mism], d[mism]) | cpu.edge_flags(o[mism], d[mism], tri_g[mism])
+        fr = cpu.closest_fragile(o[mism], d[mism], eps)
+    hard = int((~(ties | fr)).sum())
     n = tri_c.size
-    print("pixels %d  mismatched ids %d  tie/edge %d  unexplained %d" % (n, mism.sum(), ties.sum(), hard))
-    assert hard <= max_mismatch_frac * n
+    print("pixels %d  mismatched ids %d  tie/edge %d  within eps %d  unexplained %d"
+          % (n, mism.sum(), ties.sum(), (fr & ~ties).sum(), hard))
+    assert hard == 0
     same = ~mism
     hit = same & (tri_c >= 0)
     dg = gpu_out["depth"].ravel()[hit]
@@ -72,7 +80,7 @@ def test_cornell_passes_match_oracle(cornell):
     cam = scenes.cornell_camera()
     g = gpu.camera_pass(cam)
     c = cpu.camera_pass(cam)
-    _compare_pass(g, c, cpu, cam, max_mismatch_frac=0.0)
+    _compare_pass(g, c, cpu, cam, fp32_eps(gpu.info()))
     same = g["tri"] == c["tri"]
     assert np.array_equal(g["instance"][same], c["instance"][same])
     hit = same & (c["tri"] >= 0)
@@ -98,7 +106,7 @@ def test_icosphere_primary_subsample(icosphere):
     cam = scenes.small_body_camera(m, resolution=512, seed=1)
     g = gpu.camera_pass(cam)
     c = cpu.camera_pass(cam)
-    frac, tie = _compare_pass(g, c, cpu, cam, max_mismatch_frac=1e-4)
+    frac, tie = _compare_pass(g, c, cpu, cam, fp32_eps(gpu.info()))
     print("icosphere 512^2: id mismatch fraction %.2e (tie/edge %.2e)" % (frac, tie))
 
 
@@ -143,15 +151,20 @@ def test_sun_pass_shadows(icosphere):
     # oracle shadow rays from the oracle hit points, offset like the GPU
     p = c["xyz"].reshape(-1, 3)[hit.ravel()]
     n = c["normal"].reshape(-1, 3)[hit.ravel()]
-    off = max(1e-4, 1e-6 * 2300.0)
-    o = p - off * n
+    o = p - gpu.info()["self_offset"] * n
     dd = np.repeat(np.asarray(sun)[None, :], o.shape[0], axis=0)
     _, occ, _, _ = cpu.trace(o, dd, any_hit=True)
     lit_c = occ < 0
     lit_g = r["lit"].ravel()[hit.ravel()].astype(bool)
     same_id = r["tri"].ravel()[hit.ravel()] == c["tri"].ravel()[hit.ravel()]
     disagree = (lit_c != lit_g) & same_id
-    print("shadow disagreement %d of %d" % (disagree.sum(), hit.sum()))
+    # every disagreement is a terminator or an edge within FP32 resolution:
+    # the shadow ray's occlusion flips between triangles grown and shrunk by eps
+    eps = fp32_eps(gpu.info())
+    fr = cpu.shadow_fragile(o[disagree], dd[disagree], eps)
+    print("shadow rays %d, disagreements %d, all within eps (terminator / edge): %s; fragile rays overall %.4f"
+          % (hit.sum(), disagree.sum(), bool(fr.all()), cpu.shadow_fragile(o, dd, eps).mean()))
+    assert fr.all()
     assert disagree.mean() < 2e-3
     assert np.all(r["lit"][~hit] == 0)
 

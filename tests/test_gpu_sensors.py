@@ -18,6 +18,7 @@ import pytest
 import oracle
 from crt import capi, scenes
 from flash_ref import flash_subrays
+from render_parity import classify, fp32_eps
 
 pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -44,20 +45,32 @@ def _light():
     return dict(scenes.cornell_light(), type="area")
 
 
+def _same_keys(gpu, cpu, cam, lights, spp, nb, seed, tag):
+    # identical RNG keys and the library's self-intersection offset (1e-4,
+    # the reference's, at this scale): every pixel that differs must carry a
+    # fragile oracle decision (tests/render_parity.py)
+    info = gpu.info()
+    rgba, u8, spp_g, tm = gpu.render(cam, lights, spp, spp, 0.0, nb, seed=seed)
+    rgb_c, spp_c, frag = cpu.render(cam, lights, spp, spp, 0.0, nb, seed=seed, offset=info["self_offset"],
+                                    eps=fp32_eps(info))
+    r = classify(rgba[..., :3], rgb_c, frag)
+    print("%s: agree %.5f, differ %d (closest %d, shadow %d, unexplained %d), image mean rel %.2e (%.2e agreeing)"
+          % (tag, r["agree"], r["differ"], r["closest"], r["shadow"], r["unexplained"], r["mean_rel"],
+             r["mean_rel_agreeing"]))
+    assert r["unexplained"] == 0
+    assert r["mean_rel_agreeing"] < 1e-4
+    return rgba, u8, spp_g, r
+
+
 def test_render_matches_oracle_same_keys(cornell):
     gpu, cpu = cornell
     cam = scenes.cornell_camera()
     cam["resolution"] = [128, 128]
-    rgba, u8, spp, tm = gpu.render(cam, [_light()], 16, 16, 0.0, 4, seed=7)
-    rgb_c, spp_c = cpu.render(cam, [_light()], 16, 16, 0.0, 4, seed=7)
+    assert gpu.info()["self_offset"] == pytest.approx(1e-4)
+    rgba, u8, spp, r = _same_keys(gpu, cpu, cam, [_light()], 16, 4, 7, "Cornell 128^2 x 16 spp x 4 bounces")
     g = rgba[..., :3]
     assert np.all(spp == 16) and np.all(rgba[..., 3] == 1.0)
-    diff = np.abs(g - rgb_c)
-    scale = np.maximum(np.abs(rgb_c), 1e-3)
-    close = np.all(diff <= 1e-3 * scale + 1e-5, axis=-1)
-    print("render same-key agreement: %.4f of pixels, mean |diff| %.3e" % (close.mean(), diff.mean()))
-    assert close.mean() > 0.97
-    assert abs(g.mean() - rgb_c.mean()) < 2e-3 * rgb_c.mean()
+    assert r["agree"] > 0.99
     # uint8 image = trunc(clamp(256 x, 0, 255)), alpha 255
     assert np.array_equal(u8[..., :3], np.clip(g * 256.0, 0, 255).astype(np.uint8))
     assert np.all(u8[..., 3] == 255)
@@ -72,14 +85,9 @@ def test_render_odd_resolutions_match_oracle(cornell, res):
     cam = scenes.cornell_camera()
     cam["resolution"] = list(res)
     cam["sensor_size"] = [cam["sensor_size"][0], cam["sensor_size"][0] * res[1] / res[0]]
-    rgba, _, spp, _ = gpu.render(cam, [_light()], 4, 4, 0.0, 3, seed=9)
-    rgb_c, _ = cpu.render(cam, [_light()], 4, 4, 0.0, 3, seed=9)
+    rgba, _, spp, _ = _same_keys(gpu, cpu, cam, [_light()], 4, 3, 9, "Cornell %dx%d" % tuple(res))
     assert rgba.shape == (res[1], res[0], 4) and np.all(spp == 4)
-    diff = np.abs(rgba[..., :3] - rgb_c)
-    close = np.all(diff <= 1e-3 * np.maximum(np.abs(rgb_c), 1e-3) + 1e-5, axis=-1)
     assert np.all(np.isfinite(rgba))
-    if close.size >= 20:   # a few pixels may differ where FP32 and FP64 geometry part ways
-        assert close.mean() >= 0.9
     p = gpu.camera_pass(cam)
     c = cpu.camera_pass(cam)
     mism = p["tri"] != c["tri"]
@@ -133,10 +141,8 @@ def test_render_point_and_directional_lights(cornell):
     cam["resolution"] = [64, 64]
     lights = [{"type": "point", "intensity": 0.05, "position": np.array([0.5, 0.9, 0.5])},
               {"type": "directional", "intensity": 0.2, "direction": np.array([0.3, 0.2, 1.0])}]
-    rgba, _, _, _ = gpu.render(cam, lights, 4, 4, 0.0, 2, seed=1)
-    rgb_c, _ = cpu.render(cam, lights, 4, 4, 0.0, 2, seed=1)
-    close = np.all(np.abs(rgba[..., :3] - rgb_c) <= 1e-3 * np.maximum(np.abs(rgb_c), 1e-3) + 1e-5, axis=-1)
-    assert close.mean() > 0.97
+    _, _, _, r = _same_keys(gpu, cpu, cam, lights, 4, 2, 1, "Cornell point + directional")
+    assert r["agree"] > 0.99
 
 
 def test_lidar_batch_matches_reference(icosphere):
@@ -233,7 +239,9 @@ def test_radiation_pressure_matches_oracle(icosphere):
     print("SRP: F gpu %s oracle %s; hits %d vs %d" % (F, Fc, nh, nhc))
     assert abs(nh - nhc) <= 2e-4 * nhc
     assert np.linalg.norm(F - Fc) < 1e-4 * np.linalg.norm(Fc)
-    assert np.linalg.norm(T - Tc) < 1e-4 * np.linalg.norm(Fc) * 2400.0
+    print("SRP: force rel err %.2e, torque rel err %.2e" % (np.linalg.norm(F - Fc) / np.linalg.norm(Fc),
+                                                            np.linalg.norm(T - Tc) / np.linalg.norm(Tc)))
+    assert np.linalg.norm(T - Tc) < 1e-4 * np.linalg.norm(Tc)
 
 
 def test_radiation_pressure_sphere_closed_form():

@@ -310,3 +310,34 @@ def test_vectorised_counter_rng_matches_oracle():
         got = flash_ref.u01(seed, a, b, c)
         want = np.array([oracle.u01(seed, int(x), b, c) for x in a])
         assert np.array_equal(got, want)
+
+
+def test_fragility_classifiers_closed_forms():
+    """crt_oracle_closest_fragile / shadow_fragile on one triangle in z = 0
+    (legs of 10 along x and y): a ray's decision is fragile exactly when it
+    passes within eps of an edge (stretched by 1/|cos| for grazing rays), or
+    a shadow ray's blocker lies within eps of its end."""
+    v = np.array([[0, 0, 0], [10, 0, 0], [0, 10, 0]], dtype=np.float32)
+    s = oracle.Scene(v, np.array([[0, 1, 2]], dtype=np.uint32))
+    down = np.array([0.0, 0.0, -1.0])
+    o = np.array([[3.0, 3.0, 5.0],        # interior, far from every edge
+                  [5.0, 5e-4, 5.0],       # 5e-4 inside the edge y = 0
+                  [5.0, -5e-4, 5.0],      # 5e-4 outside it (a near miss)
+                  [5.0, -5e-3, 5.0]])     # a clear miss
+    d = np.repeat(down[None], 4, axis=0)
+    assert s.closest_fragile(o, d, 1e-3).tolist() == [False, True, True, False]
+    assert s.closest_fragile(o, d, 1e-4).tolist() == [False, False, False, False]
+    # grazing: |cos| = 0.01, 0.05 inside the edge x = 0 -> 5e-4 across the ray
+    g = np.array([-1.0, 0.0, -0.01])
+    og = np.array([[0.05, 5.0, 0.0]]) - 5.0 * g
+    assert s.closest_fragile(og, g[None], 1e-3).tolist() == [True]
+    assert s.closest_fragile(og, g[None], 1e-4).tolist() == [False]
+    # shadow rays (upwards through the triangle from below)
+    up = np.array([[0.0, 0.0, 1.0]] * 4)
+    os_ = np.array([[3.0, 3.0, -1.0],     # blocked well inside [0, tmax]
+                    [5.0, 5e-4, -1.0],    # blocked 5e-4 from the edge
+                    [3.0, 3.0, -1.0],     # blocker 5e-4 beyond tmax
+                    [3.0, 3.0, -1.0]])    # blocker far beyond tmax
+    tmax = np.array([10.0, 10.0, 1.0 - 5e-4, 0.5])
+    assert s.shadow_fragile(os_, up, 1e-3, tmax).tolist() == [False, True, True, False]
+    assert s.shadow_fragile(os_, up, 1e-5, tmax).tolist() == [False, False, False, False]
