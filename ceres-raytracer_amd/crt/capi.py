@@ -119,6 +119,8 @@ SIGNATURES = {
     "crt_device_init": (C.c_int, [C.c_int]),
     "crt_last_error": (C.c_char_p, []),
     "crt_version": (C.c_int, []),
+    "crt_timing_log_enable": (C.c_int, [C.c_int]),
+    "crt_timing_log_read": (C.c_int, [C.c_char_p, _p, _p]),
     "crt_scene_create": (C.c_int, [_p, C.c_uint32, _p, C.c_uint32, _p, _p, C.c_uint32, _p, _p, _p, _p]),
     "crt_scene_create_f64": (C.c_int, [_p, C.c_uint32, _p, C.c_uint32, _p, _p, C.c_uint32, _p, _p, _p, _p]),
     "crt_scene_info": (C.c_int, [_p, _p]),
@@ -287,6 +289,19 @@ def device_copy(dst: DeviceBuffer, src: DeviceBuffer):
     n = min(dst.nbytes, src.nbytes)
     if _hip().hipMemcpy(C.c_void_p(dst.ptr), C.c_void_p(src.ptr), C.c_size_t(n), 3) != 0:
         raise RuntimeError("hipMemcpy failed")
+
+
+def timing_log(on: bool):
+    """Enable / disable the library's kernel timing log (crt_timing_log_enable)."""
+    _check(load().crt_timing_log_enable(int(bool(on))))
+
+
+def timing_log_read(kernel: str):
+    """(count, mean ms) of the logged launches of `kernel` (events waited for)."""
+    n = C.c_uint64()
+    ms = C.c_double()
+    _check(load().crt_timing_log_read(kernel.encode(), C.byref(n), C.byref(ms)))
+    return n.value, ms.value
 
 
 def device_synchronize():

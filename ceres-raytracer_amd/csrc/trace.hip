@@ -341,6 +341,25 @@ struct EventTimer {
     }
 };
 
+// Kernel timing log (crt_timing_log_*): while enabled, entry points record
+// a hipEvent pair around each of their dominant kernels on the launch
+// stream, without synchronising, so a benchmark can time kernels inside its
+// own timed region (streams overlapping) the way a profiler sees them.
+struct TlogEntry {
+    const char* kernel;
+    hipEvent_t a, b;
+};
+std::mutex g_tlog_mu;
+bool g_tlog_on = false;
+std::vector<TlogEntry> g_tlog;
+
+hipEvent_t tlog_event(hipStream_t st) {
+    hipEvent_t e = nullptr;
+    CRT_HIP_CHECK(hipEventCreate(&e));
+    CRT_HIP_CHECK(hipEventRecord(e, st));
+    return e;
+}
+
 // Device output staging: use the caller's device pointer directly, or a
 // workspace buffer followed by a copy to the host pointer.
 struct OutSlot {
@@ -405,6 +424,37 @@ void crt::scene_make_view(crt_scene* s, const float* host_vertices) {
 extern "C" {
 
 int crt_version(void) { return 100; }
+
+int crt_timing_log_enable(int on) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(g_tlog_mu);
+        g_tlog_on = on != 0;
+    });
+}
+
+int crt_timing_log_read(const char* kernel, uint64_t* count, double* mean_ms) {
+    return guarded([&] {
+        CRT_REQUIRE(kernel && count && mean_ms, CRT_ERR_INVALID, "NULL argument");
+        std::vector<TlogEntry> mine;
+        {
+            std::lock_guard<std::mutex> lk(g_tlog_mu);
+            std::vector<TlogEntry> rest;
+            for (const TlogEntry& e : g_tlog) (std::strcmp(e.kernel, kernel) == 0 ? mine : rest).push_back(e);
+            g_tlog.swap(rest);
+        }
+        double sum = 0.0;
+        for (const TlogEntry& e : mine) {
+            CRT_HIP_CHECK(hipEventSynchronize(e.b));
+            float f = 0.f;
+            CRT_HIP_CHECK(hipEventElapsedTime(&f, e.a, e.b));
+            sum += f;
+            (void)hipEventDestroy(e.a);
+            (void)hipEventDestroy(e.b);
+        }
+        *count = mine.size();
+        *mean_ms = mine.empty() ? 0.0 : sum / (double)mine.size();
+    });
+}
 
 const char* crt_last_error(void) { return g_last_error.c_str(); }
 
@@ -696,16 +746,29 @@ int crt_sun_pass(crt_scene* scene, const crt_camera* camera, const double sun_di
         EventTimer tm(timing != nullptr);
         const uint32_t blocks = cdiv((uint64_t)cam.w, PIX_TILE) * cdiv((uint64_t)cam.h, PIX_TILE);
         CRT_HIP_CHECK(hipMemsetAsync(qcount, 0, 4, st));
+        bool log;
+        {
+            std::lock_guard<std::mutex> lk(g_tlog_mu);
+            log = g_tlog_on;
+        }
+        hipEvent_t la = log ? tlog_event(st) : nullptr;
         EventTimer::rec(tm.a, st);
         hipLaunchKernelGGL(k_primary_sun, dim3(blocks), dim3(LANE_BLOCK), lane_lds_bytes(scene->view), st,
                            scene->view, cam, d_depth, d_tri, d_lit, ws.h.as<ShadowRay>(),
                            shadow_slots(ws.h.as<ShadowRay>(), npix), qcount, offset);
         EventTimer::rec(tm.b, st);
+        hipEvent_t lb = log ? tlog_event(st) : nullptr, lc = log ? tlog_event(st) : nullptr;
         hipLaunchKernelGGL(k_shadow_sun, dim3(cdiv(npix, TRACE_BLOCK / QUAD)), dim3(TRACE_BLOCK),
                            quad_lds_bytes(scene->view), st, scene->view, ws.h.as<ShadowRay>(),
                            shadow_slots(ws.h.as<ShadowRay>(), npix), qcount, sun, d_lit);
         CRT_HIP_CHECK(hipGetLastError());
         EventTimer::rec(tm.c, st);
+        if (log) {
+            hipEvent_t ld = tlog_event(st);
+            std::lock_guard<std::mutex> lk(g_tlog_mu);
+            g_tlog.push_back({"k_primary_sun", la, lb});
+            g_tlog.push_back({"k_shadow_sun", lc, ld});
+        }
         uint32_t nshadow = 0;
         if (timing) CRT_HIP_CHECK(hipMemcpyAsync(&nshadow, qcount, 4, hipMemcpyDeviceToHost, st));
         if (!dev) {

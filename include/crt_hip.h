@@ -147,6 +147,15 @@ int         crt_device_init(int device);
 const char* crt_last_error(void);
 int         crt_version(void);
 
+/* Kernel timing log (benchmarks): while enabled, crt_sun_pass records a
+ * hipEvent pair around each of its kernels (k_primary_sun, k_shadow_sun) on
+ * the launch stream without synchronising, so kernels can be timed inside a
+ * timed region where streams overlap.  crt_timing_log_read waits for the
+ * logged events of one kernel, returns their count and mean duration and
+ * drops them from the log. */
+int crt_timing_log_enable(int on);
+int crt_timing_log_read(const char* kernel, uint64_t* count, double* mean_ms);
+
 /* ---- scene ------------------------------------------------------------- */
 /* Upload a merged triangle mesh (all entities already transformed to the
  * scene frame) and build the BVH on the GPU.
