@@ -19,10 +19,12 @@ class BodyFixedEntity(RigidBody):
     :param color: RGB albedo (default [1,1,1])
     :param geometry_type: "obj"
     :param smooth_shading: interpolate the OBJ vertex normals
+    :param material: (additive) a crt.materials material, default ColoredLambertian
+    :param uv: (additive) per-vertex texture coordinates (n_vertices, 2)
     """
 
     def __init__(self, geometry_path: str, color: ArrayLike = [1, 1, 1], geometry_type: str = "obj",
-                 smooth_shading: bool = False, **kwargs):
+                 smooth_shading: bool = False, material=None, uv=None, **kwargs):
         super(BodyFixedEntity, self).__init__(**kwargs)
         self.geometry_path = geometry_path
         self.geometry_type = geometry_type
@@ -32,6 +34,12 @@ class BodyFixedEntity(RigidBody):
                                          list(self.color))
         self.set_pose(self.position, self.rotation)
         self.set_scale(self.scale)
+        # additive: a crt.materials material (the reference hard-codes
+        # ColoredLambertian) and per-vertex UVs for its texture maps
+        self.material = None
+        if material is not None or uv is not None:
+            from crt import materials
+            materials.apply(self, material, uv)
 
 
 class BodyFixedGroup(RigidBody):

@@ -108,6 +108,32 @@ class ServerStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class Material(C.Structure):
+    """crt_material: src/crt/materials/material.hpp:90-184."""
+    _fields_ = [("type", C.c_uint32), ("texture", C.c_int32), ("spec_texture", C.c_int32), ("alpha", C.c_float),
+                ("color", C.c_float * 3), ("coeffs", C.c_float * 3)]
+
+
+class Texture(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("rgba", C.c_void_p)]
+
+
+MATERIAL_TYPES = {"lambertian": 0, "textured_lambertian": 1, "blinn_phong": 2, "mirror": 3}
+
+
+def make_material(m: dict) -> Material:
+    """dict(type=lambertian|textured_lambertian|blinn_phong|mirror, color=(r,g,b),
+    texture=index or -1, spec_texture=index or -1, coeffs=(ka,kd,ks), alpha=24)."""
+    x = Material()
+    x.type = MATERIAL_TYPES[m.get("type", "lambertian")]
+    x.texture = int(m.get("texture", -1))
+    x.spec_texture = int(m.get("spec_texture", -1))
+    x.alpha = float(m.get("alpha", 24.0))
+    x.color[:] = [float(c) for c in m.get("color", (1.0, 1.0, 1.0))]
+    x.coeffs[:] = [float(c) for c in m.get("coeffs", (0.0, 1.0, 0.0))]
+    return x
+
+
 REQ_RENDER, REQ_LIDAR, REQ_FLUX = 0, 1, 2
 
 _p = C.c_void_p
@@ -124,6 +150,7 @@ SIGNATURES = {
     "crt_scene_create": (C.c_int, [_p, C.c_uint32, _p, C.c_uint32, _p, _p, C.c_uint32, _p, _p, _p, _p]),
     "crt_scene_create_f64": (C.c_int, [_p, C.c_uint32, _p, C.c_uint32, _p, _p, C.c_uint32, _p, _p, _p, _p]),
     "crt_scene_info": (C.c_int, [_p, _p]),
+    "crt_scene_set_materials": (C.c_int, [_p, C.c_uint32, _p, _p, C.c_uint32, _p]),
     "crt_scene_destroy": (None, [_p]),
     "crt_scene_serialize": (C.c_int, [_p, _p, _p]),
     "crt_scene_deserialize": (C.c_int, [_p, C.c_uint64, _p]),
@@ -356,6 +383,19 @@ class Scene:
                       C.byref(opts), C.byref(h)))
         self.h = h
         self.n_tris = i.shape[0]
+
+    def set_materials(self, materials, uv=None, textures=()):
+        """One material dict per instance (see make_material), per-vertex UVs
+        (V, 2) and RGBA8 textures (H, W, 4) referenced by index
+        (crt_scene_set_materials)."""
+        mats = (Material * max(len(materials), 1))(*[make_material(m) for m in materials])
+        keep = [np.ascontiguousarray(t, dtype=np.uint8) for t in textures]
+        for t in keep:
+            if t.ndim != 3 or t.shape[2] != 4:
+                raise ValueError("textures must be (H, W, 4) RGBA8 arrays")
+        tex = (Texture * max(len(keep), 1))(*[Texture(t.shape[1], t.shape[0], t.ctypes.data) for t in keep])
+        uvs = None if uv is None else np.ascontiguousarray(uv, dtype=np.float32).reshape(-1, 2)
+        _check(load().crt_scene_set_materials(self.h, len(materials), mats, ptr(uvs), len(keep), tex))
 
     def close(self):
         if getattr(self, "h", None) and self.h.value:

@@ -58,6 +58,21 @@ __host__ __device__ inline uint32_t make_leaf_ref(uint32_t first, uint32_t count
 __host__ __device__ inline uint32_t leaf_first(uint32_t ref) { return (ref >> 4) & 0x07FFFFFFu; }
 __host__ __device__ inline uint32_t leaf_count(uint32_t ref) { return (ref & 15u) + 1u; }
 
+// Per-instance material (crt_scene_set_materials), 48 B:
+// type 0 ColoredLambertianMaterial, 1 TexturedLambertianMaterial, 2
+// TexturedBlinnPhongMaterial, 3 MirrorMaterial (src/crt/materials/
+// material.hpp:90-184); tex / spec_tex index a texture or are -1 for the
+// constant colour / (ka, kd, ks).
+struct MatDev {
+    uint32_t type;
+    int32_t tex;
+    int32_t spec_tex;
+    float alpha;
+    float color[4];
+    float coeffs[4];
+};
+static_assert(sizeof(MatDev) == 48, "material record is 48 B");
+
 // Device view of a scene (passed by value to kernels).
 struct SceneView {
     const float4* nodes;        // n_wide_nodes * 8 * 2 float4
@@ -68,6 +83,10 @@ struct SceneView {
     const uint32_t* tri_inst;   // n_tris, original order
     const float* inst_color;    // n_instances * 3
     const uint8_t* inst_smooth; // n_instances
+    const MatDev* mats;         // n_instances, or null: ColoredLambertian with inst_color
+    const float2* vuv;          // n_vertices (u, v), or null: zero (the reference OBJ loader sets none)
+    const uint32_t* texels;     // RGBA8 texels of every texture, row-major, row 0 first
+    const uint4* tex_info;      // per texture: first texel, width, height
     uint32_t n_tris;
     uint32_t n_instances;
     uint32_t stack_cap;         // per-octet stack entries

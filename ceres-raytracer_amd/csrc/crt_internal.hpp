@@ -146,6 +146,9 @@ struct crt_scene {
     int device = 0;
     uint32_t n_tris = 0, n_vertices = 0, n_instances = 0;
     crt::DevBuf vertices, indices, tri_vidx, vnormals, tri_inst, inst_color, inst_smooth;
+    // materials (crt_scene_set_materials; empty: ColoredLambertian with inst_color)
+    crt::DevBuf mats, vuv, texels, tex_info;
+    uint32_t n_textures = 0;
     crt::BuildResult bvh;
     crt::SceneView view{};
     crt_bvh_info info{};

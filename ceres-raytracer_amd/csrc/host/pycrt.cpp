@@ -141,6 +141,27 @@ struct DirectionalLight : Pose {
     double intensity, direction[3];
 };
 
+using u8arr = py::array_t<uint8_t, py::array::c_style | py::array::forcecast>;
+
+// RGBA8 copy of an (H, W, 4) or (H, W, 3) uint8 image (alpha 255 added).
+std::shared_ptr<std::vector<uint8_t>> rgba_copy(u8arr img, uint32_t& w, uint32_t& h) {
+    if (img.ndim() != 3 || (img.shape(2) != 3 && img.shape(2) != 4) || img.shape(0) < 1 || img.shape(1) < 1)
+        throw py::value_error("texture must be an (H, W, 3) or (H, W, 4) uint8 array");
+    h = (uint32_t)img.shape(0);
+    w = (uint32_t)img.shape(1);
+    const size_t c = (size_t)img.shape(2);
+    auto out = std::make_shared<std::vector<uint8_t>>((size_t)w * h * 4, 255);
+    const uint8_t* p = img.data();
+    for (size_t i = 0; i < (size_t)w * h; ++i)
+        for (size_t k = 0; k < c; ++k) (*out)[4 * i + k] = p[c * i + k];
+    return out;
+}
+
+struct Texture {
+    uint32_t w = 0, h = 0;
+    std::shared_ptr<std::vector<uint8_t>> rgba;
+};
+
 struct Entity : Pose {
     std::string geometry_path, geometry_type;
     bool smooth_shading;
@@ -148,6 +169,50 @@ struct Entity : Pose {
     double scale = 1.0;
     uint32_t id = 0;
     std::shared_ptr<crt_host::Mesh> mesh;   // loaded at construction (entity.hpp:31-67)
+    // material (additive: the reference hard-codes ColoredLambertian,
+    // src/crt/rendering_dynamic/entity.hpp:51-53) and per-vertex UVs
+    uint32_t mat_type = CRT_MAT_LAMBERTIAN;
+    float mat_alpha = 24.0f;
+    float mat_coeffs[3] = {0.f, 1.f, 0.f};
+    Texture tex, spec_tex;
+    std::vector<float> uv;
+
+    // kind: "lambertian" (the entity colour), "textured_lambertian",
+    // "blinn_phong", "mirror"; texture: image or None (the entity colour);
+    // specular: (ka, kd, ks) or an image whose R, G, B hold them.
+    void set_material(const std::string& kind, py::object texture, py::object specular, double alpha) {
+        Texture t, st;
+        float co[3] = {0.f, 1.f, 0.f};
+        uint32_t type;
+        if (kind == "lambertian") type = CRT_MAT_LAMBERTIAN;
+        else if (kind == "textured_lambertian") type = CRT_MAT_TEX_LAMBERTIAN;
+        else if (kind == "blinn_phong") type = CRT_MAT_BLINN_PHONG;
+        else if (kind == "mirror") type = CRT_MAT_MIRROR;
+        else throw py::value_error("unknown material '" + kind + "' (lambertian, textured_lambertian, blinn_phong, mirror)");
+        if (!texture.is_none()) t.rgba = rgba_copy(texture.cast<u8arr>(), t.w, t.h);
+        if (!specular.is_none()) {
+            py::array sa = py::array::ensure(specular);
+            if (sa && sa.ndim() == 3) {
+                st.rgba = rgba_copy(specular.cast<u8arr>(), st.w, st.h);
+            } else {
+                auto v = specular.cast<std::vector<double>>();
+                if (v.size() != 3) throw py::value_error("specular needs (ka, kd, ks) or an image");
+                for (int k = 0; k < 3; ++k) co[k] = (float)v[k];
+            }
+        }
+        if (!std::isfinite(alpha)) throw py::value_error("alpha must be finite");
+        mat_type = type;
+        mat_alpha = (float)alpha;
+        for (int k = 0; k < 3; ++k) mat_coeffs[k] = co[k];
+        tex = t;
+        spec_tex = st;
+    }
+    void set_uv(darr uvs) {
+        if (mesh && (size_t)uvs.size() != 2 * mesh->n_vertices())
+            throw py::value_error("uv needs (n_vertices, 2) values");
+        uv.assign(uvs.data(), uvs.data() + uvs.size());
+    }
+    bool default_material() const { return mat_type == CRT_MAT_LAMBERTIAN && !tex.rgba && uv.empty(); }
 
     void load() {
         std::string t = geometry_type;
@@ -231,6 +296,42 @@ std::shared_ptr<GpuScene> build_scene(const std::vector<const Entity*>& ents, ui
                                    inst.data(), VN.data(), (uint32_t)ents.size(), col.data(), sm.data(), &opts, &h));
     }
     g->h = h;
+    // materials, textures and UVs, when any entity sets them
+    bool any = false, any_uv = false;
+    for (const Entity* e : ents) {
+        any = any || !e->default_material();
+        any_uv = any_uv || !e->uv.empty();
+    }
+    if (any) {
+        std::vector<crt_material> mats;
+        std::vector<crt_texture> texs;
+        std::vector<float> uvs;
+        for (const Entity* e : ents) {
+            crt_material m{};
+            m.type = e->mat_type;
+            m.alpha = e->mat_alpha;
+            for (int k = 0; k < 3; ++k) { m.color[k] = e->color[k]; m.coeffs[k] = e->mat_coeffs[k]; }
+            m.texture = m.spec_texture = -1;
+            if (e->tex.rgba) {
+                m.texture = (int32_t)texs.size();
+                texs.push_back({e->tex.w, e->tex.h, e->tex.rgba->data()});
+            }
+            if (e->spec_tex.rgba) {
+                m.spec_texture = (int32_t)texs.size();
+                texs.push_back({e->spec_tex.w, e->spec_tex.h, e->spec_tex.rgba->data()});
+            }
+            mats.push_back(m);
+            if (any_uv) {
+                if (!e->uv.empty() && e->uv.size() != 2 * e->mesh->n_vertices())
+                    throw py::value_error("entity uv needs (n_vertices, 2) values");
+                if (e->uv.empty()) uvs.insert(uvs.end(), 2 * e->mesh->n_vertices(), 0.0f);
+                else uvs.insert(uvs.end(), e->uv.begin(), e->uv.end());
+            }
+        }
+        py::gil_scoped_release nogil;
+        check(crt_scene_set_materials(h, (uint32_t)mats.size(), mats.data(), any_uv ? uvs.data() : nullptr,
+                                      (uint32_t)texs.size(), texs.empty() ? nullptr : texs.data()));
+    }
     return g;
 }
 
@@ -452,6 +553,9 @@ PYBIND11_MODULE(_crt, m) {
         }));
     bind_pose(ent);
     ent.def("set_scale", &Entity::set_scale).def_readonly("scale", &Entity::scale);
+    ent.def("set_material", &Entity::set_material, py::arg("kind"), py::arg("texture") = py::none(),
+            py::arg("specular") = py::none(), py::arg("alpha") = 24.0)
+        .def("set_uv", &Entity::set_uv);
 
     auto bfe = py::class_<BodyFixedEntity, Entity>(m, "BodyFixedEntity").def(
         py::init([](const std::string& p, const std::string& t, bool s, py::list c) {

@@ -416,12 +416,29 @@ __device__ inline float3 cosine_sample(float3 n, float r1, float r2, float omr) 
     return make_float3(nx.x * x + ny.x * y + n.x * z, nx.y * x + ny.y * y + n.y * z, nx.z * x + ny.z * y + n.z * z);
 }
 
+// ImageUVMap::operator() (src/crt/materials/material.hpp:50-60): nearest
+// texel x = (size_t)(u W + 0.5), y = (size_t)(v H + 0.5), channels / 255.
+// Indices are clamped to the image, where the reference reads past the end
+// of its pixel array for u or v near 1.
+__device__ inline float3 tex_lookup(const SceneView& s, int t, float u, float v) {
+    const uint4 ti = s.tex_info[t];
+    const double fx = (double)(u * (float)ti.y) + 0.5, fy = (double)(v * (float)ti.z) + 0.5;
+    const int64_t x = min(fx > 0.0 ? (int64_t)fx : (int64_t)0, (int64_t)ti.y - 1);
+    const int64_t y = min(fy > 0.0 ? (int64_t)fy : (int64_t)0, (int64_t)ti.z - 1);
+    const uint32_t p = s.texels[(size_t)ti.x + (size_t)y * ti.y + (size_t)x];
+    return make_float3((float)(p & 0xFFu) / 255.f, (float)((p >> 8) & 0xFFu) / 255.f,
+                       (float)((p >> 16) & 0xFFu) / 255.f);
+}
+
 // Shading of a path that hit triangle `slot` (leaf order) with barycentrics
-// (hu, hv) at this bounce: one shadow record per light at live slot `ls`
-// (records ls*nl .. ls*nl+nl-1) and, unless this is the last bounce, the
-// next ray of path k with the pending bounce factor in w.w.
+// (hu, hv) at this bounce, arriving along din: one shadow record per light at
+// live slot `ls` (records ls*nl .. ls*nl+nl-1) and, unless this is the last
+// bounce, the next ray of path k with the pending bounce factor in w.w.
+// The instance's material (src/crt/materials/material.hpp:90-184) computes
+// the light term and samples the next direction; without a material table
+// every instance is ColoredLambertian with its colour.
 __device__ inline void shade_hit(const Ctx& C, int bounce, uint32_t k, uint32_t cam, uint32_t pix, uint32_t smp,
-                                 int32_t slot, float hu, float hv, uint32_t ls) {
+                                 int32_t slot, float hu, float hv, uint32_t ls, float3 din) {
     const SceneView& s = C.s;
     const float4 a = s.tris[3 * slot], b = s.tris[3 * slot + 1], c = s.tris[3 * slot + 2];
     const uint32_t orig = __float_as_uint(a.w);
@@ -432,7 +449,8 @@ __device__ inline void shade_hit(const Ctx& C, int bounce, uint32_t k, uint32_t 
     const float3 e2 = make_float3(c.x - a.x, c.y - a.y, c.z - a.z);
     const float3 gn = norm3(cross3(e1, e2));
     float3 sn = gn;
-    if (s.vnormals && inst >= 1 && inst <= s.n_instances && s.inst_smooth[inst - 1]) {
+    const bool inst_ok = inst >= 1 && inst <= s.n_instances;
+    if (s.vnormals && inst_ok && s.inst_smooth[inst - 1]) {
         const uint32_t i0 = s.tri_vidx[3 * orig], i1 = s.tri_vidx[3 * orig + 1], i2 = s.tri_vidx[3 * orig + 2];
         sn = norm3(make_float3(u * s.vnormals[3 * i1] + v * s.vnormals[3 * i2] + w * s.vnormals[3 * i0],
                                u * s.vnormals[3 * i1 + 1] + v * s.vnormals[3 * i2 + 1] + w * s.vnormals[3 * i0 + 1],
@@ -440,28 +458,82 @@ __device__ inline void shade_hit(const Ctx& C, int bounce, uint32_t k, uint32_t 
     }
     float3 p = make_float3(u * b.x + v * c.x + w * a.x, u * b.y + v * c.y + w * a.y, u * b.z + v * c.z + w * a.z);
     p = make_float3(p.x - C.offset * gn.x, p.y - C.offset * gn.y, p.z - C.offset * gn.z);
-    const float* col = s.inst_color + 3 * (inst >= 1 && inst <= s.n_instances ? inst - 1 : 0);
+    // material: colour map (constant or texture at the interpolated UV) and,
+    // for Blinn-Phong, the ka / kd / ks map
+    uint32_t mtype = 0;
+    float3 col, kc = make_float3(0.f, 0.f, 0.f);
+    float alpha = 0.f;
+    if (!s.mats) {
+        const float* ic = s.inst_color + 3 * (inst_ok ? inst - 1 : 0);
+        col = make_float3(ic[0], ic[1], ic[2]);
+    } else {
+        const MatDev M = s.mats[inst_ok ? inst - 1 : 0];
+        mtype = M.type;
+        alpha = M.alpha;
+        col = make_float3(M.color[0], M.color[1], M.color[2]);
+        kc = make_float3(M.coeffs[0], M.coeffs[1], M.coeffs[2]);
+        if ((mtype == 1 || mtype == 2) && (M.tex >= 0 || (mtype == 2 && M.spec_tex >= 0))) {
+            // unidirectional.hpp:55: u uv1 + v uv2 first, then + w uv0, in float
+            float iu = 0.f, iv = 0.f;
+            if (s.vuv) {
+                const uint32_t j0 = s.tri_vidx[3 * orig], j1 = s.tri_vidx[3 * orig + 1], j2 = s.tri_vidx[3 * orig + 2];
+                const float2 t0 = s.vuv[j0], t1 = s.vuv[j1], t2 = s.vuv[j2];
+                iu = u * t1.x + v * t2.x + w * t0.x;
+                iv = u * t1.y + v * t2.y + w * t0.y;
+            }
+            if (M.tex >= 0) col = tex_lookup(s, M.tex, iu, iv);
+            if (mtype == 2 && M.spec_tex >= 0) kc = tex_lookup(s, M.spec_tex, iu, iv);
+        }
+    }
     const uint64_t key = rng_key(C.cams[cam].seed, pix, smp);
     for (int l = 0; l < C.nl; ++l) {
         float3 ld;
         float tmax, irr;
         light_sample(C, cam, l, p, key, bounce, ld, tmax, irr);
-        // ColoredLambertianMaterial::compute: colour * (-L.n_s), no max(0, .)
-        const float ldn = -dot3(ld, sn);
+        float3 lc;
+        if (mtype == 3) {
+            // MirrorMaterial::compute is 0 whatever the shadow ray finds: no
+            // traversal (tmax < 0 rejects every node and triangle)
+            lc = make_float3(0.f, 0.f, 0.f);
+            tmax = -1.0f;
+        } else if (mtype == 2) {
+            // TexturedBlinnPhongMaterial::compute (material.hpp:152-160):
+            // clamp(map (ka + kd diffuse) + ks spec), spec = pow(n.h, alpha)
+            const float diffuse = -dot3(ld, sn);
+            const float3 hv3 = norm3(make_float3(ld.x - din.x, ld.y - din.y, ld.z - din.z));
+            const float spec = powf(dot3(norm3(sn), hv3), alpha);
+            const float sp = kc.z * spec * 1.0f;
+            lc = make_float3(fminf(fmaxf(col.x * (kc.x + kc.y * diffuse) + sp, 0.f), 1.f) * irr,
+                             fminf(fmaxf(col.y * (kc.x + kc.y * diffuse) + sp, 0.f), 1.f) * irr,
+                             fminf(fmaxf(col.z * (kc.x + kc.y * diffuse) + sp, 0.f), 1.f) * irr);
+        } else {
+            // (Colored|Textured)LambertianMaterial::compute: colour * (-L.n_s), no max(0, .)
+            const float ldn = -dot3(ld, sn);
+            lc = make_float3(col.x * ldn * irr, col.y * ldn * irr, col.z * ldn * irr);
+        }
         ShadowRec r;
         r.o = make_float4(p.x, p.y, p.z, tmax);
         r.d = make_float4(ld.x, ld.y, ld.z, __uint_as_float((uint32_t)slot));
-        r.c = make_float4(col[0] * ldn * irr, col[1] * ldn * irr, col[2] * ldn * irr, 0.f);
+        r.c = make_float4(lc.x, lc.y, lc.z, 0.f);
         C.shadow[ls * (uint32_t)C.nl + (uint32_t)l] = r;
     }
     if (bounce < C.nb - 1) {
-        const double r1 = rng_u01k(key, rng_dim(bounce, RNG_BRDF));
-        const float r2 = (float)rng_u01k(key, rng_dim(bounce, RNG_BRDF + 1));
-        const float omr = (float)(1.0 - r1);   // the bounce weight (1 - r1), as the reference's float(1 - r1)
-        const float3 nd = cosine_sample(sn, (float)r1, r2, omr);
+        float3 nd;
+        float bw;
+        if (mtype == 3) {
+            // MirrorMaterial::sample (material.hpp:179-182): d - 2 n (d.n), factor 1
+            const float dn = dot3(din, sn);
+            nd = make_float3(din.x - sn.x * 2.0f * dn, din.y - sn.y * 2.0f * dn, din.z - sn.z * 2.0f * dn);
+            bw = 1.0f;
+        } else {
+            const double r1 = rng_u01k(key, rng_dim(bounce, RNG_BRDF));
+            const float r2 = (float)rng_u01k(key, rng_dim(bounce, RNG_BRDF + 1));
+            bw = (float)(1.0 - r1);   // the bounce weight (1 - r1), as the reference's float(1 - r1)
+            nd = cosine_sample(sn, (float)r1, r2, bw);
+        }
         C.st.o[k] = make_float4(p.x, p.y, p.z, 0.f);
         C.st.d[k] = make_float4(nd.x, nd.y, nd.z, 0.f);
-        C.st.w[k].w = omr;
+        C.st.w[k].w = bw;
     }
 }
 
@@ -481,7 +553,12 @@ __global__ __launch_bounds__(TB) void k_shade(Ctx C, int bounce, int qsel) {
     if (has_hit) C.live[ls] = k;
     if (!has_hit) return;
     const float2 hb = C.st.h[k];
-    shade_hit(C, bounce, k, m.x, m.y, m.z, (int32_t)m.w, hb.x, hb.y, ls);
+    float3 din = make_float3(0.f, 0.f, 0.f);
+    if (C.s.mats) {   // Blinn-Phong and mirrors use the incoming direction
+        const float4 d4 = C.st.d[k];
+        din = make_float3(d4.x, d4.y, d4.z);
+    }
+    shade_hit(C, bounce, k, m.x, m.y, m.z, (int32_t)m.w, hb.x, hb.y, ls, din);
 }
 
 // After k_shade: shadow rays = nl per live path; tally traced rays.
@@ -545,14 +622,15 @@ __global__ __launch_bounds__(TB) void k_accumulate(Ctx C, int bounce, int next) 
 // k_shadow_quad and k_accumulate in the same order, so images are
 // bit-identical to the one-kernel-per-stage pipeline, with less path-state
 // traffic and fewer launches per bounce.
-__device__ inline void finish_extend(const Ctx& C, int bounce, uint32_t k, const HitRec& h, uint32_t ls) {
+__device__ inline void finish_extend(const Ctx& C, int bounce, uint32_t k, const HitRec& h, uint32_t ls,
+                                     float3 din) {
     if (h.slot < 0) {
         C.samples[k] = C.st.L[k];   // path ends without a hit
         return;
     }
     C.live[ls] = k;
     const uint4 m = C.st.m[k];   // cam, pix, sample
-    shade_hit(C, bounce, k, m.x, m.y, m.z, h.slot, h.u, h.v, ls);
+    shade_hit(C, bounce, k, m.x, m.y, m.z, h.slot, h.u, h.v, ls, din);
 }
 
 CRT_LANE_KERNEL void k_extend_shade_lane(Ctx C, int bounce, int qsel, int gen) {
@@ -588,7 +666,7 @@ CRT_LANE_KERNEL void k_extend_shade_lane(Ctx C, int bounce, int qsel, int gen) {
     const RayPre r = make_ray_pre(o.x, o.y, o.z, d.x, d.y, d.z);
     const HitRec h = trace_lane<false>(C.s, r, 0.0f, __int_as_float(0x7f800000), stk);
     const uint32_t ls = wave_append(h.slot >= 0, C.cnt + 3);   // live slot
-    finish_extend(C, bounce, k, h, ls);
+    finish_extend(C, bounce, k, h, ls, make_float3(d.x, d.y, d.z));
 }
 
 // Octet form for the bounces after the first.  Each octet traces R paths
@@ -604,6 +682,7 @@ void k_extend_shade_octet(Ctx C, int bounce, int qsel, int R) {
     uint32_t k_mine = 0;
     HitRec h_mine;
     h_mine.slot = -1;
+    float3 d_mine = make_float3(0.f, 0.f, 0.f);
     bool mine = false;
 #pragma unroll 1
     for (int p = 0; p < R; ++p) {
@@ -616,11 +695,12 @@ void k_extend_shade_octet(Ctx C, int bounce, int qsel, int R) {
         if (sub == p) {   // the hit is octet-uniform
             k_mine = k;
             h_mine = h;
+            d_mine = make_float3(d.x, d.y, d.z);
             mine = true;
         }
     }
     const uint32_t ls = wave_append(mine && h_mine.slot >= 0, C.cnt + 3);
-    if (mine) finish_extend(C, bounce, k_mine, h_mine, ls);
+    if (mine) finish_extend(C, bounce, k_mine, h_mine, ls, d_mine);
 }
 
 // One quad traces the shadow rays of R live paths (R = 1..4, by launch

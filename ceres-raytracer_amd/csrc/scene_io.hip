@@ -3,7 +3,8 @@
 // data, both BVH layouts) saved once and loaded without a rebuild, e.g. to
 // warm-start a server's resident meshes.
 //
-// Blob = Header, then the sections in Header order, each padded to 16 B.
+// Blob = Header, then the sections in Header order, each padded to 16 B
+// (mesh, shading data, both BVH layouts, materials, UVs, textures).
 // Everything is validated before any device memory is touched: magic,
 // version, sizes against the counts, an FNV-1a 64 checksum of the sections,
 // and the structure itself -- every node reference in range, inner children
@@ -20,6 +21,7 @@
 
 namespace crt {
 void scene_bind_view(crt_scene* s);
+uint64_t scene_device_bytes(const crt_scene* s);
 }
 
 namespace {
@@ -27,10 +29,10 @@ namespace {
 using namespace crt;
 
 constexpr char MAGIC[8] = {'C', 'R', 'T', 'S', 'C', 'N', '\0', '\1'};
-constexpr uint32_t VERSION = 3;   // 3: valid-slot mask in byte 3 of the compressed node header
+constexpr uint32_t VERSION = 3;   // 3: valid-slot mask in the compressed node header; material sections
 
 enum Section { S_VERTICES, S_INDICES, S_VNORMALS, S_TRI_INST, S_INST_COLOR, S_INST_SMOOTH, S_NODES, S_CNODES,
-               S_TRIS, N_SECTIONS };
+               S_TRIS, S_MATS, S_VUV, S_TEX_INFO, S_TEXELS, N_SECTIONS };
 
 struct Header {
     char magic[8];
@@ -38,6 +40,8 @@ struct Header {
     uint32_t header_bytes;
     uint32_t n_tris, n_vertices, n_instances, has_vnormals;
     uint32_t n_nodes, n_leaves, max_depth, pad_;
+    uint32_t has_mats, has_vuv, n_textures, pad2_;   // materials (crt_scene_set_materials)
+    uint64_t n_texels;
     double origin[3];
     double center[3];
     double radius;
@@ -67,6 +71,10 @@ const DevBuf* section_buf(const crt_scene* s, int k) {
         case S_INST_SMOOTH: return &s->inst_smooth;
         case S_NODES: return &s->bvh.nodes;
         case S_CNODES: return &s->bvh.cnodes;
+        case S_MATS: return &s->mats;
+        case S_VUV: return &s->vuv;
+        case S_TEX_INFO: return &s->tex_info;
+        case S_TEXELS: return &s->texels;
         default: return &s->bvh.tris;
     }
 }
@@ -81,6 +89,10 @@ void expected_sizes(const Header& h, uint64_t out[N_SECTIONS]) {
     out[S_NODES] = (uint64_t)h.n_nodes * 256;
     out[S_CNODES] = (uint64_t)h.n_nodes * 96;
     out[S_TRIS] = (uint64_t)h.n_tris * 48;
+    out[S_MATS] = h.has_mats ? (uint64_t)h.n_instances * sizeof(MatDev) : 0;
+    out[S_VUV] = h.has_vuv ? (uint64_t)h.n_vertices * 8 : 0;
+    out[S_TEX_INFO] = (uint64_t)h.n_textures * 16;
+    out[S_TEXELS] = h.n_textures ? std::max<uint64_t>(h.n_texels, 1) * 4 : 0;
 }
 
 // Structure checks on the host copies (see the file comment).
@@ -134,6 +146,18 @@ void validate(const Header& h, const uint8_t* const sec[N_SECTIONS]) {
         }
     }
     CRT_REQUIRE(max_depth <= h.max_depth && h.max_depth <= 64, CRT_ERR_INVALID, "scene blob: BVH deeper than stated");
+    // materials: types, texture indices and texel ranges
+    const uint4* ti = reinterpret_cast<const uint4*>(sec[S_TEX_INFO]);
+    for (uint32_t t = 0; t < h.n_textures; ++t)
+        CRT_REQUIRE(ti[t].y >= 1 && ti[t].z >= 1 && (uint64_t)ti[t].x + (uint64_t)ti[t].y * ti[t].z <= h.n_texels,
+                    CRT_ERR_INVALID, "scene blob: texture outside the texel section");
+    if (h.has_mats) {
+        const MatDev* m = reinterpret_cast<const MatDev*>(sec[S_MATS]);
+        for (uint32_t i = 0; i < h.n_instances; ++i)
+            CRT_REQUIRE(m[i].type <= CRT_MAT_MIRROR && m[i].tex >= -1 && m[i].tex < (int64_t)h.n_textures &&
+                            m[i].spec_tex >= -1 && m[i].spec_tex < (int64_t)h.n_textures,
+                        CRT_ERR_INVALID, "scene blob: bad material");
+    }
 }
 
 }  // namespace
@@ -152,6 +176,10 @@ int crt_scene_serialize(const crt_scene* scene, void* buffer, uint64_t* size) {
         h.n_vertices = scene->n_vertices;
         h.n_instances = scene->n_instances;
         h.has_vnormals = scene->vnormals.p ? 1u : 0u;
+        h.has_mats = scene->mats.p ? 1u : 0u;
+        h.has_vuv = scene->vuv.p ? 1u : 0u;
+        h.n_textures = scene->n_textures;
+        h.n_texels = scene->n_textures ? scene->texels.bytes / 4 : 0;
         h.n_nodes = scene->bvh.n_nodes;
         h.n_leaves = scene->bvh.n_leaves;
         h.max_depth = scene->bvh.max_depth;
@@ -204,6 +232,9 @@ int crt_scene_deserialize(const void* buffer, uint64_t size, crt_scene** out_sce
         CRT_REQUIRE(h.n_tris > 0 && h.n_tris < (1u << 27) && h.n_vertices > 0 && h.n_instances > 0 &&
                         h.n_nodes > 0 && h.n_nodes < (1u << 23),
                     CRT_ERR_INVALID, "scene blob: bad counts");
+        CRT_REQUIRE(h.has_mats <= 1 && h.has_vuv <= 1 && h.n_texels < (1ull << 32) &&
+                        (h.n_textures == 0 || h.has_mats),
+                    CRT_ERR_INVALID, "scene blob: bad material counts");
         uint64_t expect[N_SECTIONS];
         expected_sizes(h, expect);
         uint64_t total = pad16(sizeof(Header));
@@ -232,6 +263,7 @@ int crt_scene_deserialize(const void* buffer, uint64_t size, crt_scene** out_sce
         s->n_tris = h.n_tris;
         s->n_vertices = h.n_vertices;
         s->n_instances = h.n_instances;
+        s->n_textures = h.n_textures;
         for (int k = 0; k < N_SECTIONS; ++k) {
             if (!expect[k]) continue;
             DevBuf* d = const_cast<DevBuf*>(section_buf(s.get(), k));
@@ -253,9 +285,7 @@ int crt_scene_deserialize(const void* buffer, uint64_t size, crt_scene** out_sce
         }
         s->view.radius = h.radius;
         scene_bind_view(s.get());
-        s->info.device_bytes = s->vertices.bytes + s->indices.bytes + s->tri_vidx.bytes + s->tri_inst.bytes +
-                               s->inst_color.bytes + s->inst_smooth.bytes + s->vnormals.bytes + s->bvh.nodes.bytes +
-                               s->bvh.cnodes.bytes + s->bvh.tris.bytes;
+        s->info.device_bytes = scene_device_bytes(s.get());
         CRT_REQUIRE(trace_lds_bytes(s->view) <= 64u * 1024u && lane_lds_bytes(s->view) <= 64u * 1024u &&
                         quad_lds_bytes(s->view) <= 64u * 1024u,
                     CRT_ERR_INVALID, "scene blob: BVH too deep for the LDS traversal stack");

@@ -373,6 +373,7 @@ struct OutSlot {
 // ---- host helpers shared with other translation units ----
 void scene_make_view(crt_scene* s, const float* host_vertices);
 void scene_bind_view(crt_scene* s);
+uint64_t scene_device_bytes(const crt_scene* s);
 void noalloc_setup_current_device();   // sensors.hip
 
 }  // namespace crt
@@ -391,6 +392,10 @@ void crt::scene_bind_view(crt_scene* s) {
     v.tri_inst = s->tri_inst.as<uint32_t>();
     v.inst_color = s->inst_color.as<float>();
     v.inst_smooth = s->inst_smooth.as<uint8_t>();
+    v.mats = s->mats.p ? s->mats.as<MatDev>() : nullptr;
+    v.vuv = s->vuv.p ? s->vuv.as<float2>() : nullptr;
+    v.texels = s->texels.p ? s->texels.as<uint32_t>() : nullptr;
+    v.tex_info = s->tex_info.p ? s->tex_info.as<uint4>() : nullptr;
     v.n_tris = s->n_tris;
     v.n_instances = s->n_instances;
     v.stack_cap = std::max<uint32_t>(8u, 7u * s->bvh.max_depth + 1u);
@@ -398,6 +403,12 @@ void crt::scene_bind_view(crt_scene* s) {
     for (int k = 0; k < 3; ++k) s->info.sphere[k] = v.center[k] + v.origin[k];   // world
     s->info.sphere[3] = v.radius;
     s->info.stack_capacity = v.stack_cap;
+}
+
+uint64_t crt::scene_device_bytes(const crt_scene* s) {
+    return s->vertices.bytes + s->indices.bytes + s->tri_vidx.bytes + s->tri_inst.bytes + s->inst_color.bytes +
+           s->inst_smooth.bytes + s->vnormals.bytes + s->bvh.nodes.bytes + s->bvh.cnodes.bytes + s->bvh.tris.bytes +
+           s->mats.bytes + s->vuv.bytes + s->texels.bytes + s->tex_info.bytes;
 }
 
 void crt::scene_make_view(crt_scene* s, const float* host_vertices) {
@@ -568,9 +579,7 @@ int scene_create_impl(const VT* vertices, uint32_t n_vertices, const uint32_t* i
         inf.build_ms = s->bvh.build_ms;
         inf.sah_cost = s->bvh.sah_cost;
         for (int k = 0; k < 6; ++k) inf.bounds[k] = s->bvh.bounds[k] + s->view.origin[k % 3];   // world, min/max xyz
-        inf.device_bytes = s->vertices.bytes + s->indices.bytes + s->tri_vidx.bytes + s->tri_inst.bytes +
-                           s->inst_color.bytes + s->inst_smooth.bytes + s->vnormals.bytes + s->bvh.nodes.bytes +
-                           s->bvh.cnodes.bytes + s->bvh.tris.bytes;
+        inf.device_bytes = scene_device_bytes(s.get());
         scene_make_view(s.get(), local.data());
         inf.self_offset = self_offset(s.get());
         CRT_REQUIRE(s->bvh.n_nodes < (1u << 23), CRT_ERR_INVALID, "BVH exceeds 2^23 wide nodes");
@@ -600,6 +609,73 @@ int crt_scene_create_f64(const double* vertices, uint32_t n_vertices, const uint
                          crt_scene** out_scene) {
     return scene_create_impl(vertices, n_vertices, indices, n_tris, tri_instance, vertex_normals, n_instances,
                              instance_color, instance_smooth, opts, out_scene);
+}
+
+int crt_scene_set_materials(crt_scene* scene, uint32_t n_materials, const crt_material* materials,
+                            const float* vertex_uv, uint32_t n_textures, const crt_texture* textures) {
+    return guarded([&] {
+        CRT_REQUIRE(scene != nullptr, CRT_ERR_INVALID, "scene is NULL");
+        CRT_REQUIRE(materials != nullptr && n_materials == scene->n_instances, CRT_ERR_INVALID,
+                    "one material per instance is required");
+        CRT_REQUIRE(n_textures == 0 || textures != nullptr, CRT_ERR_INVALID, "textures is NULL");
+        std::vector<MatDev> m(n_materials);
+        for (uint32_t i = 0; i < n_materials; ++i) {
+            const crt_material& x = materials[i];
+            CRT_REQUIRE(x.type <= CRT_MAT_MIRROR, CRT_ERR_INVALID, "unknown material type");
+            CRT_REQUIRE(x.texture >= -1 && x.texture < (int64_t)n_textures && x.spec_texture >= -1 &&
+                            x.spec_texture < (int64_t)n_textures,
+                        CRT_ERR_INVALID, "material texture index out of range");
+            CRT_REQUIRE(std::isfinite(x.alpha), CRT_ERR_INVALID, "material alpha must be finite");
+            for (int k = 0; k < 3; ++k)
+                CRT_REQUIRE(std::isfinite(x.color[k]) && std::isfinite(x.coeffs[k]), CRT_ERR_INVALID,
+                            "material colour / coefficients must be finite");
+            MatDev& d = m[i];
+            d.type = x.type;
+            d.tex = x.texture;
+            d.spec_tex = x.spec_texture;
+            d.alpha = x.alpha;
+            for (int k = 0; k < 3; ++k) { d.color[k] = x.color[k]; d.coeffs[k] = x.coeffs[k]; }
+            d.color[3] = d.coeffs[3] = 0.f;
+        }
+        std::vector<uint4> info(n_textures);
+        uint64_t total = 0;
+        for (uint32_t t = 0; t < n_textures; ++t) {
+            const crt_texture& x = textures[t];
+            CRT_REQUIRE(x.width >= 1 && x.height >= 1 && x.rgba != nullptr, CRT_ERR_INVALID,
+                        "texture must be at least 1 x 1 with pixel data");
+            CRT_REQUIRE(total + (uint64_t)x.width * x.height < (1ull << 32), CRT_ERR_INVALID,
+                        "textures exceed 2^32 texels");
+            info[t] = make_uint4((uint32_t)total, x.width, x.height, 0u);
+            total += (uint64_t)x.width * x.height;
+        }
+        std::vector<uint32_t> texels((size_t)total);
+        for (uint32_t t = 0; t < n_textures; ++t)
+            std::memcpy(texels.data() + info[t].x, textures[t].rgba, (size_t)info[t].y * info[t].z * 4);
+        if (vertex_uv)
+            for (uint64_t i = 0; i < 2ull * scene->n_vertices; ++i)
+                CRT_REQUIRE(std::isfinite(vertex_uv[i]), CRT_ERR_INVALID, "vertex UVs must be finite");
+        crt::DevBuf dm, duv, dtx, dti;
+        dm.alloc(m.size() * sizeof(MatDev));
+        CRT_HIP_CHECK(hipMemcpy(dm.p, m.data(), m.size() * sizeof(MatDev), hipMemcpyHostToDevice));
+        if (vertex_uv) {
+            duv.alloc((size_t)scene->n_vertices * 8);
+            CRT_HIP_CHECK(hipMemcpy(duv.p, vertex_uv, (size_t)scene->n_vertices * 8, hipMemcpyHostToDevice));
+        }
+        if (n_textures) {
+            dti.alloc(info.size() * sizeof(uint4));
+            CRT_HIP_CHECK(hipMemcpy(dti.p, info.data(), info.size() * sizeof(uint4), hipMemcpyHostToDevice));
+            dtx.alloc(std::max<size_t>(texels.size(), 1) * 4);
+            if (!texels.empty())
+                CRT_HIP_CHECK(hipMemcpy(dtx.p, texels.data(), texels.size() * 4, hipMemcpyHostToDevice));
+        }
+        scene->mats = std::move(dm);
+        scene->vuv = std::move(duv);
+        scene->texels = std::move(dtx);
+        scene->tex_info = std::move(dti);
+        scene->n_textures = n_textures;
+        scene_bind_view(scene);
+        scene->info.device_bytes = scene_device_bytes(scene);
+    });
 }
 
 int crt_scene_info(const crt_scene* scene, crt_bvh_info* info) {

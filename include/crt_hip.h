@@ -188,6 +188,39 @@ int crt_scene_create_f64(const double* vertices, uint32_t n_vertices,
                          const uint8_t* instance_smooth,
                          const crt_build_opts* opts,
                          crt_scene** out_scene);
+/* ---- materials (src/crt/materials/material.hpp:90-184) -------------------
+ * The reference's Entity hard-codes ColoredLambertianMaterial
+ * (src/crt/rendering_dynamic/entity.hpp:51-53) and never reaches its other
+ * materials; this additive call gives each instance one of them.  Without it
+ * every instance is ColoredLambertian with its instance colour.          */
+#define CRT_MAT_LAMBERTIAN      0u  /* ColoredLambertianMaterial: colour (-L.n)          */
+#define CRT_MAT_TEX_LAMBERTIAN  1u  /* TexturedLambertianMaterial: map(uv) (-L.n)       */
+#define CRT_MAT_BLINN_PHONG     2u  /* TexturedBlinnPhongMaterial: clamp(map (ka + kd diffuse)
+                                       + ks pow(n.h, alpha)), cosine-sampled bounce       */
+#define CRT_MAT_MIRROR          3u  /* MirrorMaterial: no direct light, bounce d - 2 n (d.n) */
+
+typedef struct {
+    uint32_t type;            /* CRT_MAT_* */
+    int32_t  texture;         /* colour map: index into `textures`, or -1 for the constant `color` */
+    int32_t  spec_texture;    /* Blinn-Phong ka / kd / ks map (texel R, G, B): index, or -1 for `coeffs` */
+    float    alpha;           /* Blinn-Phong exponent (the reference's default is 24) */
+    float    color[3];        /* constant colour map */
+    float    coeffs[3];       /* constant ka, kd, ks */
+} crt_material;
+
+typedef struct {
+    uint32_t width, height;
+    const uint8_t* rgba;      /* width x height RGBA8, row-major, row 0 first (lodepng's layout) */
+} crt_texture;
+
+/* materials: one per instance (n_materials == the scene's instance count,
+ * instance id i uses materials[i - 1]); vertex_uv: n_vertices x 2 floats or
+ * NULL (all zero, as the reference's OBJ loader leaves them); textures are
+ * copied to the device.  Texel lookup is the reference's nearest texel at
+ * (size_t)(u W + 0.5), (size_t)(v H + 0.5), clamped to the image.  Not to be
+ * called while other calls on the scene run. */
+int crt_scene_set_materials(crt_scene* scene, uint32_t n_materials, const crt_material* materials,
+                            const float* vertex_uv, uint32_t n_textures, const crt_texture* textures);
 int  crt_scene_info(const crt_scene* scene, crt_bvh_info* info);
 void crt_scene_destroy(crt_scene* scene);
 

@@ -18,9 +18,11 @@ class _Header(C.Structure):
     _fields_ = [("magic", C.c_char * 8), ("version", C.c_uint32), ("header_bytes", C.c_uint32),
                 ("n_tris", C.c_uint32), ("n_vertices", C.c_uint32), ("n_instances", C.c_uint32),
                 ("has_vnormals", C.c_uint32), ("n_nodes", C.c_uint32), ("n_leaves", C.c_uint32),
-                ("max_depth", C.c_uint32), ("pad_", C.c_uint32), ("origin", C.c_double * 3),
+                ("max_depth", C.c_uint32), ("pad_", C.c_uint32), ("has_mats", C.c_uint32),
+                ("has_vuv", C.c_uint32), ("n_textures", C.c_uint32), ("pad2_", C.c_uint32),
+                ("n_texels", C.c_uint64), ("origin", C.c_double * 3),
                 ("center", C.c_double * 3), ("radius", C.c_double), ("bvh_bounds", C.c_float * 6),
-                ("info", capi.BvhInfo), ("section_bytes", C.c_uint64 * 9), ("checksum", C.c_uint64)]
+                ("info", capi.BvhInfo), ("section_bytes", C.c_uint64 * 13), ("checksum", C.c_uint64)]
 
 
 def _pad16(b):

@@ -341,3 +341,61 @@ def test_fragility_classifiers_closed_forms():
     tmax = np.array([10.0, 10.0, 1.0 - 5e-4, 0.5])
     assert s.shadow_fragile(os_, up, 1e-3, tmax).tolist() == [False, True, True, False]
     assert s.shadow_fragile(os_, up, 1e-5, tmax).tolist() == [False, False, False, False]
+
+
+# ---- materials (src/crt/materials/material.hpp:50-60, 90-184) -------------
+def test_oracle_blinn_phong_closed_forms():
+    # ka only: clamp(colour ka) (no angle term); ks only with alpha 0:
+    # clamp(ks pow(x, 0)) = clamp(ks); kd only equals ColoredLambertian
+    import material_scenes as ms
+    cam_hit = None
+    for mat, expect in (({"type": "blinn_phong", "coeffs": (1.0, 0.0, 0.0)}, lambda c: np.minimum(c, 1.0)),
+                        ({"type": "blinn_phong", "coeffs": (0.0, 0.0, 0.6), "alpha": 0.0},
+                         lambda c: np.full(3, 0.6, np.float32))):
+        sc = ms.lit_plane(mat)
+        s = ms.build(sc, gpu=False)
+        rgb, _ = s.render(sc["camera"], sc["lights"], 1, 1, 0.0, 1, seed=1)
+        hit = rgb.sum(-1) > 0
+        cam_hit = hit.mean()
+        want = ms.TWO_PI_F * expect(np.asarray(sc["colors"][0], np.float32)).astype(np.float32) * np.float32(0.8)
+        assert hit.mean() > 0.5
+        assert np.allclose(rgb[hit], want, rtol=1e-6, atol=0)
+    sc = ms.lit_plane({"type": "blinn_phong", "coeffs": (0.0, 1.0, 0.0)})
+    bp, _ = ms.build(sc, gpu=False).render(sc["camera"], sc["lights"], 1, 1, 0.0, 1, seed=1)
+    sc["materials"] = [{"type": "lambertian", "color": sc["colors"][0]}]
+    lam, _ = ms.build(sc, gpu=False).render(sc["camera"], sc["lights"], 1, 1, 0.0, 1, seed=1)
+    assert np.array_equal(bp, lam) and cam_hit > 0.5
+
+
+def test_oracle_textures_closed_forms():
+    import material_scenes as ms
+    # a constant texture equals ColoredLambertian with texel / 255
+    sc = ms.mixed_cornell()
+    sc["textures"] = [np.full((3, 5, 4), 200, np.uint8)] + sc["textures"][1:]
+    sc["materials"] = [m for m in sc["materials"]]
+    sc["materials"][2] = {"type": "textured_lambertian", "texture": 0}
+    a, _ = ms.build(sc, gpu=False).render(dict(sc["camera"], resolution=[48, 48]), sc["lights"], 4, 4, 0.0, 3, seed=2)
+    sc["materials"][2] = {"type": "lambertian", "color": (np.float32(200) / np.float32(255),) * 3}
+    b, _ = ms.build(sc, gpu=False).render(dict(sc["camera"], resolution=[48, 48]), sc["lights"], 4, 4, 0.0, 3, seed=2)
+    assert np.array_equal(a, b)
+    # the checker pattern follows the nearest-texel rule
+    sc = ms.checker_plane()
+    rgb, _ = ms.build(sc, gpu=False).render(sc["camera"], sc["lights"], 1, 1, 0.0, 1, seed=1)
+    want = ms.checker_expected(sc["camera"])
+    sel = want >= 0
+    white = rgb[..., 0] > 0.5
+    assert sel.mean() > 0.5 and np.array_equal(white[sel], want[sel] == 1)
+
+
+def test_oracle_mirror_reflection_closed_form():
+    import material_scenes as ms
+    sc = ms.mirror_room()
+    rgb, _ = ms.build(sc, gpu=False).render(sc["camera"], sc["lights"], 1, 1, 0.0, 2, seed=1)
+    want = ms.mirror_expected(sc["camera"])
+    sel = want != 0
+    got = np.sign(rgb[..., 0] - rgb[..., 1])
+    assert sel.mean() > 0.3
+    assert np.array_equal(got[sel], want[sel])
+    # bounce 0 on the mirror adds nothing: one bounce renders black there
+    one, _ = ms.build(sc, gpu=False).render(sc["camera"], sc["lights"], 1, 1, 0.0, 1, seed=1)
+    assert np.all(one[sel] == 0)
