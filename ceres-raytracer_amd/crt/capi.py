@@ -53,7 +53,8 @@ class BvhInfo(C.Structure):
     _fields_ = [("n_tris", C.c_uint32), ("n_wide_nodes", C.c_uint32), ("n_leaves", C.c_uint32),
                 ("max_depth", C.c_uint32), ("stack_capacity", C.c_uint32), ("pad_", C.c_uint32),
                 ("build_ms", C.c_double), ("sah_cost", C.c_double), ("bounds", C.c_double * 6),
-                ("sphere", C.c_double * 4), ("device_bytes", C.c_uint64), ("self_offset", C.c_double)]
+                ("sphere", C.c_double * 4), ("device_bytes", C.c_uint64), ("self_offset", C.c_double),
+                ("refit_ms", C.c_double)]
 
 
 class Timing(C.Structure):
@@ -151,6 +152,7 @@ SIGNATURES = {
     "crt_scene_create_f64": (C.c_int, [_p, C.c_uint32, _p, C.c_uint32, _p, _p, C.c_uint32, _p, _p, _p, _p]),
     "crt_scene_info": (C.c_int, [_p, _p]),
     "crt_scene_set_materials": (C.c_int, [_p, C.c_uint32, _p, _p, C.c_uint32, _p]),
+    "crt_scene_refit": (C.c_int, [_p, _p, _p, C.c_double, _p]),
     "crt_scene_destroy": (None, [_p]),
     "crt_scene_serialize": (C.c_int, [_p, _p, _p]),
     "crt_scene_deserialize": (C.c_int, [_p, C.c_uint64, _p]),
@@ -396,6 +398,16 @@ class Scene:
         tex = (Texture * max(len(keep), 1))(*[Texture(t.shape[1], t.shape[0], t.ctypes.data) for t in keep])
         uvs = None if uv is None else np.ascontiguousarray(uv, dtype=np.float32).reshape(-1, 2)
         _check(load().crt_scene_set_materials(self.h, len(materials), mats, ptr(uvs), len(keep), tex))
+
+    def refit(self, vertices, vertex_normals=None, max_sah_growth=0.0) -> bool:
+        """Move the vertices (same triangles) and refit the BVH on the GPU;
+        returns True when the SAH had grown past max_sah_growth (default
+        1.5) and the BVH was rebuilt instead (crt_scene_refit)."""
+        v = np.ascontiguousarray(vertices, dtype=np.float64).reshape(-1, 3)
+        vn = None if vertex_normals is None else np.ascontiguousarray(vertex_normals, dtype=np.float32)
+        rb = C.c_uint32(0)
+        _check(load().crt_scene_refit(self.h, ptr(v), ptr(vn), float(max_sah_growth), C.byref(rb)))
+        return bool(rb.value)
 
     def close(self):
         if getattr(self, "h", None) and self.h.value:

@@ -856,7 +856,165 @@ __global__ __launch_bounds__(BLK) void k_compress(const float4* __restrict__ nod
 
 inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
+// ---- SAH of the wide BVH (node visit cost 1, triangle cost 1) ----------
+// Per block, the sum over child records of area x (leaf ? triangles : 1);
+// the host adds the block sums in order (deterministic) and the root.
+__global__ __launch_bounds__(BLK) void k_sah_blocks(const float4* __restrict__ nodes, uint32_t n_rec,
+                                                    double* __restrict__ partial) {
+    __shared__ double red[BLK];
+    const uint32_t i = blockIdx.x * BLK + threadIdx.x;
+    double a = 0.0;
+    if (i < n_rec) {
+        const float4 l = nodes[2 * (size_t)i], h = nodes[2 * (size_t)i + 1];
+        const uint32_t ref = __float_as_uint(l.w);
+        if (ref != REF_EMPTY) {
+            const double dx = fmax(0.0, (double)h.x - l.x), dy = fmax(0.0, (double)h.y - l.y),
+                         dz = fmax(0.0, (double)h.z - l.z);
+            a = (dx * dy + dy * dz + dz * dx) * ((ref & REF_LEAF) ? (double)leaf_count(ref) : 1.0);
+        }
+    }
+    red[threadIdx.x] = a;
+    __syncthreads();
+    for (int w = BLK / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// ---- refit (crt_scene_refit) -------------------------------------------
+// Leaf-ordered triangles rewritten in place from new vertices (the original
+// triangle index sits in p0.w).
+__global__ __launch_bounds__(BLK) void k_refit_emit(const float* __restrict__ V, const uint32_t* __restrict__ I,
+                                                    uint32_t n, float4* __restrict__ tris) {
+    const uint32_t k = blockIdx.x * BLK + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t t = __float_as_uint(tris[3 * k].w);
+    const uint32_t a = I[3 * t], b = I[3 * t + 1], c = I[3 * t + 2];
+    tris[3 * k + 0] = make_float4(V[3 * a], V[3 * a + 1], V[3 * a + 2], __uint_as_float(t));
+    tris[3 * k + 1] = make_float4(V[3 * b], V[3 * b + 1], V[3 * b + 2], 0.f);
+    tris[3 * k + 2] = make_float4(V[3 * c], V[3 * c + 1], V[3 * c + 2], 0.f);
+}
+
+// One wide level: child k of node i gets the exact union of what it holds
+// (its triangles for a leaf, the exact child boxes of its node otherwise,
+// already refitted since deeper levels run first), kept unpadded in `exact`
+// and written padded into the node -- the build pads each child box once
+// from its exact box in the same way, so unmoved vertices refit to the same
+// nodes bit for bit.
+__global__ __launch_bounds__(BLK) void k_refit_level(float4* __restrict__ nodes, float4* __restrict__ exact,
+                                                     const float4* __restrict__ tris, uint32_t first,
+                                                     uint32_t count) {
+    const uint32_t t = blockIdx.x * BLK + threadIdx.x;
+    if (t >= count * 8u) return;
+    const uint32_t i = first + t / 8u, k = t % 8u;
+    float4* rec = nodes + (size_t)i * 16u + 2u * k;
+    const uint32_t ref = __float_as_uint(rec[0].w);
+    if (ref == REF_EMPTY) return;
+    const float inf = __int_as_float(0x7f800000);
+    float lx = inf, ly = inf, lz = inf, hx = -inf, hy = -inf, hz = -inf;
+    if (ref & REF_LEAF) {
+        const uint32_t f = leaf_first(ref), c = leaf_count(ref);
+        for (uint32_t j = 0; j < 3u * c; ++j) {
+            const float4 p = tris[3 * (size_t)f + j];
+            lx = fminf(lx, p.x); ly = fminf(ly, p.y); lz = fminf(lz, p.z);
+            hx = fmaxf(hx, p.x); hy = fmaxf(hy, p.y); hz = fmaxf(hz, p.z);
+        }
+    } else {
+        for (uint32_t m = 0; m < 8u; ++m) {
+            if (__float_as_uint(nodes[(size_t)ref * 16u + 2u * m].w) == REF_EMPTY) continue;
+            const float4 el = exact[(size_t)ref * 16u + 2u * m], eh = exact[(size_t)ref * 16u + 2u * m + 1u];
+            lx = fminf(lx, el.x); ly = fminf(ly, el.y); lz = fminf(lz, el.z);
+            hx = fmaxf(hx, eh.x); hy = fmaxf(hy, eh.y); hz = fmaxf(hz, eh.z);
+        }
+    }
+    exact[(size_t)i * 16u + 2u * k] = make_float4(lx, ly, lz, 0.f);
+    exact[(size_t)i * 16u + 2u * k + 1u] = make_float4(hx, hy, hz, 0.f);
+    rec[0] = make_float4(pad_down(lx), pad_down(ly), pad_down(lz), rec[0].w);
+    rec[1] = make_float4(pad_up(hx), pad_up(hy), pad_up(hz), rec[1].w);
+}
+
 }  // namespace
+
+double wide_sah(const BuildResult& b, hipStream_t st) {
+    const uint32_t n_rec = b.n_nodes * 8u;
+    const uint32_t nb = cdiv(n_rec, BLK);
+    DevBuf part;
+    part.alloc((size_t)nb * 8);
+    hipLaunchKernelGGL(k_sah_blocks, dim3(nb), dim3(BLK), 0, st, b.nodes.as<float4>(), n_rec, part.as<double>());
+    CRT_HIP_CHECK(hipGetLastError());
+    std::vector<double> hp(nb);
+    float4 root[16];
+    CRT_HIP_CHECK(hipMemcpyAsync(hp.data(), part.p, (size_t)nb * 8, hipMemcpyDeviceToHost, st));
+    CRT_HIP_CHECK(hipMemcpyAsync(root, b.nodes.p, sizeof(root), hipMemcpyDeviceToHost, st));
+    CRT_HIP_CHECK(hipStreamSynchronize(st));
+    double cost = 0.0;
+    for (double x : hp) cost += x;
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int k = 0; k < 8; ++k) {
+        uint32_t ref;
+        std::memcpy(&ref, &root[2 * k].w, 4);
+        if (ref == REF_EMPTY) continue;
+        const float l[3] = {root[2 * k].x, root[2 * k].y, root[2 * k].z};
+        const float h[3] = {root[2 * k + 1].x, root[2 * k + 1].y, root[2 * k + 1].z};
+        for (int a = 0; a < 3; ++a) { lo[a] = std::min(lo[a], (double)l[a]); hi[a] = std::max(hi[a], (double)h[a]); }
+    }
+    const double dx = std::max(0.0, hi[0] - lo[0]), dy = std::max(0.0, hi[1] - lo[1]), dz = std::max(0.0, hi[2] - lo[2]);
+    const double root_area = dx * dy + dy * dz + dz * dx;
+    return root_area > 0 ? (root_area + cost) / root_area : 0.0;
+}
+
+// Level boundaries of a tree whose nodes were numbered level by level
+// (scene blobs do not store them): depth of every node from the refs.
+void wide_levels(BuildResult& b) {
+    std::vector<float4> hn((size_t)b.n_nodes * 16);
+    CRT_HIP_CHECK(hipMemcpy(hn.data(), b.nodes.p, hn.size() * 16, hipMemcpyDeviceToHost));
+    std::vector<uint32_t> depth(b.n_nodes, 0);
+    if (b.n_nodes) depth[0] = 1;
+    b.level_end.clear();
+    for (uint32_t i = 0; i < b.n_nodes; ++i) {
+        CRT_REQUIRE(i == 0 || depth[i] >= depth[i - 1], CRT_ERR_STATE, "BVH nodes are not numbered by level");
+        if (i > 0 && depth[i] != depth[i - 1]) b.level_end.push_back(i);
+        for (int k = 0; k < 8; ++k) {
+            uint32_t ref;
+            std::memcpy(&ref, &hn[(size_t)i * 16 + 2 * k].w, 4);
+            if (ref != REF_EMPTY && !(ref & REF_LEAF)) depth[ref] = depth[i] + 1;
+        }
+    }
+    b.level_end.push_back(b.n_nodes);
+}
+
+// Boxes of every wide node recomputed bottom-up from the (already uploaded)
+// new vertices, triangles re-emitted in leaf order, compressed nodes and the
+// SAH recomputed; the tree's topology is kept.
+void refit_bvh(const float* d_vertices, const uint32_t* d_indices, uint32_t n_tris, hipStream_t st, BuildResult& b,
+               DevBuf& scratch) {
+    if (b.level_end.empty()) wide_levels(b);
+    hipEvent_t e0, e1;
+    CRT_HIP_CHECK(hipEventCreate(&e0));
+    CRT_HIP_CHECK(hipEventCreate(&e1));
+    CRT_HIP_CHECK(hipEventRecord(e0, st));
+    hipLaunchKernelGGL(k_refit_emit, dim3(cdiv(n_tris, BLK)), dim3(BLK), 0, st, d_vertices, d_indices, n_tris,
+                       b.tris.as<float4>());
+    scratch.reserve((size_t)b.n_nodes * 256);
+    for (size_t lv = b.level_end.size(); lv-- > 0;) {
+        const uint32_t first = lv == 0 ? 0u : b.level_end[lv - 1];
+        const uint32_t count = b.level_end[lv] - first;
+        if (count == 0) continue;
+        hipLaunchKernelGGL(k_refit_level, dim3(cdiv((uint64_t)count * 8, BLK)), dim3(BLK), 0, st,
+                           b.nodes.as<float4>(), scratch.as<float4>(), b.tris.as<float4>(), first, count);
+    }
+    hipLaunchKernelGGL(k_compress, dim3(cdiv(b.n_nodes, BLK)), dim3(BLK), 0, st, b.nodes.as<float4>(), b.n_nodes,
+                       b.cnodes.as<float4>());
+    CRT_HIP_CHECK(hipGetLastError());
+    CRT_HIP_CHECK(hipEventRecord(e1, st));
+    b.sah_cost = wide_sah(b, st);
+    float ms = 0.f;
+    CRT_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    b.refit_ms = ms;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+}
 
 // PLOC rounds on the Morton-sorted leaves, then the tree written in the
 // unified layout (see k_ploc_emit); `vals_out` receives the triangle order.
@@ -1082,6 +1240,7 @@ void build_bvh(const float* d_vertices, const uint32_t* d_indices, uint32_t n_tr
     }
     uint32_t n_in = 1, depth = 0, fr = 0;
     uint32_t host_counters[4] = {0, 0, 0, 0};
+    out.level_end.assign(1, 1u);   // wide node ids are assigned level by level: level 1 = the root
     while (n_in > 0) {
         ++depth;
         CRT_HIP_CHECK(hipMemsetAsync(counters.as<uint32_t>() + 1, 0, 4, st));
@@ -1098,6 +1257,7 @@ void build_bvh(const float* d_vertices, const uint32_t* d_indices, uint32_t n_tr
         CRT_HIP_CHECK(hipMemcpyAsync(host_counters, counters.p, 16, hipMemcpyDeviceToHost, st));
         CRT_HIP_CHECK(hipStreamSynchronize(st));
         n_in = host_counters[1];
+        if (n_in > 0) out.level_end.push_back(host_counters[0]);
         fr ^= 1;
         CRT_REQUIRE(host_counters[0] <= node_cap, CRT_ERR_HIP, "wide node capacity exceeded");
     }
@@ -1124,39 +1284,8 @@ void build_bvh(const float* d_vertices, const uint32_t* d_indices, uint32_t n_tr
     (void)hipEventDestroy(e1);
     for (int k = 0; k < 6; ++k) out.bounds[k] = o2f(hb[6 + k]);
 
-    // SAH cost of the wide BVH (node visit cost 1, triangle cost 1), host side
-    std::vector<float4> hn((size_t)out.n_nodes * 16);
-    CRT_HIP_CHECK(hipMemcpy(hn.data(), out.nodes.p, hn.size() * 16, hipMemcpyDeviceToHost));
-    auto ha = [](const float4& lo, const float4& hi) {
-        const double dx = std::max(0.0, (double)hi.x - lo.x), dy = std::max(0.0, (double)hi.y - lo.y),
-                     dz = std::max(0.0, (double)hi.z - lo.z);
-        return dx * dy + dy * dz + dz * dx;
-    };
-    double cost = 0.0, root_area = 0.0;
-    {
-        float4 rl = make_float4(INFINITY, INFINITY, INFINITY, 0), rh = make_float4(-INFINITY, -INFINITY, -INFINITY, 0);
-        for (int k = 0; k < 8; ++k) {
-            const float4& l = hn[2 * k];
-            const float4& h = hn[2 * k + 1];
-            uint32_t ref;
-            memcpy(&ref, &l.w, 4);
-            if (ref == REF_EMPTY) continue;
-            rl.x = std::min(rl.x, l.x); rl.y = std::min(rl.y, l.y); rl.z = std::min(rl.z, l.z);
-            rh.x = std::max(rh.x, h.x); rh.y = std::max(rh.y, h.y); rh.z = std::max(rh.z, h.z);
-        }
-        root_area = ha(rl, rh);
-    }
-    cost = root_area;   // root visit
-    for (size_t i = 0; i < (size_t)out.n_nodes * 8; ++i) {
-        const float4& l = hn[2 * i];
-        const float4& h = hn[2 * i + 1];
-        uint32_t ref;
-        memcpy(&ref, &l.w, 4);
-        if (ref == REF_EMPTY) continue;
-        const double a = ha(l, h);
-        cost += (ref & REF_LEAF) ? a * (double)leaf_count(ref) : a;
-    }
-    out.sah_cost = root_area > 0 ? cost / root_area : 0.0;
+    out.sah_cost = wide_sah(out, st);
+    out.sah_at_build = out.sah_cost;
 }
 
 }  // namespace crt

@@ -99,6 +99,9 @@ struct BuildResult {
     uint32_t ploc_rounds = 0;   // 0: LBVH (Karras) tree
     double build_ms = 0.0;
     double sah_cost = 0.0;
+    double sah_at_build = 0.0;        // SAH when the topology was last built
+    double refit_ms = 0.0;            // device time of the last refit
+    std::vector<uint32_t> level_end;  // wide node ids are level-ordered: level l ends at level_end[l]
     float bounds[6] = {0, 0, 0, 0, 0, 0};
 };
 
@@ -106,6 +109,10 @@ struct BuildResult {
 // hierarchy, bottom-up refit) collapsed into 8-wide nodes.
 void build_bvh(const float* d_vertices, const uint32_t* d_indices, uint32_t n_tris,
                uint32_t max_leaf, uint32_t build_flags, hipStream_t stream, BuildResult& out);
+// Refit: same topology, boxes recomputed from new (already uploaded) vertices.
+void refit_bvh(const float* d_vertices, const uint32_t* d_indices, uint32_t n_tris, hipStream_t stream,
+               BuildResult& b, DevBuf& scratch);
+double wide_sah(const BuildResult& b, hipStream_t stream);   // node cost 1, triangle cost 1, per root area
 
 // Shared helpers of the trace / render / sensor translation units.
 // Camera in the scene frame of `s` (position moved by the scene origin).
@@ -149,6 +156,9 @@ struct crt_scene {
     // materials (crt_scene_set_materials; empty: ColoredLambertian with inst_color)
     crt::DevBuf mats, vuv, texels, tex_info;
     uint32_t n_textures = 0;
+    // refit (crt_scene_refit): build parameters for a rebuild, exact boxes
+    uint32_t max_leaf = 8, build_flags = 0;
+    crt::DevBuf refit_scratch;
     crt::BuildResult bvh;
     crt::SceneView view{};
     crt_bvh_info info{};

@@ -26,6 +26,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <random>
 #include <string>
 #include <vector>
@@ -254,15 +255,18 @@ struct GpuScene {
     }
 };
 
-std::shared_ptr<GpuScene> build_scene(const std::vector<const Entity*>& ents, uint32_t max_leaf) {
-    ensure_device();
-    // vertices stay FP64 up to the library, which rounds them to FP32 only
-    // after moving them into the scene frame (an entity 1e7 m out keeps its
-    // own resolution, as the reference's double geometry does)
+// Entities merged in FP64 (vertices stay FP64 up to the library, which
+// rounds them to FP32 only after moving them into the scene frame: an entity
+// 1e7 m out keeps its own resolution, as the reference's double geometry does).
+struct SceneInput {
     std::vector<double> V;
     std::vector<float> VN, col;
     std::vector<uint32_t> I, inst;
     std::vector<uint8_t> sm;
+};
+
+SceneInput gather(const std::vector<const Entity*>& ents) {
+    SceneInput in;
     uint32_t base = 0;
     for (size_t e = 0; e < ents.size(); ++e) {
         const Entity& en = *ents[e];
@@ -271,69 +275,124 @@ std::shared_ptr<GpuScene> build_scene(const std::vector<const Entity*>& ents, ui
         const double* R = en.rotation;
         for (size_t i = 0; i < m.n_vertices(); ++i) {
             const double x = en.scale * m.v[3 * i], y = en.scale * m.v[3 * i + 1], z = en.scale * m.v[3 * i + 2];
-            V.push_back(R[0] * x + R[1] * y + R[2] * z + en.position[0]);
-            V.push_back(R[3] * x + R[4] * y + R[5] * z + en.position[1]);
-            V.push_back(R[6] * x + R[7] * y + R[8] * z + en.position[2]);
+            in.V.push_back(R[0] * x + R[1] * y + R[2] * z + en.position[0]);
+            in.V.push_back(R[3] * x + R[4] * y + R[5] * z + en.position[1]);
+            in.V.push_back(R[6] * x + R[7] * y + R[8] * z + en.position[2]);
             const double a = m.vn[3 * i], b = m.vn[3 * i + 1], c = m.vn[3 * i + 2];
-            VN.push_back((float)(R[0] * a + R[1] * b + R[2] * c));
-            VN.push_back((float)(R[3] * a + R[4] * b + R[5] * c));
-            VN.push_back((float)(R[6] * a + R[7] * b + R[8] * c));
+            in.VN.push_back((float)(R[0] * a + R[1] * b + R[2] * c));
+            in.VN.push_back((float)(R[3] * a + R[4] * b + R[5] * c));
+            in.VN.push_back((float)(R[6] * a + R[7] * b + R[8] * c));
         }
-        for (size_t t = 0; t < m.f.size(); ++t) I.push_back(m.f[t] + base);
-        inst.insert(inst.end(), m.n_tris(), (uint32_t)(e + 1));
+        for (size_t t = 0; t < m.f.size(); ++t) in.I.push_back(m.f[t] + base);
+        in.inst.insert(in.inst.end(), m.n_tris(), (uint32_t)(e + 1));
         base += (uint32_t)m.n_vertices();
-        col.insert(col.end(), en.color, en.color + 3);
-        sm.push_back(en.smooth_shading ? 1 : 0);
+        in.col.insert(in.col.end(), en.color, en.color + 3);
+        in.sm.push_back(en.smooth_shading ? 1 : 0);
     }
-    if (I.empty()) throw py::value_error("the entities contain no triangles");
-    auto g = std::make_shared<GpuScene>();
-    g->colors = col;
-    crt_build_opts opts{max_leaf, 0};
-    crt_scene* h = nullptr;
-    {
-        py::gil_scoped_release nogil;
-        check(crt_scene_create_f64(V.data(), (uint32_t)(V.size() / 3), I.data(), (uint32_t)(I.size() / 3),
-                                   inst.data(), VN.data(), (uint32_t)ents.size(), col.data(), sm.data(), &opts, &h));
-    }
-    g->h = h;
-    // materials, textures and UVs, when any entity sets them
+    if (in.I.empty()) throw py::value_error("the entities contain no triangles");
+    return in;
+}
+
+void upload_materials(crt_scene* h, const std::vector<const Entity*>& ents) {
     bool any = false, any_uv = false;
     for (const Entity* e : ents) {
         any = any || !e->default_material();
         any_uv = any_uv || !e->uv.empty();
     }
-    if (any) {
-        std::vector<crt_material> mats;
-        std::vector<crt_texture> texs;
-        std::vector<float> uvs;
-        for (const Entity* e : ents) {
-            crt_material m{};
-            m.type = e->mat_type;
-            m.alpha = e->mat_alpha;
-            for (int k = 0; k < 3; ++k) { m.color[k] = e->color[k]; m.coeffs[k] = e->mat_coeffs[k]; }
-            m.texture = m.spec_texture = -1;
-            if (e->tex.rgba) {
-                m.texture = (int32_t)texs.size();
-                texs.push_back({e->tex.w, e->tex.h, e->tex.rgba->data()});
-            }
-            if (e->spec_tex.rgba) {
-                m.spec_texture = (int32_t)texs.size();
-                texs.push_back({e->spec_tex.w, e->spec_tex.h, e->spec_tex.rgba->data()});
-            }
-            mats.push_back(m);
-            if (any_uv) {
-                if (!e->uv.empty() && e->uv.size() != 2 * e->mesh->n_vertices())
-                    throw py::value_error("entity uv needs (n_vertices, 2) values");
-                if (e->uv.empty()) uvs.insert(uvs.end(), 2 * e->mesh->n_vertices(), 0.0f);
-                else uvs.insert(uvs.end(), e->uv.begin(), e->uv.end());
-            }
+    if (!any) return;
+    std::vector<crt_material> mats;
+    std::vector<crt_texture> texs;
+    std::vector<float> uvs;
+    for (const Entity* e : ents) {
+        crt_material m{};
+        m.type = e->mat_type;
+        m.alpha = e->mat_alpha;
+        for (int k = 0; k < 3; ++k) { m.color[k] = e->color[k]; m.coeffs[k] = e->mat_coeffs[k]; }
+        m.texture = m.spec_texture = -1;
+        if (e->tex.rgba) {
+            m.texture = (int32_t)texs.size();
+            texs.push_back({e->tex.w, e->tex.h, e->tex.rgba->data()});
         }
-        py::gil_scoped_release nogil;
-        check(crt_scene_set_materials(h, (uint32_t)mats.size(), mats.data(), any_uv ? uvs.data() : nullptr,
-                                      (uint32_t)texs.size(), texs.empty() ? nullptr : texs.data()));
+        if (e->spec_tex.rgba) {
+            m.spec_texture = (int32_t)texs.size();
+            texs.push_back({e->spec_tex.w, e->spec_tex.h, e->spec_tex.rgba->data()});
+        }
+        mats.push_back(m);
+        if (any_uv) {
+            if (!e->uv.empty() && e->uv.size() != 2 * e->mesh->n_vertices())
+                throw py::value_error("entity uv needs (n_vertices, 2) values");
+            if (e->uv.empty()) uvs.insert(uvs.end(), 2 * e->mesh->n_vertices(), 0.0f);
+            else uvs.insert(uvs.end(), e->uv.begin(), e->uv.end());
+        }
     }
+    py::gil_scoped_release nogil;
+    check(crt_scene_set_materials(h, (uint32_t)mats.size(), mats.data(), any_uv ? uvs.data() : nullptr,
+                                  (uint32_t)texs.size(), texs.empty() ? nullptr : texs.data()));
+}
+
+std::shared_ptr<GpuScene> build_scene(const std::vector<const Entity*>& ents, uint32_t max_leaf) {
+    ensure_device();
+    const SceneInput in = gather(ents);
+    auto g = std::make_shared<GpuScene>();
+    g->colors = in.col;
+    crt_build_opts opts{max_leaf, 0};
+    crt_scene* h = nullptr;
+    {
+        py::gil_scoped_release nogil;
+        check(crt_scene_create_f64(in.V.data(), (uint32_t)(in.V.size() / 3), in.I.data(), (uint32_t)(in.I.size() / 3),
+                                   in.inst.data(), in.VN.data(), (uint32_t)ents.size(), in.col.data(), in.sm.data(),
+                                   &opts, &h));
+    }
+    g->h = h;
+    upload_materials(h, ents);
     return g;
 }
+
+// Dynamic API (src/crt/rendering_dynamic/render.hpp:19-62 rebuilds the BVH
+// on every call): the last per-call scene is kept, and a call with the same
+// meshes, colours, shading flags and materials -- only poses or scales
+// changed -- refits it on the GPU (crt_scene_refit, which rebuilds by itself
+// when the SAH has grown 1.5x).  A call checks the scene out of the cache
+// while it uses it, so concurrent calls never share one.
+struct DynKey {
+    std::vector<const void*> meshes, textures;
+    std::vector<float> params;    // colour, smooth flag, material type / alpha / coefficients per entity
+    std::vector<float> uv;
+    bool operator==(const DynKey& o) const {
+        return meshes == o.meshes && textures == o.textures && params == o.params && uv == o.uv;
+    }
+};
+
+DynKey dyn_key(const std::vector<const Entity*>& ents) {
+    DynKey k;
+    for (const Entity* e : ents) {
+        k.meshes.push_back(e->mesh.get());
+        k.textures.push_back(e->tex.rgba.get());
+        k.textures.push_back(e->spec_tex.rgba.get());
+        k.params.insert(k.params.end(), {e->color[0], e->color[1], e->color[2], e->smooth_shading ? 1.f : 0.f,
+                                         (float)e->mat_type, e->mat_alpha, e->mat_coeffs[0], e->mat_coeffs[1],
+                                         e->mat_coeffs[2]});
+        k.uv.insert(k.uv.end(), e->uv.begin(), e->uv.end());
+        k.uv.push_back(-1.f);   // separator: which entity the UVs belong to
+    }
+    return k;
+}
+
+std::mutex g_dyn_mu;
+DynKey g_dyn_key;
+std::shared_ptr<GpuScene> g_dyn_scene;
+std::atomic<uint64_t> g_dyn_refits{0}, g_dyn_builds{0};
+
+struct DynScene {
+    std::shared_ptr<GpuScene> scene;
+    DynKey key;
+    ~DynScene() {   // back into the cache for the next call
+        if (!scene) return;
+        std::lock_guard<std::mutex> lk(g_dyn_mu);
+        g_dyn_scene = scene;
+        g_dyn_key = std::move(key);
+    }
+};
 
 // ---------------------------------------------------------------- lights
 std::vector<crt_light> convert_lights(py::list lights) {
@@ -482,14 +541,36 @@ BodyFixedGroup make_group(py::list entities, uint32_t max_leaf) {
     return g;
 }
 
-std::shared_ptr<GpuScene> dynamic_scene(py::list entities) {
-    // src/crt/rendering_dynamic/render.hpp:19-62: the BVH is rebuilt per call
+std::unique_ptr<DynScene> dynamic_scene(py::list entities) {
     std::vector<const Entity*> ptrs;
     for (py::handle h : entities) {
         if (!py::isinstance<Entity>(h)) throw py::type_error("expected Entity objects");
         ptrs.push_back(&h.cast<const Entity&>());
     }
-    return build_scene(ptrs, 8);
+    ensure_device();
+    auto d = std::make_unique<DynScene>();
+    d->key = dyn_key(ptrs);
+    {
+        std::lock_guard<std::mutex> lk(g_dyn_mu);
+        if (g_dyn_scene && g_dyn_key == d->key) d->scene = std::move(g_dyn_scene);
+        g_dyn_scene.reset();
+    }
+    if (d->scene) {
+        const SceneInput in = gather(ptrs);
+        uint32_t rebuilt = 0;
+        int rc;
+        {
+            py::gil_scoped_release nogil;
+            rc = crt_scene_refit(d->scene->h, in.V.data(), in.VN.data(), 0.0, &rebuilt);
+        }
+        if (rc != CRT_OK) d->scene.reset();   // a half-refitted scene never goes back to the cache
+        check(rc);
+        ++g_dyn_refits;
+    } else {
+        d->scene = build_scene(ptrs, 8);
+        ++g_dyn_builds;
+    }
+    return d;
 }
 
 template <typename T>
@@ -715,23 +796,30 @@ PYBIND11_MODULE(_crt, m) {
     // ---- dynamic (per-call BVH) module functions, src/pybind.cpp:580-718
     m.def("render", [](py::handle camera, py::list lights, py::list entities, int min_s, int max_s, double thr,
                        int nb) {
-        auto s = dynamic_scene(entities);
-        return op_render(s->h, as_camera(camera), lights, min_s, max_s, thr, nb, py::none(), nullptr);
+        auto d = dynamic_scene(entities);
+        return op_render(d->scene->h, as_camera(camera), lights, min_s, max_s, thr, nb, py::none(), nullptr);
     });
     m.def("simulate_lidar", [](py::handle lidar, py::list entities, int) {
-        auto s = dynamic_scene(entities);
-        return op_lidar(s->h, as_lidar(lidar));
+        auto d = dynamic_scene(entities);
+        return op_lidar(d->scene->h, as_lidar(lidar));
     });
     m.def("intersection_pass", [](py::handle camera, py::list entities) {
-        auto s = dynamic_scene(entities);
-        return op_pass_xyz(s->h, as_camera(camera), false);
+        auto d = dynamic_scene(entities);
+        return op_pass_xyz(d->scene->h, as_camera(camera), false);
     });
     m.def("instance_pass", [](py::handle camera, py::list entities) {
-        auto s = dynamic_scene(entities);
-        return op_pass_instance(s->h, as_camera(camera));
+        auto d = dynamic_scene(entities);
+        return op_pass_instance(d->scene->h, as_camera(camera));
     });
     m.def("normal_pass", [](py::handle camera, py::list entities) {
-        auto s = dynamic_scene(entities);
-        return op_pass_xyz(s->h, as_camera(camera), true);
+        auto d = dynamic_scene(entities);
+        return op_pass_xyz(d->scene->h, as_camera(camera), true);
     });
+    m.def("dynamic_scene_stats", []() {
+        std::lock_guard<std::mutex> lk(g_dyn_mu);
+        py::dict r;
+        r["refits"] = g_dyn_refits.load();
+        r["builds"] = g_dyn_builds.load();
+        return r;
+    }, "Counts of per-call scenes built and refitted by the dynamic API (additive).");
 }

@@ -277,6 +277,7 @@ int crt_scene_deserialize(const void* buffer, uint64_t size, crt_scene** out_sce
         s->bvh.max_depth = h.max_depth;
         s->bvh.build_ms = h.info.build_ms;
         s->bvh.sah_cost = h.info.sah_cost;
+        s->bvh.sah_at_build = h.info.sah_cost;
         for (int k = 0; k < 6; ++k) s->bvh.bounds[k] = h.bvh_bounds[k];
         s->info = h.info;
         for (int k = 0; k < 3; ++k) {

@@ -484,6 +484,36 @@ int crt_device_init(int device) {
 
 namespace {
 
+// Scene frame: a body whose box contains the world origin keeps origin 0
+// (its FP32 vertices stay exact; moving them by the box centre could push
+// coordinates across a binade and round them, tilting small triangles'
+// normals by ulp / edge); any other body takes its box centre rounded to a
+// multiple of the FP32 ulp of its largest coordinate, which makes
+// vertex - origin exact for FP32 input vertices of the origin's magnitude (a
+// body far from the world origin).  FP64 input is rounded once, after the
+// subtraction.  `local` receives the vertices in the frame.
+template <typename VT>
+void choose_frame(const VT* vertices, uint32_t n_vertices, double origin[3], std::vector<float>& local) {
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    double amax = 0.0;
+    for (uint64_t i = 0; i < n_vertices; ++i)
+        for (int k = 0; k < 3; ++k) {
+            const double x = vertices[3 * i + k];
+            lo[k] = std::min(lo[k], x);
+            hi[k] = std::max(hi[k], x);
+            amax = std::max(amax, std::fabs(x));
+        }
+    int ex = 0;
+    std::frexp(amax, &ex);
+    const double q = std::ldexp(1.0, ex - 24);   // >= ulp of every vertex coordinate
+    const bool around_zero = lo[0] <= 0.0 && hi[0] >= 0.0 && lo[1] <= 0.0 && hi[1] >= 0.0 && lo[2] <= 0.0 &&
+                             hi[2] >= 0.0;
+    for (int k = 0; k < 3; ++k)
+        origin[k] = (amax > 0.0 && !around_zero) ? std::round(0.5 * (lo[k] + hi[k]) / q) * q : 0.0;
+    local.resize((size_t)n_vertices * 3);
+    for (uint64_t i = 0; i < 3ull * n_vertices; ++i) local[i] = (float)((double)vertices[i] - origin[i % 3]);
+}
+
 // Shared by crt_scene_create (FP32 vertices) and crt_scene_create_f64 (FP64
 // vertices): the frame origin is chosen from the vertices at their own
 // precision and subtracted before the one rounding to FP32, so an FP64 mesh
@@ -514,33 +544,8 @@ int scene_create_impl(const VT* vertices, uint32_t n_vertices, const uint32_t* i
         s->n_tris = n_tris;
         s->n_vertices = n_vertices;
         s->n_instances = ni;
-        // scene frame: a body whose box contains the world origin keeps
-        // origin 0 (its FP32 vertices stay exact; moving them by the box
-        // centre could push coordinates across a binade and round them,
-        // tilting small triangles' normals by ulp / edge); any other body
-        // takes its box centre rounded to a multiple of the FP32 ulp of its
-        // largest coordinate, which makes vertex - origin exact for FP32
-        // input vertices of the origin's magnitude (a body far from the world
-        // origin).  FP64 input is rounded once, after the subtraction.
-        double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-        double amax = 0.0;
-        for (uint64_t i = 0; i < n_vertices; ++i)
-            for (int k = 0; k < 3; ++k) {
-                const double x = vertices[3 * i + k];
-                lo[k] = std::min(lo[k], x);
-                hi[k] = std::max(hi[k], x);
-                amax = std::max(amax, std::fabs(x));
-            }
-        int ex = 0;
-        std::frexp(amax, &ex);
-        const double q = std::ldexp(1.0, ex - 24);   // >= ulp of every vertex coordinate
-        const bool around_zero = lo[0] <= 0.0 && hi[0] >= 0.0 && lo[1] <= 0.0 && hi[1] >= 0.0 && lo[2] <= 0.0 &&
-                                 hi[2] >= 0.0;
-        for (int k = 0; k < 3; ++k)
-            s->view.origin[k] = (amax > 0.0 && !around_zero) ? std::round(0.5 * (lo[k] + hi[k]) / q) * q : 0.0;
-        std::vector<float> local((size_t)n_vertices * 3);
-        for (uint64_t i = 0; i < 3ull * n_vertices; ++i)
-            local[i] = (float)((double)vertices[i] - s->view.origin[i % 3]);
+        std::vector<float> local;
+        choose_frame(vertices, n_vertices, s->view.origin, local);
         s->vertices.alloc((size_t)n_vertices * 12);
         s->indices.alloc((size_t)n_tris * 12);
         s->tri_vidx.alloc((size_t)n_tris * 12);
@@ -570,6 +575,8 @@ int scene_create_impl(const VT* vertices, uint32_t n_vertices, const uint32_t* i
         CRT_REQUIRE(max_leaf <= 8u, CRT_ERR_INVALID, "max_leaf_size must be in 1..8");
         const uint32_t bflags = opts ? opts->flags : 0u;
         CRT_REQUIRE((bflags & ~(CRT_BUILD_PLOC | CRT_BUILD_GREEDY_WIDE)) == 0u, CRT_ERR_INVALID, "unknown build flags");
+        s->max_leaf = max_leaf;
+        s->build_flags = bflags;
         build_bvh(s->vertices.as<float>(), s->indices.as<uint32_t>(), n_tris, max_leaf, bflags, nullptr, s->bvh);
         crt_bvh_info& inf = s->info;
         inf.n_tris = n_tris;
@@ -675,6 +682,67 @@ int crt_scene_set_materials(crt_scene* scene, uint32_t n_materials, const crt_ma
         scene->n_textures = n_textures;
         scene_bind_view(scene);
         scene->info.device_bytes = scene_device_bytes(scene);
+    });
+}
+
+int crt_scene_refit(crt_scene* scene, const double* vertices, const float* vertex_normals, double max_sah_growth,
+                    uint32_t* rebuilt) {
+    return guarded([&] {
+        CRT_REQUIRE(scene != nullptr && vertices != nullptr, CRT_ERR_INVALID, "NULL argument");
+        crt_scene* s = scene;
+        const uint32_t nv = s->n_vertices;
+        for (uint64_t i = 0; i < 3ull * nv; ++i)
+            CRT_REQUIRE(std::isfinite(vertices[i]) && std::fabs(vertices[i]) <= 1e18, CRT_ERR_INVALID,
+                        "vertex coordinate is not finite or exceeds 1e18");
+        const double growth = max_sah_growth > 0.0 ? max_sah_growth : 1.5;
+        std::vector<float> local;
+        double origin[3];
+        choose_frame(vertices, nv, origin, local);
+        CRT_HIP_CHECK(hipMemcpy(s->vertices.p, local.data(), (size_t)nv * 12, hipMemcpyHostToDevice));
+        if (vertex_normals) {
+            if (!s->vnormals.p) s->vnormals.alloc((size_t)nv * 12);
+            CRT_HIP_CHECK(hipMemcpy(s->vnormals.p, vertex_normals, (size_t)nv * 12, hipMemcpyHostToDevice));
+        }
+        for (int k = 0; k < 3; ++k) s->view.origin[k] = origin[k];
+        refit_bvh(s->vertices.as<float>(), s->indices.as<uint32_t>(), s->n_tris, nullptr, s->bvh, s->refit_scratch);
+        bool rb = false;
+        if (s->bvh.sah_cost > growth * s->bvh.sah_at_build) {
+            // boxes have grown apart (entities moved relative to each other): rebuild
+            const uint32_t max_leaf = s->max_leaf, flags = s->build_flags;
+            build_bvh(s->vertices.as<float>(), s->indices.as<uint32_t>(), s->n_tris, max_leaf, flags, nullptr, s->bvh);
+            rb = true;
+        } else {
+            // scene-frame bounds = the root's children (exact boxes of the refit)
+            float4 ex[16];
+            CRT_HIP_CHECK(hipMemcpy(ex, s->refit_scratch.p, sizeof(ex), hipMemcpyDeviceToHost));
+            float4 root[16];
+            CRT_HIP_CHECK(hipMemcpy(root, s->bvh.nodes.p, sizeof(root), hipMemcpyDeviceToHost));
+            float b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+            for (int k = 0; k < 8; ++k) {
+                uint32_t ref;
+                std::memcpy(&ref, &root[2 * k].w, 4);
+                if (ref == REF_EMPTY) continue;
+                b[0] = std::min(b[0], ex[2 * k].x); b[1] = std::min(b[1], ex[2 * k].y); b[2] = std::min(b[2], ex[2 * k].z);
+                b[3] = std::max(b[3], ex[2 * k + 1].x); b[4] = std::max(b[4], ex[2 * k + 1].y);
+                b[5] = std::max(b[5], ex[2 * k + 1].z);
+            }
+            for (int k = 0; k < 6; ++k) s->bvh.bounds[k] = b[k];
+        }
+        crt_bvh_info& inf = s->info;
+        inf.n_wide_nodes = s->bvh.n_nodes;
+        inf.n_leaves = s->bvh.n_leaves;
+        inf.max_depth = s->bvh.max_depth;
+        if (rb) inf.build_ms = s->bvh.build_ms;
+        inf.refit_ms = rb ? 0.0 : s->bvh.refit_ms;
+        inf.sah_cost = s->bvh.sah_cost;
+        for (int k = 0; k < 6; ++k) inf.bounds[k] = s->bvh.bounds[k] + s->view.origin[k % 3];
+        scene_make_view(s, local.data());
+        inf.self_offset = self_offset(s);
+        inf.device_bytes = scene_device_bytes(s);
+        CRT_REQUIRE(trace_lds_bytes(s->view) <= 64u * 1024u && lane_lds_bytes(s->view) <= 64u * 1024u &&
+                        quad_lds_bytes(s->view) <= 64u * 1024u,
+                    CRT_ERR_INVALID, "BVH too deep for the LDS traversal stack");
+        if (rebuilt) *rebuilt = rb ? 1u : 0u;
     });
 }
 

@@ -121,6 +121,7 @@ typedef struct {
     double   self_offset;     /* secondary-ray origin offset along the geometric normal:
                                  max(1e-4, 1e-6 x the scene's half extent), the reference's
                                  1e-4 (unidirectional.hpp:61) at unit scale */
+    double   refit_ms;        /* device time of the last crt_scene_refit (0: built) */
 } crt_bvh_info;
 
 /* Kernel timings of the last call, measured with hipEvents on the launch stream. */
@@ -221,6 +222,21 @@ typedef struct {
  * called while other calls on the scene run. */
 int crt_scene_set_materials(crt_scene* scene, uint32_t n_materials, const crt_material* materials,
                             const float* vertex_uv, uint32_t n_textures, const crt_texture* textures);
+/* ---- refit for moving geometry (dynamic scenes) --------------------------
+ * The reference rebuilds its SweepSAH + PRO BVH on every call of the
+ * dynamic API (src/crt/rendering_dynamic/render.hpp:19-62,
+ * src/crt/passes.hpp:200-357, simulate_lidar.hpp:7-59) and ships a
+ * refitter (lib/bvh/hierarchy_refitter.hpp:11-43).  Here: new vertex
+ * positions (FP64, n_vertices x 3, same order and triangles as at creation)
+ * and optionally new vertex normals; the tree keeps its topology and every
+ * box is recomputed bottom-up on the GPU (unmoved vertices give the built
+ * nodes bit for bit).  When the refitted SAH exceeds max_sah_growth (<= 0:
+ * 1.5) times the SAH of the last full build the BVH is rebuilt instead
+ * (*rebuilt = 1, may be NULL).  Closest hits equal those of a scene created
+ * from the new vertices except which triangle wins an exact tie.  Not to be
+ * called while other calls on the scene run. */
+int crt_scene_refit(crt_scene* scene, const double* vertices, const float* vertex_normals, double max_sah_growth,
+                    uint32_t* rebuilt);
 int  crt_scene_info(const crt_scene* scene, crt_bvh_info* info);
 void crt_scene_destroy(crt_scene* scene);
 
