@@ -97,7 +97,8 @@ class Request(C.Structure):
     _fields_ = [("kind", C.c_uint32), ("scene", C.c_uint32), ("camera", Camera), ("spp", C.c_int32),
                 ("bounces", C.c_int32), ("seed", C.c_uint64), ("n_lights", C.c_uint32), ("sub_rays", C.c_uint32),
                 ("lights", Light * 2), ("sigma", C.c_double), ("flux", FluxOpts),
-                ("rgba", C.c_void_p), ("lidar", C.c_void_p), ("force_torque", C.c_void_p)]
+                ("rgba", C.c_void_p), ("lidar", C.c_void_p), ("force_torque", C.c_void_p),
+                ("light_list", C.c_void_p)]
 
 
 class ServerStats(C.Structure):

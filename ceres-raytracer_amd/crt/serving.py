@@ -56,8 +56,12 @@ class Server:
             r.seed = int(req.get("seed", 0))
             lights = req.get("lights", [])
             r.n_lights = len(lights)
-            for k, l in enumerate(lights):
-                r.lights[k] = capi.make_light(l)
+            if len(lights) <= 2:
+                for k, l in enumerate(lights):
+                    r.lights[k] = capi.make_light(l)
+            else:   # any number of lights through light_list (copied at submission)
+                arr = (capi.Light * len(lights))(*[capi.make_light(l) for l in lights])
+                r.light_list = C.cast(arr, C.c_void_p)
             out["rgba"] = np.zeros((h, w, 4), dtype=np.float32)
             r.rgba = out["rgba"].ctypes.data
         elif kind == "lidar":
@@ -72,6 +76,7 @@ class Server:
             f.pressure, f.rho_s, f.rho_d = float(req["pressure"]), float(req["rho_s"]), float(req["rho_d"])
             f.grid = int(req["grid"])
             f.radius = 0.0
+            f.ref_point[:] = [float(x) for x in req.get("ref_point", (0.0, 0.0, 0.0))]
             r.flux = f
             out["force_torque"] = np.zeros(6)
             r.force_torque = out["force_torque"].ctypes.data

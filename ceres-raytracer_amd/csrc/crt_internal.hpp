@@ -129,6 +129,8 @@ struct RenderWorkspace;
 RenderWorkspace* new_render_workspace();
 void delete_render_workspace(RenderWorkspace* w);
 size_t render_bytes_estimate(uint64_t pixels, int spp, int n_lights);
+size_t render_workspace_bytes(const RenderWorkspace* w);   // device bytes it holds
+void check_light(const crt_light& L);                      // raise on bad light parameters
 // `lights` holds n_lights lights shared by every camera, or, with
 // per_camera_lights, n_cams x n_lights (camera c's lights at c * n_lights).
 int render_batch(crt_scene* scene, uint32_t n_cams, const crt_camera* cams, const int32_t* max_samples,
@@ -146,6 +148,16 @@ void flash_lidar_impl(crt_scene* scene, const crt_flash_opts* opts, double* sing
                       uint8_t* n_returns, uint32_t flags, void* stream, crt_timing* timing, Workspace& ws);
 void radiation_pressure_impl(crt_scene* scene, const crt_flux_opts* opts, double force[3], double torque[3],
                              uint64_t* n_hits, void* stream, crt_timing* timing, Workspace& ws);
+void check_flash_opts(const crt_flash_opts* opts);   // raise CRT_ERR_INVALID on bad options
+void check_flux_opts(const crt_flux_opts* opts);
+// Batched scans / flux grids of one scene, one launch (sequence) each; the
+// device segment table sits in ws.e, `table_bytes` keeps its host copy until
+// the caller synchronises.  outs: 3 device pointers (single, first, last) per
+// scan; out7: device, 7 doubles per flux request.
+void flash_lidar_multi(crt_scene* scene, uint32_t n, const crt_flash_opts* opts, double* const* outs,
+                       hipStream_t st, Workspace& ws, std::vector<uint8_t>& table_bytes);
+void radiation_pressure_multi(crt_scene* scene, uint32_t n, const crt_flux_opts* opts, double* out7,
+                              hipStream_t st, Workspace& ws, std::vector<uint8_t>& table_bytes);
 
 }  // namespace crt
 

@@ -54,7 +54,6 @@ namespace {
 
 inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
-constexpr int MAX_LIGHTS = 8;
 constexpr int TB = 256;
 constexpr uint32_t NO_LIST = 0xFFFFFFFFu;
 // default extend mode: one lane per camera ray from this many paths per pass
@@ -73,9 +72,10 @@ struct LightDev {
 struct CamDev {
     CamView v;
     uint64_t seed;
-    uint32_t pix_base;   // first global pixel of this camera
-    uint32_t jitter;     // max_samples != 1
-    LightDev lights[MAX_LIGHTS];   // this camera's lights (scene frame); the count is batch-wide
+    uint32_t pix_base;     // first global pixel of this camera
+    uint32_t jitter;       // max_samples != 1
+    uint32_t light_base;   // this camera's lights (scene frame) start here in the light array;
+    uint32_t pad_;         // the count is batch-wide
 };
 
 struct Segment {
@@ -136,6 +136,7 @@ struct Ctx {
     float offset;
     uint32_t grab;
     const CamDev* cams;
+    const LightDev* lights;
     const Segment* segs;
     const PassCtl* ctl;
     const uint32_t* plist;
@@ -362,7 +363,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_persistent(Ctx C, int qse
 // `key` = rng_key(camera seed, pixel, sample)
 __device__ inline void light_sample(const Ctx& C, uint32_t cam, int l, float3 p, uint64_t key, int bounce,
                                     float3& dir, float& tmax, float& irr) {
-    const LightDev& L = C.cams[cam].lights[l];
+    const LightDev& L = C.lights[C.cams[cam].light_base + (uint32_t)l];
     if (L.type == CRT_LIGHT_POINT) {
         // src/crt/lights/point_light.hpp:32-39
         const float3 dd = make_float3(L.pos[0] - p.x, L.pos[1] - p.y, L.pos[2] - p.z);
@@ -881,15 +882,41 @@ void launch_pass(const Ctx& C, uint64_t cap_paths, uint32_t cap_fold, const crt_
 // Grow-only device buffers of one render caller (a scene's default
 // workspace, or one per serving worker so that workers never allocate).
 struct RenderWorkspace {
-    DevBuf d_cams, d_segs, d_ctl, d_plist, mean, spp, active, counters, tally, pstate, q0, q1, live, shadow, samples,
-        o_rgba, o_u8;
+    DevBuf d_cams, d_lights, d_segs, d_ctl, d_plist, mean, spp, active, counters, tally, pstate, q0, q1, live, shadow,
+        samples, o_rgba, o_u8;
+    size_t bytes() const {
+        return d_cams.bytes + d_lights.bytes + d_segs.bytes + d_ctl.bytes + d_plist.bytes + mean.bytes + spp.bytes +
+               active.bytes + counters.bytes + tally.bytes + pstate.bytes + q0.bytes + q1.bytes + live.bytes +
+               shadow.bytes + samples.bytes + o_rgba.bytes + o_u8.bytes;
+    }
 };
 
 RenderWorkspace* new_render_workspace() { return new RenderWorkspace(); }
 void delete_render_workspace(RenderWorkspace* w) { delete w; }
 
+size_t render_workspace_bytes(const RenderWorkspace* w) { return w ? w->bytes() : 0; }
+
+void check_light(const crt_light& L) {
+    CRT_REQUIRE(L.type >= 0 && L.type <= 2, CRT_ERR_TYPE, "unknown light type");
+    bool fin = std::isfinite(L.intensity) && std::isfinite(L.size[0]) && std::isfinite(L.size[1]);
+    for (int k = 0; k < 3; ++k) fin = fin && std::isfinite(L.position[k]) && std::isfinite(L.direction[k]);
+    for (int k = 0; k < 9; ++k) fin = fin && std::isfinite(L.rotation[k]);
+    CRT_REQUIRE(fin, CRT_ERR_INVALID, "light parameters must be finite");
+    if (L.type == CRT_LIGHT_DIRECTIONAL)
+        CRT_REQUIRE(L.direction[0] != 0.0 || L.direction[1] != 0.0 || L.direction[2] != 0.0, CRT_ERR_INVALID,
+                    "directional light needs a non-zero direction");
+}
+
+// Paths per pass: 2^23, fewer with many lights so that the shadow records
+// (48 B per path and light) stay within 4 GiB; a camera's pixels always fit.
+uint64_t pass_budget(int n_lights) {
+    const uint64_t per = (uint64_t)std::max(n_lights, 1) * sizeof(ShadowRec);
+    return std::max<uint64_t>((uint64_t)1 << 16, std::min<uint64_t>((uint64_t)1 << 23, ((uint64_t)4 << 30) / per));
+}
+
 size_t render_bytes_estimate(uint64_t pixels, int spp, int n_lights) {
-    const uint64_t paths = std::min<uint64_t>(pixels * (uint64_t)std::max(spp, 1), (uint64_t)1 << 23);
+    const uint64_t paths = std::max<uint64_t>(pixels, std::min<uint64_t>(pixels * (uint64_t)std::max(spp, 1),
+                                                                         pass_budget(n_lights)));
     return (size_t)(paths * (PATH_BYTES + 12 + 16 + std::max(n_lights, 1) * sizeof(ShadowRec)) +
                     pixels * (16 + 4 + 1 + 4 + 16 + 4));
 }
@@ -907,7 +934,7 @@ int render_batch(crt_scene* scene, uint32_t n_cams, const crt_camera* cams, cons
     CRT_REQUIRE(scene != nullptr, CRT_ERR_INVALID, "scene is NULL");
     CRT_REQUIRE(n_cams >= 1 && cams != nullptr, CRT_ERR_INVALID, "no cameras");
     CRT_REQUIRE(opts != nullptr, CRT_ERR_INVALID, "render options are NULL");
-    CRT_REQUIRE(n_lights <= (uint32_t)MAX_LIGHTS, CRT_ERR_INVALID, "at most 8 lights per render");
+    CRT_REQUIRE(n_lights <= (1u << 20), CRT_ERR_INVALID, "too many lights");
     CRT_REQUIRE(n_lights == 0 || lights != nullptr, CRT_ERR_INVALID, "lights is NULL");
     CRT_REQUIRE(opts->num_bounces >= 0, CRT_ERR_INVALID, "num_bounces must be >= 0");
     crt_batch_opts B{};
@@ -959,11 +986,15 @@ int render_batch(crt_scene* scene, uint32_t n_cams, const crt_camera* cams, cons
     Ctx C;
     std::memset(&C, 0, sizeof(C));
     C.s = scene->view;
-    for (uint32_t c = 0; c < n_cams; ++c) {
+    // lights: one slice per camera (or one shared slice), in the scene frame
+    const uint32_t n_slices = per_camera_lights ? n_cams : 1u;
+    std::vector<LightDev> hl((size_t)n_slices * n_lights);
+    for (uint32_t c = 0; c < n_cams; ++c) hc[c].light_base = per_camera_lights ? c * n_lights : 0u;
+    for (uint32_t c = 0; c < n_slices; ++c) {
         for (uint32_t l = 0; l < n_lights; ++l) {
-            const crt_light& L = lights[(per_camera_lights ? c * n_lights : 0u) + l];
-            CRT_REQUIRE(L.type >= 0 && L.type <= 2, CRT_ERR_TYPE, "unknown light type");
-            LightDev& D = hc[c].lights[l];
+            const crt_light& L = lights[c * n_lights + l];
+            check_light(L);
+            LightDev& D = hl[(size_t)c * n_lights + l];
             D.type = L.type;
             D.intensity = (float)L.intensity;
             for (int k = 0; k < 3; ++k) D.pos[k] = (float)(L.position[k] - scene->view.origin[k]);   // scene frame
@@ -986,7 +1017,7 @@ int render_batch(crt_scene* scene, uint32_t n_cams, const crt_camera* cams, cons
     C.grab = B.grab;
 
     // ---- pass planning ----
-    const uint64_t budget = (uint64_t)1 << 23;   // paths per pass
+    const uint64_t budget = pass_budget(C.nl);   // paths per pass
     int S = (int)B.samples_per_pass;
     if (S <= 0) {
         int smax = 1;
@@ -1000,6 +1031,7 @@ int render_batch(crt_scene* scene, uint32_t n_cams, const crt_camera* cams, cons
            &spp = W.spp, &active = W.active, &counters = W.counters, &tally = W.tally, &pstate = W.pstate,
            &q0 = W.q0, &q1 = W.q1, &live = W.live, &shadow = W.shadow, &samples = W.samples;
     d_cams.reserve(sizeof(CamDev) * n_cams);
+    W.d_lights.reserve(std::max<size_t>(hl.size(), 1) * sizeof(LightDev));
     d_segs.reserve(sizeof(Segment) * (n_cams + 1));
     d_ctl.reserve(sizeof(PassCtl));
     d_plist.reserve((size_t)P * 4);
@@ -1015,11 +1047,14 @@ int render_batch(crt_scene* scene, uint32_t n_cams, const crt_camera* cams, cons
     tally.reserve(16);
     samples.reserve(cap_paths * 16);
     CRT_HIP_CHECK(hipMemcpyAsync(d_cams.p, hc.data(), sizeof(CamDev) * n_cams, hipMemcpyHostToDevice, st));
+    if (!hl.empty())
+        CRT_HIP_CHECK(hipMemcpyAsync(W.d_lights.p, hl.data(), hl.size() * sizeof(LightDev), hipMemcpyHostToDevice, st));
     CRT_HIP_CHECK(hipMemsetAsync(mean.p, 0, (size_t)P * 16, st));
     CRT_HIP_CHECK(hipMemsetAsync(spp.p, 0, (size_t)P * 4, st));
     CRT_HIP_CHECK(hipMemsetAsync(tally.p, 0, 16, st));
     hipLaunchKernelGGL(k_fill_u8, dim3(std::max(1u, cdiv(P, TB))), dim3(TB), 0, st, active.as<uint8_t>(), P, (uint8_t)1);
     C.cams = d_cams.as<CamDev>();
+    C.lights = W.d_lights.as<LightDev>();
     C.segs = d_segs.as<Segment>();
     C.ctl = d_ctl.as<PassCtl>();
     C.plist = d_plist.as<uint32_t>();

@@ -107,13 +107,14 @@ __device__ inline void flash_dir(const FlashParams& F, uint32_t beam, uint32_t k
 // in consecutive lanes of one wave (64 / per beams per wave, the remaining
 // lanes idle), so the returns are reduced across lanes and the per-ray ranges
 // never leave registers.  No early exit: every lane takes part in the shuffles.
-CRT_LANE_KERNEL void k_flash(SceneView s, FlashParams F, uint32_t n_beams, double* __restrict__ single,
-                                                       double* __restrict__ first, double* __restrict__ last,
-                                                       uint8_t* __restrict__ n_returns) {
-    const LaneStack stk = lane_stack(s);
+// `block` is the scan's own block index (one wave per block).
+__device__ inline void flash_block(const SceneView& s, const FlashParams& F, uint32_t n_beams, uint32_t block,
+                                   double* __restrict__ single, double* __restrict__ first,
+                                   double* __restrict__ last, uint8_t* __restrict__ n_returns,
+                                   const LaneStack& stk) {
     const uint32_t per = F.sub_rays + 1, groups = LANE_BLOCK / per;
     const uint32_t lane = threadIdx.x, g = lane / per, k = lane - g * per;
-    const uint64_t beam64 = (uint64_t)blockIdx.x * groups + g;
+    const uint64_t beam64 = (uint64_t)block * groups + g;
     const bool live = g < groups && beam64 < n_beams;
     const uint32_t beam = (uint32_t)beam64;
     double rg = 0.0;
@@ -142,6 +143,38 @@ CRT_LANE_KERNEL void k_flash(SceneView s, FlashParams F, uint32_t n_beams, doubl
     }
 }
 
+CRT_LANE_KERNEL void k_flash(SceneView s, FlashParams F, uint32_t n_beams, double* __restrict__ single,
+                             double* __restrict__ first, double* __restrict__ last, uint8_t* __restrict__ n_returns) {
+    flash_block(s, F, n_beams, blockIdx.x, single, first, last, n_returns, lane_stack(s));
+}
+
+// A batch of scans in one launch: scan i owns blocks [block_begin, +blocks)
+// and computes exactly what its own k_flash launch would.
+struct FlashSeg {
+    FlashParams F;
+    uint32_t block_begin, n_beams;
+    double *single, *first, *last;
+    uint8_t* n_returns;
+};
+
+// largest i < n with key(i) <= x, wave-uniform (scalar loads)
+template <typename F>
+__device__ inline uint32_t uniform_search(uint32_t n, uint32_t x, F key) {
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (key(mid) <= x) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+CRT_LANE_KERNEL void k_flash_multi(SceneView s, const FlashSeg* __restrict__ segs, uint32_t nseg) {
+    const uint32_t i = uniform_search(nseg, blockIdx.x, [&](uint32_t j) { return segs[j].block_begin; });
+    const FlashSeg& g = segs[i];
+    flash_block(s, g.F, g.n_beams, blockIdx.x - g.block_begin, g.single, g.first, g.last, g.n_returns,
+                lane_stack(s));
+}
+
 // ---- radiation pressure ----
 struct Comp {   // compensated FP64 accumulator (value, error)
     double s, c;
@@ -166,14 +199,15 @@ struct FluxParams {
 
 // Block partials: one wave of 64 rays per block; 7 sums each (Fx,Fy,Fz,
 // Tx,Ty,Tz, hits), reduced across the wave with shuffles, then combined
-// across blocks with TwoSum compensation.
-CRT_LANE_KERNEL void k_flux(SceneView s, FluxParams F, double* __restrict__ partial, uint32_t nblocks) {
-    const LaneStack stk = lane_stack(s);
+// across blocks with TwoSum compensation.  `block` is the request's own
+// block index; partials are component-major over its nblocks.
+__device__ inline void flux_block(const SceneView& s, const FluxParams& F, double* __restrict__ partial,
+                                  uint32_t nblocks, uint32_t block, const LaneStack& stk) {
     // one 8 x 8 tile of the grid per wave (tiles row-major): a wave's
     // parallel rays cover a compact patch and share BVH nodes
     const uint32_t tiles = (F.grid + 7u) / 8u;
-    const uint32_t i = (blockIdx.x % tiles) * 8u + (threadIdx.x & 7u);
-    const uint32_t j = (blockIdx.x / tiles) * 8u + (threadIdx.x >> 3);
+    const uint32_t i = (block % tiles) * 8u + (threadIdx.x & 7u);
+    const uint32_t j = (block / tiles) * 8u + (threadIdx.x >> 3);
     double term[7] = {0, 0, 0, 0, 0, 0, 0};
     if (i < F.grid && j < F.grid) {
         const double x = -F.radius + (i + 0.5) * F.h, y = -F.radius + (j + 0.5) * F.h;
@@ -222,8 +256,20 @@ CRT_LANE_KERNEL void k_flux(SceneView s, FluxParams F, double* __restrict__ part
     for (int w = 32; w > 0; w >>= 1)
         for (int k = 0; k < 7; ++k) acc[k] += __shfl_xor(acc[k], w, 64);
     if (threadIdx.x == 0)
-        for (int k = 0; k < 7; ++k) partial[(size_t)k * nblocks + blockIdx.x] = acc[k];   // component-major
+        for (int k = 0; k < 7; ++k) partial[(size_t)k * nblocks + block] = acc[k];   // component-major
 }
+
+CRT_LANE_KERNEL void k_flux(SceneView s, FluxParams F, double* __restrict__ partial, uint32_t nblocks) {
+    flux_block(s, F, partial, nblocks, blockIdx.x, lane_stack(s));
+}
+
+// A batch of flux grids in one launch: request i owns blocks
+// [block_begin, +nblocks) and partials at part_off (7 x nblocks doubles).
+struct FluxSeg {
+    FluxParams F;
+    uint32_t block_begin, nblocks;
+    uint64_t part_off;
+};
 
 // Cross-block reduction in two compensated (TwoSum) stages: FLUX_SLICES
 // blocks per component each sum a contiguous slice of the block partials,
@@ -244,17 +290,21 @@ __device__ inline Comp block_comp_sum(Comp acc, Comp* red) {
     return red[0];
 }
 
-__global__ __launch_bounds__(FLUX_SLICE_THREADS) void k_flux_slices(const double* __restrict__ partial,
-                                                                    uint32_t nblocks, Comp* __restrict__ slices) {
-    __shared__ Comp red[FLUX_SLICE_THREADS];
-    const int k = blockIdx.y;   // component 0..6
+__device__ inline void flux_slice(const double* __restrict__ partial, uint32_t nblocks, int k, uint32_t slice,
+                                  Comp* __restrict__ slices, Comp* red) {
     const uint32_t per = (nblocks + FLUX_SLICES - 1) / FLUX_SLICES;
-    const uint32_t lo = blockIdx.x * per, hi = min(lo + per, nblocks);
+    const uint32_t lo = slice * per, hi = min(lo + per, nblocks);
     Comp acc{0.0, 0.0};
     for (uint32_t b = lo + threadIdx.x; b < hi; b += FLUX_SLICE_THREADS)
         acc = comp_add(acc, Comp{partial[(size_t)k * nblocks + b], 0.0});
     const Comp r = block_comp_sum(acc, red);
-    if (threadIdx.x == 0) slices[k * FLUX_SLICES + blockIdx.x] = r;
+    if (threadIdx.x == 0) slices[k * FLUX_SLICES + slice] = r;
+}
+
+__global__ __launch_bounds__(FLUX_SLICE_THREADS) void k_flux_slices(const double* __restrict__ partial,
+                                                                    uint32_t nblocks, Comp* __restrict__ slices) {
+    __shared__ Comp red[FLUX_SLICE_THREADS];
+    flux_slice(partial, nblocks, (int)blockIdx.y, blockIdx.x, slices, red);   // component blockIdx.y
 }
 
 __global__ __launch_bounds__(FLUX_SLICES) void k_flux_final(const Comp* __restrict__ slices,
@@ -263,6 +313,33 @@ __global__ __launch_bounds__(FLUX_SLICES) void k_flux_final(const Comp* __restri
     const int k = blockIdx.x;   // component 0..6
     const Comp r = block_comp_sum(slices[k * FLUX_SLICES + threadIdx.x], red);
     if (threadIdx.x == 0) out[k] = r.s + r.c;
+}
+
+// Batched forms: the same per-request reduction tree (FLUX_SLICES slices per
+// component, then one combine), so each request's result is bit-identical
+// to its own k_flux / k_flux_slices / k_flux_final launches.
+CRT_LANE_KERNEL void k_flux_multi(SceneView s, const FluxSeg* __restrict__ segs, uint32_t nseg,
+                                  double* __restrict__ partial) {
+    const uint32_t i = uniform_search(nseg, blockIdx.x, [&](uint32_t j) { return segs[j].block_begin; });
+    const FluxSeg& g = segs[i];
+    flux_block(s, g.F, partial + g.part_off, g.nblocks, blockIdx.x - g.block_begin, lane_stack(s));
+}
+
+__global__ __launch_bounds__(FLUX_SLICE_THREADS) void k_flux_slices_multi(const FluxSeg* __restrict__ segs,
+                                                                          const double* __restrict__ partial,
+                                                                          Comp* __restrict__ slices) {
+    __shared__ Comp red[FLUX_SLICE_THREADS];
+    const uint32_t r = blockIdx.x / FLUX_SLICES;
+    flux_slice(partial + segs[r].part_off, segs[r].nblocks, (int)blockIdx.y, blockIdx.x % FLUX_SLICES,
+               slices + (size_t)r * 7 * FLUX_SLICES, red);
+}
+
+__global__ __launch_bounds__(FLUX_SLICES) void k_flux_final_multi(const Comp* __restrict__ slices,
+                                                                  double* __restrict__ out) {
+    __shared__ Comp red[FLUX_SLICES];
+    const uint32_t r = blockIdx.x / 7, k = blockIdx.x % 7;
+    const Comp c = block_comp_sum(slices[((size_t)r * 7 + k) * FLUX_SLICES + threadIdx.x], red);
+    if (threadIdx.x == 0) out[7 * r + k] = c.s + c.c;
 }
 
 // Events only when the caller asked for timing (see EventTimer in trace.hip).
@@ -377,10 +454,8 @@ namespace crt {
 void flash_lidar_impl(crt_scene* scene, const crt_flash_opts* opts, double* single, double* first, double* last,
                       uint8_t* n_returns, uint32_t flags, void* stream, crt_timing* timing, Workspace& ws) {
     {
-        CRT_REQUIRE(scene != nullptr && opts != nullptr, CRT_ERR_INVALID, "NULL argument");
-        check_camera_args(&opts->beams);
-        CRT_REQUIRE(opts->sub_rays < (uint32_t)LANE_BLOCK, CRT_ERR_INVALID, "at most 63 sub-rays per beam");
-        CRT_REQUIRE(opts->sigma >= 0.0 && std::isfinite(opts->sigma), CRT_ERR_INVALID, "sigma must be >= 0");
+        CRT_REQUIRE(scene != nullptr, CRT_ERR_INVALID, "NULL argument");
+        check_flash_opts(opts);
         hipStream_t st = (hipStream_t)stream;
         FlashParams F;
         F.cam = make_cam_view(opts->beams, scene);
@@ -507,6 +582,94 @@ void radiation_pressure_impl(crt_scene* scene, const crt_flux_opts* opts, double
         }
     }
 }
+
+void check_flash_opts(const crt_flash_opts* opts) {
+    CRT_REQUIRE(opts != nullptr, CRT_ERR_INVALID, "NULL argument");
+    check_camera_args(&opts->beams);
+    CRT_REQUIRE(opts->sub_rays < (uint32_t)LANE_BLOCK, CRT_ERR_INVALID, "at most 63 sub-rays per beam");
+    CRT_REQUIRE(opts->sigma >= 0.0 && std::isfinite(opts->sigma), CRT_ERR_INVALID, "sigma must be >= 0");
+    const uint64_t nb = (uint64_t)std::floor(opts->beams.resolution[0]) * (uint64_t)std::floor(opts->beams.resolution[1]);
+    CRT_REQUIRE(nb * (opts->sub_rays + 1) < (1ull << 32), CRT_ERR_INVALID, "flash LiDAR too large");
+}
+
+void check_flux_opts(const crt_flux_opts* opts) {
+    CRT_REQUIRE(opts != nullptr, CRT_ERR_INVALID, "NULL argument");
+    CRT_REQUIRE(opts->grid >= 1 && opts->grid <= 65536, CRT_ERR_INVALID, "grid must be in 1..65536");
+    const double sl = std::sqrt(opts->sun_dir[0] * opts->sun_dir[0] + opts->sun_dir[1] * opts->sun_dir[1] +
+                                opts->sun_dir[2] * opts->sun_dir[2]);
+    CRT_REQUIRE(sl > 0 && std::isfinite(sl), CRT_ERR_INVALID, "sun_dir must be a finite non-zero vector");
+    bool fin = std::isfinite(opts->pressure) && std::isfinite(opts->rho_s) && std::isfinite(opts->rho_d) &&
+               std::isfinite(opts->radius);
+    for (int k = 0; k < 3; ++k) fin = fin && std::isfinite(opts->center[k]) && std::isfinite(opts->ref_point[k]);
+    CRT_REQUIRE(fin, CRT_ERR_INVALID, "flux options must be finite");
+}
+
+// Several flash-LiDAR scans of one scene in one launch; outputs (single /
+// first / last per scan, each beams long) are device pointers.  The device
+// segment table lives in ws.e; `table` keeps its host copy alive until the
+// caller synchronises the stream.
+void flash_lidar_multi(crt_scene* scene, uint32_t n, const crt_flash_opts* opts, double* const* outs,
+                       hipStream_t st, Workspace& ws, std::vector<uint8_t>& table_bytes) {
+    table_bytes.resize((size_t)n * sizeof(FlashSeg));
+    FlashSeg* table = reinterpret_cast<FlashSeg*>(table_bytes.data());
+    uint32_t blocks = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        check_flash_opts(&opts[i]);
+        FlashSeg& g = table[i];
+        g.F.cam = make_cam_view(opts[i].beams, scene);
+        g.F.sub_rays = opts[i].sub_rays;
+        g.F.sigma = opts[i].sigma;
+        g.F.seed = opts[i].seed;
+        g.n_beams = (uint32_t)((uint64_t)g.F.cam.w * g.F.cam.h);
+        g.block_begin = blocks;
+        g.single = outs[3 * i];
+        g.first = outs[3 * i + 1];
+        g.last = outs[3 * i + 2];
+        g.n_returns = nullptr;
+        const uint64_t nb = cdiv(g.n_beams, LANE_BLOCK / (g.F.sub_rays + 1));
+        CRT_REQUIRE((uint64_t)blocks + nb < (1ull << 31), CRT_ERR_INVALID, "LiDAR batch too large");
+        blocks += (uint32_t)nb;
+    }
+    if (blocks == 0) return;
+    ws.e.reserve(table_bytes.size());
+    CRT_HIP_CHECK(hipMemcpyAsync(ws.e.p, table, table_bytes.size(), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_flash_multi, dim3(blocks), dim3(LANE_BLOCK), lane_lds_bytes(scene->view), st, scene->view,
+                       ws.e.as<FlashSeg>(), n);
+    CRT_HIP_CHECK(hipGetLastError());
+}
+
+// Several flux grids of one scene in one launch sequence; out7 (device, 7 x n
+// doubles) receives force xyz, torque xyz and hits per request.
+void radiation_pressure_multi(crt_scene* scene, uint32_t n, const crt_flux_opts* opts, double* out7,
+                              hipStream_t st, Workspace& ws, std::vector<uint8_t>& table_bytes) {
+    table_bytes.resize((size_t)n * sizeof(FluxSeg));
+    FluxSeg* table = reinterpret_cast<FluxSeg*>(table_bytes.data());
+    uint64_t blocks = 0, parts = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        check_flux_opts(&opts[i]);
+        FluxSeg& g = table[i];
+        g.F = flux_params(scene, &opts[i]);
+        g.nblocks = flux_blocks(g.F);
+        g.block_begin = (uint32_t)blocks;
+        g.part_off = parts;
+        blocks += g.nblocks;
+        parts += 7ull * g.nblocks;
+        CRT_REQUIRE(blocks < (1ull << 31), CRT_ERR_INVALID, "flux batch too large");
+    }
+    if (n == 0) return;
+    ws.e.reserve(table_bytes.size());
+    ws.h.reserve((size_t)parts * sizeof(double) + (size_t)n * 7 * FLUX_SLICES * sizeof(Comp));
+    double* part = ws.h.as<double>();
+    Comp* slices = reinterpret_cast<Comp*>(part + parts);
+    CRT_HIP_CHECK(hipMemcpyAsync(ws.e.p, table, table_bytes.size(), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_flux_multi, dim3((uint32_t)blocks), dim3(LANE_BLOCK), lane_lds_bytes(scene->view), st,
+                       scene->view, ws.e.as<FluxSeg>(), n, part);
+    hipLaunchKernelGGL(k_flux_slices_multi, dim3(FLUX_SLICES * n, 7), dim3(FLUX_SLICE_THREADS), 0, st,
+                       ws.e.as<FluxSeg>(), part, slices);
+    hipLaunchKernelGGL(k_flux_final_multi, dim3(7 * n), dim3(FLUX_SLICES), 0, st, slices, out7);
+    CRT_HIP_CHECK(hipGetLastError());
+}
+
 }  // namespace crt
 
 extern "C" {

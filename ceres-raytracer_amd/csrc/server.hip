@@ -14,9 +14,16 @@
 //     join the next batch as soon as a worker is free, while the other
 //     workers' batches are still on the device: continuous batching.
 //   * renders of a batch go through render_batch (one set of launches for all
-//     cameras); the flash-LiDAR scans of a batch are enqueued back to back
-//     with one synchronisation; flux requests run back to back.  Images and
-//     scans return through the worker's pinned staging buffer.
+//     cameras); the flash-LiDAR scans of a batch run as one launch over a
+//     segment table (flash_lidar_multi), its flux grids as one launch
+//     sequence (radiation_pressure_multi); one synchronisation per batch.
+//     Images, scans and forces return through the worker's pinned staging.
+//   * every option is validated at submission; a batch that still fails at
+//     run time is rerun request by request, so an error reaches only its own
+//     request; the scene of a request is resolved at submission.
+//   * the memory budget covers the admitted requests and the device
+//     workspaces the workers keep between batches (a worker frees its
+//     workspaces when keeping them would not let the next batch fit).
 // Every result is a pure function of its request (counter-based RNG keyed by
 // the request seed), so it is the same alone and in any mix.
 // `batching = 0` gives the plain baseline: one worker, one request per
@@ -42,6 +49,8 @@ using Clock = std::chrono::steady_clock;
 struct Job {
     uint64_t ticket;
     crt_request req;
+    crt_scene* scene = nullptr;       // resolved at submission
+    std::vector<crt_light> lights;    // the request's lights (copied at submission)
     size_t mem;
     Clock::time_point submitted, done_at;
     bool done = false;
@@ -85,21 +94,46 @@ size_t request_bytes(const crt_request& r) {
         case CRT_REQ_RENDER:
             return render_bytes_estimate(px, r.spp, (int)r.n_lights) + px * 16;
         case CRT_REQ_LIDAR:
-            return (size_t)px * 24;   // per-beam outputs; sub-ray ranges stay in registers
+            return (size_t)px * 24 + 512;   // per-beam outputs; sub-ray ranges stay in registers
         case CRT_REQ_FLUX: {
             const uint64_t n = (uint64_t)r.flux.grid * r.flux.grid;
-            return (size_t)(n / LANE_BLOCK + 1) * 7 * 16 + 256;
+            return (size_t)(n / LANE_BLOCK + 1) * 7 * 8 + 7 * 64 * 16 + 512;
         }
         default:
             return 0;
     }
 }
 
+// Everything a request's batch would check, at submission (raises).
+void validate_request(const crt_request& r, uint32_t n_scenes) {
+    CRT_REQUIRE(r.kind <= CRT_REQ_FLUX, CRT_ERR_TYPE, "unknown request kind");
+    CRT_REQUIRE(r.scene < n_scenes, CRT_ERR_INVALID, "unknown scene id");
+    if (r.kind == CRT_REQ_RENDER) {
+        check_camera_args(&r.camera);
+        CRT_REQUIRE(r.spp >= 1 && r.bounces >= 0, CRT_ERR_INVALID, "bad render request (spp >= 1, bounces >= 0)");
+        CRT_REQUIRE(r.light_list != nullptr || r.n_lights <= 2, CRT_ERR_INVALID,
+                    "more than 2 lights need light_list");
+        CRT_REQUIRE(r.n_lights <= (1u << 20), CRT_ERR_INVALID, "too many lights");
+        const crt_light* L = r.light_list ? r.light_list : r.lights;
+        for (uint32_t l = 0; l < r.n_lights; ++l) check_light(L[l]);
+        const uint64_t px = (uint64_t)std::floor(r.camera.resolution[0]) * (uint64_t)std::floor(r.camera.resolution[1]);
+        CRT_REQUIRE(px < (1ull << 31), CRT_ERR_INVALID, "camera too large");
+    } else if (r.kind == CRT_REQ_LIDAR) {
+        crt_flash_opts fo{};
+        fo.beams = r.camera;
+        fo.sub_rays = r.sub_rays;
+        fo.sigma = r.sigma;
+        check_flash_opts(&fo);
+    } else {
+        check_flux_opts(&r.flux);
+    }
+}
+
 // Renders batch together when they share the scene, the bounce count and
 // the number of lights (every camera carries its own lights).
-bool same_render_setup(const crt_request& a, const crt_request& b) {
-    if (a.kind != CRT_REQ_RENDER || b.kind != CRT_REQ_RENDER) return false;
-    return a.scene == b.scene && a.bounces == b.bounces && a.n_lights == b.n_lights;
+bool same_render_setup(const Job& a, const Job& b) {
+    if (a.req.kind != CRT_REQ_RENDER || b.req.kind != CRT_REQ_RENDER) return false;
+    return a.scene == b.scene && a.req.bounces == b.req.bounces && a.req.n_lights == b.req.n_lights;
 }
 
 }  // namespace
@@ -113,10 +147,25 @@ struct crt_server {
     std::deque<std::shared_ptr<crt::Job>> queue;
     std::unordered_map<uint64_t, std::shared_ptr<crt::Job>> jobs;
     uint64_t next_ticket = 1;
-    size_t reserved = 0;
     bool stop = false;
     int device = 0;
     std::vector<std::thread> workers;
+    // per worker: its stream, device workspaces and pinned staging (another
+    // worker may free the workspaces of an idle worker, under `mu`), the
+    // device bytes they keep between batches and the reservation of the
+    // batch it runs (admission counts the max of the two)
+    struct Worker {
+        hipStream_t st = nullptr;
+        crt::Workspace ws;
+        crt::RenderWorkspace* rws = nullptr;
+        crt::PinnedBuf pin;
+        ~Worker() {
+            if (rws) crt::delete_render_workspace(rws);
+            if (st) (void)hipStreamDestroy(st);
+        }
+    };
+    std::vector<std::unique_ptr<Worker>> wstate;
+    std::vector<size_t> held, running;
     // statistics
     std::vector<double> latencies_ms;
     uint64_t completed = 0, rejected = 0, rays = 0, batches = 0, inflight = 0;
@@ -127,10 +176,10 @@ struct crt_server {
 namespace crt {
 namespace {
 
-void run_batch(crt_server* S, std::vector<std::shared_ptr<Job>>& batch, hipStream_t st, Workspace& ws,
-               RenderWorkspace* rws, PinnedBuf& pin) {
+void run_batch(std::vector<std::shared_ptr<Job>>& batch, hipStream_t st, Workspace& ws, RenderWorkspace* rws,
+               PinnedBuf& pin, const crt_batch_opts& ropts) {
     const crt_request& head = batch[0]->req;
-    crt_scene* scene = S->scenes[head.scene];
+    crt_scene* scene = batch[0]->scene;
     const auto pixels = [](const crt_request& r) {
         return (uint64_t)std::floor(r.camera.resolution[0]) * (uint64_t)std::floor(r.camera.resolution[1]);
     };
@@ -144,7 +193,7 @@ void run_batch(crt_server* S, std::vector<std::shared_ptr<Job>>& batch, hipStrea
             cams.push_back(j->req.camera);
             spp.push_back(j->req.spp);
             seeds.push_back(j->req.seed);
-            for (uint32_t l = 0; l < head.n_lights; ++l) lights.push_back(j->req.lights[l]);
+            lights.insert(lights.end(), j->lights.begin(), j->lights.end());
             total += pixels(j->req);
         }
         crt_render_opts o{};
@@ -152,15 +201,16 @@ void run_batch(crt_server* S, std::vector<std::shared_ptr<Job>>& batch, hipStrea
         o.max_samples = 1;
         o.noise_threshold = 0.0;
         o.num_bounces = head.bounces;
-        crt_batch_opts b = S->opts.render;
+        crt_batch_opts b = ropts;
         // the batch's images (packed in camera order) stay on the device,
         // then come back through the pinned staging in one copy
         ws.a.reserve((size_t)total * 16);
         float* d_rgba = ws.a.as<float>();
         crt_timing tm{};
-        render_batch(scene, (uint32_t)batch.size(), cams.data(), spp.data(), seeds.data(),
-                     lights.empty() ? nullptr : lights.data(), head.n_lights, true, &o, &b, d_rgba, nullptr,
-                     nullptr, CRT_OUT_DEVICE, st, &tm, rws);
+        const int rc = render_batch(scene, (uint32_t)batch.size(), cams.data(), spp.data(), seeds.data(),
+                                    lights.empty() ? nullptr : lights.data(), head.n_lights, true, &o, &b, d_rgba,
+                                    nullptr, nullptr, CRT_OUT_DEVICE, st, &tm, rws);
+        if (rc != CRT_OK) throw Error(rc, g_last_error);
         float* h = pin.get<float>((size_t)total * 16);
         CRT_HIP_CHECK(hipMemcpyAsync(h, d_rgba, (size_t)total * 16, hipMemcpyDeviceToHost, st));
         CRT_HIP_CHECK(hipStreamSynchronize(st));
@@ -172,26 +222,32 @@ void run_batch(crt_server* S, std::vector<std::shared_ptr<Job>>& batch, hipStrea
             off += px;
         }
     } else if (head.kind == CRT_REQ_LIDAR) {
-        // every scan of the batch is enqueued (device outputs single / first
-        // / last, 3 x px doubles per request, back to back), one copy back
+        // every scan of the batch in one launch (single / first / last, 3 x px
+        // doubles per request, back to back), one copy back
         uint64_t total = 0;
         for (auto& j : batch) total += pixels(j->req);
         ws.b.reserve((size_t)total * 24);
         double* d = ws.b.as<double>();
+        std::vector<crt_flash_opts> fo(batch.size());
+        std::vector<double*> outs(3 * batch.size());
         uint64_t off = 0;
-        for (auto& j : batch) {
-            const crt_request& r = j->req;
-            crt_flash_opts fo{};
-            fo.beams = r.camera;
-            fo.sub_rays = r.sub_rays;
-            fo.sigma = r.sigma;
-            fo.seed = r.seed;
+        for (size_t i = 0; i < batch.size(); ++i) {
+            const crt_request& r = batch[i]->req;
+            fo[i] = crt_flash_opts{};
+            fo[i].beams = r.camera;
+            fo[i].sub_rays = r.sub_rays;
+            fo[i].sigma = r.sigma;
+            fo[i].seed = r.seed;
             const uint64_t px = pixels(r);
             double* dr = d + 3 * off;
-            flash_lidar_impl(scene, &fo, dr, dr + px, dr + 2 * px, nullptr, CRT_OUT_DEVICE, st, nullptr, ws);
-            j->rays = px * (uint64_t)(r.sub_rays + 1);
+            outs[3 * i] = dr;
+            outs[3 * i + 1] = dr + px;
+            outs[3 * i + 2] = dr + 2 * px;
+            batch[i]->rays = px * (uint64_t)(r.sub_rays + 1);
             off += px;
         }
+        std::vector<uint8_t> table;
+        flash_lidar_multi(scene, (uint32_t)batch.size(), fo.data(), outs.data(), st, ws, table);
         double* h = pin.get<double>((size_t)total * 24);
         CRT_HIP_CHECK(hipMemcpyAsync(h, d, (size_t)total * 24, hipMemcpyDeviceToHost, st));
         CRT_HIP_CHECK(hipStreamSynchronize(st));
@@ -202,36 +258,61 @@ void run_batch(crt_server* S, std::vector<std::shared_ptr<Job>>& batch, hipStrea
             off += px;
         }
     } else {
-        for (auto& j : batch) {
-            crt_request& r = j->req;
-            crt_timing tm{};
-            double f[3], t[3];
-            radiation_pressure_impl(scene, &r.flux, f, t, nullptr, st, &tm, ws);
-            if (r.force_torque) {
-                std::memcpy(r.force_torque, f, sizeof(f));
-                std::memcpy(r.force_torque + 3, t, sizeof(t));
-            }
-            j->rays = tm.rays_traced + tm.shadow_rays;
+        // every flux grid of the batch in one launch sequence, one copy back
+        std::vector<crt_flux_opts> fo(batch.size());
+        for (size_t i = 0; i < batch.size(); ++i) {
+            fo[i] = batch[i]->req.flux;
+            batch[i]->rays = (uint64_t)fo[i].grid * fo[i].grid;
         }
+        ws.f.reserve(batch.size() * 7 * sizeof(double));
+        std::vector<uint8_t> table;
+        radiation_pressure_multi(scene, (uint32_t)batch.size(), fo.data(), ws.f.as<double>(), st, ws, table);
+        double* h = pin.get<double>(batch.size() * 7 * sizeof(double));
+        CRT_HIP_CHECK(hipMemcpyAsync(h, ws.f.p, batch.size() * 7 * sizeof(double), hipMemcpyDeviceToHost, st));
+        CRT_HIP_CHECK(hipStreamSynchronize(st));
+        for (size_t i = 0; i < batch.size(); ++i)
+            if (batch[i]->req.force_torque) std::memcpy(batch[i]->req.force_torque, h + 7 * i, 6 * sizeof(double));
     }
 }
 
+size_t workspace_bytes(const Workspace& ws, const RenderWorkspace* rws) {
+    return ws.a.bytes + ws.b.bytes + ws.c.bytes + ws.d.bytes + ws.e.bytes + ws.f.bytes + ws.g.bytes + ws.h.bytes +
+           ws.counters.bytes + render_workspace_bytes(rws);
+}
+
+// Admission of `need` bytes on worker w (under S->mu): the other workers'
+// footprints (a running one: max of what it keeps and its batch's
+// reservation; an idle one: what it keeps) plus w's own.  Returns 0 (does
+// not fit), 1 (fits keeping w's workspaces), 2 (fits once w frees them) or 3
+// (fits once w and the idle workers free theirs).
+int admit(const crt_server* S, int w, size_t need) {
+    size_t busy = 0, idle = 0;
+    for (size_t v = 0; v < S->held.size(); ++v) {
+        if ((int)v == w) continue;
+        if (S->running[v]) busy += std::max(S->held[v], S->running[v]);
+        else idle += S->held[v];
+    }
+    const size_t B = S->opts.memory_budget;
+    if (busy + idle + std::max(S->held[w], need) <= B) return 1;
+    if (busy + idle + need <= B) return 2;
+    if (busy + need <= B) return 3;
+    return 0;
+}
+
 void worker_main(crt_server* S, int index) {
-    (void)index;
     (void)hipSetDevice(S->device);
-    hipStream_t st = nullptr;
-    (void)hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
-    Workspace ws;
-    std::unique_ptr<RenderWorkspace, void (*)(RenderWorkspace*)> rws(new_render_workspace(), delete_render_workspace);
-    PinnedBuf pin;
+    crt_server::Worker& me = *S->wstate[index];
     const bool batching = S->opts.batching != 0;
+    const auto pixels = [](const crt_request& r) {
+        return (uint64_t)std::floor(r.camera.resolution[0]) * (uint64_t)std::floor(r.camera.resolution[1]);
+    };
     while (true) {
         std::vector<std::shared_ptr<Job>> batch;
         size_t mem = 0;
         {
             std::unique_lock<std::mutex> lk(S->mu);
             S->cv_work.wait(lk, [&] {
-                return S->stop || (!S->queue.empty() && S->reserved + S->queue.front()->mem <= S->opts.memory_budget);
+                return S->stop || (!S->queue.empty() && admit(S, index, S->queue.front()->mem) != 0);
             });
             if (S->stop && S->queue.empty()) break;
             if (S->queue.empty()) continue;
@@ -239,41 +320,22 @@ void worker_main(crt_server* S, int index) {
             S->queue.pop_front();
             batch.push_back(head);
             mem = head->mem;
-            if (batching && head->req.kind == CRT_REQ_RENDER) {
-                uint64_t paths = (uint64_t)std::floor(head->req.camera.resolution[0]) *
-                                 (uint64_t)std::floor(head->req.camera.resolution[1]) * head->req.spp;
-                for (auto it = S->queue.begin(); it != S->queue.end() && batch.size() < S->opts.max_batch;) {
-                    const crt_request& r = (*it)->req;
-                    const uint64_t p = (uint64_t)std::floor(r.camera.resolution[0]) *
-                                       (uint64_t)std::floor(r.camera.resolution[1]) * r.spp;
-                    if (same_render_setup(head->req, r) && paths + p <= S->opts.max_batch_paths &&
-                        S->reserved + mem + (*it)->mem <= S->opts.memory_budget) {
-                        paths += p;
-                        mem += (*it)->mem;
-                        batch.push_back(*it);
-                        it = S->queue.erase(it);
-                    } else {
-                        ++it;
-                    }
-                }
-            } else if (batching) {
-                // LiDAR / flux requests of the same scene and kind ride along;
-                // a LiDAR batch holds at most max_batch_paths rays (it shares
-                // one device buffer and one pinned staging copy)
-                const auto lidar_rays = [](const crt_request& r) {
-                    return r.kind == CRT_REQ_LIDAR ? (uint64_t)std::floor(r.camera.resolution[0]) *
-                                                         (uint64_t)std::floor(r.camera.resolution[1]) *
-                                                         (uint64_t)(r.sub_rays + 1)
-                                                   : 0u;
+            if (batching) {
+                // requests that share the head's launches join it, within the
+                // batch limits and the memory budget
+                const bool render = head->req.kind == CRT_REQ_RENDER;
+                const auto work = [&](const Job& j) {
+                    return render ? pixels(j.req) * (uint64_t)j.req.spp
+                                  : (j.req.kind == CRT_REQ_LIDAR ? pixels(j.req) * (uint64_t)(j.req.sub_rays + 1) : 0u);
                 };
-                uint64_t rays = lidar_rays(head->req);
+                uint64_t amount = work(*head);
                 for (auto it = S->queue.begin(); it != S->queue.end() && batch.size() < S->opts.max_batch;) {
-                    const crt_request& r = (*it)->req;
-                    if (r.kind == head->req.kind && r.scene == head->req.scene &&
-                        (r.kind != CRT_REQ_LIDAR || rays + lidar_rays(r) <= S->opts.max_batch_paths) &&
-                        S->reserved + mem + (*it)->mem <= S->opts.memory_budget) {
-                        rays += lidar_rays(r);
-                        mem += (*it)->mem;
+                    const Job& j = **it;
+                    const bool joins = render ? same_render_setup(*head, j)
+                                              : (j.req.kind == head->req.kind && j.scene == head->scene);
+                    if (joins && amount + work(j) <= S->opts.max_batch_paths && admit(S, index, mem + j.mem) != 0) {
+                        amount += work(j);
+                        mem += j.mem;
                         batch.push_back(*it);
                         it = S->queue.erase(it);
                     } else {
@@ -281,33 +343,65 @@ void worker_main(crt_server* S, int index) {
                     }
                 }
             }
-            S->reserved += mem;
+            const int a = admit(S, index, mem);
+            if (a >= 2) {   // free what this worker keeps (idle, so nothing of its stream is in flight)
+                me.ws = Workspace();
+                delete_render_workspace(me.rws);
+                me.rws = new_render_workspace();
+                S->held[index] = 0;
+            }
+            if (a == 3)
+                for (size_t v = 0; v < S->wstate.size(); ++v)
+                    if ((int)v != index && S->running[v] == 0 && S->held[v] > 0) {
+                        crt_server::Worker& o = *S->wstate[v];
+                        o.ws = Workspace();
+                        delete_render_workspace(o.rws);
+                        o.rws = new_render_workspace();
+                        S->held[v] = 0;
+                    }
+            S->running[index] = std::max<size_t>(mem, 1);
             S->inflight += batch.size();
         }
-        int status = CRT_OK;
-        std::string err;
-        try {
-            run_batch(S, batch, st, ws, rws.get(), pin);
-        } catch (const Error& e) {
-            status = e.code;
-            err = e.what();
-        } catch (const std::exception& e) {
-            status = CRT_ERR_HIP;
-            err = e.what();
-        }
+        // run the batch; if it fails, rerun its requests one by one so an
+        // error reaches only its own request
+        std::vector<int> status(batch.size(), CRT_OK);
+        std::vector<std::string> err(batch.size());
+        const auto run = [&](std::vector<std::shared_ptr<Job>>& b, size_t first) {
+            try {
+                run_batch(b, me.st, me.ws, me.rws, me.pin, S->opts.render);
+                return true;
+            } catch (const Error& e) {
+                for (size_t i = 0; i < b.size(); ++i) { status[first + i] = e.code; err[first + i] = e.what(); }
+            } catch (const std::exception& e) {
+                for (size_t i = 0; i < b.size(); ++i) { status[first + i] = CRT_ERR_HIP; err[first + i] = e.what(); }
+            }
+            (void)hipGetLastError();
+            (void)hipStreamSynchronize(me.st);
+            return false;
+        };
+        if (!run(batch, 0) && batch.size() > 1)
+            for (size_t i = 0; i < batch.size(); ++i) {
+                std::vector<std::shared_ptr<Job>> one{batch[i]};
+                status[i] = CRT_OK;
+                err[i].clear();
+                run(one, i);
+            }
+        const size_t keep = workspace_bytes(me.ws, me.rws);
         const auto now = Clock::now();
         {
             std::lock_guard<std::mutex> lk(S->mu);
-            S->reserved -= mem;
+            S->running[index] = 0;
+            S->held[index] = keep;
             S->inflight -= batch.size();
             ++S->batches;
-            for (auto& j : batch) {
+            for (size_t i = 0; i < batch.size(); ++i) {
+                auto& j = batch[i];
                 j->done = true;
                 j->done_at = now;
-                j->status = status;
-                j->error = err;
+                j->status = status[i];
+                j->error = err[i];
                 ++S->completed;
-                S->rays += j->rays;
+                if (status[i] == CRT_OK) S->rays += j->rays;
                 S->latencies_ms.push_back(std::chrono::duration<double, std::milli>(now - j->submitted).count());
             }
             S->last_done = now;
@@ -315,7 +409,6 @@ void worker_main(crt_server* S, int index) {
         S->cv_done.notify_all();
         S->cv_work.notify_all();
     }
-    if (st) (void)hipStreamDestroy(st);
 }
 
 }  // namespace
@@ -340,6 +433,14 @@ int crt_server_create(const crt_server_opts* opts, crt_server** out) {
             S->opts.max_batch = 1;
         }
         CRT_HIP_CHECK(hipGetDevice(&S->device));
+        for (uint32_t k = 0; k < S->opts.n_workers; ++k) {
+            std::unique_ptr<crt_server::Worker> w(new crt_server::Worker());
+            CRT_HIP_CHECK(hipStreamCreateWithFlags(&w->st, hipStreamNonBlocking));   // default priority
+            w->rws = new_render_workspace();
+            S->wstate.push_back(std::move(w));
+        }
+        S->held.assign(S->opts.n_workers, 0);
+        S->running.assign(S->opts.n_workers, 0);
         crt_server* raw = S.get();
         for (uint32_t k = 0; k < S->opts.n_workers; ++k) S->workers.emplace_back(worker_main, raw, (int)k);
         *out = S.release();
@@ -361,14 +462,19 @@ int crt_server_submit(crt_server* S, const crt_request* req, uint64_t* ticket) {
         CRT_REQUIRE(req->kind <= CRT_REQ_FLUX, CRT_ERR_TYPE, "unknown request kind");
         auto j = std::make_shared<Job>();
         j->req = *req;
+        if (req->kind == CRT_REQ_RENDER && req->n_lights) {
+            const crt_light* L = req->light_list ? req->light_list : req->lights;
+            CRT_REQUIRE(req->light_list != nullptr || req->n_lights <= 2, CRT_ERR_INVALID,
+                        "more than 2 lights need light_list");
+            j->lights.assign(L, L + req->n_lights);
+        }
+        j->req.light_list = nullptr;
         j->mem = request_bytes(*req);
         j->submitted = Clock::now();
         {
             std::lock_guard<std::mutex> lk(S->mu);
-            CRT_REQUIRE(req->scene < S->scenes.size(), CRT_ERR_INVALID, "unknown scene id");
-            if (req->kind != CRT_REQ_FLUX) check_camera_args(&req->camera);
-            if (req->kind == CRT_REQ_RENDER)
-                CRT_REQUIRE(req->spp >= 1 && req->n_lights <= 2, CRT_ERR_INVALID, "bad render request");
+            validate_request(*req, (uint32_t)S->scenes.size());
+            j->scene = S->scenes[req->scene];
             if (j->mem > S->opts.memory_budget) {
                 ++S->rejected;
                 throw Error(CRT_ERR_ADMISSION, "request needs more device memory than the server budget");

@@ -288,7 +288,8 @@ int crt_sun_pass(crt_scene* scene, const crt_camera* camera, const double sun_di
  * sampling (min/max samples, noise threshold) and the reference's weights.
  * Outputs (any may be NULL): rgba f32 [H][W][4] radiance (alpha 1), rgba_u8
  * [H][W][4] = trunc(clamp(256 x, 0, 255)) like the reference's uint8 image,
- * spp_used int32 [H][W].  At most 8 lights.                                */
+ * spp_used int32 [H][W].  Any number of lights (the paths of a pass shrink
+ * with many lights so their shadow records stay within 4 GiB).            */
 int crt_render(crt_scene* scene, const crt_camera* camera, const crt_light* lights, uint32_t n_lights,
                const crt_render_opts* opts, float* rgba, uint8_t* rgba_u8, int32_t* spp_used,
                uint32_t flags, void* stream, crt_timing* timing);
@@ -387,10 +388,16 @@ int crt_radiation_pressure_noalloc(crt_scene* scene, const crt_flux_opts* opts, 
 /* ---- serving (new) ---------------------------------------------------------
  * A server keeps scenes resident, admits requests by device-memory budget and
  * runs them on n_workers streams; each worker batches every queued request
- * that can share its launches (continuous batching).  batching = 0 is the
- * plain baseline: one worker, one request per launch sequence.  Outputs are
- * host buffers owned by the caller until crt_server_wait returns.
- * Results depend only on the request (its seed), never on the mix.           */
+ * that can share its launches (continuous batching: the renders of one scene
+ * and bounce count in one set of launches, a scene's flash-LiDAR scans in one
+ * launch, its flux grids in one launch sequence).  batching = 0 is the plain
+ * baseline: one worker, one request per launch sequence.  Outputs are host
+ * buffers owned by the caller until crt_server_wait returns.  Every option is
+ * validated at submission (bad requests are refused there, never inside a
+ * batch); a batch that fails at run time is rerun request by request, so a
+ * failure reaches only its own request.  The memory budget covers the
+ * admitted requests and the device workspaces the workers keep between
+ * batches.  Results depend only on the request (its seed), never on the mix. */
 #define CRT_REQ_RENDER 0u
 #define CRT_REQ_LIDAR  1u
 #define CRT_REQ_FLUX   2u
@@ -414,14 +421,15 @@ typedef struct {
     int32_t spp;                /* render samples per pixel                         */
     int32_t bounces;            /* render bounces                                   */
     uint64_t seed;              /* render / LiDAR RNG seed                          */
-    uint32_t n_lights;          /* render lights (<= 2)                             */
+    uint32_t n_lights;          /* render lights                                    */
     uint32_t sub_rays;          /* LiDAR sub-rays per beam                          */
-    crt_light lights[2];
+    crt_light lights[2];        /* the lights when light_list is NULL (n_lights <= 2) */
     double sigma;               /* LiDAR sub-ray angular sigma (rad)                */
     crt_flux_opts flux;         /* radiation-pressure request                       */
     float* rgba;                /* render output [H][W][4] f32                      */
     double* lidar;              /* LiDAR output: single, first, last, each [H][W]    */
     double* force_torque;       /* flux output: force xyz, torque xyz               */
+    const crt_light* light_list;/* n_lights lights, any number (copied at submission), or NULL */
 } crt_request;
 
 typedef struct {

@@ -82,3 +82,95 @@ def test_admission_rejects_oversized_request(meshes):
     assert out["rgba"].shape == (16, 16, 4) and lat > 0
     assert srv.stats()["rejected"] == 1
     srv.close()
+
+
+def test_invalid_requests_refused_at_submission(meshes):
+    # every option a batch would check is checked at submission: bad requests
+    # raise there and never share a batch with valid ones
+    ms, sc = meshes
+    srv = serving.Server(batching=True, n_workers=2)
+    for s in sc:
+        srv.add_scene(s)
+    cam = scenes.small_body_camera(ms[0], resolution=48, seed=2)
+    sun = {"type": "directional", "intensity": 1.0, "direction": scenes.sun_direction(2)}
+    good = [{"kind": "lidar", "scene": 0, "camera": cam, "sub_rays": 8, "sigma": 5e-4, "seed": 1},
+            {"kind": "flux", "scene": 1, "sun_dir": scenes.sun_direction(3), "pressure": 4.56e-6, "rho_s": 0.3,
+             "rho_d": 0.6, "grid": 256},
+            {"kind": "render", "scene": 0, "camera": cam, "spp": 2, "bounces": 2, "seed": 4, "lights": [sun]}]
+    bad = [dict(good[0], sub_rays=64), dict(good[0], sigma=-1.0), dict(good[1], grid=0), dict(good[1], grid=70000),
+           dict(good[1], sun_dir=(0.0, 0.0, 0.0)), dict(good[2], spp=0), dict(good[2], scene=7),
+           dict(good[2], lights=[{"type": "directional", "intensity": 1.0, "direction": (0.0, 0.0, 0.0)}])]
+    tickets = []
+    for k in range(3):
+        for g in good:
+            tickets.append((g, srv.submit(g)))
+        for b in bad:
+            with pytest.raises((ValueError, TypeError)):
+                srv.submit(b)
+    for g, tk in tickets:
+        out, _ = srv.result(tk)
+        if g["kind"] == "lidar":
+            o = sc[0].flash_lidar(g["camera"], g["sub_rays"], g["sigma"], g["seed"])
+            assert np.array_equal(np.stack([o["single"], o["first"], o["last"]]), out["lidar"])
+        elif g["kind"] == "flux":
+            f, tq, _, _ = sc[1].radiation_pressure(g["sun_dir"], g["pressure"], g["rho_s"], g["rho_d"], g["grid"])
+            assert np.array_equal(np.concatenate([f, tq]), out["force_torque"])
+        else:
+            rgba, _, _, _ = sc[0].render(g["camera"], g["lights"], 2, 2, 0.0, 2, seed=g["seed"])
+            assert np.array_equal(rgba, out["rgba"])
+    st = srv.stats()
+    assert st["completed"] == len(tickets)
+    srv.close()
+
+
+def test_many_lights_served_and_plain_match_oracle(meshes):
+    # no cap on the number of lights (the reference accepts any list):
+    # 11 lights in a served request equal the plain render, and the plain
+    # render matches the oracle at the same RNG keys
+    import oracle
+    from render_parity import classify, fp32_eps
+    ms, sc = meshes
+    rng = np.random.default_rng(4)
+    lights = [{"type": "directional", "intensity": 0.1, "direction": scenes.random_unit(rng)} for _ in range(8)] + \
+             [{"type": "point", "intensity": 5e8, "position": scenes.random_unit(rng) * 9000.0} for _ in range(3)]
+    cam = scenes.small_body_camera(ms[0], resolution=64, seed=8)
+    srv = serving.Server(batching=True, n_workers=2)
+    srv.add_scene(sc[0])
+    req = {"kind": "render", "scene": 0, "camera": cam, "spp": 4, "bounces": 2, "seed": 12, "lights": lights}
+    out, _ = srv.result(srv.submit(req))
+    srv.close()
+    rgba, _, _, _ = sc[0].render(cam, lights, 4, 4, 0.0, 2, seed=12)
+    assert np.array_equal(rgba, out["rgba"])
+    cpu = oracle.Scene(ms[0].vertices, ms[0].indices)
+    info = sc[0].info()
+    rgb, _, frag = cpu.render(cam, lights, 4, 4, 0.0, 2, seed=12, offset=info["self_offset"], eps=fp32_eps(info))
+    r = classify(rgba[..., :3], rgb, frag)
+    print("11 lights: agree %.5f, differ %d, unexplained %d" % (r["agree"], r["differ"], r["unexplained"]))
+    assert r["unexplained"] == 0 and r["agree"] > 0.99
+
+
+def test_tight_budget_counts_retained_workspaces(meshes):
+    # a budget of about two renders' working memory, three workers: the
+    # workspaces a worker keeps between batches count against the budget,
+    # idle workers' are freed when a batch needs the room, and every request
+    # completes (no wait on memory that nobody would release)
+    ms, sc = meshes
+    cam = scenes.small_body_camera(ms[0], resolution=128, seed=1)
+    sun = {"type": "directional", "intensity": 1.0, "direction": scenes.sun_direction(2)}
+    req = {"kind": "render", "scene": 0, "camera": cam, "spp": 4, "bounces": 2, "seed": 1, "lights": [sun]}
+    probe = serving.Server(batching=True, n_workers=1)
+    probe.add_scene(sc[0])
+    probe.result(probe.submit(req))
+    probe.close()
+    # the estimate of one such request: render_bytes_estimate + image
+    px = 128 * 128
+    est = px * 4 * (5 * 16 + 8 + 12 + 16 + 48) + px * (16 + 4 + 1 + 4 + 16 + 4) + px * 16
+    srv = serving.Server(batching=True, n_workers=3, memory_budget=int(2.2 * est), max_batch=1)
+    srv.add_scene(sc[0])
+    ts = [srv.submit(dict(req, seed=100 + k)) for k in range(12)]
+    for k, t in enumerate(ts):
+        out, _ = srv.result(t)
+        ref, _, _, _ = sc[0].render(cam, [sun], 4, 4, 0.0, 2, seed=100 + k)
+        assert np.array_equal(out["rgba"], ref)
+    assert srv.stats()["completed"] == 12
+    srv.close()
