@@ -468,7 +468,7 @@ def run_served(args, capi, scenes):
                [m.n_tris for m in meshes], resident / 1e9), "requests": args.requests}}
     # plain = the drop-in API called once per request, one request at a time
     # (the server's batching = 0 mode: one worker, no batching)
-    modes = (("served", True, 3), ("plain", False, 1))
+    modes = tuple(m for m in (("served", True, 3), ("plain", False, 1)) if m[0] in args.modes.split(","))
     # the same render launch parameters for every mode: lane traversal of
     # camera rays; no hipGraph (a request batch renders a few passes, and a
     # capture + instantiate per batch costs more than it saves)
@@ -490,6 +490,55 @@ def run_served(args, capi, scenes):
         out[mode] = {"mrays_s": st["mrays_per_s"], "wall_s": st["wall_s"], "batches": st["batches"],
                      "rays": st["rays"], "p50_ms_saturated": st["p50_ms"], "p99_ms_saturated": st["p99_ms"]}
         srv.close()
+    # Batching without concurrency: the same saturated request set with each
+    # (kind, scene) group alone on one worker, groups one after another,
+    # largest request first (it runs alone while the rest of its group queues
+    # and then goes as one batch), budget and batch limits raised so nothing
+    # splits a group further.  Against plain it separates what merged
+    # launches gain from what the served mode's concurrent streams gain.
+    if args.no_grouped or "plain" not in out or "served" not in out:
+        modes = () if args.no_rate else modes
+        return _served_rate(args, out, modes, meshes, sc, ropts)
+    reqs = serving.mixed_workload(meshes, args.requests, 1e6, seed=7)
+    groups = {}
+    for _, r in reqs:
+        groups.setdefault((r["kind"], r["scene"]), []).append(r)
+    size = {"render": lambda r: r["camera"]["resolution"][0] * r["camera"]["resolution"][1] * r["spp"],
+            "lidar": lambda r: r["camera"]["resolution"][0] * r["camera"]["resolution"][1],
+            "flux": lambda r: r["grid"] ** 2}
+    srv = serving.Server(batching=True, n_workers=1, render_opts=ropts, memory_budget=160 << 30, max_batch=1 << 16,
+                         max_batch_paths=1 << 40)
+    for s in sc:
+        srv.add_scene(s)
+    grp = {}
+    for (kind, sid), rs in sorted(groups.items()):
+        rs = sorted(rs, key=size[kind], reverse=True)
+        rays0 = srv.stats()["rays"]
+        t0 = time.perf_counter()
+        ts = [srv.submit(r) for r in rs]
+        for t in ts:
+            srv.result(t)
+        dt = time.perf_counter() - t0
+        c = grp.setdefault(kind, {"requests": 0, "rays": 0, "wall_s": 0.0})
+        c["requests"] += len(rs)
+        c["rays"] += srv.stats()["rays"] - rays0
+        c["wall_s"] += dt
+    srv.close()
+    for c in grp.values():
+        c["mrays_s"] = c["rays"] / c["wall_s"] / 1e6
+    wall = sum(c["wall_s"] for c in grp.values())
+    out["grouped_one_stream"] = {"by_kind": grp, "wall_s": wall,
+                                 "mrays_s": sum(c["rays"] for c in grp.values()) / wall / 1e6,
+                                 "speedup_vs_plain": out["plain"]["wall_s"] / wall}
+    log("grouped, one stream: %.3f s (%s); plain %.3f s, served %.3f s" % (
+        wall, ", ".join("%s %.3f s" % (k, c["wall_s"]) for k, c in grp.items()), out["plain"]["wall_s"],
+        out["served"]["wall_s"]))
+    return _served_rate(args, out, () if args.no_rate else modes, meshes, sc, ropts)
+
+
+def _served_rate(args, out, modes, meshes, sc, ropts):
+    """Latency of each mode at a fixed Poisson arrival rate."""
+    from crt import serving
     rate = args.rate
     for mode, batching, nw in modes:
         srv = serving.Server(batching=batching, n_workers=nw, render_opts=ropts)
@@ -501,8 +550,10 @@ def run_served(args, capi, scenes):
         out[mode].update({"rate_hz": rate, "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
                           "mrays_s_at_rate": st["mrays_per_s"]})
         srv.close()
-    out["value"] = out["served"]["mrays_s"]
-    out["speedup_vs_plain"] = out["served"]["mrays_s"] / max(out["plain"]["mrays_s"], 1e-9)
+    first = "served" if "served" in out else "plain"
+    out["value"] = out[first]["mrays_s"]
+    if "served" in out and "plain" in out:
+        out["speedup_vs_plain"] = out["served"]["mrays_s"] / max(out["plain"]["mrays_s"], 1e-9)
     return out
 
 
@@ -522,6 +573,9 @@ def main():
     ap.add_argument("--requests", type=int, default=300)
     ap.add_argument("--rate", type=float, default=100.0)
     ap.add_argument("--small", action="store_true", help="served: small meshes (quick check)")
+    ap.add_argument("--modes", default="served,plain", help="served: comma list of served, plain")
+    ap.add_argument("--no-rate", action="store_true", help="served: skip the fixed-rate latency runs")
+    ap.add_argument("--no-grouped", action="store_true", help="served: skip the one-stream grouped run")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()

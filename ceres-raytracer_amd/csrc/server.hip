@@ -57,6 +57,7 @@ struct Job {
     int status = CRT_OK;
     std::string error;
     uint64_t rays = 0;
+    uint32_t passed_over = 0;         // times a later request was taken first (scene affinity)
 };
 
 // Page-locked host staging, grow-only (powers of two), one per worker.
@@ -166,6 +167,7 @@ struct crt_server {
     };
     std::vector<std::unique_ptr<Worker>> wstate;
     std::vector<size_t> held, running;
+    std::vector<const crt_scene*> running_scene;   // per worker, null when idle
     // statistics
     std::vector<double> latencies_ms;
     uint64_t completed = 0, rejected = 0, rays = 0, batches = 0, inflight = 0;
@@ -299,6 +301,33 @@ int admit(const crt_server* S, int w, size_t need) {
     return 0;
 }
 
+// The next batch's head (under S->mu), or queue.end() to wait.  The oldest
+// request, unless it is on a scene no other worker is tracing and one of the
+// next AFFINITY_WINDOW requests is on such a scene: taking that one keeps the
+// concurrent batches on one BVH, which then stays in L2 / MALL instead of
+// two large meshes evicting each other.  The oldest request is passed over
+// at most AFFINITY_SKIPS times, and a request that does not fit the memory
+// budget is never passed (it waits at the front, as in FIFO order).
+constexpr int AFFINITY_WINDOW = 8;
+constexpr uint32_t AFFINITY_SKIPS = 8;
+
+std::deque<std::shared_ptr<Job>>::iterator pick_head(crt_server* S, int w) {
+    auto front = S->queue.begin();
+    if (front == S->queue.end() || admit(S, w, (*front)->mem) == 0) return S->queue.end();
+    const auto traced = [&](const crt_scene* sc) {
+        for (size_t v = 0; v < S->running_scene.size(); ++v)
+            if ((int)v != w && S->running_scene[v] == sc) return true;
+        return false;
+    };
+    bool others = false;
+    for (size_t v = 0; v < S->running_scene.size(); ++v) others |= (int)v != w && S->running_scene[v] != nullptr;
+    if (!others || (*front)->passed_over >= AFFINITY_SKIPS || traced((*front)->scene)) return front;
+    int k = 0;
+    for (auto it = std::next(front); it != S->queue.end() && k < AFFINITY_WINDOW; ++it, ++k)
+        if (traced((*it)->scene) && admit(S, w, (*it)->mem) != 0) return it;
+    return front;
+}
+
 void worker_main(crt_server* S, int index) {
     (void)hipSetDevice(S->device);
     crt_server::Worker& me = *S->wstate[index];
@@ -311,13 +340,16 @@ void worker_main(crt_server* S, int index) {
         size_t mem = 0;
         {
             std::unique_lock<std::mutex> lk(S->mu);
+            // on stop, queued requests still run (close drains the queue)
             S->cv_work.wait(lk, [&] {
-                return S->stop || (!S->queue.empty() && admit(S, index, S->queue.front()->mem) != 0);
+                return (S->stop && S->queue.empty()) || pick_head(S, index) != S->queue.end();
             });
             if (S->stop && S->queue.empty()) break;
-            if (S->queue.empty()) continue;
-            auto head = S->queue.front();
-            S->queue.pop_front();
+            const auto pick = pick_head(S, index);
+            if (pick == S->queue.end()) continue;
+            auto head = *pick;
+            if (pick != S->queue.begin()) ++S->queue.front()->passed_over;
+            S->queue.erase(pick);
             batch.push_back(head);
             mem = head->mem;
             if (batching) {
@@ -360,6 +392,7 @@ void worker_main(crt_server* S, int index) {
                         S->held[v] = 0;
                     }
             S->running[index] = std::max<size_t>(mem, 1);
+            S->running_scene[index] = head->scene;
             S->inflight += batch.size();
         }
         // run the batch; if it fails, rerun its requests one by one so an
@@ -391,6 +424,7 @@ void worker_main(crt_server* S, int index) {
         {
             std::lock_guard<std::mutex> lk(S->mu);
             S->running[index] = 0;
+            S->running_scene[index] = nullptr;
             S->held[index] = keep;
             S->inflight -= batch.size();
             ++S->batches;
@@ -441,6 +475,7 @@ int crt_server_create(const crt_server_opts* opts, crt_server** out) {
         }
         S->held.assign(S->opts.n_workers, 0);
         S->running.assign(S->opts.n_workers, 0);
+        S->running_scene.assign(S->opts.n_workers, nullptr);
         crt_server* raw = S.get();
         for (uint32_t k = 0; k < S->opts.n_workers; ++k) S->workers.emplace_back(worker_main, raw, (int)k);
         *out = S.release();
