@@ -115,6 +115,37 @@ def roofline(stats, prim_rays, sh_rays, t_prim_ms, t_sh_ms, ms_per_step):
     return roof
 
 
+def bvh_quality(scenes, scene, mesh, cam, sun, stats):
+    """Traversal work per ray of the GPU BVH on the workload's rays, beside
+    the reference BVH's statistics for the same rays: every camera ray
+    (closest hit), then one sun ray (any hit) per hit from the hit point moved
+    max(1e-4, 1e-6 extent) along the inward normal of the hit triangle, as the
+    reference's statistics were taken.  The reference's traversal step tests
+    two child boxes; here a visit of an 8-wide node tests its valid children."""
+    o, d = scenes.camera_rays(cam)
+    c = scene.trace_stats(o, d)
+    hit = c["tri"] >= 0
+    v = np.asarray(mesh.vertices, dtype=np.float64)
+    f = np.asarray(mesh.indices, dtype=np.int64)[c["tri"][hit]]
+    p0, p1, p2 = v[f[:, 0]], v[f[:, 1]], v[f[:, 2]]
+    nin = np.cross(p0 - p1, p2 - p0)
+    nin /= np.linalg.norm(nin, axis=1, keepdims=True)
+    xyz = o[hit] + c["t"][hit][:, None] * d[hit]
+    so = xyz - max(1e-4, 1e-6 * float(np.max(np.abs(v)))) * nin
+    sd = np.repeat(np.asarray(sun, dtype=np.float64)[None, :], so.shape[0], axis=0)
+    a = scene.trace_stats(so, sd, any_hit=True)
+    rc, ra = stats["closest"], stats["shadow"]
+    return {"rays": "all %d camera rays (closest) + %d sun rays from their hits (any hit)" % (o.shape[0], so.shape[0]),
+            "closest": {"wide_nodes": float(c["nodes"].mean()), "box_tests": float(c["boxes"].mean()),
+                        "triangle_tests": float(c["tris"].mean()),
+                        "reference_box_tests": 2.0 * rc["traversal_steps_mean"],
+                        "reference_triangle_tests": rc["intersections_mean"]},
+            "shadow": {"wide_nodes": float(a["nodes"].mean()), "box_tests": float(a["boxes"].mean()),
+                       "triangle_tests": float(a["tris"].mean()),
+                       "reference_box_tests": 2.0 * ra["traversal_steps_mean"],
+                       "reference_triangle_tests": ra["intersections_mean"]}}
+
+
 def cpu_baseline(mesh, cam, sun, budget_s=12.0, gpu_depth=None, gpu_tri=None):
     """Time a bounded sample of the same workload on the host cores.
 
@@ -271,10 +302,24 @@ def run_batched(args, capi, scenes):
         cams.append(scenes.small_body_camera(mesh, resolution=side, seed=10000 + k, range_km=(20.0, 500.0)))
     sun = {"type": "directional", "intensity": 1.0, "direction": scenes.sun_direction(2)}
     spp, nb = 8, 3
+    seeds = list(range(len(cams)))
+    if args.render_once:
+        # one warm-up and one render of the whole batch on one stream with the
+        # default launch parameters (a short, clean run to profile)
+        scene.render_batch(cams[:8], [sun], spp, spp, 0.0, nb, seeds=seeds[:8], outputs=False)
+        t0 = time.perf_counter()
+        _, tmb = scene.render_batch(cams, [sun], spp, spp, 0.0, nb, seeds=seeds, outputs=False)
+        t1 = time.perf_counter() - t0
+        rays = tmb["rays_traced"] + tmb["shadow_rays"]
+        log("batched, one stream, default launch: %.1f ms, %d rays" % (t1 * 1e3, rays))
+        return {"metric": "batched render Mrays/s (256 cameras, 8 spp, 3 bounces, L8 body)", "value": rays / t1 / 1e6,
+                "unit": "Mrays/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+                "config": {"workload": "batched path tracing: %d cameras, spp 8, bounces 3, sun; one render, one "
+                                       "stream, default launch parameters" % len(cams)},
+                "one_stream_ms": t1 * 1e3, "rays": rays, "timing": tmb}
     log("autotuning on a subset")
     best = autotune.tune(scene, cams, [sun], spp, nb, log=log)
     log("best launch parameters: %s" % best)
-    seeds = list(range(len(cams)))
     scene.render_batch(cams[:8], [sun], spp, spp, 0.0, nb, seeds=seeds[:8], outputs=False, **best)
     # one batched render of all cameras on one stream
     times1 = []
@@ -575,6 +620,8 @@ def main():
     ap.add_argument("--small", action="store_true", help="served: small meshes (quick check)")
     ap.add_argument("--modes", default="served,plain", help="served: comma list of served, plain")
     ap.add_argument("--no-rate", action="store_true", help="served: skip the fixed-rate latency runs")
+    ap.add_argument("--render-once", action="store_true",
+                    help="batched: one render of the batch on one stream, default launch parameters (profiling)")
     ap.add_argument("--no-grouped", action="store_true", help="served: skip the one-stream grouped run")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -662,6 +709,10 @@ def main():
         with open(STATS_PATH) as fh:
             stats = json.load(fh)
         roof = roofline(stats, prim_rays, sh_rays, avg_trace, avg_shadow, ms_per_step)
+    quality = None
+    if stats is not None and args.resolution == 2048:
+        log("traversal statistics (untimed)")
+        quality = bvh_quality(scenes, scene, mesh, cam, sun, stats)
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline")
@@ -691,7 +742,7 @@ def main():
                       "note": "mean launch durations over the timed region (hipEvents on each launch stream; "
                               "passes on three streams overlap, so kernels share the GPU)"},
         "bvh": {"wide_nodes": info["n_wide_nodes"], "leaves": info["n_leaves"], "depth": info["max_depth"],
-                "build_ms": info["build_ms"], "sah_cost": info["sah_cost"]},
+                "build_ms": info["build_ms"], "sah_cost": info["sah_cost"], "traversal_per_ray": quality},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -159,6 +159,7 @@ SIGNATURES = {
     "crt_scene_deserialize": (C.c_int, [_p, C.c_uint64, _p]),
     "crt_pass": (C.c_int, [_p, _p, _p, _p, _p, _p, _p, C.c_uint32, _p, _p]),
     "crt_trace_rays": (C.c_int, [_p, C.c_uint64, _p, _p, _p, _p, _p, _p, _p, _p, C.c_uint32, _p, _p]),
+    "crt_trace_stats": (C.c_int, [_p, C.c_uint64, _p, _p, C.c_uint32, _p, _p, _p, _p]),
     "crt_sun_pass": (C.c_int, [_p, _p, _p, _p, _p, _p, C.c_uint32, _p, _p]),
     "crt_render": (C.c_int, [_p, _p, _p, C.c_uint32, _p, _p, _p, _p, C.c_uint32, _p, _p]),
     "crt_render_batch": (C.c_int, [_p, C.c_uint32, _p, _p, _p, _p, C.c_uint32, _p, _p, _p, _p, _p, C.c_uint32,
@@ -489,6 +490,19 @@ class Scene:
         _check(load().crt_trace_rays(self.h, n, ptr(o), ptr(d), ptr(tmin), ptr(tmax), ptr(t), ptr(tri), ptr(u),
                                      ptr(v), flags, None, C.byref(tm)))
         return t, tri, u, v
+
+    def trace_stats(self, o, d, any_hit=False):
+        """Traversal statistics per ray (crt_trace_stats): dict of `nodes`,
+        `boxes`, `tris` (uint32, n) and the hit `t`, `tri` of the same rays."""
+        o = np.ascontiguousarray(o, dtype=np.float64).reshape(-1, 3)
+        d = np.ascontiguousarray(d, dtype=np.float64).reshape(-1, 3)
+        n = o.shape[0]
+        counts = np.empty((n, 3), dtype=np.uint32)
+        t = np.empty(n)
+        tri = np.empty(n, dtype=np.int32)
+        _check(load().crt_trace_stats(self.h, n, ptr(o), ptr(d), CRT_ANY_HIT if any_hit else 0, ptr(counts), ptr(t),
+                                      ptr(tri), None))
+        return {"nodes": counts[:, 0], "boxes": counts[:, 1], "tris": counts[:, 2], "t": t, "tri": tri}
 
     def sun_pass(self, cam: dict, sun_dir, outputs=None, device=False, stream=None, timing=True):
         """Primary closest hit + one sun shadow ray per hit.  With device=True,

@@ -344,6 +344,26 @@ def small_body_camera(mesh: Mesh, resolution=2048, seed=1, range_km=(50.0, 200.0
             "z_positive": False, "position": pos, "rotation": look_at_rotation(pos)}
 
 
+def camera_rays(cam: dict):
+    """FP64 rays of every pixel (row-major, integer pixel coordinates) of a
+    pinhole camera dict, as SimpleCamera::pixel_to_ray forms them
+    (src/crt/cameras/simple_camera.hpp:61-81): normalised camera-frame
+    direction ((u - cx) / sx, (cy - v) / sy, +-f), rotated by R^T.
+    Returns (origins, dirs), each (H*W, 3)."""
+    w, h = int(np.floor(cam["resolution"][0])), int(np.floor(cam["resolution"][1]))
+    sx = cam["resolution"][0] / cam["sensor_size"][0]
+    sy = cam["resolution"][1] / cam["sensor_size"][1]
+    f = cam["focal_length"] if cam.get("z_positive", False) else -cam["focal_length"]
+    v, u = np.meshgrid(np.arange(h, dtype=np.float64), np.arange(w, dtype=np.float64), indexing="ij")
+    dc = np.stack([(u.ravel() - cam["resolution"][0] / 2.0) / sx, (cam["resolution"][1] / 2.0 - v.ravel()) / sy,
+                   np.full(w * h, float(f))], axis=1)
+    dc /= np.linalg.norm(dc, axis=1, keepdims=True)
+    R = np.asarray(cam["rotation"], dtype=np.float64).reshape(3, 3)
+    d = dc @ R   # row k = R^T dc_k
+    o = np.broadcast_to(np.asarray(cam["position"], dtype=np.float64), d.shape).copy()
+    return o, d
+
+
 def sun_direction(seed=2):
     """Unit vector from the body towards the sun (uniform on the sphere)."""
     return random_unit(np.random.default_rng(seed))

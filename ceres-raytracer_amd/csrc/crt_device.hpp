@@ -540,14 +540,23 @@ __device__ inline uint32_t cnode_test(const float4* __restrict__ cn, const RayPr
     return hits;
 }
 
+// Traversal counters of one ray, the counterpart of the reference
+// traverser's Statistics (lib/bvh/single_ray_traverser.hpp:130-160): wide
+// nodes visited, child boxes tested (the valid children of the node's mask
+// at each visit; the reference tests two per traversal step) and triangles
+// tested (a leaf counts all its triangles, as intersect_leaf does).
+struct TraversalStats {
+    uint32_t nodes, boxes, tris;
+};
+
 // Lane traversal in "while-while" form (Aila and Laine, HPG 2009): a lane
 // that reaches a leaf parks it and keeps descending until it meets a second
 // leaf or every lane of the wave holds one; then the wave tests triangles
 // together, so inner-node and leaf work of different lanes are not
 // interleaved step by step.
-template <bool ANY_HIT>
+template <bool ANY_HIT, bool STATS = false>
 __device__ inline HitRec trace_lane(const SceneView& s, const RayPre& r, float tmin, float tmax,
-                                    const LaneStack& stk) {
+                                    const LaneStack& stk, TraversalStats* ts = nullptr) {
     HitRec hit;
     hit.t = tmax; hit.u = 0.f; hit.v = 0.f; hit.slot = -1;
     float best = tmax;
@@ -559,6 +568,10 @@ __device__ inline HitRec trace_lane(const SceneView& s, const RayPre& r, float t
         while (cur != 0u && !(cur & REF_LEAF)) {
             uint32_t k0, ref0;
             float tn0, tn1;
+            if (STATS) {
+                ++ts->nodes;
+                ts->boxes += (uint32_t)__popc((cur & 0xFFu) & (__float_as_uint(s.cnodes[(size_t)(cur >> 8) * 6u].w) >> 24));
+            }
             const uint32_t hits = cnode_test(s.cnodes + (size_t)(cur >> 8) * 6u, r, tmin, best, cur & 0xFFu, k0,
                                              ref0, tn0, tn1);
             uint32_t next = 0u;
@@ -589,6 +602,7 @@ __device__ inline HitRec trace_lane(const SceneView& s, const RayPre& r, float t
             const uint32_t first = leaf_first(lr);
             const uint32_t count = leaf_count(lr);
             const float4* __restrict__ tp = s.tris + (size_t)first * 3u;
+            if (STATS) ts->tris += count;
             for (uint32_t k = 0; k < count; ++k) {
                 float t, u, v;
                 if (tri_test(r, tp[3u * k], tp[3u * k + 1u], tp[3u * k + 2u], tmin, best, t, u, v)) {

@@ -78,9 +78,10 @@ namespace {
 // Closest/any hit of one FP64 ray (unit direction d, origin in the scene
 // frame): rebase in FP64, trace in FP32 (one ray per lane), then t, u, v of
 // the hit triangle re-evaluated in FP64.  Returns false on a miss.
-template <bool ANY>
+template <bool ANY, bool STATS = false>
 __device__ inline bool trace_f64(const SceneView& s, const double o[3], const double d[3], double tmin,
-                                 double tmax, const LaneStack& stk, double& t_out, double& u, double& v, int& slot) {
+                                 double tmax, const LaneStack& stk, double& t_out, double& u, double& v, int& slot,
+                                 TraversalStats* ts = nullptr) {
     double t0;
     if (!rebase_t0(s, o, d, t0)) return false;
     if (t0 > tmax) return false;
@@ -88,7 +89,7 @@ __device__ inline bool trace_f64(const SceneView& s, const double o[3], const do
     const RayPre r = make_ray_pre((float)ro[0], (float)ro[1], (float)ro[2], (float)d[0], (float)d[1], (float)d[2]);
     const float ltmin = (float)fmax(tmin - t0, 0.0);
     const float ltmax = tmax >= 3.0e38 ? __int_as_float(0x7f800000) : (float)(tmax - t0);
-    const HitRec h = trace_lane<ANY>(s, r, ltmin, ltmax, stk);
+    const HitRec h = trace_lane<ANY, STATS>(s, r, ltmin, ltmax, stk, ts);
     if (h.slot < 0) return false;
     t_out = t0 + (double)h.t;
     u = h.u;
@@ -201,9 +202,10 @@ struct RayOut {
     int32_t* tri;
     double* u;
     double* v;
+    uint32_t* counts;   // traversal statistics: nodes, boxes, triangles per ray (k_trace_rays<ANY, true>)
 };
 
-template <bool ANY>
+template <bool ANY, bool STATS = false>
 CRT_LANE_KERNEL void k_trace_rays(SceneView s, RayIn in, RayOut out, uint64_t n) {
     const LaneStack stk = lane_stack(s);
     const uint64_t ray = (uint64_t)blockIdx.x * LANE_BLOCK + threadIdx.x;
@@ -219,13 +221,19 @@ CRT_LANE_KERNEL void k_trace_rays(SceneView s, RayIn in, RayOut out, uint64_t n)
     double t = 0.0;
     double u = 0.0, v = 0.0;
     int slot = -1;
+    TraversalStats ts{0u, 0u, 0u};
     if (len > 0.0 && !(tmax < tmin)) {
         d[0] /= len; d[1] /= len; d[2] /= len;
         // distances along the unit direction
         tmin *= len;
         tmax = tmax >= 1.0e300 ? tmax : tmax * len;
-        hit = trace_f64<ANY>(s, o, d, tmin, tmax, stk, t, u, v, slot);
+        hit = trace_f64<ANY, STATS>(s, o, d, tmin, tmax, stk, t, u, v, slot, &ts);
         if (hit) t /= len;
+    }
+    if (STATS) {
+        out.counts[3 * ray] = ts.nodes;
+        out.counts[3 * ray + 1] = ts.boxes;
+        out.counts[3 * ray + 2] = ts.tris;
     }
     if (out.t) out.t[ray] = hit ? t : -1.0;
     if (out.tri) out.tri[ray] = hit ? (int32_t)__float_as_uint(s.tris[3 * slot].w) : -1;
@@ -797,9 +805,13 @@ int crt_pass(crt_scene* scene, const crt_camera* camera, double* xyz, double* no
     });
 }
 
-int crt_trace_rays(crt_scene* scene, uint64_t n, const double* origins, const double* dirs, const double* tmin,
-                   const double* tmax, double* t, int32_t* tri, double* u, double* v, uint32_t flags, void* stream,
-                   crt_timing* timing) {
+}  // extern "C"
+
+namespace crt {
+namespace {
+int trace_rays_impl(crt_scene* scene, uint64_t n, const double* origins, const double* dirs, const double* tmin,
+                    const double* tmax, double* t, int32_t* tri, double* u, double* v, uint32_t* counts,
+                    uint32_t flags, void* stream, crt_timing* timing) {
     return guarded([&] {
         CRT_REQUIRE(scene != nullptr, CRT_ERR_INVALID, "scene is NULL");
         CRT_REQUIRE(origins && dirs, CRT_ERR_INVALID, "origins/dirs are NULL");
@@ -812,7 +824,7 @@ int crt_trace_rays(crt_scene* scene, uint64_t n, const double* origins, const do
         std::lock_guard<std::mutex> slot_lock(slot.mu);
         Workspace& ws = slot.ws;
         RayIn in{origins, dirs, tmin, tmax};
-        RayOut out{t, tri, u, v};
+        RayOut out{t, tri, u, v, counts};
         if (!dev) {
             // stage inputs: o(3n) d(3n) tmin(n) tmax(n) in ws.f; outputs in ws.g
             const size_t in_bytes = n * 8 * 8;
@@ -824,22 +836,28 @@ int crt_trace_rays(crt_scene* scene, uint64_t n, const double* origins, const do
             in.d = base + 3 * n;
             if (tmin) { CRT_HIP_CHECK(hipMemcpyAsync(base + 6 * n, tmin, n * 8, hipMemcpyHostToDevice, st)); in.tmin = base + 6 * n; }
             if (tmax) { CRT_HIP_CHECK(hipMemcpyAsync(base + 7 * n, tmax, n * 8, hipMemcpyHostToDevice, st)); in.tmax = base + 7 * n; }
-            ws.g.reserve(n * 28);
+            ws.g.reserve(n * 40);
             char* ob = ws.g.as<char>();
             out.t = t ? reinterpret_cast<double*>(ob) : nullptr;
             out.u = u ? reinterpret_cast<double*>(ob + n * 8) : nullptr;
             out.v = v ? reinterpret_cast<double*>(ob + n * 16) : nullptr;
             out.tri = tri ? reinterpret_cast<int32_t*>(ob + n * 24) : nullptr;
+            out.counts = counts ? reinterpret_cast<uint32_t*>(ob + n * 28) : nullptr;
         }
         EventTimer tm(timing != nullptr);
         const uint32_t blocks = cdiv(n, LANE_BLOCK);
         EventTimer::rec(tm.a, st);
-        if (any)
-            hipLaunchKernelGGL(k_trace_rays<true>, dim3(blocks), dim3(LANE_BLOCK), lane_lds_bytes(scene->view), st,
-                               scene->view, in, out, n);
+        const size_t lds = lane_lds_bytes(scene->view);
+        if (counts && any)
+            hipLaunchKernelGGL((k_trace_rays<true, true>), dim3(blocks), dim3(LANE_BLOCK), lds, st, scene->view, in,
+                               out, n);
+        else if (counts)
+            hipLaunchKernelGGL((k_trace_rays<false, true>), dim3(blocks), dim3(LANE_BLOCK), lds, st, scene->view, in,
+                               out, n);
+        else if (any)
+            hipLaunchKernelGGL(k_trace_rays<true>, dim3(blocks), dim3(LANE_BLOCK), lds, st, scene->view, in, out, n);
         else
-            hipLaunchKernelGGL(k_trace_rays<false>, dim3(blocks), dim3(LANE_BLOCK), lane_lds_bytes(scene->view), st,
-                               scene->view, in, out, n);
+            hipLaunchKernelGGL(k_trace_rays<false>, dim3(blocks), dim3(LANE_BLOCK), lds, st, scene->view, in, out, n);
         CRT_HIP_CHECK(hipGetLastError());
         EventTimer::rec(tm.b, st);
         if (!dev) {
@@ -847,6 +865,7 @@ int crt_trace_rays(crt_scene* scene, uint64_t n, const double* origins, const do
             if (u) CRT_HIP_CHECK(hipMemcpyAsync(u, out.u, n * 8, hipMemcpyDeviceToHost, st));
             if (v) CRT_HIP_CHECK(hipMemcpyAsync(v, out.v, n * 8, hipMemcpyDeviceToHost, st));
             if (tri) CRT_HIP_CHECK(hipMemcpyAsync(tri, out.tri, n * 4, hipMemcpyDeviceToHost, st));
+            if (counts) CRT_HIP_CHECK(hipMemcpyAsync(counts, out.counts, n * 12, hipMemcpyDeviceToHost, st));
         }
         if (!dev || timing) CRT_HIP_CHECK(hipStreamSynchronize(st));
         if (timing) {
@@ -857,6 +876,23 @@ int crt_trace_rays(crt_scene* scene, uint64_t n, const double* origins, const do
             timing->total_ms = ms;
         }
     });
+}
+}  // namespace
+}  // namespace crt
+
+extern "C" {
+
+int crt_trace_rays(crt_scene* scene, uint64_t n, const double* origins, const double* dirs, const double* tmin,
+                   const double* tmax, double* t, int32_t* tri, double* u, double* v, uint32_t flags, void* stream,
+                   crt_timing* timing) {
+    return trace_rays_impl(scene, n, origins, dirs, tmin, tmax, t, tri, u, v, nullptr, flags, stream, timing);
+}
+
+int crt_trace_stats(crt_scene* scene, uint64_t n, const double* origins, const double* dirs, uint32_t flags,
+                    uint32_t* counts, double* t, int32_t* tri, void* stream) {
+    if (counts == nullptr) return guarded([] { CRT_REQUIRE(false, CRT_ERR_INVALID, "counts is NULL"); });
+    return trace_rays_impl(scene, n, origins, dirs, nullptr, nullptr, t, tri, nullptr, nullptr, counts,
+                           flags & ~(uint32_t)CRT_OUT_DEVICE, stream, nullptr);
 }
 
 int crt_sun_pass(crt_scene* scene, const crt_camera* camera, const double sun_dir[3], double* depth, int32_t* tri_id,

@@ -274,6 +274,17 @@ int crt_trace_rays(crt_scene* scene, uint64_t n,
                    double* t, int32_t* tri, double* u, double* v,
                    uint32_t flags, void* stream, crt_timing* timing);
 
+/* Traversal statistics <- SingleRayTraverser::traverse(ray, intersector,
+ * Statistics&) (lib/bvh/single_ray_traverser.hpp:130-160): the rays of
+ * crt_trace_rays (tmin 0, tmax +inf; CRT_ANY_HIT for any hit) traced by the
+ * same lane traversal, with per ray in `counts` (n x 3 uint32, host) the
+ * wide nodes visited, the child boxes tested (valid children of the visited
+ * nodes; the reference's traversal_steps test two boxes each) and the
+ * triangles tested (a leaf counts all its triangles, like the reference's
+ * intersections).  t and tri as crt_trace_rays (may be NULL).  Host arrays.  */
+int crt_trace_stats(crt_scene* scene, uint64_t n, const double* origins, const double* dirs,
+                    uint32_t flags, uint32_t* counts, double* t, int32_t* tri, void* stream);
+
 /* ---- primary + directional-sun shadow pass (new) ------------------------ *
  * The camera's closest hits plus one any-hit shadow ray per hit towards the
  * sun direction (unit vector from the scene towards the sun).  Shadow rays
