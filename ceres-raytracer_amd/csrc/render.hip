@@ -142,6 +142,8 @@ struct Ctx {
     const uint32_t* plist;
     PathPlanes st;
     uint32_t* queue[2];
+    uint32_t* sorted;      // a bounce's queue ordered by ray-origin cell (coherence sort)
+    uint32_t* bins;        // SORT_BINS counts, then SORT_BINS cursors
     uint32_t* live;
     ShadowRec* shadow;
     unsigned long long* tally;   // [0] closest-hit rays, [1] shadow rays
@@ -264,6 +266,99 @@ __global__ __launch_bounds__(TB) void k_raygen(Ctx C) {
     }
     const uint32_t slot = wave_append(alive, C.cnt + 0);
     if (alive) C.queue[0][slot] = (uint32_t)k;
+}
+
+// ---- coherence sort (crt_batch_opts.coherence_sort) ----------------------
+// Optionally, before the extension rays of a bounce after the first are
+// traced, the bounce's queue is reordered by the cell of the ray origin (a
+// 16^3 grid over the scene's bounding cube, Morton order), so that the rays
+// of a wave start in the same part of the tree.  A counting sort (histogram,
+// scan, scatter); paths of one cell keep no particular order, which changes
+// nothing in the image: every path's result depends only on its own state
+// and RNG keys.  Off by default: the queues already follow the pixel tiles
+// of their paths, so origins are coherent without it (256 cameras of the L8
+// body, 8 spp, 3 bounces: 191 ms without, 193 ms with).
+constexpr uint32_t SORT_BINS = 4096;
+constexpr int SORT_TB = 256;
+constexpr int SORT_ITEMS = 16;
+constexpr uint32_t SORT_TILE = (uint32_t)SORT_TB * SORT_ITEMS;
+
+__device__ inline uint32_t origin_cell(const SceneView& s, float4 o) {
+    const float r = (float)s.radius;
+    const float inv = 16.0f / (2.0f * r);
+    const float lo[3] = {(float)s.center[0] - r, (float)s.center[1] - r, (float)s.center[2] - r};
+    const float p[3] = {o.x, o.y, o.z};
+    uint32_t key = 0u;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const uint32_t c = (uint32_t)fminf(fmaxf((p[a] - lo[a]) * inv, 0.0f), 15.0f);
+        key |= ((c & 1u) | ((c & 2u) << 2) | ((c & 4u) << 4) | ((c & 8u) << 6)) << a;
+    }
+    return key;
+}
+
+__global__ __launch_bounds__(SORT_TB) void k_sort_zero(uint32_t* __restrict__ bins) {
+    for (uint32_t i = threadIdx.x; i < SORT_BINS; i += SORT_TB) bins[i] = 0u;
+}
+
+__global__ __launch_bounds__(SORT_TB) void k_sort_hist(Ctx C, int qsel) {
+    __shared__ uint32_t h[SORT_BINS];
+    for (uint32_t i = threadIdx.x; i < SORT_BINS; i += SORT_TB) h[i] = 0u;
+    __syncthreads();
+    const uint32_t n = C.cnt[qsel];
+    const uint32_t base = blockIdx.x * SORT_TILE;
+    for (int j = 0; j < SORT_ITEMS; ++j) {
+        const uint32_t q = base + (uint32_t)j * SORT_TB + threadIdx.x;
+        if (q < n) atomicAdd(&h[origin_cell(C.s, C.st.o[C.queue[qsel][q]])], 1u);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < SORT_BINS; i += SORT_TB)
+        if (h[i] != 0u) atomicAdd(&C.bins[i], h[i]);
+}
+
+// One block: cursors = exclusive prefix of the counts.
+__global__ __launch_bounds__(1024) void k_sort_scan(uint32_t* __restrict__ bins) {
+    __shared__ uint32_t part[1024];
+    constexpr uint32_t PER = SORT_BINS / 1024;
+    const uint32_t t = threadIdx.x;
+    uint32_t v[PER], sum = 0u;
+    for (uint32_t j = 0; j < PER; ++j) { v[j] = bins[t * PER + j]; sum += v[j]; }
+    part[t] = sum;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+        const uint32_t x = t >= off ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += x;
+        __syncthreads();
+    }
+    uint32_t run = part[t] - sum;
+    for (uint32_t j = 0; j < PER; ++j) { bins[SORT_BINS + t * PER + j] = run; run += v[j]; }
+}
+
+__global__ __launch_bounds__(SORT_TB) void k_sort_scatter(Ctx C, int qsel) {
+    __shared__ uint32_t h[SORT_BINS];
+    __shared__ uint32_t at[SORT_BINS];
+    for (uint32_t i = threadIdx.x; i < SORT_BINS; i += SORT_TB) h[i] = 0u;
+    __syncthreads();
+    const uint32_t n = C.cnt[qsel];
+    const uint32_t base = blockIdx.x * SORT_TILE;
+    uint32_t path[SORT_ITEMS], slot[SORT_ITEMS];   // slot = cell << 12 | rank in the block's cell
+    for (int j = 0; j < SORT_ITEMS; ++j) {
+        const uint32_t q = base + (uint32_t)j * SORT_TB + threadIdx.x;
+        path[j] = 0xFFFFFFFFu;
+        slot[j] = 0u;
+        if (q < n) {
+            path[j] = C.queue[qsel][q];
+            const uint32_t c = origin_cell(C.s, C.st.o[path[j]]);
+            slot[j] = (c << 12) | atomicAdd(&h[c], 1u);   // rank < SORT_TILE = 4096
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < SORT_BINS; i += SORT_TB)
+        if (h[i] != 0u) at[i] = atomicAdd(&C.bins[SORT_BINS + i], h[i]);
+    __syncthreads();
+    for (int j = 0; j < SORT_ITEMS; ++j)
+        if (path[j] != 0xFFFFFFFFu) C.sorted[at[slot[j] >> 12] + (slot[j] & 0xFFFu)] = path[j];
 }
 
 // ---- traversal kernels -------------------------------------------------
@@ -634,7 +729,7 @@ __device__ inline void finish_extend(const Ctx& C, int bounce, uint32_t k, const
     shade_hit(C, bounce, k, m.x, m.y, m.z, h.slot, h.u, h.v, ls, din);
 }
 
-CRT_LANE_KERNEL void k_extend_shade_lane(Ctx C, int bounce, int qsel, int gen) {
+CRT_LANE_KERNEL void k_extend_shade_lane(Ctx C, int bounce, int qsel, int gen, int sorted) {
     const LaneStack stk = lane_stack(C.s);
     const uint32_t q = blockIdx.x * LANE_BLOCK + threadIdx.x;
     uint32_t k = 0;
@@ -658,7 +753,7 @@ CRT_LANE_KERNEL void k_extend_shade_lane(Ctx C, int bounce, int qsel, int gen) {
         }
         (void)wave_append(alive, C.cnt + qsel);   // count of traced camera rays (tally)
     } else if (q < C.cnt[qsel]) {
-        k = C.queue[qsel][q];
+        k = sorted ? C.sorted[q] : C.queue[qsel][q];
         o = C.st.o[k];
         d = C.st.d[k];
         alive = true;
@@ -676,7 +771,7 @@ CRT_LANE_KERNEL void k_extend_shade_lane(Ctx C, int bounce, int qsel, int gen) {
 // atomic per wave appends them.  Those rays leave the body more often than
 // camera rays hit it; camera rays traced by octets use k_trace + k_shade.
 __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8)))
-void k_extend_shade_octet(Ctx C, int bounce, int qsel, int R) {
+void k_extend_shade_octet(Ctx C, int bounce, int qsel, int R, int sorted) {
     OCTET_SETUP(C)
     const uint32_t q0 = (blockIdx.x * OCTETS_PER_BLOCK + local) * (uint32_t)R;
     const uint32_t n = C.cnt[qsel];
@@ -689,7 +784,7 @@ void k_extend_shade_octet(Ctx C, int bounce, int qsel, int R) {
     for (int p = 0; p < R; ++p) {
         const uint32_t q = q0 + (uint32_t)p;
         if (q >= n) break;   // octet-uniform
-        const uint32_t k = C.queue[qsel][q];
+        const uint32_t k = sorted ? C.sorted[q] : C.queue[qsel][q];
         const float4 o = C.st.o[k], d = C.st.d[k];
         const RayPre r = make_ray_pre(o.x, o.y, o.z, d.x, d.y, d.z);
         const HitRec h = trace_octet(C.s, r, 0.0f, __int_as_float(0x7f800000), sref, sd, sub, lane_base);
@@ -831,15 +926,25 @@ void launch_pass(const Ctx& C, uint64_t cap_paths, uint32_t cap_fold, const crt_
         // paths per octet / quad: enough groups for ~32k waves, at most one path per lane
         const int ro = cap_paths >= (1u << 21) ? 8 : cap_paths >= (1u << 20) ? 4 : cap_paths >= (1u << 19) ? 2 : 1;
         const int rq = cap_paths >= (1u << 21) ? 4 : cap_paths >= (1u << 20) ? 2 : 1;
+        const bool sort = B.coherence_sort == 1;
         for (int b = 0; b < C.nb; ++b) {
             const int cur = b & 1, nxt = cur ^ 1;
             hipLaunchKernelGGL(k_clear, dim3(1), dim3(32), 0, st, C.cnt, (1u << nxt) | (1u << 3));
+            const bool sorted = sort && b > 0 && (lane_at(b) || B.separate_kernels != 2);
+            if (sorted) {
+                const uint32_t gs = std::max(1u, cdiv(cap_paths, SORT_TILE));
+                hipLaunchKernelGGL(k_sort_zero, dim3(1), dim3(SORT_TB), 0, st, C.bins);
+                hipLaunchKernelGGL(k_sort_hist, dim3(gs), dim3(SORT_TB), 0, st, C, cur);
+                hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, st, C.bins);
+                hipLaunchKernelGGL(k_sort_scatter, dim3(gs), dim3(SORT_TB), 0, st, C, cur);
+            }
             if (lane_at(b))
                 hipLaunchKernelGGL(k_extend_shade_lane, dim3(std::max(1u, cdiv(cap_paths, LANE_BLOCK))),
-                                   dim3(LANE_BLOCK), lane_lds_bytes(C.s), st, C, b, cur, (int)(gen && b == 0));
+                                   dim3(LANE_BLOCK), lane_lds_bytes(C.s), st, C, b, cur, (int)(gen && b == 0),
+                                   (int)sorted);
             else if (b > 0 && B.separate_kernels != 2)
                 hipLaunchKernelGGL(k_extend_shade_octet, dim3(std::max(1u, cdiv(cap_paths, OCTETS_PER_BLOCK * ro))),
-                                   dim3(TRACE_BLOCK), lds, st, C, b, cur, ro);
+                                   dim3(TRACE_BLOCK), lds, st, C, b, cur, ro, (int)sorted);
             else {
                 hipLaunchKernelGGL(k_trace, dim3(go), dim3(TRACE_BLOCK), lds, st, C, cur);
                 hipLaunchKernelGGL(k_shade, dim3(gp), dim3(TB), 0, st, C, b, cur);
@@ -882,11 +987,12 @@ void launch_pass(const Ctx& C, uint64_t cap_paths, uint32_t cap_fold, const crt_
 // Grow-only device buffers of one render caller (a scene's default
 // workspace, or one per serving worker so that workers never allocate).
 struct RenderWorkspace {
-    DevBuf d_cams, d_lights, d_segs, d_ctl, d_plist, mean, spp, active, counters, tally, pstate, q0, q1, live, shadow,
-        samples, o_rgba, o_u8;
+    DevBuf d_cams, d_lights, d_segs, d_ctl, d_plist, mean, spp, active, counters, tally, pstate, q0, q1, qs, bins,
+        live, shadow, samples, o_rgba, o_u8;
     size_t bytes() const {
         return d_cams.bytes + d_lights.bytes + d_segs.bytes + d_ctl.bytes + d_plist.bytes + mean.bytes + spp.bytes +
-               active.bytes + counters.bytes + tally.bytes + pstate.bytes + q0.bytes + q1.bytes + live.bytes +
+               active.bytes + counters.bytes + tally.bytes + pstate.bytes + q0.bytes + q1.bytes + qs.bytes +
+               bins.bytes + live.bytes +
                shadow.bytes + samples.bytes + o_rgba.bytes + o_u8.bytes;
     }
 };
@@ -1042,6 +1148,8 @@ int render_batch(crt_scene* scene, uint32_t n_cams, const crt_camera* cams, cons
     pstate.reserve(cap_paths * PATH_BYTES);
     q0.reserve(cap_paths * 4);
     q1.reserve(cap_paths * 4);
+    W.qs.reserve(cap_paths * 4);
+    W.bins.reserve(2 * SORT_BINS * 4);
     live.reserve(cap_paths * 4);
     shadow.reserve(cap_paths * std::max(1, C.nl) * sizeof(ShadowRec));
     tally.reserve(16);
@@ -1069,6 +1177,8 @@ int render_batch(crt_scene* scene, uint32_t n_cams, const crt_camera* cams, cons
     }
     C.queue[0] = q0.as<uint32_t>();
     C.queue[1] = q1.as<uint32_t>();
+    C.sorted = W.qs.as<uint32_t>();
+    C.bins = W.bins.as<uint32_t>();
     C.live = live.as<uint32_t>();
     C.shadow = shadow.as<ShadowRec>();
     C.tally = tally.as<unsigned long long>();

@@ -329,6 +329,10 @@ typedef struct {
                                  per stage; 2: fused, except octet-traced bounces, which use
                                  separate trace and shade kernels.  Persistent traversal always
                                  uses separate kernels.  All give bit-identical images. */
+    uint32_t coherence_sort;  /* 1: order each bounce's extension rays (after the first) by the cell
+                                 of their origin before the fused trace kernels; 0 (default): queue
+                                 order, which already follows the pixel tiles of the paths.  The
+                                 image is the same either way. */
 } crt_batch_opts;
 int crt_render_batch(crt_scene* scene, uint32_t n_cams, const crt_camera* cams, const int32_t* max_samples,
                      const uint64_t* seeds, const crt_light* lights, uint32_t n_lights,

@@ -102,3 +102,19 @@ def test_fused_equals_separate_kernels(body, extend_mode, fused, n_lights, bounc
         assert np.array_equal(a[1], b[1])
     assert tms["rays_traced"] == tmf["rays_traced"]
     assert tms["shadow_rays"] == tmf["shadow_rays"]
+
+
+def test_coherence_sort_same_images(body):
+    # ordering the bounce queues by ray-origin cell changes only the order in
+    # which paths are traced: every image is bit-identical, on the octet
+    # (extend_mode 3) and the all-lane (extend_mode 2) bounce kernels
+    m, s = body
+    cams = _cams(m, 5, seed=9)
+    sun = {"type": "directional", "intensity": 1.0, "direction": scenes.sun_direction(2)}
+    seeds = [21 + k for k in range(len(cams))]
+    for em in (3, 2):
+        ref, _ = s.render_batch(cams, [sun], 4, 4, 0.0, 3, seeds=seeds, extend_mode=em)
+        out, tm = s.render_batch(cams, [sun], 4, 4, 0.0, 3, seeds=seeds, extend_mode=em, coherence_sort=1)
+        for k in range(len(cams)):
+            assert np.array_equal(out[k][0], ref[k][0]), (em, k)
+        assert tm["rays_traced"] > 0
