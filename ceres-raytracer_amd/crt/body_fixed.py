@@ -66,9 +66,25 @@ class BodyFixedGroup(RigidBody):
     def transform_to_body(self, position: ArrayLike, rotation: ArrayLike) -> Tuple[np.ndarray, np.ndarray]:
         """FP64 pose in the group frame: p' = scale R_g (p - p_g), R' = (R_g R^T)^T
         (crt/body_fixed.py:87-101; the reference multiplies by the scale)."""
-        relative_position = self.scale * (self.rotation @ (np.asarray(position, dtype=np.float64) - self.position))
-        relative_rotation = (self.rotation @ np.asarray(rotation, dtype=np.float64).T).T
-        return relative_position, relative_rotation
+        p, r = self.transform_to_body_batch(np.asarray(position, dtype=np.float64).reshape(1, 3),
+                                            np.asarray(rotation, dtype=np.float64).reshape(3, 3, 1))
+        return p[0], r[:, :, 0]
+
+    def transform_to_body_batch(self, positions: ArrayLike, rotations: ArrayLike) -> Tuple[np.ndarray, np.ndarray]:
+        """transform_to_body of N poses at once: positions (N, 3), rotations
+        (3, 3, N).  Every product is summed left to right over j in FP64, so
+        pose k gets the bits transform_to_body gives it alone."""
+        G = np.asarray(self.rotation, dtype=np.float64)
+        d = np.asarray(positions, dtype=np.float64) - np.asarray(self.position, dtype=np.float64)
+        rp = np.empty_like(d)
+        for i in range(3):
+            rp[:, i] = self.scale * (G[i, 0] * d[:, 0] + G[i, 1] * d[:, 1] + G[i, 2] * d[:, 2])
+        R = np.asarray(rotations, dtype=np.float64)
+        rr = np.empty_like(R)
+        for i in range(3):   # R' = (G R^T)^T = R G^T: R'[i, l] = sum_j R[i, j] G[l, j]
+            for l in range(3):
+                rr[i, l, :] = R[i, 0, :] * G[l, 0] + R[i, 1, :] * G[l, 1] + R[i, 2, :] * G[l, 2]
+        return rp, rr
 
     def _to_body(self, obj):
         p, r = self.transform_to_body(obj.position, obj.rotation)
@@ -107,13 +123,7 @@ class BodyFixedGroup(RigidBody):
         return self._cpp.simulate_lidar(lidar._cpp, num_rays)
 
     def batch_simulate_lidar(self, lidar: Lidar, num_rays: int = 1):
-        positions = lidar.batch_positions
-        rotations = lidar.batch_rotations
-        n = positions.shape[0]
-        rel_p = np.zeros(positions.shape)
-        rel_r = np.zeros(rotations.shape)
-        for k in range(n):
-            rel_p[k, :], rel_r[:, :, k] = self.transform_to_body(positions[k, :], rotations[:, :, k])
+        rel_p, rel_r = self.transform_to_body_batch(lidar.batch_positions, lidar.batch_rotations)
         lidar.batch_set_pose(rel_p, rel_r)
         return self._cpp.batch_simulate_lidar(lidar._cpp, num_rays)
 

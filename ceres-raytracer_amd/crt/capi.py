@@ -297,6 +297,13 @@ class DeviceBuffer:
         if _hip().hipMemset(self._p, int(byte), C.c_size_t(self.nbytes)) != 0:
             raise RuntimeError("hipMemset failed")
 
+    def from_host(self, a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        if _hip().hipMemcpy(self._p, a.ctypes.data, C.c_size_t(a.nbytes), 1) != 0:   # host to device
+            raise RuntimeError("hipMemcpy failed")
+        return self
+
     def to_host(self, out: np.ndarray) -> np.ndarray:
         assert out.nbytes <= self.nbytes and out.flags["C_CONTIGUOUS"]
         if _hip().hipMemcpy(out.ctypes.data, self._p, C.c_size_t(out.nbytes), 2) != 0:   # device to host
@@ -547,6 +554,13 @@ class Scene:
         _check(load().crt_lidar_batch(self.h, pos.shape[0], ptr(pos), ptr(rot), int(bool(z_positive)), ptr(out), 0,
                                       None, C.byref(tm)))
         return out
+
+    def lidar_batch_device(self, n, positions, rotations, ranges, z_positive=False, stream=None):
+        """crt_lidar_batch on device-resident poses: `positions` (n x 3 f64),
+        `rotations` (n x 9 f64, row-major R per pose) and `ranges` (n f64) are
+        device addresses; only enqueues the work on `stream`."""
+        _check(load().crt_lidar_batch(self.h, int(n), ptr(positions), ptr(rotations), int(bool(z_positive)),
+                                      ptr(ranges), CRT_OUT_DEVICE, stream, None))
 
     def flash_lidar(self, beams: dict, sub_rays=8, sigma=5e-4, seed=0, outputs=None, device=False, stream=None,
                     timing=True):

@@ -6,7 +6,7 @@ from numpy.typing import ArrayLike
 
 import _crt
 from crt.rigid_body import RigidBody, _spice
-from crt._validate_values import validate_position, validate_rotation
+from crt._validate_values import validate_positions, validate_rotation, validate_rotations
 
 
 class Lidar(ABC):
@@ -15,16 +15,13 @@ class Lidar(ABC):
 
         The reference caps N at 100000 (src/crt/lidars/lidar.hpp:11); the HIP
         core has no cap."""
-        batch_positions = np.asarray(batch_positions, dtype=np.float64)
+        batch_positions = np.asarray(batch_positions, dtype=np.float64).reshape(-1, 3)
         batch_rotations = np.asarray(batch_rotations, dtype=np.float64)
         n = batch_positions.shape[0]
-        if batch_rotations.shape[:2] != (3, 3) or batch_rotations.shape[2] < n:
+        if batch_rotations.ndim != 3 or batch_rotations.shape[:2] != (3, 3) or batch_rotations.shape[2] < n:
             raise ValueError("batch_rotations must have shape (3,3,N)")
-        self.batch_positions = np.stack([validate_position(p) for p in batch_positions]) if n else \
-            np.zeros((0, 3))
-        rot = np.zeros((3, 3, n))
-        for k in range(n):
-            rot[:, :, k] = validate_rotation(batch_rotations[:, :, k])
+        self.batch_positions = validate_positions(batch_positions)
+        rot = validate_rotations(batch_rotations[:, :, :n])
         self.batch_rotations = rot
         self._cpp.batch_set_pose(np.ascontiguousarray(self.batch_positions), np.ascontiguousarray(rot), n)
 
