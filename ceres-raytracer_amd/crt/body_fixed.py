@@ -12,6 +12,23 @@ from crt.lights import Light
 from crt.rigid_body import RigidBody
 
 
+def _to_body(group, positions, rotations):
+    """p' = s G (p - p_g) and R' = (G R^T)^T = R G^T for N poses (positions
+    (N, 3), rotations (3, 3, N)), every product summed left to right over j
+    in FP64 so that one pose and a batch give the same bits."""
+    G = np.asarray(group.rotation, dtype=np.float64)
+    d = np.asarray(positions, dtype=np.float64) - np.asarray(group.position, dtype=np.float64)
+    rp = np.empty_like(d)
+    for i in range(3):
+        rp[:, i] = group.scale * (G[i, 0] * d[:, 0] + G[i, 1] * d[:, 1] + G[i, 2] * d[:, 2])
+    R = np.asarray(rotations, dtype=np.float64)
+    rr = np.empty_like(R)
+    for i in range(3):   # R'[i, l] = sum_j R[i, j] G[l, j]
+        for l in range(3):
+            rr[i, l, :] = R[i, 0, :] * G[l, 0] + R[i, 1, :] * G[l, 1] + R[i, 2, :] * G[l, 2]
+    return rp, rr
+
+
 class BodyFixedEntity(RigidBody):
     """Mesh, colour and pose of one member of a BodyFixedGroup.
 
@@ -66,25 +83,14 @@ class BodyFixedGroup(RigidBody):
     def transform_to_body(self, position: ArrayLike, rotation: ArrayLike) -> Tuple[np.ndarray, np.ndarray]:
         """FP64 pose in the group frame: p' = scale R_g (p - p_g), R' = (R_g R^T)^T
         (crt/body_fixed.py:87-101; the reference multiplies by the scale)."""
-        p, r = self.transform_to_body_batch(np.asarray(position, dtype=np.float64).reshape(1, 3),
-                                            np.asarray(rotation, dtype=np.float64).reshape(3, 3, 1))
+        p, r = _to_body(self, np.asarray(position, dtype=np.float64).reshape(1, 3),
+                        np.asarray(rotation, dtype=np.float64).reshape(3, 3, 1))
         return p[0], r[:, :, 0]
 
     def transform_to_body_batch(self, positions: ArrayLike, rotations: ArrayLike) -> Tuple[np.ndarray, np.ndarray]:
         """transform_to_body of N poses at once: positions (N, 3), rotations
-        (3, 3, N).  Every product is summed left to right over j in FP64, so
-        pose k gets the bits transform_to_body gives it alone."""
-        G = np.asarray(self.rotation, dtype=np.float64)
-        d = np.asarray(positions, dtype=np.float64) - np.asarray(self.position, dtype=np.float64)
-        rp = np.empty_like(d)
-        for i in range(3):
-            rp[:, i] = self.scale * (G[i, 0] * d[:, 0] + G[i, 1] * d[:, 1] + G[i, 2] * d[:, 2])
-        R = np.asarray(rotations, dtype=np.float64)
-        rr = np.empty_like(R)
-        for i in range(3):   # R' = (G R^T)^T = R G^T: R'[i, l] = sum_j R[i, j] G[l, j]
-            for l in range(3):
-                rr[i, l, :] = R[i, 0, :] * G[l, 0] + R[i, 1, :] * G[l, 1] + R[i, 2, :] * G[l, 2]
-        return rp, rr
+        (3, 3, N); pose k gets the bits transform_to_body gives it alone."""
+        return _to_body(self, positions, rotations)
 
     def _to_body(self, obj):
         p, r = self.transform_to_body(obj.position, obj.rotation)
