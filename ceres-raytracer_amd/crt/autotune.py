@@ -1,17 +1,31 @@
-"""Autotuner for batched rendering launch parameters.
+"""Autotuner for batched rendering launch parameters (crt_batch_opts).
 
-Tunes crt_batch_opts -- samples per pass, hipGraph replay of a pass,
-persistent traversal kernels, persistent blocks per CU, rays claimed per
-atomic and the traversal form of extension rays -- by timing a representative
-subset of the batch (the first cameras at their own resolution, capped only
-for very large ones: small images misjudge the traversal forms) and keeps the
-winner in a JSON cache keyed by device, scene size, bounces and light count.
-Every candidate renders the same images (the RNG is keyed per camera/pixel/
-sample; only the traversal form can flip an exact tie between two
-triangles), so only speed is compared.
+Two stages, timed on a representative subset of the batch (the first
+cameras at their own resolution, capped only for very large ones: small
+images misjudge the traversal forms):
+
+  1. launch presets: samples per pass, hipGraph replay of a pass, persistent
+     traversal kernels (blocks per CU, rays claimed per atomic), the
+     traversal form of extension rays and the kernel split;
+  2. around the stage-1 winner, one knob at a time: paths per octet in the
+     octet bounce kernels (rays_per_octet), live paths per quad in the
+     shadow kernel (rays_per_quad) and the camera pixel tile shape
+     (tile_width).
+
+Every candidate renders the same images: the RNG is keyed per camera /
+pixel / sample, and only the traversal form (lanes vs octets) can flip an
+exact tie between two triangles; the stage-2 knobs change no traversal.
+So only speed is compared.
+
+Winners are cached per process, keyed by device, library build (a hash of
+libcrt_hip.so), scene size, bounces and light count.  A file cache is used
+only when asked for (`cache_path`, or the CRT_AUTOTUNE_CACHE environment
+variable), so a choice never outlives the build or the process that measured
+it unless the caller keeps it on purpose.
 """
 from __future__ import annotations
 
+import hashlib
 import json
 import os
 import time
@@ -20,9 +34,7 @@ import numpy as np
 
 from crt import capi
 
-CACHE = os.environ.get("CRT_AUTOTUNE_CACHE", os.path.join(os.path.expanduser("~"), ".cache", "crt", "autotune.json"))
-
-CANDIDATES = [
+PRESETS = [
     dict(samples_per_pass=0, use_graph=False, persistent=False, extend_mode=1),
     dict(samples_per_pass=0, use_graph=True, persistent=False, extend_mode=1),
     dict(samples_per_pass=0, use_graph=False, persistent=False, extend_mode=3),
@@ -38,6 +50,17 @@ CANDIDATES = [
     dict(samples_per_pass=0, use_graph=True, persistent=True, blocks_per_cu=32, grab=4, extend_mode=3),
     dict(samples_per_pass=0, use_graph=True, persistent=True, blocks_per_cu=64, grab=1, extend_mode=1),
 ]
+CANDIDATES = PRESETS   # name kept for callers of the first version
+
+# stage 2: values tried for each knob (0 = the library's choice by pass size)
+KNOBS = {
+    "rays_per_octet": [0, 1, 2, 4, 8],
+    "rays_per_quad": [0, 1, 2, 4],
+    "tile_width": [8, 16, 4, 32],
+}
+
+_memory = {}
+_lib_hash = None
 
 
 def _device_name():
@@ -54,59 +77,105 @@ def _device_name():
     return "unknown"
 
 
-def _key(scene: capi.Scene, bounces, n_lights):
+def _build_id():
+    """Hash of the loaded libcrt_hip.so: a rebuilt library is a new key."""
+    global _lib_hash
+    if _lib_hash is None:
+        h = hashlib.sha256()
+        with open(capi.LIB_PATH, "rb") as fh:
+            for chunk in iter(lambda: fh.read(1 << 20), b""):
+                h.update(chunk)
+        _lib_hash = h.hexdigest()[:16]
+    return _lib_hash
+
+
+def cache_key(scene: capi.Scene, bounces, n_lights):
     n = scene.n_tris
     bucket = int(np.round(np.log2(max(n, 1))))
-    return "v3|%s|tris2^%d|b%d|l%d" % (_device_name(), bucket, bounces, n_lights)
+    return "v4|%s|%s|tris2^%d|b%d|l%d" % (_device_name(), _build_id(), bucket, bounces, n_lights)
 
 
-def load_cache():
+def _cache_path(cache_path):
+    return cache_path or os.environ.get("CRT_AUTOTUNE_CACHE") or None
+
+
+def load_cache(path):
     try:
-        with open(CACHE) as fh:
+        with open(path) as fh:
             return json.load(fh)
     except (OSError, ValueError):
         return {}
 
 
-def save_cache(cache):
-    os.makedirs(os.path.dirname(CACHE), exist_ok=True)
-    tmp = CACHE + ".tmp"
+def save_cache(path, cache):
+    d = os.path.dirname(os.path.abspath(path))
+    os.makedirs(d, exist_ok=True)
+    tmp = path + ".tmp"
     with open(tmp, "w") as fh:
         json.dump(cache, fh, indent=1, sort_keys=True)
-    os.replace(tmp, CACHE)
+    os.replace(tmp, path)
 
 
-def tune(scene: capi.Scene, cams, lights, spp, bounces, max_cams=48, max_res=1024, repeats=1, use_cache=True,
-         log=None):
-    """Return the fastest crt_batch_opts (dict) for this kind of batch."""
-    key = _key(scene, bounces, len(lights))
-    cache = load_cache() if use_cache else {}
-    if key in cache:
-        return cache[key]["opts"]
+def _subset(cams, max_cams, max_res):
     sub = []
     for c in cams[:max_cams]:
         c = dict(c)
         r = min(int(c["resolution"][0]), max_res)
-        scale = r / float(c["resolution"][0])
         c["resolution"] = [r, r]
         c["sensor_size"] = [c["sensor_size"][0], c["sensor_size"][1]]
         sub.append(c)
-    results = []
-    for cand in CANDIDATES:
+    return sub
+
+
+def tune(scene: capi.Scene, cams, lights, spp, bounces, max_cams=48, max_res=1024, repeats=1, use_cache=True,
+         cache_path=None, stage2=True, log=None):
+    """Return the fastest crt_batch_opts (dict) for this kind of batch."""
+    key = cache_key(scene, bounces, len(lights))
+    path = _cache_path(cache_path)
+    if use_cache:
+        if key in _memory:
+            return dict(_memory[key]["opts"])
+        if path:
+            disk = load_cache(path)
+            if key in disk:
+                _memory[key] = disk[key]
+                return dict(disk[key]["opts"])
+    sub = _subset(cams, max_cams, max_res)
+    seeds = list(range(len(sub)))
+
+    def timed(cand):
         best = None
         for _ in range(repeats + 1):
             t0 = time.perf_counter()
-            _, tm = scene.render_batch(sub, lights, spp, spp, 0.0, bounces, seeds=list(range(len(sub))),
-                                       outputs=False, **cand)
+            scene.render_batch(sub, lights, spp, spp, 0.0, bounces, seeds=seeds, outputs=False, **cand)
             dt = time.perf_counter() - t0
             best = dt if best is None else min(best, dt)
-        results.append((best, cand))
         if log:
             log("autotune %s: %.2f ms" % (cand, best * 1e3))
+        return best
+
+    results = [(timed(c), dict(c)) for c in PRESETS]
     results.sort(key=lambda x: x[0])
-    winner = results[0][1]
-    cache[key] = {"opts": winner, "ms": results[0][0] * 1e3,
-                  "all": [{"opts": c, "ms": t * 1e3} for t, c in results]}
+    best_t, best = results[0]
+    if stage2:
+        for knob, values in KNOBS.items():
+            if knob == "rays_per_octet" and best.get("extend_mode", 0) == 2:
+                continue        # no octet bounce kernel in the all-lane form
+            if knob == "rays_per_quad" and best.get("persistent", False):
+                continue        # the persistent form has its own shadow kernel
+            for v in values:
+                if best.get(knob, 0 if knob != "tile_width" else 8) == v:
+                    continue
+                cand = dict(best, **{knob: v})
+                t = timed(cand)
+                results.append((t, cand))
+                if t < best_t:
+                    best_t, best = t, cand
+    entry = {"opts": best, "ms": best_t * 1e3, "all": [{"opts": c, "ms": t * 1e3} for t, c in results]}
     if use_cache:
-        save_cache(cache)
-    return winner
+        _memory[key] = entry
+        if path:
+            disk = load_cache(path)
+            disk[key] = entry
+            save_cache(path, disk)
+    return dict(best)

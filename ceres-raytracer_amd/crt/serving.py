@@ -25,7 +25,9 @@ class Server:
         bo = capi.BatchOpts(int(ro.get("samples_per_pass", 0)), int(bool(ro.get("use_graph", False))),
                             int(bool(ro.get("persistent", False))), int(ro.get("blocks_per_cu", 0)),
                             int(ro.get("grab", 0)), int(ro.get("extend_mode", 0)),
-                            int(ro.get("separate_kernels", 0)), int(ro.get("coherence_sort", 0)))
+                            int(ro.get("separate_kernels", 0)), int(ro.get("coherence_sort", 0)),
+                            int(ro.get("rays_per_octet", 0)), int(ro.get("rays_per_quad", 0)),
+                            int(ro.get("tile_width", 0)))
         o = capi.ServerOpts(int(memory_budget), int(n_workers), int(max_batch), int(max_batch_paths),
                             int(bool(batching)), 0, bo)
         h = C.c_void_p()

@@ -628,28 +628,36 @@ __device__ inline HitRec trace_lane(const SceneView& s, const RayPre& r, float t
 }
 
 // ------------------------------------------------------------------------
-// Pixel order for coherent waves: the i-th pixel of a w x h image in 8x8
-// tiles (row-major over tiles, Z-order inside a tile), then the pixels that
-// no whole tile covers (the right strip row by row, then the bottom rows).
-// A bijection of [0, w*h): consecutive i stay in compact 2^k x 2^k blocks,
-// so a wave's rays share BVH nodes.
+// Pixel order for coherent waves: the i-th pixel of a w x h image in tiles
+// of 64 pixels, `tw` wide (8: 8x8, Z-order inside; 4, 16, 32: 4x16, 16x4,
+// 32x2, row-major inside), row-major over the tiles, then the pixels that no
+// whole tile covers (the right strip row by row, then the bottom rows).
+// A bijection of [0, w*h): consecutive i stay in compact blocks, so a wave's
+// rays share BVH nodes.
 // ------------------------------------------------------------------------
-__host__ __device__ inline uint32_t tile_order(uint32_t i, uint32_t w, uint32_t h) {
-    const uint32_t tw = w >> 3, th = h >> 3;
-    const uint32_t nfull = tw * th * 64u;
+__host__ __device__ inline uint32_t tile_order(uint32_t i, uint32_t w, uint32_t h, uint32_t tw = 8u) {
+    const uint32_t th = 64u / tw;
+    const uint32_t ntx = w / tw, nty = h / th;
+    const uint32_t nfull = ntx * nty * 64u;
     if (i < nfull) {
         const uint32_t tile = i >> 6, z = i & 63u;
-        const uint32_t tx = tile % tw, ty = tile / tw;
-        const uint32_t x = (z & 1u) | ((z >> 1) & 2u) | ((z >> 2) & 4u);
-        const uint32_t y = ((z >> 1) & 1u) | ((z >> 2) & 2u) | ((z >> 3) & 4u);
-        return (ty * 8u + y) * w + tx * 8u + x;
+        const uint32_t tx = tile % ntx, ty = tile / ntx;
+        uint32_t x, y;
+        if (tw == 8u) {
+            x = (z & 1u) | ((z >> 1) & 2u) | ((z >> 2) & 4u);
+            y = ((z >> 1) & 1u) | ((z >> 2) & 2u) | ((z >> 3) & 4u);
+        } else {
+            x = z % tw;
+            y = z / tw;
+        }
+        return (ty * th + y) * w + tx * tw + x;
     }
     i -= nfull;
-    const uint32_t rw = w - tw * 8u;
-    const uint32_t nright = th * 8u * rw;
-    if (i < nright) return (i / rw) * w + tw * 8u + i % rw;
+    const uint32_t rw = w - ntx * tw;
+    const uint32_t nright = nty * th * rw;
+    if (i < nright) return (i / rw) * w + ntx * tw + i % rw;
     i -= nright;
-    return (th * 8u + i / w) * w + i % w;
+    return (nty * th + i / w) * w + i % w;
 }
 
 // ------------------------------------------------------------------------

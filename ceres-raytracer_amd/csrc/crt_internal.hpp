@@ -131,6 +131,7 @@ void delete_render_workspace(RenderWorkspace* w);
 size_t render_bytes_estimate(uint64_t pixels, int spp, int n_lights);
 size_t render_workspace_bytes(const RenderWorkspace* w);   // device bytes it holds
 void check_light(const crt_light& L);                      // raise on bad light parameters
+void check_batch_opts(const crt_batch_opts& B);            // raise on bad launch parameters
 // `lights` holds n_lights lights shared by every camera, or, with
 // per_camera_lights, n_cams x n_lights (camera c's lights at c * n_lights).
 int render_batch(crt_scene* scene, uint32_t n_cams, const crt_camera* cams, const int32_t* max_samples,

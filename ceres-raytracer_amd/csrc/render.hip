@@ -135,6 +135,7 @@ struct Ctx {
     float thr;
     float offset;
     uint32_t grab;
+    uint32_t tile_w;       // camera pixel tiles: 64 pixels, tile_w wide (tile_order)
     const CamDev* cams;
     const LightDev* lights;
     const Segment* segs;
@@ -216,7 +217,7 @@ __device__ inline bool gen_camera_path(const Ctx& C, const PassCtl& ctl, uint64_
         const uint32_t i = local / sg.s_count;
         const uint32_t smp = sg.s_first + local % sg.s_count;
         const CamDev& cd = C.cams[sg.cam];
-        const uint32_t pix = sg.plist_off == NO_LIST ? tile_order(i, (uint32_t)cd.v.w, (uint32_t)cd.v.h)
+        const uint32_t pix = sg.plist_off == NO_LIST ? tile_order(i, (uint32_t)cd.v.w, (uint32_t)cd.v.h, C.tile_w)
                                                      : C.plist[sg.plist_off + i];
         const int px = (int)(pix % (uint32_t)cd.v.w), py = (int)(pix / (uint32_t)cd.v.w);
         double u = px, v = py;
@@ -853,7 +854,7 @@ __global__ __launch_bounds__(TB) void k_fold(Ctx C) {
     const Segment sg = C.segs[si];
     const uint32_t i = t - sg.fold_begin;
     const CamDev& cd = C.cams[sg.cam];
-    const uint32_t pix = sg.plist_off == NO_LIST ? tile_order(i, (uint32_t)cd.v.w, (uint32_t)cd.v.h)
+    const uint32_t pix = sg.plist_off == NO_LIST ? tile_order(i, (uint32_t)cd.v.w, (uint32_t)cd.v.h, C.tile_w)
                                                  : C.plist[sg.plist_off + i];
     const uint32_t g = cd.pix_base + pix;
     float4 m = C.mean[g];
@@ -924,8 +925,10 @@ void launch_pass(const Ctx& C, uint64_t cap_paths, uint32_t cap_fold, const crt_
         const bool gen = C.nb > 0 && lane_at(0);
         if (!gen && C.nb > 0) hipLaunchKernelGGL(k_raygen, dim3(gp), dim3(TB), 0, st, C);
         // paths per octet / quad: enough groups for ~32k waves, at most one path per lane
-        const int ro = cap_paths >= (1u << 21) ? 8 : cap_paths >= (1u << 20) ? 4 : cap_paths >= (1u << 19) ? 2 : 1;
-        const int rq = cap_paths >= (1u << 21) ? 4 : cap_paths >= (1u << 20) ? 2 : 1;
+        const int ro = B.rays_per_octet != 0 ? (int)B.rays_per_octet
+                       : cap_paths >= (1u << 21) ? 8 : cap_paths >= (1u << 20) ? 4 : cap_paths >= (1u << 19) ? 2 : 1;
+        const int rq = B.rays_per_quad != 0 ? (int)B.rays_per_quad
+                       : cap_paths >= (1u << 21) ? 4 : cap_paths >= (1u << 20) ? 2 : 1;
         const bool sort = B.coherence_sort == 1;
         for (int b = 0; b < C.nb; ++b) {
             const int cur = b & 1, nxt = cur ^ 1;
@@ -1002,6 +1005,17 @@ void delete_render_workspace(RenderWorkspace* w) { delete w; }
 
 size_t render_workspace_bytes(const RenderWorkspace* w) { return w ? w->bytes() : 0; }
 
+void check_batch_opts(const crt_batch_opts& B) {
+    CRT_REQUIRE(B.extend_mode <= 3, CRT_ERR_INVALID, "extend_mode must be 0..3");
+    CRT_REQUIRE(B.separate_kernels <= 2, CRT_ERR_INVALID, "separate_kernels must be 0..2");
+    CRT_REQUIRE(B.coherence_sort <= 1, CRT_ERR_INVALID, "coherence_sort must be 0 or 1");
+    CRT_REQUIRE(B.rays_per_octet <= 8, CRT_ERR_INVALID, "rays_per_octet must be 0 (by pass size) or 1..8");
+    CRT_REQUIRE(B.rays_per_quad <= 4, CRT_ERR_INVALID, "rays_per_quad must be 0 (by pass size) or 1..4");
+    CRT_REQUIRE(B.tile_width == 0 || B.tile_width == 4 || B.tile_width == 8 || B.tile_width == 16 ||
+                    B.tile_width == 32,
+                CRT_ERR_INVALID, "tile_width must be 0 (8), 4, 8, 16 or 32");
+}
+
 void check_light(const crt_light& L) {
     CRT_REQUIRE(L.type >= 0 && L.type <= 2, CRT_ERR_TYPE, "unknown light type");
     bool fin = std::isfinite(L.intensity) && std::isfinite(L.size[0]) && std::isfinite(L.size[1]);
@@ -1047,6 +1061,8 @@ int render_batch(crt_scene* scene, uint32_t n_cams, const crt_camera* cams, cons
     if (bopts) B = *bopts;
     if (B.grab == 0) B.grab = 4;
     if (B.blocks_per_cu == 0) B.blocks_per_cu = 8;
+    if (B.tile_width == 0) B.tile_width = 8;
+    check_batch_opts(B);
     hipStream_t st = (hipStream_t)stream;
     // stream capture is not allowed on the legacy null stream: graph replay
     // on a null stream runs on a private stream (this call synchronises at
@@ -1121,6 +1137,7 @@ int render_batch(crt_scene* scene, uint32_t n_cams, const crt_camera* cams, cons
     C.thr = (float)opts->noise_threshold;
     C.offset = self_offset(scene);
     C.grab = B.grab;
+    C.tile_w = B.tile_width;
 
     // ---- pass planning ----
     const uint64_t budget = pass_budget(C.nl);   // paths per pass

@@ -458,6 +458,7 @@ int crt_server_create(const crt_server_opts* opts, crt_server** out) {
         *out = nullptr;
         std::unique_ptr<crt_server> S(new crt_server());
         S->opts = *opts;
+        check_batch_opts(S->opts.render);
         if (S->opts.memory_budget == 0) S->opts.memory_budget = (uint64_t)16 << 30;
         if (S->opts.n_workers == 0) S->opts.n_workers = 3;
         if (S->opts.max_batch == 0) S->opts.max_batch = 64;

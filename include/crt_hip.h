@@ -333,6 +333,12 @@ typedef struct {
                                  of their origin before the fused trace kernels; 0 (default): queue
                                  order, which already follows the pixel tiles of the paths.  The
                                  image is the same either way. */
+    uint32_t rays_per_octet;  /* octet bounce kernels: paths one octet traces in turn (1..8); 0 = by
+                                 pass size (8 from 2^21 paths, 4 from 2^20, 2 from 2^19, else 1) */
+    uint32_t rays_per_quad;   /* shadow kernel: live paths one quad tests in turn (1..4); 0 = by pass
+                                 size (4 from 2^21 paths, 2 from 2^20, else 1) */
+    uint32_t tile_width;      /* camera rays of a wave: a tile of 64 pixels this wide (4, 8, 16 or
+                                 32; 0 = 8, i.e. 8x8) */
 } crt_batch_opts;
 int crt_render_batch(crt_scene* scene, uint32_t n_cams, const crt_camera* cams, const int32_t* max_samples,
                      const uint64_t* seeds, const crt_light* lights, uint32_t n_lights,
