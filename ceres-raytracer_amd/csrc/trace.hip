@@ -938,6 +938,14 @@ int crt_sun_pass(crt_scene* scene, const crt_camera* camera, const double sun_di
                            shadow_slots(ws.h.as<ShadowRay>(), npix), qcount, offset);
         EventTimer::rec(tm.b, st);
         hipEvent_t lb = log ? tlog_event(st) : nullptr, lc = log ? tlog_event(st) : nullptr;
+        // one one-wave block per 16 pixels, though only hits have a shadow
+        // ray: blocks past the queue exit after one load, and blocks
+        // dispatched in order keep the rays in flight one contiguous stretch
+        // of the queue (neighbouring pixels sharing BVH nodes in L2).  On the
+        // 2048^2 L8 pass (1.8M shadow rays, 57 % of the blocks empty) this
+        // takes 0.35 ms; a grid of 32 blocks per CU striding over the queue
+        // 0.48 ms (the rays in flight span the image), with an atomic claim
+        // of 16 rays per wave 1.42 ms (one contended counter).
         hipLaunchKernelGGL(k_shadow_sun, dim3(cdiv(npix, TRACE_BLOCK / QUAD)), dim3(TRACE_BLOCK),
                            quad_lds_bytes(scene->view), st, scene->view, ws.h.as<ShadowRay>(),
                            shadow_slots(ws.h.as<ShadowRay>(), npix), qcount, sun, d_lit);
