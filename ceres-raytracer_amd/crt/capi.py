@@ -21,6 +21,8 @@ CRT_ERR_OOM = 3
 CRT_ERR_HIP = 4
 CRT_ERR_STATE = 5
 CRT_ERR_ADMISSION = 6
+CRT_ERR_CANCELLED = 7
+CRT_ERR_TIMEOUT = 8
 
 CRT_OUT_DEVICE = 1
 CRT_ANY_HIT = 2
@@ -105,7 +107,7 @@ class Request(C.Structure):
 class ServerStats(C.Structure):
     _fields_ = [("completed", C.c_uint64), ("rejected", C.c_uint64), ("rays", C.c_uint64), ("batches", C.c_uint64),
                 ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("mean_ms", C.c_double), ("wall_s", C.c_double),
-                ("mrays_per_s", C.c_double)]
+                ("mrays_per_s", C.c_double), ("cancelled", C.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -169,6 +171,8 @@ SIGNATURES = {
     "crt_server_add_scene": (C.c_int, [_p, _p, _p]),
     "crt_server_submit": (C.c_int, [_p, _p, _p]),
     "crt_server_wait": (C.c_int, [_p, C.c_uint64, _p]),
+    "crt_server_wait_for": (C.c_int, [_p, C.c_uint64, C.c_double, _p]),
+    "crt_server_cancel": (C.c_int, [_p, C.c_uint64]),
     "crt_server_drain": (C.c_int, [_p]),
     "crt_server_stats_get": (C.c_int, [_p, _p]),
     "crt_server_destroy": (None, [_p]),
@@ -207,6 +211,8 @@ def _check(code):
             raise TypeError(msg)
         if code == CRT_ERR_OOM:
             raise MemoryError(msg)
+        if code == CRT_ERR_TIMEOUT:
+            raise TimeoutError(msg)
         raise CrtError(code, msg)
 
 

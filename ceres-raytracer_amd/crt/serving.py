@@ -87,10 +87,27 @@ class Server:
         self._keep[t.value] = out
         return t.value
 
-    def result(self, ticket: int):
+    def result(self, ticket: int, timeout_ms=None):
+        """(outputs, latency ms) of a request.  With timeout_ms, raises
+        TimeoutError if it has not finished by then (ask again later); a
+        cancelled request raises CrtError (CRT_ERR_CANCELLED)."""
         lat = C.c_double()
-        capi._check(capi.load().crt_server_wait(self.h, ticket, C.byref(lat)))
-        return self._keep.pop(ticket), lat.value
+        t = float("inf") if timeout_ms is None else float(timeout_ms)
+        code = capi.load().crt_server_wait_for(self.h, ticket, t, C.byref(lat))
+        if code == capi.CRT_ERR_TIMEOUT:
+            capi._check(code)
+        out = self._keep.pop(ticket, None)
+        capi._check(code)
+        return out, lat.value
+
+    def cancel(self, ticket: int) -> bool:
+        """Cancel a request that is still queued; False if it is already
+        running or finished (its result then comes as usual)."""
+        code = capi.load().crt_server_cancel(self.h, ticket)
+        if code == capi.CRT_ERR_STATE:
+            return False
+        capi._check(code)
+        return True
 
     def drain(self):
         capi._check(capi.load().crt_server_drain(self.h))

@@ -33,8 +33,10 @@
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
+#include <cmath>
 #include <cstring>
 #include <deque>
+#include <limits>
 #include <memory>
 #include <mutex>
 #include <thread>
@@ -170,7 +172,7 @@ struct crt_server {
     std::vector<const crt_scene*> running_scene;   // per worker, null when idle
     // statistics
     std::vector<double> latencies_ms;
-    uint64_t completed = 0, rejected = 0, rays = 0, batches = 0, inflight = 0;
+    uint64_t completed = 0, rejected = 0, cancelled = 0, rays = 0, batches = 0, inflight = 0;
     bool have_first = false;
     crt::Clock::time_point first_submit, last_done;
 };
@@ -528,18 +530,51 @@ int crt_server_submit(crt_server* S, const crt_request* req, uint64_t* ticket) {
     });
 }
 
-int crt_server_wait(crt_server* S, uint64_t ticket, double* latency_ms) {
+int crt_server_wait_for(crt_server* S, uint64_t ticket, double timeout_ms, double* latency_ms) {
     return guarded([&] {
         CRT_REQUIRE(S != nullptr, CRT_ERR_INVALID, "NULL server");
+        CRT_REQUIRE(timeout_ms >= 0.0, CRT_ERR_INVALID, "timeout_ms must be >= 0 (infinity: no limit)");
         std::shared_ptr<Job> j;
         std::unique_lock<std::mutex> lk(S->mu);
         auto it = S->jobs.find(ticket);
         CRT_REQUIRE(it != S->jobs.end(), CRT_ERR_INVALID, "unknown ticket");
         j = it->second;
-        S->cv_done.wait(lk, [&] { return j->done; });
+        if (std::isinf(timeout_ms)) {
+            S->cv_done.wait(lk, [&] { return j->done; });
+        } else if (!S->cv_done.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms),
+                                        [&] { return j->done; })) {
+            throw Error(CRT_ERR_TIMEOUT, "request not finished within the timeout");
+        }
         if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(j->done_at - j->submitted).count();
         S->jobs.erase(ticket);
         if (j->status != CRT_OK) throw Error(j->status, j->error);
+    });
+}
+
+int crt_server_wait(crt_server* S, uint64_t ticket, double* latency_ms) {
+    return crt_server_wait_for(S, ticket, std::numeric_limits<double>::infinity(), latency_ms);
+}
+
+int crt_server_cancel(crt_server* S, uint64_t ticket) {
+    return guarded([&] {
+        CRT_REQUIRE(S != nullptr, CRT_ERR_INVALID, "NULL server");
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            auto it = S->jobs.find(ticket);
+            CRT_REQUIRE(it != S->jobs.end(), CRT_ERR_INVALID, "unknown ticket");
+            const std::shared_ptr<Job> j = it->second;
+            CRT_REQUIRE(!j->done, CRT_ERR_STATE, "request already finished");
+            auto q = std::find(S->queue.begin(), S->queue.end(), j);
+            CRT_REQUIRE(q != S->queue.end(), CRT_ERR_STATE, "request is running");
+            S->queue.erase(q);
+            j->done = true;
+            j->done_at = Clock::now();
+            j->status = CRT_ERR_CANCELLED;
+            j->error = "request cancelled before it ran";
+            ++S->cancelled;
+        }
+        S->cv_done.notify_all();
+        S->cv_work.notify_all();   // the queue's head changed
     });
 }
 
@@ -558,6 +593,7 @@ int crt_server_stats_get(crt_server* S, crt_server_stats* out) {
         std::memset(out, 0, sizeof(*out));
         out->completed = S->completed;
         out->rejected = S->rejected;
+        out->cancelled = S->cancelled;
         out->rays = S->rays;
         out->batches = S->batches;
         std::vector<double> l = S->latencies_ms;

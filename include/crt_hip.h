@@ -62,6 +62,8 @@ extern "C" {
 #define CRT_ERR_HIP          4   /* HIP runtime error (RuntimeError)         */
 #define CRT_ERR_STATE        5   /* scene not built / wrong call order       */
 #define CRT_ERR_ADMISSION    6   /* serving: request exceeds memory budget   */
+#define CRT_ERR_CANCELLED    7   /* serving: request cancelled before it ran  */
+#define CRT_ERR_TIMEOUT      8   /* serving: wait timed out (ticket still valid) */
 
 /* ---- flags ------------------------------------------------------------- */
 #define CRT_OUT_DEVICE       1u  /* output (and input ray) pointers are device pointers */
@@ -461,12 +463,20 @@ typedef struct {
     double p50_ms, p99_ms, mean_ms;   /* submit -> completion latency           */
     double wall_s;              /* first submission .. last completion              */
     double mrays_per_s;         /* rays / wall_s                                    */
+    uint64_t cancelled;         /* requests cancelled while queued                  */
 } crt_server_stats;
 
 int  crt_server_create(const crt_server_opts* opts, crt_server** out);
 int  crt_server_add_scene(crt_server* server, crt_scene* scene, uint32_t* scene_id);
 int  crt_server_submit(crt_server* server, const crt_request* request, uint64_t* ticket);
 int  crt_server_wait(crt_server* server, uint64_t ticket, double* latency_ms);
+/* crt_server_wait with a time limit: CRT_ERR_TIMEOUT if the request has not
+ * finished within timeout_ms (the ticket stays valid; wait again later). */
+int  crt_server_wait_for(crt_server* server, uint64_t ticket, double timeout_ms, double* latency_ms);
+/* Cancel a request that is still queued: it never runs, and its wait returns
+ * CRT_ERR_CANCELLED.  A request already running or finished is not cancelled
+ * (CRT_ERR_STATE); its wait returns its result as usual. */
+int  crt_server_cancel(crt_server* server, uint64_t ticket);
 int  crt_server_drain(crt_server* server);
 int  crt_server_stats_get(crt_server* server, crt_server_stats* stats);
 void crt_server_destroy(crt_server* server);

@@ -174,3 +174,42 @@ def test_tight_budget_counts_retained_workspaces(meshes):
         assert np.array_equal(out["rgba"], ref)
     assert srv.stats()["completed"] == 12
     srv.close()
+
+
+def test_cancel_queued_requests_and_wait_timeout(meshes):
+    # one worker, one request at a time: everything behind a large render is
+    # still queued right after submission
+    ms, sc = meshes
+    srv = serving.Server(batching=False)
+    srv.add_scene(sc[0])
+    sun = {"type": "directional", "intensity": 1.0, "direction": scenes.sun_direction(2)}
+    big = {"kind": "render", "scene": 0, "camera": scenes.small_body_camera(ms[0], resolution=1024, seed=3),
+           "spp": 16, "bounces": 2, "seed": 5, "lights": [sun]}
+    cam = scenes.small_body_camera(ms[0], resolution=48, seed=4)
+    small = [{"kind": "render", "scene": 0, "camera": cam, "spp": 2, "bounces": 2, "seed": 10 + k, "lights": [sun]}
+             for k in range(8)]
+    tb = srv.submit(big)
+    ts = [srv.submit(r) for r in small]
+    # a zero timeout on a request waiting behind the large one times out, and
+    # the ticket stays valid
+    with pytest.raises(TimeoutError):
+        srv.result(ts[-1], timeout_ms=0.0)
+    cancelled = [k for k in range(0, 8, 2) if srv.cancel(ts[k])]
+    assert len(cancelled) >= 3          # the large render keeps the worker busy meanwhile
+    out_big, _ = srv.result(tb)
+    assert out_big["rgba"].shape == (1024, 1024, 4)
+    for k, t in enumerate(ts):
+        if k in cancelled:
+            with pytest.raises(capi.CrtError) as e:
+                srv.result(t)
+            assert e.value.code == capi.CRT_ERR_CANCELLED
+        else:
+            out, _ = srv.result(t, timeout_ms=60_000.0)
+            ref, _, _, _ = sc[0].render(cam, [sun], 2, 2, 0.0, 2, seed=10 + k)
+            assert np.array_equal(out["rgba"], ref)
+    st = srv.stats()
+    assert st["cancelled"] == len(cancelled) and st["completed"] == 9 - len(cancelled)
+    # a finished request's ticket is gone
+    with pytest.raises(ValueError):
+        srv.cancel(tb)
+    srv.close()
