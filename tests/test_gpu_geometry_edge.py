@@ -27,6 +27,12 @@ def _compare(v, i, o, d, max_unexplained=0):
     explained = np.zeros(len(o), bool)
     if mism.any():
         explained[mism] = cpu.tie_flags(o[mism], d[mism]) | cpu.edge_flags(o[mism], d[mism], trg[mism])
+        # beyond the fixed barycentric band: decisions within FP32 resolution
+        # (2^-20 of the scene's size, stretched by 1/|cos| for grazing rays)
+        rest = mism & ~explained
+        if rest.any():
+            from render_parity import fp32_eps
+            explained[rest] = cpu.closest_fragile(o[rest], d[rest], fp32_eps(gpu.info()))
     unexplained = int(mism.sum() - explained.sum())
     print("rays %d  hits %d  mismatched %d  tie/edge %d  unexplained %d" % (
         len(o), int((trc >= 0).sum()), int(mism.sum()), int(explained.sum()), unexplained))
@@ -69,8 +75,9 @@ def test_planar_mesh_zero_extent_axis():
     o[:g, 2] = rng.uniform(-0.05, 0.05, size=g)
     d = tgt - o
     # near-parallel rays see a needle-thin sheared triangle: FP32 vs FP64 may
-    # pick the neighbour a little beyond the 1e-5 barycentric graze band
-    trg, trc = _compare(v, i, o, d, max_unexplained=n // 2000)
+    # pick the neighbour beyond the 1e-5 barycentric graze band, but within
+    # FP32 resolution of the edge (the 1/|cos| stretch of closest_fragile)
+    trg, trc = _compare(v, i, o, d)
     assert (trc >= 0).sum() > n // 2
 
 

@@ -153,8 +153,17 @@ def test_lidar_batch_matches_reference(icosphere):
     hit_both = (r > 0) & (ref > 0)
     rel = np.abs(r[hit_both] - ref[hit_both]) / ref[hit_both]
     bad = np.where((r > 0) != (ref > 0))[0]
-    print("lidar: %d beams, hit-status disagreements %d, max rel range err %.2e" % (r.size, bad.size, rel.max()))
-    assert bad.size <= 1
+    # a hit-status disagreement is allowed only where the beam's closest-hit
+    # decision lies within FP32 resolution of an edge (oracle classification);
+    # beam: origin = position, direction R^T (0, 0, -1) = -(third row of R)
+    fragile = np.zeros(0, bool)
+    if bad.size:
+        R = g["rotation"].reshape(-1, 3, 3)[bad]
+        from render_parity import fp32_eps
+        fragile = cpu.closest_fragile(g["position"][bad], -R[:, 2, :], fp32_eps(gpu.info()))
+    print("lidar: %d beams, hit-status disagreements %d (%d near an edge), max rel range err %.2e"
+          % (r.size, bad.size, int(fragile.sum()), rel.max()))
+    assert int((~fragile).sum()) == 0
     assert rel.max() < 1e-5
     assert np.allclose(r[:8], g["single_range"], rtol=1e-5)
 
