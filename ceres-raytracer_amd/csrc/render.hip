@@ -534,6 +534,9 @@ __device__ inline float3 tex_lookup(const SceneView& s, int t, float u, float v)
 // The instance's material (src/crt/materials/material.hpp:90-184) computes
 // the light term and samples the next direction; without a material table
 // every instance is ColoredLambertian with its colour.
+// MATS: the scene has a material table (crt_scene_set_materials); without one
+// the kernels are instantiated with no material code at all (fewer registers).
+template <bool MATS>
 __device__ inline void shade_hit(const Ctx& C, int bounce, uint32_t k, uint32_t cam, uint32_t pix, uint32_t smp,
                                  int32_t slot, float hu, float hv, uint32_t ls, float3 din) {
     const SceneView& s = C.s;
@@ -560,7 +563,7 @@ __device__ inline void shade_hit(const Ctx& C, int bounce, uint32_t k, uint32_t 
     uint32_t mtype = 0;
     float3 col, kc = make_float3(0.f, 0.f, 0.f);
     float alpha = 0.f;
-    if (!s.mats) {
+    if (!MATS) {
         const float* ic = s.inst_color + 3 * (inst_ok ? inst - 1 : 0);
         col = make_float3(ic[0], ic[1], ic[2]);
     } else {
@@ -634,6 +637,7 @@ __device__ inline void shade_hit(const Ctx& C, int bounce, uint32_t k, uint32_t 
     }
 }
 
+template <bool MATS>
 __global__ __launch_bounds__(TB) void k_shade(Ctx C, int bounce, int qsel) {
     const uint32_t q = blockIdx.x * TB + threadIdx.x;
     const uint32_t n = C.cnt[qsel];
@@ -651,11 +655,11 @@ __global__ __launch_bounds__(TB) void k_shade(Ctx C, int bounce, int qsel) {
     if (!has_hit) return;
     const float2 hb = C.st.h[k];
     float3 din = make_float3(0.f, 0.f, 0.f);
-    if (C.s.mats) {   // Blinn-Phong and mirrors use the incoming direction
+    if (MATS) {   // Blinn-Phong and mirrors use the incoming direction
         const float4 d4 = C.st.d[k];
         din = make_float3(d4.x, d4.y, d4.z);
     }
-    shade_hit(C, bounce, k, m.x, m.y, m.z, (int32_t)m.w, hb.x, hb.y, ls, din);
+    shade_hit<MATS>(C, bounce, k, m.x, m.y, m.z, (int32_t)m.w, hb.x, hb.y, ls, din);
 }
 
 // After k_shade: shadow rays = nl per live path; tally traced rays.
@@ -719,6 +723,7 @@ __global__ __launch_bounds__(TB) void k_accumulate(Ctx C, int bounce, int next) 
 // k_shadow_quad and k_accumulate in the same order, so images are
 // bit-identical to the one-kernel-per-stage pipeline, with less path-state
 // traffic and fewer launches per bounce.
+template <bool MATS>
 __device__ inline void finish_extend(const Ctx& C, int bounce, uint32_t k, const HitRec& h, uint32_t ls,
                                      float3 din) {
     if (h.slot < 0) {
@@ -727,9 +732,10 @@ __device__ inline void finish_extend(const Ctx& C, int bounce, uint32_t k, const
     }
     C.live[ls] = k;
     const uint4 m = C.st.m[k];   // cam, pix, sample
-    shade_hit(C, bounce, k, m.x, m.y, m.z, h.slot, h.u, h.v, ls, din);
+    shade_hit<MATS>(C, bounce, k, m.x, m.y, m.z, h.slot, h.u, h.v, ls, din);
 }
 
+template <bool MATS>
 CRT_LANE_KERNEL void k_extend_shade_lane(Ctx C, int bounce, int qsel, int gen, int sorted) {
     const LaneStack stk = lane_stack(C.s);
     const uint32_t q = blockIdx.x * LANE_BLOCK + threadIdx.x;
@@ -763,7 +769,7 @@ CRT_LANE_KERNEL void k_extend_shade_lane(Ctx C, int bounce, int qsel, int gen, i
     const RayPre r = make_ray_pre(o.x, o.y, o.z, d.x, d.y, d.z);
     const HitRec h = trace_lane<false>(C.s, r, 0.0f, __int_as_float(0x7f800000), stk);
     const uint32_t ls = wave_append(h.slot >= 0, C.cnt + 3);   // live slot
-    finish_extend(C, bounce, k, h, ls, make_float3(d.x, d.y, d.z));
+    finish_extend<MATS>(C, bounce, k, h, ls, make_float3(d.x, d.y, d.z));
 }
 
 // Octet form for the bounces after the first.  Each octet traces R paths
@@ -771,6 +777,7 @@ CRT_LANE_KERNEL void k_extend_shade_lane(Ctx C, int bounce, int qsel, int gen, i
 // shades path p, so a wave carries 8R paths through the shading and one
 // atomic per wave appends them.  Those rays leave the body more often than
 // camera rays hit it; camera rays traced by octets use k_trace + k_shade.
+template <bool MATS>
 __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8)))
 void k_extend_shade_octet(Ctx C, int bounce, int qsel, int R, int sorted) {
     OCTET_SETUP(C)
@@ -797,7 +804,7 @@ void k_extend_shade_octet(Ctx C, int bounce, int qsel, int R, int sorted) {
         }
     }
     const uint32_t ls = wave_append(mine && h_mine.slot >= 0, C.cnt + 3);
-    if (mine) finish_extend(C, bounce, k_mine, h_mine, ls, d_mine);
+    if (mine) finish_extend<MATS>(C, bounce, k_mine, h_mine, ls, d_mine);
 }
 
 // One quad traces the shadow rays of R live paths (R = 1..4, by launch
@@ -907,8 +914,16 @@ __global__ __launch_bounds__(TB) void k_fill_u8(uint8_t* p, uint32_t n, uint8_t 
 
 // Launch the kernels of one pass.  Grids are sized for `cap_paths` /
 // `cap_fold` so that the same sequence can be captured once and replayed.
+void launch_shade(const Ctx& C, bool mats, uint32_t gp, int b, int cur, hipStream_t st) {
+    if (mats)
+        hipLaunchKernelGGL(k_shade<true>, dim3(gp), dim3(TB), 0, st, C, b, cur);
+    else
+        hipLaunchKernelGGL(k_shade<false>, dim3(gp), dim3(TB), 0, st, C, b, cur);
+}
+
 void launch_pass(const Ctx& C, uint64_t cap_paths, uint32_t cap_fold, const crt_batch_opts& B, int num_cus,
                  hipStream_t st) {
+    const bool mats = C.s.mats != nullptr;
     const size_t lds = trace_lds_bytes(C.s);
     const uint32_t gp = std::max(1u, cdiv(cap_paths, TB));
     const uint32_t go = std::max(1u, cdiv(cap_paths, OCTETS_PER_BLOCK));
@@ -941,16 +956,25 @@ void launch_pass(const Ctx& C, uint64_t cap_paths, uint32_t cap_fold, const crt_
                 hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, st, C.bins);
                 hipLaunchKernelGGL(k_sort_scatter, dim3(gs), dim3(SORT_TB), 0, st, C, cur);
             }
-            if (lane_at(b))
-                hipLaunchKernelGGL(k_extend_shade_lane, dim3(std::max(1u, cdiv(cap_paths, LANE_BLOCK))),
-                                   dim3(LANE_BLOCK), lane_lds_bytes(C.s), st, C, b, cur, (int)(gen && b == 0),
-                                   (int)sorted);
-            else if (b > 0 && B.separate_kernels != 2)
-                hipLaunchKernelGGL(k_extend_shade_octet, dim3(std::max(1u, cdiv(cap_paths, OCTETS_PER_BLOCK * ro))),
-                                   dim3(TRACE_BLOCK), lds, st, C, b, cur, ro, (int)sorted);
-            else {
+            const dim3 gl(std::max(1u, cdiv(cap_paths, LANE_BLOCK)));
+            const dim3 goct(std::max(1u, cdiv(cap_paths, OCTETS_PER_BLOCK * ro)));
+            if (lane_at(b)) {
+                if (mats)
+                    hipLaunchKernelGGL(k_extend_shade_lane<true>, gl, dim3(LANE_BLOCK), lane_lds_bytes(C.s), st, C, b,
+                                       cur, (int)(gen && b == 0), (int)sorted);
+                else
+                    hipLaunchKernelGGL(k_extend_shade_lane<false>, gl, dim3(LANE_BLOCK), lane_lds_bytes(C.s), st, C,
+                                       b, cur, (int)(gen && b == 0), (int)sorted);
+            } else if (b > 0 && B.separate_kernels != 2) {
+                if (mats)
+                    hipLaunchKernelGGL(k_extend_shade_octet<true>, goct, dim3(TRACE_BLOCK), lds, st, C, b, cur, ro,
+                                       (int)sorted);
+                else
+                    hipLaunchKernelGGL(k_extend_shade_octet<false>, goct, dim3(TRACE_BLOCK), lds, st, C, b, cur, ro,
+                                       (int)sorted);
+            } else {
                 hipLaunchKernelGGL(k_trace, dim3(go), dim3(TRACE_BLOCK), lds, st, C, cur);
-                hipLaunchKernelGGL(k_shade, dim3(gp), dim3(TB), 0, st, C, b, cur);
+                launch_shade(C, mats, gp, b, cur, st);
             }
             hipLaunchKernelGGL(k_shadow_accum, dim3(std::max(1u, cdiv(cap_paths, (TRACE_BLOCK / QUAD) * rq))),
                                dim3(TRACE_BLOCK), qlds, st, C, b, cur, nxt, rq);
@@ -970,7 +994,7 @@ void launch_pass(const Ctx& C, uint64_t cap_paths, uint32_t cap_fold, const crt_
             hipLaunchKernelGGL(k_trace_persistent, dim3(std::min(gpers, go)), dim3(TRACE_BLOCK), lds, st, C, cur);
         else
             hipLaunchKernelGGL(k_trace, dim3(go), dim3(TRACE_BLOCK), lds, st, C, cur);
-        hipLaunchKernelGGL(k_shade, dim3(gp), dim3(TB), 0, st, C, b, cur);
+        launch_shade(C, mats, gp, b, cur, st);
         hipLaunchKernelGGL(k_tally, dim3(1), dim3(32), 0, st, C, cur);
         if (C.nl > 0) {
             if (B.persistent)
