@@ -339,6 +339,8 @@ void worker_main(crt_server* S, int index) {
     };
     while (true) {
         std::vector<std::shared_ptr<Job>> batch;
+        std::vector<Workspace> drop_ws;
+        std::vector<RenderWorkspace*> drop_rws;
         size_t mem = 0;
         {
             std::unique_lock<std::mutex> lk(S->mu);
@@ -377,26 +379,28 @@ void worker_main(crt_server* S, int index) {
                     }
                 }
             }
+            // free what this worker and, if needed, the idle ones keep (an
+            // idle worker has nothing in flight on its stream); the buffers
+            // are only taken here and released after the lock, because
+            // hipFree may wait for the device
             const int a = admit(S, index, mem);
-            if (a >= 2) {   // free what this worker keeps (idle, so nothing of its stream is in flight)
-                me.ws = Workspace();
-                delete_render_workspace(me.rws);
-                me.rws = new_render_workspace();
-                S->held[index] = 0;
-            }
+            const auto take = [&](crt_server::Worker& w, size_t v) {
+                drop_ws.push_back(std::move(w.ws));
+                w.ws = Workspace();
+                drop_rws.push_back(w.rws);
+                w.rws = new_render_workspace();
+                S->held[v] = 0;
+            };
+            if (a >= 2) take(me, (size_t)index);
             if (a == 3)
                 for (size_t v = 0; v < S->wstate.size(); ++v)
-                    if ((int)v != index && S->running[v] == 0 && S->held[v] > 0) {
-                        crt_server::Worker& o = *S->wstate[v];
-                        o.ws = Workspace();
-                        delete_render_workspace(o.rws);
-                        o.rws = new_render_workspace();
-                        S->held[v] = 0;
-                    }
+                    if ((int)v != index && S->running[v] == 0 && S->held[v] > 0) take(*S->wstate[v], v);
             S->running[index] = std::max<size_t>(mem, 1);
             S->running_scene[index] = head->scene;
             S->inflight += batch.size();
         }
+        drop_ws.clear();
+        for (RenderWorkspace* r : drop_rws) delete_render_workspace(r);
         // run the batch; if it fails, rerun its requests one by one so an
         // error reaches only its own request
         std::vector<int> status(batch.size(), CRT_OK);
