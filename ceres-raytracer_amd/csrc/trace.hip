@@ -633,6 +633,8 @@ int crt_scene_set_materials(crt_scene* scene, uint32_t n_materials, const crt_ma
         CRT_REQUIRE(materials != nullptr && n_materials == scene->n_instances, CRT_ERR_INVALID,
                     "one material per instance is required");
         CRT_REQUIRE(n_textures == 0 || textures != nullptr, CRT_ERR_INVALID, "textures is NULL");
+        // work already enqueued on any stream may still read the old tables
+        CRT_HIP_CHECK(hipDeviceSynchronize());
         std::vector<MatDev> m(n_materials);
         for (uint32_t i = 0; i < n_materials; ++i) {
             const crt_material& x = materials[i];
@@ -703,6 +705,9 @@ int crt_scene_refit(crt_scene* scene, const double* vertices, const float* verte
             CRT_REQUIRE(std::isfinite(vertices[i]) && std::fabs(vertices[i]) <= 1e18, CRT_ERR_INVALID,
                         "vertex coordinate is not finite or exceeds 1e18");
         const double growth = max_sah_growth > 0.0 ? max_sah_growth : 1.5;
+        // work already enqueued on any stream may still read the old nodes
+        // and triangles
+        CRT_HIP_CHECK(hipDeviceSynchronize());
         std::vector<float> local;
         double origin[3];
         choose_frame(vertices, nv, origin, local);
@@ -761,7 +766,11 @@ int crt_scene_info(const crt_scene* scene, crt_bvh_info* info) {
     });
 }
 
-void crt_scene_destroy(crt_scene* scene) { delete scene; }
+void crt_scene_destroy(crt_scene* scene) {
+    if (!scene) return;
+    (void)hipDeviceSynchronize();   // enqueued work may still read the scene
+    delete scene;
+}
 
 int crt_pass(crt_scene* scene, const crt_camera* camera, double* xyz, double* normal, uint32_t* instance,
              double* depth, int32_t* tri_id, uint32_t flags, void* stream, crt_timing* timing) {

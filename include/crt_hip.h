@@ -221,7 +221,8 @@ typedef struct {
  * NULL (all zero, as the reference's OBJ loader leaves them); textures are
  * copied to the device.  Texel lookup is the reference's nearest texel at
  * (size_t)(u W + 0.5), (size_t)(v H + 0.5), clamped to the image.  Not to be
- * called while other calls on the scene run. */
+ * called while other calls on the scene run; it first waits for the work
+ * already enqueued on the device. */
 int crt_scene_set_materials(crt_scene* scene, uint32_t n_materials, const crt_material* materials,
                             const float* vertex_uv, uint32_t n_textures, const crt_texture* textures);
 /* ---- refit for moving geometry (dynamic scenes) --------------------------
@@ -236,7 +237,8 @@ int crt_scene_set_materials(crt_scene* scene, uint32_t n_materials, const crt_ma
  * 1.5) times the SAH of the last full build the BVH is rebuilt instead
  * (*rebuilt = 1, may be NULL).  Closest hits equal those of a scene created
  * from the new vertices except which triangle wins an exact tie.  Not to be
- * called while other calls on the scene run. */
+ * called while other calls on the scene run; it first waits for the work
+ * already enqueued on the device (e.g. device-output calls not yet done). */
 int crt_scene_refit(crt_scene* scene, const double* vertices, const float* vertex_normals, double max_sah_growth,
                     uint32_t* rebuilt);
 int  crt_scene_info(const crt_scene* scene, crt_bvh_info* info);
