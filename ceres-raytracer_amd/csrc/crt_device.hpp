@@ -181,17 +181,33 @@ __device__ inline RayPre make_ray_pre(float ox, float oy, float oz, float dx, fl
     return r;
 }
 
+// Component K (0..2) of (a, b, c), or component k chosen per lane (K = 3).
+template <int K>
+__device__ inline float pick3(float a, float b, float c, int k) {
+    if constexpr (K == 0) return a;
+    else if constexpr (K == 1) return b;
+    else if constexpr (K == 2) return c;
+    else return sel3(a, b, c, k);
+}
+
 // Watertight ray/triangle test.  Returns true on hit with tmin <= t <= tmax;
-// u = weight of p1, v = weight of p2 (reference convention).
+// u = weight of p1, v = weight of p2 (reference convention).  KX, KY, KZ:
+// the ray's axis permutation when the caller knows it for the whole wave
+// (then no per-lane selects), or 3 to take it from the ray; either way the
+// arithmetic, and so the result, is the same.
+template <int KX = 3, int KY = 3, int KZ = 3>
 __device__ inline bool tri_test(const RayPre& r, float4 a, float4 b, float4 c,
                                 float tmin, float tmax, float& t_out, float& u_out, float& v_out) {
     // vertices relative to the origin, permuted into (kx, ky, kz)
     const float Ax0 = a.x - r.ox, Ay0 = a.y - r.oy, Az0 = a.z - r.oz;
     const float Bx0 = b.x - r.ox, By0 = b.y - r.oy, Bz0 = b.z - r.oz;
     const float Cx0 = c.x - r.ox, Cy0 = c.y - r.oy, Cz0 = c.z - r.oz;
-    const float Akx = sel3(Ax0, Ay0, Az0, r.kx), Aky = sel3(Ax0, Ay0, Az0, r.ky), Akz = sel3(Ax0, Ay0, Az0, r.kz);
-    const float Bkx = sel3(Bx0, By0, Bz0, r.kx), Bky = sel3(Bx0, By0, Bz0, r.ky), Bkz = sel3(Bx0, By0, Bz0, r.kz);
-    const float Ckx = sel3(Cx0, Cy0, Cz0, r.kx), Cky = sel3(Cx0, Cy0, Cz0, r.ky), Ckz = sel3(Cx0, Cy0, Cz0, r.kz);
+    const float Akx = pick3<KX>(Ax0, Ay0, Az0, r.kx), Aky = pick3<KY>(Ax0, Ay0, Az0, r.ky),
+                Akz = pick3<KZ>(Ax0, Ay0, Az0, r.kz);
+    const float Bkx = pick3<KX>(Bx0, By0, Bz0, r.kx), Bky = pick3<KY>(Bx0, By0, Bz0, r.ky),
+                Bkz = pick3<KZ>(Bx0, By0, Bz0, r.kz);
+    const float Ckx = pick3<KX>(Cx0, Cy0, Cz0, r.kx), Cky = pick3<KY>(Cx0, Cy0, Cz0, r.ky),
+                Ckz = pick3<KZ>(Cx0, Cy0, Cz0, r.kz);
     // shear: one explicit FMA per coordinate, so a vertex shared by two
     // triangles gets identical bits in both
     const float Ax = fmaf(-r.sx, Akz, Akx);
@@ -233,6 +249,43 @@ __device__ inline bool tri_test(const RayPre& r, float4 a, float4 b, float4 c,
     u_out = V * rcp;
     v_out = W * rcp;
     return true;
+}
+
+// The ray's axis permutation as a code 0..5 (2 kz + 1 if kx and ky are
+// swapped), and a call of f with the permutation as template arguments
+// (PermTag<KX, KY, KZ>) for a code that is the same on the whole wave.
+template <int X, int Y, int Z>
+struct PermTag {
+    static constexpr int kx = X, ky = Y, kz = Z;
+};
+
+__device__ inline int perm_code(const RayPre& r) {
+    const int kx0 = r.kz == 2 ? 0 : r.kz + 1;
+    return 2 * r.kz + (r.kx != kx0 ? 1 : 0);
+}
+
+// Wave-uniform permutation code of the active lanes, or -1 when they differ.
+__device__ inline int wave_perm(const RayPre& r) {
+    const int pc = perm_code(r);
+    const int pc0 = __builtin_amdgcn_readfirstlane(pc);
+    return __ballot(pc != pc0) == 0ull ? pc0 : -1;
+}
+
+template <bool ON, typename F>
+__device__ inline bool perm_dispatch(int pc, F&& f) {
+    if constexpr (!ON) {
+        return f(PermTag<3, 3, 3>{});
+    } else {
+        switch (pc) {
+            case 0: return f(PermTag<1, 2, 0>{});
+            case 1: return f(PermTag<2, 1, 0>{});
+            case 2: return f(PermTag<2, 0, 1>{});
+            case 3: return f(PermTag<0, 2, 1>{});
+            case 4: return f(PermTag<0, 1, 2>{});
+            case 5: return f(PermTag<1, 0, 2>{});
+            default: return f(PermTag<3, 3, 3>{});
+        }
+    }
 }
 
 // ------------------------------------------------------------------------
@@ -382,11 +435,15 @@ __device__ inline bool hint_blocks(const SceneView& s, const RayPre& r, uint32_t
     return tri_test(r, s.tris[ti], s.tris[ti + 1], s.tris[ti + 2], tmin, tmax, t, u, v);
 }
 
+template <bool FIXED_PERM = true>
 __device__ inline bool trace_quad_any(const SceneView& s, const RayPre& r, float tmin, float tmax, uint32_t* stk,
                                       int j, int lane_base) {
     const float4* __restrict__ nodes = s.nodes;
     const float4* __restrict__ tris = s.tris;
     const uint32_t cap = s.stack_cap;
+    // leaves test with a fixed axis permutation when the wave shares one
+    // (FIXED_PERM: off where the extra code would cost occupancy)
+    const int pc = FIXED_PERM ? wave_perm(r) : -1;
     uint32_t cur = 0u;
     uint32_t sp = 0u;
     while (true) {
@@ -420,16 +477,20 @@ __device__ inline bool trace_quad_any(const SceneView& s, const RayPre& r, float
             // leaf of at most 8 triangles: lane j tests j and j + 4
             const uint32_t first = leaf_first(cur);
             const uint32_t count = leaf_count(cur);
-            float t, u, v;
-            bool hit = false;
-            if ((uint32_t)j < count) {
-                const size_t ti = (size_t)(first + (uint32_t)j) * 3u;
-                hit = tri_test(r, tris[ti], tris[ti + 1], tris[ti + 2], tmin, tmax, t, u, v);
-            }
-            if (!hit && (uint32_t)j + 4u < count) {
-                const size_t ti = (size_t)(first + (uint32_t)j + 4u) * 3u;
-                hit = tri_test(r, tris[ti], tris[ti + 1], tris[ti + 2], tmin, tmax, t, u, v);
-            }
+            const bool hit = perm_dispatch<FIXED_PERM>(pc, [&](auto P) {
+                constexpr int KX = decltype(P)::kx, KY = decltype(P)::ky, KZ = decltype(P)::kz;
+                float t, u, v;
+                bool h = false;
+                if ((uint32_t)j < count) {
+                    const size_t ti = (size_t)(first + (uint32_t)j) * 3u;
+                    h = tri_test<KX, KY, KZ>(r, tris[ti], tris[ti + 1], tris[ti + 2], tmin, tmax, t, u, v);
+                }
+                if (!h && (uint32_t)j + 4u < count) {
+                    const size_t ti = (size_t)(first + (uint32_t)j + 4u) * 3u;
+                    h = tri_test<KX, KY, KZ>(r, tris[ti], tris[ti + 1], tris[ti + 2], tmin, tmax, t, u, v);
+                }
+                return h;
+            });
             if (((__ballot(hit) >> lane_base) & 0xFull) != 0ull) return true;
         }
         if (sp == 0u) return false;
@@ -554,12 +615,15 @@ struct TraversalStats {
 // leaf or every lane of the wave holds one; then the wave tests triangles
 // together, so inner-node and leaf work of different lanes are not
 // interleaved step by step.
-template <bool ANY_HIT, bool STATS = false>
+template <bool ANY_HIT, bool STATS = false, bool FIXED_PERM = true>
 __device__ inline HitRec trace_lane(const SceneView& s, const RayPre& r, float tmin, float tmax,
                                     const LaneStack& stk, TraversalStats* ts = nullptr) {
     HitRec hit;
     hit.t = tmax; hit.u = 0.f; hit.v = 0.f; hit.slot = -1;
     float best = tmax;
+    // leaves test with a fixed axis permutation when the wave shares one
+    // (FIXED_PERM: off where the extra code would cost more than it saves)
+    const int pc = FIXED_PERM ? wave_perm(r) : -1;
     uint32_t cur = 0xFFu;      // group (node << 8 | mask), leaf ref, or 0 = nothing
     uint32_t leaf = 0u;        // parked leaf, 0 = none
     uint32_t sp = 0u;
@@ -603,16 +667,21 @@ __device__ inline HitRec trace_lane(const SceneView& s, const RayPre& r, float t
             const uint32_t count = leaf_count(lr);
             const float4* __restrict__ tp = s.tris + (size_t)first * 3u;
             if (STATS) ts->tris += count;
-            for (uint32_t k = 0; k < count; ++k) {
-                float t, u, v;
-                if (tri_test(r, tp[3u * k], tp[3u * k + 1u], tp[3u * k + 2u], tmin, best, t, u, v)) {
-                    // t <= best: exact ties go to the later triangle, like the
-                    // "last tested wins" rule of lib/bvh/single_ray_traverser.hpp:55-60
-                    hit.t = t; hit.u = u; hit.v = v; hit.slot = (int)(first + k);
-                    if (ANY_HIT) return hit;
-                    best = t;
+            const bool any = perm_dispatch<FIXED_PERM>(pc, [&](auto P) {
+                constexpr int KX = decltype(P)::kx, KY = decltype(P)::ky, KZ = decltype(P)::kz;
+                for (uint32_t k = 0; k < count; ++k) {
+                    float t, u, v;
+                    if (tri_test<KX, KY, KZ>(r, tp[3u * k], tp[3u * k + 1u], tp[3u * k + 2u], tmin, best, t, u, v)) {
+                        // t <= best: exact ties go to the later triangle, like the
+                        // "last tested wins" rule of lib/bvh/single_ray_traverser.hpp:55-60
+                        hit.t = t; hit.u = u; hit.v = v; hit.slot = (int)(first + k);
+                        if (ANY_HIT) return true;
+                        best = t;
+                    }
                 }
-            }
+                return false;
+            });
+            if (ANY_HIT && any) return hit;
             if (parked) {
                 leaf = 0u;
             } else {

@@ -434,7 +434,7 @@ CRT_LANE_KERNEL void k_extend_lane(Ctx C, int qsel) {
     const uint32_t k = C.queue[qsel][q];
     const float4 o = C.st.o[k], d = C.st.d[k];
     const RayPre r = make_ray_pre(o.x, o.y, o.z, d.x, d.y, d.z);
-    const HitRec h = trace_lane<false>(C.s, r, 0.0f, __int_as_float(0x7f800000), stk);
+    const HitRec h = trace_lane<false, false, false>(C.s, r, 0.0f, __int_as_float(0x7f800000), stk);
     C.st.m[k].w = (uint32_t)h.slot;
     C.st.h[k] = make_float2(h.u, h.v);
 }
@@ -767,7 +767,7 @@ CRT_LANE_KERNEL void k_extend_shade_lane(Ctx C, int bounce, int qsel, int gen, i
     }
     if (!alive) return;
     const RayPre r = make_ray_pre(o.x, o.y, o.z, d.x, d.y, d.z);
-    const HitRec h = trace_lane<false>(C.s, r, 0.0f, __int_as_float(0x7f800000), stk);
+    const HitRec h = trace_lane<false, false, false>(C.s, r, 0.0f, __int_as_float(0x7f800000), stk);
     const uint32_t ls = wave_append(h.slot >= 0, C.cnt + 3);   // live slot
     finish_extend<MATS>(C, bounce, k, h, ls, make_float3(d.x, d.y, d.z));
 }
@@ -809,7 +809,8 @@ void k_extend_shade_octet(Ctx C, int bounce, int qsel, int R, int sorted) {
 
 // One quad traces the shadow rays of R live paths (R = 1..4, by launch
 // size) one after another; lane p of the quad then accumulates path p.
-__global__ __launch_bounds__(TRACE_BLOCK) void k_shadow_accum(Ctx C, int bounce, int qsel, int next, int R) {
+__global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8)))
+void k_shadow_accum(Ctx C, int bounce, int qsel, int next, int R) {
     QUAD_SETUP(C)
     const uint32_t q0 = (blockIdx.x * (TRACE_BLOCK / QUAD) + qlocal) * (uint32_t)R;
     const uint32_t n = C.cnt[3];
@@ -830,7 +831,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow_accum(Ctx C, int bounce,
             const float4 o = C.shadow[i].o, d = C.shadow[i].d;
             const RayPre r = make_ray_pre(o.x, o.y, o.z, d.x, d.y, d.z);
             const bool occ = hint_blocks(C.s, r, __float_as_uint(d.w), 0.0f, o.w) ||
-                             trace_quad_any(C.s, r, 0.0f, o.w, qstk, qj, qbase);
+                             trace_quad_any<false>(C.s, r, 0.0f, o.w, qstk, qj, qbase);
             if (!occ) {
                 const float4 c = C.shadow[i].c;
                 lr[0] += c.x; lr[1] += c.y; lr[2] += c.z;
