@@ -32,7 +32,9 @@
 // whose octets / quads claim `grab` entries per atomic) runs k_raygen, then
 // per bounce k_trace / k_extend_lane -> k_shade -> k_tally -> k_shadow_quad ->
 // k_accumulate with the same arithmetic, so both give bit-identical images.
-// Queues are compacted with a wave ballot + one atomic per wave.  All kernels
+// Queues are compacted with a wave ballot + one atomic per wave, and are
+// sharded (SHARDS counters per queue, see shard_append) because device-scope
+// atomics on one address serialise across the XCDs.  All kernels
 // read their sizes from device memory, so a pass can be captured once in a
 // hipGraph and replayed (the host only rewrites the segment table between
 // passes).  Counters are cleared by a kernel, never by a memset node.
@@ -142,18 +144,52 @@ struct Ctx {
     const PassCtl* ctl;
     const uint32_t* plist;
     PathPlanes st;
-    uint32_t* queue[2];
-    uint32_t* sorted;      // a bounce's queue ordered by ray-origin cell (coherence sort)
+    uint32_t region;       // entries per shard of the sharded buffers below
+    uint32_t* queue[2];    // path ids to extend, sharded
+    uint32_t* qcnt[2];     // their shard counters
+    uint32_t* sorted;      // a bounce's queue ordered by ray-origin cell (coherence sort), dense
     uint32_t* bins;        // SORT_BINS counts, then SORT_BINS cursors
-    uint32_t* live;
-    ShadowRec* shadow;
+    uint32_t* live;        // paths with a hit at this bounce, sharded
+    uint32_t* lcnt;        // their shard counters
+    ShadowRec* shadow;     // nl records per live entry: live position p -> p*nl .. p*nl+nl-1
     unsigned long long* tally;   // [0] closest-hit rays, [1] shadow rays
     float4* samples;
     float4* mean;
     int32_t* spp;
     uint8_t* active;
-    uint32_t* cnt;   // [0],[1] queues, [2] shadow, [3] live, [4] active list, [5],[6] persistent work
+    uint32_t* cnt;   // CNT_* below, then the shard counters
 };
+
+// Plain counters (C.cnt[...]).
+constexpr int CNT_ACTIVE = 4;        // adaptive: active pixel list
+constexpr int CNT_WORK_EXT = 5;      // persistent extension rays: next claim
+constexpr int CNT_WORK_SHADOW = 6;   // persistent shadow rays: next claim
+constexpr int CNT_SORTED = 7;        // entries of the sorted queue
+constexpr int CNT_OVERFLOW = 8;      // a shard filled up (ruled out by the region size; reported)
+constexpr int CNT_PLAIN = 32;
+
+// Sharded append buffers.  A wave appends its ballot with one device-scope
+// atomic; on a single address those serialise across the 8 XCDs (about 4 ns
+// each: 3 ms per bounce of 2^23 paths, more than the shading).  So a queue
+// has SHARDS counters, each in its own 128-B line, and shard s owns entries
+// [s*region, s*region + region) of the buffer.  Block b appends to shard
+// b % SHARDS; a consumer block b reads chunk b / SHARDS of shard b % SHARDS
+// and appends what it keeps to the same shard.  A producer over n inputs
+// with at most MAX_BLOCK_ITEMS per block fills a shard with at most
+// n/SHARDS + MAX_BLOCK_ITEMS entries, and a consumer never appends more to a
+// shard than it read from it, so region = ceil(cap/SHARDS) + MAX_BLOCK_ITEMS
+// never overflows (the appends check it anyway).
+constexpr uint32_t SHARDS = 64;
+constexpr uint32_t SHARD_STRIDE = 32;        // counters 128 B apart
+constexpr uint32_t MAX_BLOCK_ITEMS = 256;    // inputs of one block, any kernel
+constexpr uint32_t NO_POS = 0xFFFFFFFFu;
+constexpr int SET_Q0 = 0, SET_Q1 = 1, SET_LIVE = 2, N_SETS = 3;
+static_assert(SHARDS == 64, "shard_total reduces one counter per lane of a wave");
+static_assert(TB <= (int)MAX_BLOCK_ITEMS && LANE_BLOCK <= (int)MAX_BLOCK_ITEMS && TRACE_BLOCK <= (int)MAX_BLOCK_ITEMS,
+              "block inputs exceed MAX_BLOCK_ITEMS");
+
+inline uint32_t shard_region(uint64_t cap) { return cdiv(cap, SHARDS) + MAX_BLOCK_ITEMS; }
+inline size_t counter_words() { return CNT_PLAIN + (size_t)N_SETS * SHARDS * SHARD_STRIDE; }
 
 __device__ inline float dot3(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 __device__ inline float3 cross3(float3 a, float3 b) {
@@ -173,6 +209,52 @@ __device__ inline uint32_t wave_append(bool want, uint32_t* counter) {
     if (m != 0ull && lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
     base = __shfl(base, leader < 0 ? 0 : leader, 64);
     return base + (uint32_t)__popcll(m & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
+}
+
+__device__ inline uint32_t block_shard() { return blockIdx.x & (SHARDS - 1); }
+
+// Position in a sharded buffer of this lane's entry, appended to shard s of
+// `counters`; NO_POS if !want (or if the shard were full).
+__device__ inline uint32_t shard_append(const Ctx& C, bool want, uint32_t* counters, uint32_t s) {
+    const uint32_t off = wave_append(want, counters + s * SHARD_STRIDE);
+    if (!want) return NO_POS;
+    if (off >= C.region) {
+        atomicOr(C.cnt + CNT_OVERFLOW, 1u);
+        return NO_POS;
+    }
+    return s * C.region + off;
+}
+
+// Input t (< items) of this block in a sharded buffer: chunk blockIdx.x /
+// SHARDS of shard blockIdx.x % SHARDS.  False past the shard's count.
+__device__ inline bool shard_entry(const Ctx& C, const uint32_t* counters, uint32_t items, uint32_t t,
+                                   uint32_t& pos) {
+    const uint32_t s = block_shard();
+    const uint32_t off = (blockIdx.x / SHARDS) * items + t;
+    pos = s * C.region + off;
+    return off < counters[s * SHARD_STRIDE];
+}
+
+// Extension ray input t of this block: from the sharded queue qsel, or,
+// when `sorted`, entry blockIdx.x * items + t of the dense sorted queue.
+__device__ inline bool queue_entry(const Ctx& C, int qsel, int sorted, uint32_t items, uint32_t t, uint32_t& k) {
+    if (sorted) {
+        const uint32_t q = blockIdx.x * items + t;
+        if (q >= C.cnt[CNT_SORTED]) return false;
+        k = C.sorted[q];
+        return true;
+    }
+    uint32_t pos;
+    if (!shard_entry(C, C.qcnt[qsel], items, t, pos)) return false;
+    k = C.queue[qsel][pos];
+    return true;
+}
+
+// Sum of a set of shard counters, in every lane; all 64 lanes of the wave call.
+__device__ inline uint32_t shard_total(const uint32_t* counters) {
+    uint32_t v = counters[(threadIdx.x & 63) * SHARD_STRIDE];
+    for (int o = 32; o > 0; o >>= 1) v += (uint32_t)__shfl_xor((int)v, o, 64);
+    return v;
 }
 
 // largest segment index with key(seg) <= x
@@ -200,9 +282,13 @@ __device__ inline uint32_t seg_search_wave(uint32_t nseg, uint64_t x, F key) {
     return lo;
 }
 
-__global__ void k_clear(uint32_t* cnt, uint32_t mask) {
+// Clears the plain counters in `mask` and the shard counter sets in `sets`
+// (one block of SHARDS threads).
+__global__ void k_clear(uint32_t* cnt, uint32_t mask, uint32_t sets) {
     const uint32_t i = threadIdx.x;
-    if (i < 32 && ((mask >> i) & 1u)) cnt[i] = 0u;
+    if (i < (uint32_t)CNT_PLAIN && ((mask >> i) & 1u)) cnt[i] = 0u;
+    for (int j = 0; j < N_SETS; ++j)
+        if ((sets >> j) & 1u) cnt[CNT_PLAIN + ((size_t)j * SHARDS + i) * SHARD_STRIDE] = 0u;
 }
 
 // Camera path k of the pass: FP64 camera ray (jittered), rebased to the
@@ -265,8 +351,8 @@ __global__ __launch_bounds__(TB) void k_raygen(Ctx C) {
         C.st.m[k] = make_uint4(ps.cam, ps.pix, ps.sample, (uint32_t)ps.slot);
         C.st.h[k] = make_float2(ps.hu, ps.hv);
     }
-    const uint32_t slot = wave_append(alive, C.cnt + 0);
-    if (alive) C.queue[0][slot] = (uint32_t)k;
+    const uint32_t pos = shard_append(C, alive, C.qcnt[0], block_shard());
+    if (pos != NO_POS) C.queue[0][pos] = (uint32_t)k;
 }
 
 // ---- coherence sort (crt_batch_opts.coherence_sort) ----------------------
@@ -302,23 +388,24 @@ __global__ __launch_bounds__(SORT_TB) void k_sort_zero(uint32_t* __restrict__ bi
     for (uint32_t i = threadIdx.x; i < SORT_BINS; i += SORT_TB) bins[i] = 0u;
 }
 
+// Both passes read the sharded queue like every consumer (SORT_TILE entries
+// per block); the sorted queue is dense.
 __global__ __launch_bounds__(SORT_TB) void k_sort_hist(Ctx C, int qsel) {
     __shared__ uint32_t h[SORT_BINS];
     for (uint32_t i = threadIdx.x; i < SORT_BINS; i += SORT_TB) h[i] = 0u;
     __syncthreads();
-    const uint32_t n = C.cnt[qsel];
-    const uint32_t base = blockIdx.x * SORT_TILE;
     for (int j = 0; j < SORT_ITEMS; ++j) {
-        const uint32_t q = base + (uint32_t)j * SORT_TB + threadIdx.x;
-        if (q < n) atomicAdd(&h[origin_cell(C.s, C.st.o[C.queue[qsel][q]])], 1u);
+        uint32_t pos;
+        if (shard_entry(C, C.qcnt[qsel], SORT_TILE, (uint32_t)j * SORT_TB + threadIdx.x, pos))
+            atomicAdd(&h[origin_cell(C.s, C.st.o[C.queue[qsel][pos]])], 1u);
     }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < SORT_BINS; i += SORT_TB)
         if (h[i] != 0u) atomicAdd(&C.bins[i], h[i]);
 }
 
-// One block: cursors = exclusive prefix of the counts.
-__global__ __launch_bounds__(1024) void k_sort_scan(uint32_t* __restrict__ bins) {
+// One block: cursors = exclusive prefix of the counts; *total = their sum.
+__global__ __launch_bounds__(1024) void k_sort_scan(uint32_t* __restrict__ bins, uint32_t* __restrict__ total) {
     __shared__ uint32_t part[1024];
     constexpr uint32_t PER = SORT_BINS / 1024;
     const uint32_t t = threadIdx.x;
@@ -334,6 +421,7 @@ __global__ __launch_bounds__(1024) void k_sort_scan(uint32_t* __restrict__ bins)
     }
     uint32_t run = part[t] - sum;
     for (uint32_t j = 0; j < PER; ++j) { bins[SORT_BINS + t * PER + j] = run; run += v[j]; }
+    if (t == 1023) *total = part[t];
 }
 
 __global__ __launch_bounds__(SORT_TB) void k_sort_scatter(Ctx C, int qsel) {
@@ -341,15 +429,13 @@ __global__ __launch_bounds__(SORT_TB) void k_sort_scatter(Ctx C, int qsel) {
     __shared__ uint32_t at[SORT_BINS];
     for (uint32_t i = threadIdx.x; i < SORT_BINS; i += SORT_TB) h[i] = 0u;
     __syncthreads();
-    const uint32_t n = C.cnt[qsel];
-    const uint32_t base = blockIdx.x * SORT_TILE;
     uint32_t path[SORT_ITEMS], slot[SORT_ITEMS];   // slot = cell << 12 | rank in the block's cell
     for (int j = 0; j < SORT_ITEMS; ++j) {
-        const uint32_t q = base + (uint32_t)j * SORT_TB + threadIdx.x;
+        uint32_t pos;
         path[j] = 0xFFFFFFFFu;
         slot[j] = 0u;
-        if (q < n) {
-            path[j] = C.queue[qsel][q];
+        if (shard_entry(C, C.qcnt[qsel], SORT_TILE, (uint32_t)j * SORT_TB + threadIdx.x, pos)) {
+            path[j] = C.queue[qsel][pos];
             const uint32_t c = origin_cell(C.s, C.st.o[path[j]]);
             slot[j] = (c << 12) | atomicAdd(&h[c], 1u);   // rank < SORT_TILE = 4096
         }
@@ -384,9 +470,9 @@ __device__ inline void extend_one(const Ctx& C, uint32_t k, uint32_t* sref, floa
 // Extension rays: one per octet (closest hit).
 __global__ __launch_bounds__(TRACE_BLOCK) void k_trace(Ctx C, int qsel) {
     OCTET_SETUP(C)
-    const uint32_t q = blockIdx.x * OCTETS_PER_BLOCK + local;
-    if (q >= C.cnt[qsel]) return;
-    extend_one(C, C.queue[qsel][q], sref, sd, sub, lane_base);
+    uint32_t k;
+    if (!queue_entry(C, qsel, 0, OCTETS_PER_BLOCK, local, k)) return;   // octet-uniform
+    extend_one(C, k, sref, sd, sub, lane_base);
 }
 
 // Shadow rays: one per quad (any hit, see trace_quad_any).
@@ -404,34 +490,40 @@ __device__ inline void shadow_quad(const Ctx& C, uint32_t i, uint32_t* stk, int 
     if (j == 0) C.shadow[i].c.w = occ ? 1.0f : 0.0f;
 }
 
+// The shadow records of shard s are records s*region*nl .. + count(s)*nl;
+// block b tests chunk b / SHARDS of shard b % SHARDS.
 __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow_quad(Ctx C) {
     QUAD_SETUP(C)
-    const uint32_t q = blockIdx.x * (TRACE_BLOCK / QUAD) + qlocal;
-    if (q >= C.cnt[2]) return;   // quad-uniform
-    shadow_quad(C, q, qstk, qj, qbase);
+    const uint32_t s = block_shard(), nl = (uint32_t)C.nl;
+    const uint32_t local = (blockIdx.x / SHARDS) * (TRACE_BLOCK / QUAD) + qlocal;
+    if (local >= C.lcnt[s * SHARD_STRIDE] * nl) return;   // quad-uniform
+    shadow_quad(C, s * C.region * nl + local, qstk, qj, qbase);
 }
 
-// Persistent form: quads claim `grab` consecutive shadow rays per atomic.
-__global__ __launch_bounds__(TRACE_BLOCK) void k_shadow_quad_persistent(Ctx C) {
+// Persistent form: quads claim `grab` consecutive records per atomic over
+// the first `span` entries' records of every shard.
+__global__ __launch_bounds__(TRACE_BLOCK) void k_shadow_quad_persistent(Ctx C, uint32_t span) {
     QUAD_SETUP(C)
-    uint32_t* work = C.cnt + 6;
-    const uint32_t n = C.cnt[2];
+    uint32_t* work = C.cnt + CNT_WORK_SHADOW;
+    const uint32_t nl = (uint32_t)C.nl, per = span * nl, n = SHARDS * per;
     while (true) {
         uint32_t base = 0;
         if (qj == 0) base = atomicAdd(work, C.grab);
         base = (uint32_t)__shfl((int)base, qbase, 64);
         if (base >= n) break;
         const uint32_t end = min(base + C.grab, n);
-        for (uint32_t q = base; q < end; ++q) shadow_quad(C, q, qstk, qj, qbase);
+        for (uint32_t q = base; q < end; ++q) {
+            const uint32_t s = q / per, local = q - s * per;
+            if (local < C.lcnt[s * SHARD_STRIDE] * nl) shadow_quad(C, s * C.region * nl + local, qstk, qj, qbase);
+        }
     }
 }
 
 // One extension ray per lane (coherent rays: camera rays).
 CRT_LANE_KERNEL void k_extend_lane(Ctx C, int qsel) {
     const LaneStack stk = lane_stack(C.s);
-    const uint32_t q = blockIdx.x * LANE_BLOCK + threadIdx.x;
-    if (q >= C.cnt[qsel]) return;
-    const uint32_t k = C.queue[qsel][q];
+    uint32_t k;
+    if (!queue_entry(C, qsel, 0, LANE_BLOCK, threadIdx.x, k)) return;
     const float4 o = C.st.o[k], d = C.st.d[k];
     const RayPre r = make_ray_pre(o.x, o.y, o.z, d.x, d.y, d.z);
     const HitRec h = trace_lane<false, false, false>(C.s, r, 0.0f, __int_as_float(0x7f800000), stk);
@@ -439,19 +531,24 @@ CRT_LANE_KERNEL void k_extend_lane(Ctx C, int qsel) {
     C.st.h[k] = make_float2(h.u, h.v);
 }
 
-// Persistent form: octets claim `grab` consecutive extension rays per atomic.
-__global__ __launch_bounds__(TRACE_BLOCK) void k_trace_persistent(Ctx C, int qsel) {
+// Persistent form: octets claim `grab` consecutive queue slots per atomic
+// over the first `span` slots of every shard.
+__global__ __launch_bounds__(TRACE_BLOCK) void k_trace_persistent(Ctx C, int qsel, uint32_t span) {
     OCTET_SETUP(C)
     (void)local;
-    uint32_t* work = C.cnt + 5;
-    const uint32_t n = C.cnt[qsel];
+    uint32_t* work = C.cnt + CNT_WORK_EXT;
+    const uint32_t n = SHARDS * span;
     while (true) {
         uint32_t base = 0;
         if (sub == 0) base = atomicAdd(work, C.grab);
         base = (uint32_t)__shfl((int)base, lane_base, 64);
         if (base >= n) break;
         const uint32_t end = min(base + C.grab, n);
-        for (uint32_t q = base; q < end; ++q) extend_one(C, C.queue[qsel][q], sref, sd, sub, lane_base);
+        for (uint32_t q = base; q < end; ++q) {
+            const uint32_t s = q / span, off = q - s * span;
+            if (off < C.qcnt[qsel][s * SHARD_STRIDE])
+                extend_one(C, C.queue[qsel][s * C.region + off], sref, sd, sub, lane_base);
+        }
     }
 }
 
@@ -639,20 +736,18 @@ __device__ inline void shade_hit(const Ctx& C, int bounce, uint32_t k, uint32_t 
 
 template <bool MATS>
 __global__ __launch_bounds__(TB) void k_shade(Ctx C, int bounce, int qsel) {
-    const uint32_t q = blockIdx.x * TB + threadIdx.x;
-    const uint32_t n = C.cnt[qsel];
     bool has_hit = false;
     uint32_t k = 0;
     uint4 m = make_uint4(0u, 0u, 0u, 0u);
-    if (q < n) {
-        k = C.queue[qsel][q];
+    if (queue_entry(C, qsel, 0, TB, threadIdx.x, k)) {
         m = C.st.m[k];
         if ((int32_t)m.w < 0) C.samples[k] = C.st.L[k];   // path ends without a hit
         else has_hit = true;
     }
-    const uint32_t ls = wave_append(has_hit, C.cnt + 3);   // live slot = shadow slots ls*nl .. ls*nl+nl-1
-    if (has_hit) C.live[ls] = k;
-    if (!has_hit) return;
+    // live position = shadow records ls*nl .. ls*nl+nl-1
+    const uint32_t ls = shard_append(C, has_hit, C.lcnt, block_shard());
+    if (ls == NO_POS) return;
+    C.live[ls] = k;
     const float2 hb = C.st.h[k];
     float3 din = make_float3(0.f, 0.f, 0.f);
     if (MATS) {   // Blinn-Phong and mirrors use the incoming direction
@@ -662,14 +757,18 @@ __global__ __launch_bounds__(TB) void k_shade(Ctx C, int bounce, int qsel) {
     shade_hit<MATS>(C, bounce, k, m.x, m.y, m.z, (int32_t)m.w, hb.x, hb.y, ls, din);
 }
 
-// After k_shade: shadow rays = nl per live path; tally traced rays.
-__global__ void k_tally(Ctx C, int qsel) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        C.cnt[2] = C.cnt[3] * (uint32_t)C.nl;
-        C.tally[0] += C.cnt[qsel];
-        C.tally[1] += C.cnt[2];
+// Traced rays of a bounce: the extension rays of queue qsel and nl shadow
+// rays per live path.  One wave.
+__device__ inline void tally_bounce(const Ctx& C, int qsel) {
+    const uint32_t ext = shard_total(C.qcnt[qsel]);
+    const uint32_t lv = shard_total(C.lcnt);
+    if (threadIdx.x == 0) {
+        C.tally[0] += ext;
+        C.tally[1] += (unsigned long long)lv * (uint32_t)C.nl;
     }
 }
+
+__global__ void k_tally(Ctx C, int qsel) { tally_bounce(C, qsel); }
 
 // Direct light of this bounce (sum over the unoccluded lights, clamped to
 // [0,1] from bounce 1 on) times the path weight; then the weight takes the
@@ -694,11 +793,9 @@ __device__ inline bool accumulate_path(const Ctx& C, int bounce, uint32_t k, flo
 }
 
 __global__ __launch_bounds__(TB) void k_accumulate(Ctx C, int bounce, int next) {
-    const uint32_t q = blockIdx.x * TB + threadIdx.x;
-    const uint32_t n = C.cnt[3];
     bool cont = false;
-    uint32_t k = 0;
-    if (q < n) {
+    uint32_t k = 0, q;
+    if (shard_entry(C, C.lcnt, TB, threadIdx.x, q)) {
         k = C.live[q];
         float lr[3] = {0.f, 0.f, 0.f};
         for (int l = 0; l < C.nl; ++l) {
@@ -707,8 +804,8 @@ __global__ __launch_bounds__(TB) void k_accumulate(Ctx C, int bounce, int next) 
         }
         cont = accumulate_path(C, bounce, k, lr);
     }
-    const uint32_t slot = wave_append(cont, C.cnt + next);
-    if (cont) C.queue[next][slot] = k;
+    const uint32_t pos = shard_append(C, cont, C.qcnt[next], block_shard());
+    if (pos != NO_POS) C.queue[next][pos] = k;
 }
 
 // ---- fused bounce kernels ------------------------------------------------
@@ -730,6 +827,7 @@ __device__ inline void finish_extend(const Ctx& C, int bounce, uint32_t k, const
         C.samples[k] = C.st.L[k];   // path ends without a hit
         return;
     }
+    if (ls == NO_POS) return;   // shard full (flagged)
     C.live[ls] = k;
     const uint4 m = C.st.m[k];   // cam, pix, sample
     shade_hit<MATS>(C, bounce, k, m.x, m.y, m.z, h.slot, h.u, h.v, ls, din);
@@ -758,9 +856,9 @@ CRT_LANE_KERNEL void k_extend_shade_lane(Ctx C, int bounce, int qsel, int gen, i
                 d = ps.d;
             }
         }
-        (void)wave_append(alive, C.cnt + qsel);   // count of traced camera rays (tally)
-    } else if (q < C.cnt[qsel]) {
-        k = sorted ? C.sorted[q] : C.queue[qsel][q];
+        // count of traced camera rays (tally)
+        (void)wave_append(alive, C.qcnt[qsel] + block_shard() * SHARD_STRIDE);
+    } else if (queue_entry(C, qsel, sorted, LANE_BLOCK, threadIdx.x, k)) {
         o = C.st.o[k];
         d = C.st.d[k];
         alive = true;
@@ -768,7 +866,7 @@ CRT_LANE_KERNEL void k_extend_shade_lane(Ctx C, int bounce, int qsel, int gen, i
     if (!alive) return;
     const RayPre r = make_ray_pre(o.x, o.y, o.z, d.x, d.y, d.z);
     const HitRec h = trace_lane<false, false, false>(C.s, r, 0.0f, __int_as_float(0x7f800000), stk);
-    const uint32_t ls = wave_append(h.slot >= 0, C.cnt + 3);   // live slot
+    const uint32_t ls = shard_append(C, h.slot >= 0, C.lcnt, block_shard());   // live position
     finish_extend<MATS>(C, bounce, k, h, ls, make_float3(d.x, d.y, d.z));
 }
 
@@ -781,8 +879,7 @@ template <bool MATS>
 __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8)))
 void k_extend_shade_octet(Ctx C, int bounce, int qsel, int R, int sorted) {
     OCTET_SETUP(C)
-    const uint32_t q0 = (blockIdx.x * OCTETS_PER_BLOCK + local) * (uint32_t)R;
-    const uint32_t n = C.cnt[qsel];
+    const uint32_t items = OCTETS_PER_BLOCK * (uint32_t)R, t0 = (uint32_t)local * (uint32_t)R;
     uint32_t k_mine = 0;
     HitRec h_mine;
     h_mine.slot = -1;
@@ -790,9 +887,8 @@ void k_extend_shade_octet(Ctx C, int bounce, int qsel, int R, int sorted) {
     bool mine = false;
 #pragma unroll 1
     for (int p = 0; p < R; ++p) {
-        const uint32_t q = q0 + (uint32_t)p;
-        if (q >= n) break;   // octet-uniform
-        const uint32_t k = sorted ? C.sorted[q] : C.queue[qsel][q];
+        uint32_t k;
+        if (!queue_entry(C, qsel, sorted, items, t0 + (uint32_t)p, k)) break;   // octet-uniform
         const float4 o = C.st.o[k], d = C.st.d[k];
         const RayPre r = make_ray_pre(o.x, o.y, o.z, d.x, d.y, d.z);
         const HitRec h = trace_octet(C.s, r, 0.0f, __int_as_float(0x7f800000), sref, sd, sub, lane_base);
@@ -803,7 +899,7 @@ void k_extend_shade_octet(Ctx C, int bounce, int qsel, int R, int sorted) {
             mine = true;
         }
     }
-    const uint32_t ls = wave_append(mine && h_mine.slot >= 0, C.cnt + 3);
+    const uint32_t ls = shard_append(C, mine && h_mine.slot >= 0, C.lcnt, block_shard());
     if (mine) finish_extend<MATS>(C, bounce, k_mine, h_mine, ls, d_mine);
 }
 
@@ -812,19 +908,15 @@ void k_extend_shade_octet(Ctx C, int bounce, int qsel, int R, int sorted) {
 __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8)))
 void k_shadow_accum(Ctx C, int bounce, int qsel, int next, int R) {
     QUAD_SETUP(C)
-    const uint32_t q0 = (blockIdx.x * (TRACE_BLOCK / QUAD) + qlocal) * (uint32_t)R;
-    const uint32_t n = C.cnt[3];
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        C.tally[0] += C.cnt[qsel];
-        C.tally[1] += n * (uint32_t)C.nl;
-    }
+    const uint32_t items = (TRACE_BLOCK / QUAD) * (uint32_t)R, t0 = (uint32_t)qlocal * (uint32_t)R;
+    if (blockIdx.x == 0) tally_bounce(C, qsel);   // block 0 is one whole wave
     float lr_mine[3] = {0.f, 0.f, 0.f};
     uint32_t k_mine = 0;
     bool mine = false;
 #pragma unroll 1
     for (int p = 0; p < R; ++p) {
-        const uint32_t q = q0 + (uint32_t)p;
-        if (q >= n) break;   // quad-uniform
+        uint32_t q;
+        if (!shard_entry(C, C.lcnt, items, t0 + (uint32_t)p, q)) break;   // quad-uniform
         float lr[3] = {0.f, 0.f, 0.f};
         for (int l = 0; l < C.nl; ++l) {
             const uint32_t i = q * (uint32_t)C.nl + (uint32_t)l;
@@ -844,8 +936,8 @@ void k_shadow_accum(Ctx C, int bounce, int qsel, int next, int R) {
         }
     }
     const bool cont = mine && accumulate_path(C, bounce, k_mine, lr_mine);
-    const uint32_t slot = wave_append(cont, C.cnt + next);
-    if (cont) C.queue[next][slot] = k_mine;
+    const uint32_t pos = shard_append(C, cont, C.qcnt[next], block_shard());
+    if (pos != NO_POS) C.queue[next][pos] = k_mine;
 }
 
 __global__ __launch_bounds__(TB) void k_zero_samples(Ctx C) {
@@ -915,11 +1007,11 @@ __global__ __launch_bounds__(TB) void k_fill_u8(uint8_t* p, uint32_t n, uint8_t 
 
 // Launch the kernels of one pass.  Grids are sized for `cap_paths` /
 // `cap_fold` so that the same sequence can be captured once and replayed.
-void launch_shade(const Ctx& C, bool mats, uint32_t gp, int b, int cur, hipStream_t st) {
+void launch_shade(const Ctx& C, bool mats, uint32_t grid, int b, int cur, hipStream_t st) {
     if (mats)
-        hipLaunchKernelGGL(k_shade<true>, dim3(gp), dim3(TB), 0, st, C, b, cur);
+        hipLaunchKernelGGL(k_shade<true>, dim3(grid), dim3(TB), 0, st, C, b, cur);
     else
-        hipLaunchKernelGGL(k_shade<false>, dim3(gp), dim3(TB), 0, st, C, b, cur);
+        hipLaunchKernelGGL(k_shade<false>, dim3(grid), dim3(TB), 0, st, C, b, cur);
 }
 
 void launch_pass(const Ctx& C, uint64_t cap_paths, uint32_t cap_fold, const crt_batch_opts& B, int num_cus,
@@ -931,7 +1023,15 @@ void launch_pass(const Ctx& C, uint64_t cap_paths, uint32_t cap_fold, const crt_
     const uint32_t gsq = std::max(1u, cdiv(cap_paths * std::max(1, C.nl), TRACE_BLOCK / QUAD));
     const size_t qlds = quad_lds_bytes(C.s);
     const uint32_t gpers = std::max(1u, B.blocks_per_cu) * (uint32_t)num_cus;
-    hipLaunchKernelGGL(k_clear, dim3(1), dim3(32), 0, st, C.cnt, 0x7Fu);
+    // entries any shard can hold in this pass, and the grid of a consumer of
+    // a sharded buffer with `items` inputs per block
+    const uint32_t span = std::min<uint32_t>(C.region, shard_region(cap_paths));
+    const auto gsh = [&](uint64_t items) { return SHARDS * cdiv(span, items); };
+    const uint32_t nl = (uint32_t)std::max(1, C.nl);
+    constexpr uint32_t QPB = TRACE_BLOCK / QUAD;
+    constexpr uint32_t ALL_SETS = (1u << N_SETS) - 1u;
+    hipLaunchKernelGGL(k_clear, dim3(1), dim3(SHARDS), 0, st, C.cnt,
+                       (1u << CNT_WORK_EXT) | (1u << CNT_WORK_SHADOW) | (1u << CNT_SORTED), ALL_SETS);
     // extend_mode 0 (default): lanes for the coherent camera rays of passes
     // large enough to fill the GPU with one lane per ray, octets otherwise
     const uint32_t em = B.extend_mode != 0 ? B.extend_mode : (cap_paths >= LANE_PASS_MIN ? 3u : 1u);
@@ -948,17 +1048,23 @@ void launch_pass(const Ctx& C, uint64_t cap_paths, uint32_t cap_fold, const crt_
         const bool sort = B.coherence_sort == 1;
         for (int b = 0; b < C.nb; ++b) {
             const int cur = b & 1, nxt = cur ^ 1;
-            hipLaunchKernelGGL(k_clear, dim3(1), dim3(32), 0, st, C.cnt, (1u << nxt) | (1u << 3));
+            hipLaunchKernelGGL(k_clear, dim3(1), dim3(SHARDS), 0, st, C.cnt, 0u, (1u << nxt) | (1u << SET_LIVE));
             const bool sorted = sort && b > 0 && (lane_at(b) || B.separate_kernels != 2);
             if (sorted) {
-                const uint32_t gs = std::max(1u, cdiv(cap_paths, SORT_TILE));
+                const uint32_t gs = gsh(SORT_TILE);
                 hipLaunchKernelGGL(k_sort_zero, dim3(1), dim3(SORT_TB), 0, st, C.bins);
                 hipLaunchKernelGGL(k_sort_hist, dim3(gs), dim3(SORT_TB), 0, st, C, cur);
-                hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, st, C.bins);
+                hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, st, C.bins, C.cnt + CNT_SORTED);
                 hipLaunchKernelGGL(k_sort_scatter, dim3(gs), dim3(SORT_TB), 0, st, C, cur);
             }
-            const dim3 gl(std::max(1u, cdiv(cap_paths, LANE_BLOCK)));
-            const dim3 goct(std::max(1u, cdiv(cap_paths, OCTETS_PER_BLOCK * ro)));
+            // a sorted queue is dense (as many blocks as its capacity needs); a
+            // sharded one takes SHARDS chunks of each shard; bounce 0 of the
+            // lane form makes its rays (one lane per path)
+            const auto grid = [&](uint32_t items) {
+                return std::max(1u, sorted || (gen && b == 0) ? cdiv(cap_paths, items) : gsh(items));
+            };
+            const dim3 gl(grid(LANE_BLOCK));
+            const dim3 goct(grid(OCTETS_PER_BLOCK * ro));
             if (lane_at(b)) {
                 if (mats)
                     hipLaunchKernelGGL(k_extend_shade_lane<true>, gl, dim3(LANE_BLOCK), lane_lds_bytes(C.s), st, C, b,
@@ -974,11 +1080,10 @@ void launch_pass(const Ctx& C, uint64_t cap_paths, uint32_t cap_fold, const crt_
                     hipLaunchKernelGGL(k_extend_shade_octet<false>, goct, dim3(TRACE_BLOCK), lds, st, C, b, cur, ro,
                                        (int)sorted);
             } else {
-                hipLaunchKernelGGL(k_trace, dim3(go), dim3(TRACE_BLOCK), lds, st, C, cur);
-                launch_shade(C, mats, gp, b, cur, st);
+                hipLaunchKernelGGL(k_trace, dim3(gsh(OCTETS_PER_BLOCK)), dim3(TRACE_BLOCK), lds, st, C, cur);
+                launch_shade(C, mats, gsh(TB), b, cur, st);
             }
-            hipLaunchKernelGGL(k_shadow_accum, dim3(std::max(1u, cdiv(cap_paths, (TRACE_BLOCK / QUAD) * rq))),
-                               dim3(TRACE_BLOCK), qlds, st, C, b, cur, nxt, rq);
+            hipLaunchKernelGGL(k_shadow_accum, dim3(gsh(QPB * rq)), dim3(TRACE_BLOCK), qlds, st, C, b, cur, nxt, rq);
         }
         if (C.nb == 0) hipLaunchKernelGGL(k_zero_samples, dim3(gp), dim3(TB), 0, st, C);
         hipLaunchKernelGGL(k_fold, dim3(std::max(1u, cdiv(cap_fold, TB))), dim3(TB), 0, st, C);
@@ -987,24 +1092,27 @@ void launch_pass(const Ctx& C, uint64_t cap_paths, uint32_t cap_fold, const crt_
     hipLaunchKernelGGL(k_raygen, dim3(gp), dim3(TB), 0, st, C);
     for (int b = 0; b < C.nb; ++b) {
         const int cur = b & 1, nxt = cur ^ 1;
-        hipLaunchKernelGGL(k_clear, dim3(1), dim3(32), 0, st, C.cnt, (1u << nxt) | (1u << 2) | (1u << 3) | (3u << 5));
+        hipLaunchKernelGGL(k_clear, dim3(1), dim3(SHARDS), 0, st, C.cnt,
+                           (1u << CNT_WORK_EXT) | (1u << CNT_WORK_SHADOW), (1u << nxt) | (1u << SET_LIVE));
         if (lane_at(b))
-            hipLaunchKernelGGL(k_extend_lane, dim3(std::max(1u, cdiv(cap_paths, LANE_BLOCK))), dim3(LANE_BLOCK),
-                               lane_lds_bytes(C.s), st, C, cur);
+            hipLaunchKernelGGL(k_extend_lane, dim3(gsh(LANE_BLOCK)), dim3(LANE_BLOCK), lane_lds_bytes(C.s), st, C,
+                               cur);
         else if (B.persistent)
-            hipLaunchKernelGGL(k_trace_persistent, dim3(std::min(gpers, go)), dim3(TRACE_BLOCK), lds, st, C, cur);
+            hipLaunchKernelGGL(k_trace_persistent, dim3(std::min(gpers, go)), dim3(TRACE_BLOCK), lds, st, C, cur,
+                               span);
         else
-            hipLaunchKernelGGL(k_trace, dim3(go), dim3(TRACE_BLOCK), lds, st, C, cur);
-        launch_shade(C, mats, gp, b, cur, st);
-        hipLaunchKernelGGL(k_tally, dim3(1), dim3(32), 0, st, C, cur);
+            hipLaunchKernelGGL(k_trace, dim3(gsh(OCTETS_PER_BLOCK)), dim3(TRACE_BLOCK), lds, st, C, cur);
+        launch_shade(C, mats, gsh(TB), b, cur, st);
+        hipLaunchKernelGGL(k_tally, dim3(1), dim3(64), 0, st, C, cur);
         if (C.nl > 0) {
             if (B.persistent)
                 hipLaunchKernelGGL(k_shadow_quad_persistent, dim3(std::min(gpers, gsq)), dim3(TRACE_BLOCK), qlds, st,
-                                   C);
+                                   C, span);
             else
-                hipLaunchKernelGGL(k_shadow_quad, dim3(gsq), dim3(TRACE_BLOCK), qlds, st, C);
+                hipLaunchKernelGGL(k_shadow_quad, dim3(SHARDS * cdiv((uint64_t)span * nl, QPB)), dim3(TRACE_BLOCK),
+                                   qlds, st, C);
         }
-        hipLaunchKernelGGL(k_accumulate, dim3(gp), dim3(TB), 0, st, C, b, nxt);
+        hipLaunchKernelGGL(k_accumulate, dim3(gsh(TB)), dim3(TB), 0, st, C, b, nxt);
     }
     if (C.nb == 0) hipLaunchKernelGGL(k_zero_samples, dim3(gp), dim3(TB), 0, st, C);
     hipLaunchKernelGGL(k_fold, dim3(std::max(1u, cdiv(cap_fold, TB))), dim3(TB), 0, st, C);
@@ -1062,7 +1170,8 @@ uint64_t pass_budget(int n_lights) {
 size_t render_bytes_estimate(uint64_t pixels, int spp, int n_lights) {
     const uint64_t paths = std::max<uint64_t>(pixels, std::min<uint64_t>(pixels * (uint64_t)std::max(spp, 1),
                                                                          pass_budget(n_lights)));
-    return (size_t)(paths * (PATH_BYTES + 12 + 16 + std::max(n_lights, 1) * sizeof(ShadowRec)) +
+    const uint64_t slots = (uint64_t)SHARDS * shard_region(paths);   // sharded queues, live list, shadow records
+    return (size_t)(paths * (PATH_BYTES + 4 + 16) + slots * (12 + std::max(n_lights, 1) * sizeof(ShadowRec)) +
                     pixels * (16 + 4 + 1 + 4 + 16 + 4));
 }
 
@@ -1186,14 +1295,18 @@ int render_batch(crt_scene* scene, uint32_t n_cams, const crt_camera* cams, cons
     mean.reserve((size_t)P * 16);
     spp.reserve((size_t)P * 4);
     active.reserve(P);
-    counters.reserve(32 * 4);
+    const uint32_t region = shard_region(cap_paths);
+    const uint64_t slots = (uint64_t)SHARDS * region;
+    // shadow records are indexed in 32 bits
+    CRT_REQUIRE(slots * (uint64_t)std::max(1, C.nl) < (1ull << 32), CRT_ERR_INVALID, "too many lights for one pass");
+    counters.reserve(counter_words() * 4);
     pstate.reserve(cap_paths * PATH_BYTES);
-    q0.reserve(cap_paths * 4);
-    q1.reserve(cap_paths * 4);
+    q0.reserve(slots * 4);
+    q1.reserve(slots * 4);
     W.qs.reserve(cap_paths * 4);
     W.bins.reserve(2 * SORT_BINS * 4);
-    live.reserve(cap_paths * 4);
-    shadow.reserve(cap_paths * std::max(1, C.nl) * sizeof(ShadowRec));
+    live.reserve(slots * 4);
+    shadow.reserve(slots * std::max(1, C.nl) * sizeof(ShadowRec));
     tally.reserve(16);
     samples.reserve(cap_paths * 16);
     CRT_HIP_CHECK(hipMemcpyAsync(d_cams.p, hc.data(), sizeof(CamDev) * n_cams, hipMemcpyHostToDevice, st));
@@ -1217,6 +1330,7 @@ int render_batch(crt_scene* scene, uint32_t n_cams, const crt_camera* cams, cons
         C.st.m = reinterpret_cast<uint4*>(base + 4 * cap_paths);
         C.st.h = reinterpret_cast<float2*>(base + 5 * cap_paths);
     }
+    C.region = region;
     C.queue[0] = q0.as<uint32_t>();
     C.queue[1] = q1.as<uint32_t>();
     C.sorted = W.qs.as<uint32_t>();
@@ -1229,6 +1343,10 @@ int render_batch(crt_scene* scene, uint32_t n_cams, const crt_camera* cams, cons
     C.spp = spp.as<int32_t>();
     C.active = active.as<uint8_t>();
     C.cnt = counters.as<uint32_t>();
+    C.qcnt[0] = C.cnt + CNT_PLAIN + (size_t)SET_Q0 * SHARDS * SHARD_STRIDE;
+    C.qcnt[1] = C.cnt + CNT_PLAIN + (size_t)SET_Q1 * SHARDS * SHARD_STRIDE;
+    C.lcnt = C.cnt + CNT_PLAIN + (size_t)SET_LIVE * SHARDS * SHARD_STRIDE;
+    hipLaunchKernelGGL(k_clear, dim3(1), dim3(SHARDS), 0, st, C.cnt, 1u << CNT_OVERFLOW, 0u);
 
     hipEvent_t e0, e1;
     CRT_HIP_CHECK(hipEventCreate(&e0));
@@ -1288,11 +1406,11 @@ int render_batch(crt_scene* scene, uint32_t n_cams, const crt_camera* cams, cons
         for (const Segment& sg : segs) done[sg.cam] += (int)sg.s_count;
         if (adaptive) {
             // remaining active pixels, per camera, in increasing pixel order
-            CRT_HIP_CHECK(hipMemsetAsync(C.cnt + 4, 0, 4, st));
+            CRT_HIP_CHECK(hipMemsetAsync(C.cnt + CNT_ACTIVE, 0, 4, st));
             hipLaunchKernelGGL(k_active_list, dim3(std::max(1u, cdiv(P, TB))), dim3(TB), 0, st, C.active, P,
-                               d_plist.as<uint32_t>(), C.cnt + 4);
+                               d_plist.as<uint32_t>(), C.cnt + CNT_ACTIVE);
             uint32_t na = 0;
-            CRT_HIP_CHECK(hipMemcpyAsync(&na, C.cnt + 4, 4, hipMemcpyDeviceToHost, st));
+            CRT_HIP_CHECK(hipMemcpyAsync(&na, C.cnt + CNT_ACTIVE, 4, hipMemcpyDeviceToHost, st));
             CRT_HIP_CHECK(hipStreamSynchronize(st));
             std::vector<uint32_t> hl(na);
             if (na) CRT_HIP_CHECK(hipMemcpy(hl.data(), d_plist.p, (size_t)na * 4, hipMemcpyDeviceToHost));
@@ -1327,8 +1445,11 @@ int render_batch(crt_scene* scene, uint32_t n_cams, const crt_camera* cams, cons
         CRT_HIP_CHECK(hipMemcpyAsync(spp_used, spp.p, (size_t)P * 4, hipMemcpyDeviceToDevice, st));
     }
     unsigned long long ht[2] = {0, 0};
+    uint32_t overflow = 0;
     CRT_HIP_CHECK(hipMemcpyAsync(ht, tally.p, 16, hipMemcpyDeviceToHost, st));
+    CRT_HIP_CHECK(hipMemcpyAsync(&overflow, C.cnt + CNT_OVERFLOW, 4, hipMemcpyDeviceToHost, st));
     CRT_HIP_CHECK(hipStreamSynchronize(st));
+    CRT_REQUIRE(overflow == 0, CRT_ERR_STATE, "render queue shard overflow");
     float ms = 0.f;
     CRT_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0);
