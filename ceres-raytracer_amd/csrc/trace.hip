@@ -302,6 +302,8 @@ CRT_LANE_KERNEL void k_primary_sun(SceneView s, CamView cam, double* depth, int3
     }
 }
 
+__global__ void k_zero_u32(uint32_t* p) { *p = 0u; }
+
 // Shadow rays are any-hit: quad traversal (16 rays per wave, two children
 // per lane) beats both one ray per lane, whose waves wait on their longest
 // unoccluded ray, and one ray per octet.  One wave (16 rays) per block.
@@ -934,7 +936,9 @@ int crt_sun_pass(crt_scene* scene, const crt_camera* camera, const double sun_di
         const float offset = self_offset(scene);
         EventTimer tm(timing != nullptr);
         const uint32_t blocks = cdiv((uint64_t)cam.w, PIX_TILE) * cdiv((uint64_t)cam.h, PIX_TILE);
-        CRT_HIP_CHECK(hipMemsetAsync(qcount, 0, 4, st));
+        // cleared by a kernel, not a memset node, so that the call can be
+        // captured in a graph
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, st, qcount);
         bool log;
         {
             std::lock_guard<std::mutex> lk(g_tlog_mu);
