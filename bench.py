@@ -602,6 +602,48 @@ def _served_rate(args, out, modes, meshes, sc, ropts):
     return out
 
 
+# The other workloads of the default run: (key, --workload, extra flags, time
+# limit in s).  The Cornell case keeps its CPU baseline (the reference's own
+# renderer on the host cores); the others skip theirs to keep the run short.
+OTHER_CONFIGS = (
+    ("C1_cornell", "cornell", [], 240),
+    ("C3_lidar_srp", "lidar_srp", ["--no-cpu-baseline"], 180),
+    ("C4_batched", "batched", ["--no-cpu-baseline"], 300),
+    ("C5_served", "served", [], 420),
+)
+
+
+def run_other_configs():
+    """Each other workload's JSON line, measured by this script in a child
+    process of its own (so a failure or a time-out there leaves the headline
+    line intact) after the headline's timed region, one after another."""
+    import subprocess
+    import tempfile
+    out = {}
+    for key, workload, extra, limit in OTHER_CONFIGS:
+        log("%s (--workload %s)" % (key, workload))
+        with tempfile.NamedTemporaryFile(suffix=".json", delete=False) as tf:
+            path = tf.name
+        t0 = time.perf_counter()
+        try:
+            r = subprocess.run([sys.executable, "-u", os.path.abspath(__file__), "--workload", workload,
+                                "--out", path, *extra], stdout=subprocess.DEVNULL, timeout=limit)
+            if r.returncode == 0:
+                with open(path) as fh:
+                    out[key] = json.loads(fh.read())
+            else:
+                out[key] = {"error": "exit status %d" % r.returncode}
+        except subprocess.TimeoutExpired:
+            out[key] = {"error": "no result within %d s" % limit}
+        except Exception as e:   # unreadable result file
+            out[key] = {"error": repr(e)}
+        finally:
+            if os.path.exists(path):
+                os.unlink(path)
+        out[key]["run_s"] = time.perf_counter() - t0
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -623,6 +665,9 @@ def main():
     ap.add_argument("--render-once", action="store_true",
                     help="batched: one render of the batch on one stream, default launch parameters (profiling)")
     ap.add_argument("--no-grouped", action="store_true", help="served: skip the one-stream grouped run")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="c2: skip the other workloads' lines (cornell, lidar_srp, batched, served) that the "
+                         "default run adds under other_configs")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
@@ -746,6 +791,10 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if not args.no_configs:
+        # the other workloads after the headline's timed region, each in its own process
+        del scene, obufs, depth, tri
+        line["other_configs"] = run_other_configs()
     print(json.dumps(line), flush=True)
     if args.out:
         with open(args.out, "w") as fh:

@@ -23,6 +23,7 @@ parser (lib/model_loaders/obj.hpp:75-77) reads back exactly the FP32 values.
 from __future__ import annotations
 
 import math
+import os
 from dataclasses import dataclass, field
 from typing import List, Tuple
 
@@ -289,7 +290,17 @@ def value_noise(p, seed=0):
 
 
 def fbm(p, octaves=6, lacunarity=2.0, gain=0.5, base_frequency=2.0, seed=0):
-    """Fractal sum of value noise normalised to [-1,1]."""
+    """Fractal sum of value noise normalised to [-1,1].  Every point is
+    independent, so large inputs are split into chunks summed on threads
+    (numpy releases the GIL inside its loops); the values are the same bits
+    as one pass over all points."""
+    chunk = 1 << 18
+    if p.shape[0] > 2 * chunk:
+        from concurrent.futures import ThreadPoolExecutor
+        parts = [p[i:i + chunk] for i in range(0, p.shape[0], chunk)]
+        with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1, len(parts))) as pool:
+            return np.concatenate(list(pool.map(
+                lambda q: fbm(q, octaves, lacunarity, gain, base_frequency, seed), parts)))
     total = np.zeros(p.shape[0])
     amp, freq, norm = 1.0, base_frequency, 0.0
     for o in range(octaves):
