@@ -138,6 +138,7 @@ struct Ctx {
     float offset;
     uint32_t grab;
     uint32_t tile_w;       // camera pixel tiles: 64 pixels, tile_w wide (tile_order)
+    int direct_fold;       // one sample per pixel per pass: a path folds its own sample (no k_fold)
     const CamDev* cams;
     const LightDev* lights;
     const Segment* segs;
@@ -294,6 +295,49 @@ __global__ void k_clear(uint32_t* cnt, uint32_t mask, uint32_t sets) {
 // Camera path k of the pass: FP64 camera ray (jittered), rebased to the
 // scene's bounding sphere, FP32 path state.  Returns false (and writes a
 // zero sample) when the ray misses the sphere.
+// Running-mean step of sample `smp` (1-based) of a pixel
+// (src/crt/do_render.hpp:100-108); false when the pixel stops (adaptive
+// threshold, from min_samples on).  Shared by k_fold and the paths that fold
+// their own sample, with contraction off, so every kernel computes the same
+// bits.
+__device__ inline bool fold_step(float4& m, float4 x, uint32_t smp, int min_samples, float thr) {
+#pragma clang fp contract(off)
+    const float inv = 1.0f / (float)smp;
+    const float rx = (x.x - m.x) * inv, ry = (x.y - m.y) * inv, rz = (x.z - m.z) * inv;
+    m.x += rx; m.y += ry; m.z += rz;
+    if ((int)smp >= min_samples) {
+        const float noise = sqrtf(rx * rx + ry * ry + rz * rz);
+        if (noise < thr) return false;
+    }
+    return true;
+}
+
+// End of path k (camera cam, camera-local pixel pix, sample smp) with
+// radiance L: written for k_fold, or, in a pass with one sample per pixel
+// (each pixel has exactly one path, so no other path touches its mean),
+// folded into the pixel's running mean at once.
+__device__ inline void end_path(const Ctx& C, uint64_t k, uint32_t cam, uint32_t pix, uint32_t smp, float4 L) {
+    if (!C.direct_fold) {
+        C.samples[k] = L;
+        return;
+    }
+    const uint32_t g = C.cams[cam].pix_base + pix;
+    float4 m = C.mean[g];
+    const bool act = fold_step(m, L, smp, C.min_samples, C.thr);
+    C.mean[g] = m;
+    C.spp[g] = (int)smp;
+    C.active[g] = act ? 1 : 0;
+}
+
+__device__ inline void end_path(const Ctx& C, uint64_t k, float4 L) {
+    if (!C.direct_fold) {
+        C.samples[k] = L;
+        return;
+    }
+    const uint4 m = C.st.m[k];   // cam, pix, sample
+    end_path(C, k, m.x, m.y, m.z, L);
+}
+
 __device__ inline bool gen_camera_path(const Ctx& C, const PassCtl& ctl, uint64_t k, PathState& ps) {
     bool alive = false;
     {
@@ -331,7 +375,7 @@ __device__ inline bool gen_camera_path(const Ctx& C, const PassCtl& ctl, uint64_
         } else {
             ps.o = make_float4(0.f, 0.f, 0.f, 0.f);
             ps.d = make_float4(0.f, 0.f, 1.f, 0.f);
-            C.samples[k] = make_float4(0.f, 0.f, 0.f, 0.f);   // camera ray misses the scene
+            end_path(C, k, sg.cam, pix, smp, make_float4(0.f, 0.f, 0.f, 0.f));   // camera ray misses the scene
         }
     }
     return alive;
@@ -741,7 +785,7 @@ __global__ __launch_bounds__(TB) void k_shade(Ctx C, int bounce, int qsel) {
     uint4 m = make_uint4(0u, 0u, 0u, 0u);
     if (queue_entry(C, qsel, 0, TB, threadIdx.x, k)) {
         m = C.st.m[k];
-        if ((int32_t)m.w < 0) C.samples[k] = C.st.L[k];   // path ends without a hit
+        if ((int32_t)m.w < 0) end_path(C, k, m.x, m.y, m.z, C.st.L[k]);   // path ends without a hit
         else has_hit = true;
     }
     // live position = shadow records ls*nl .. ls*nl+nl-1
@@ -784,7 +828,7 @@ __device__ inline bool accumulate_path(const Ctx& C, int bounce, uint32_t k, flo
     L.z += lr[2] * w.z;
     C.st.L[k] = L;
     if (bounce == C.nb - 1) {
-        C.samples[k] = L;
+        end_path(C, k, L);
         return false;
     }
     w.x *= w.w; w.y *= w.w; w.z *= w.w;
@@ -824,7 +868,7 @@ template <bool MATS>
 __device__ inline void finish_extend(const Ctx& C, int bounce, uint32_t k, const HitRec& h, uint32_t ls,
                                      float3 din) {
     if (h.slot < 0) {
-        C.samples[k] = C.st.L[k];   // path ends without a hit
+        end_path(C, k, C.st.L[k]);   // path ends without a hit
         return;
     }
     if (ls == NO_POS) return;   // shard full (flagged)
@@ -963,15 +1007,8 @@ __global__ __launch_bounds__(TB) void k_fold(Ctx C) {
     const uint64_t base = sg.path_begin + (uint64_t)i * sg.s_count;
     for (uint32_t j = 0; j < sg.s_count; ++j) {
         const uint32_t smp = sg.s_first + j;
-        const float4 x = C.samples[base + j];
-        const float inv = 1.0f / (float)smp;
-        const float rx = (x.x - m.x) * inv, ry = (x.y - m.y) * inv, rz = (x.z - m.z) * inv;
-        m.x += rx; m.y += ry; m.z += rz;
         used = (int)smp;
-        if ((int)smp >= C.min_samples) {
-            const float noise = sqrtf(rx * rx + ry * ry + rz * rz);
-            if (noise < C.thr) { act = false; break; }
-        }
+        if (!fold_step(m, C.samples[base + j], smp, C.min_samples, C.thr)) { act = false; break; }
     }
     C.mean[g] = m;
     C.spp[g] = used;
@@ -1083,10 +1120,16 @@ void launch_pass(const Ctx& C, uint64_t cap_paths, uint32_t cap_fold, const crt_
                 hipLaunchKernelGGL(k_trace, dim3(gsh(OCTETS_PER_BLOCK)), dim3(TRACE_BLOCK), lds, st, C, cur);
                 launch_shade(C, mats, gsh(TB), b, cur, st);
             }
-            hipLaunchKernelGGL(k_shadow_accum, dim3(gsh(QPB * rq)), dim3(TRACE_BLOCK), qlds, st, C, b, cur, nxt, rq);
+            // the bounce-0 queue holds every camera-ray hit, the later ones only
+            // the paths that hit again: by default bounce 0 runs half the paths
+            // per quad, twice the waves (C4: its shadow kernel 5.0 -> 4.55 ms per
+            // pass; the later bounces are slower with fewer paths per quad, their
+            // grids being sized for the whole pass)
+            const int rqb = (b == 0 && B.rays_per_quad == 0) ? std::max(1, rq / 2) : rq;
+            hipLaunchKernelGGL(k_shadow_accum, dim3(gsh(QPB * rqb)), dim3(TRACE_BLOCK), qlds, st, C, b, cur, nxt, rqb);
         }
         if (C.nb == 0) hipLaunchKernelGGL(k_zero_samples, dim3(gp), dim3(TB), 0, st, C);
-        hipLaunchKernelGGL(k_fold, dim3(std::max(1u, cdiv(cap_fold, TB))), dim3(TB), 0, st, C);
+        if (!C.direct_fold) hipLaunchKernelGGL(k_fold, dim3(std::max(1u, cdiv(cap_fold, TB))), dim3(TB), 0, st, C);
         return;
     }
     hipLaunchKernelGGL(k_raygen, dim3(gp), dim3(TB), 0, st, C);
@@ -1115,7 +1158,7 @@ void launch_pass(const Ctx& C, uint64_t cap_paths, uint32_t cap_fold, const crt_
         hipLaunchKernelGGL(k_accumulate, dim3(gsh(TB)), dim3(TB), 0, st, C, b, nxt);
     }
     if (C.nb == 0) hipLaunchKernelGGL(k_zero_samples, dim3(gp), dim3(TB), 0, st, C);
-    hipLaunchKernelGGL(k_fold, dim3(std::max(1u, cdiv(cap_fold, TB))), dim3(TB), 0, st, C);
+    if (!C.direct_fold) hipLaunchKernelGGL(k_fold, dim3(std::max(1u, cdiv(cap_fold, TB))), dim3(TB), 0, st, C);
 }
 
 }  // namespace
@@ -1281,6 +1324,9 @@ int render_batch(crt_scene* scene, uint32_t n_cams, const crt_camera* cams, cons
         for (int v : spp_c) smax = std::max(smax, v);
         S = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)smax, budget / std::max<uint64_t>(P, 1)));
     }
+    // one sample per pixel per pass (S = 1 makes every segment's s_count 1):
+    // paths fold their own samples
+    C.direct_fold = (S == 1 && C.nb > 0) ? 1 : 0;
     // capacity: the largest pass we will plan
     const uint64_t cap_paths = std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)P * S, std::max(budget, (uint64_t)P)));
 
@@ -1308,7 +1354,7 @@ int render_batch(crt_scene* scene, uint32_t n_cams, const crt_camera* cams, cons
     live.reserve(slots * 4);
     shadow.reserve(slots * std::max(1, C.nl) * sizeof(ShadowRec));
     tally.reserve(16);
-    samples.reserve(cap_paths * 16);
+    if (!C.direct_fold) samples.reserve(cap_paths * 16);
     CRT_HIP_CHECK(hipMemcpyAsync(d_cams.p, hc.data(), sizeof(CamDev) * n_cams, hipMemcpyHostToDevice, st));
     if (!hl.empty())
         CRT_HIP_CHECK(hipMemcpyAsync(W.d_lights.p, hl.data(), hl.size() * sizeof(LightDev), hipMemcpyHostToDevice, st));
