@@ -326,20 +326,22 @@ __device__ inline HitRec trace_octet(const SceneView& s, const RayPre& r, float 
             const uint32_t m = (uint32_t)((__ballot(h) >> lane_base) & 0xFFull);
             if (m != 0u) {
                 const int nh = __popc(m);
-                // rank among the hit children by tnear (>= 0, so its bits
-                // without the sign order like the value; the low 3 bits
-                // carry the lane)
+                // nearest hit child by tnear (>= 0, so its bits without the
+                // sign order like the value; the low 3 bits carry the lane),
+                // min-reduced over the octet in three DPP steps
                 const uint32_t key = h ? ((__float_as_uint(tnear) & 0x7FFFFFF8u) | (uint32_t)sub) : 0xFFFFFFFFu;
-                const uint32_t k7 = dpp<DPP_HALF_MIRROR>(key);
-                const uint32_t k1 = dpp<DPP_XOR1>(key), k2 = dpp<DPP_XOR2>(key), k3 = dpp<DPP_XOR3>(key);
-                const uint32_t k4 = dpp<DPP_XOR3>(k7), k5 = dpp<DPP_XOR2>(k7), k6 = dpp<DPP_XOR1>(k7);
-                const int rank = (int)(k1 < key) + (int)(k2 < key) + (int)(k3 < key) + (int)(k4 < key) +
-                                 (int)(k5 < key) + (int)(k6 < key) + (int)(k7 < key);
-                const uint32_t kmin = min(min(min(key, k1), min(k2, k3)), min(min(k4, k5), min(k6, k7)));
+                uint32_t kmin = min(key, dpp<DPP_XOR1>(key));
+                kmin = min(kmin, dpp<DPP_XOR2>(kmin));
+                kmin = min(kmin, dpp<DPP_HALF_MIRROR>(kmin));
                 const int first_lane = (int)(kmin & 7u);
                 const uint32_t next = (uint32_t)__shfl((int)ref, lane_base + first_lane, 64);
-                if (h && rank > 0) {
-                    const uint32_t pos = sp + (uint32_t)(nh - 1 - rank);
+                // the other hit children are pushed in child order: most
+                // bounce rays leave the body and visit every child they
+                // cross whatever the order; pops still skip entries beyond
+                // the best hit
+                if (h && sub != first_lane) {
+                    const uint32_t below = m & ((1u << sub) - 1u) & ~(1u << first_lane);
+                    const uint32_t pos = sp + (uint32_t)__popc(below);
                     if (pos < cap) {
                         stk_ref[pos] = ref;
                         stk_t[pos] = tnear;
