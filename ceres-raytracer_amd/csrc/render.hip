@@ -27,7 +27,9 @@
 //                         turn (origin triangle first), then lane p adds path
 //                         p's clamped direct light and queues it on
 // plus k_fold per pixel at the end of the pass (running mean in sample order,
-// adaptive stop).  Small passes trace camera rays by octets (k_raygen, k_trace,
+// adaptive stop); a pass with one sample per pixel has no k_fold: each path
+// folds its sample into its pixel's mean in the kernel where it ends (one
+// path per pixel, so no two paths touch a mean).  Small passes trace camera rays by octets (k_raygen, k_trace,
 // k_shade).  The stage pipeline (separate_kernels, and the persistent forms
 // whose octets / quads claim `grab` entries per atomic) runs k_raygen, then
 // per bounce k_trace / k_extend_lane -> k_shade -> k_tally -> k_shadow_quad ->
