@@ -614,9 +614,14 @@ struct TraversalStats {
 
 // Lane traversal in "while-while" form (Aila and Laine, HPG 2009): a lane
 // that reaches a leaf parks it and keeps descending until it meets a second
-// leaf or every lane of the wave holds one; then the wave tests triangles
-// together, so inner-node and leaf work of different lanes are not
-// interleaved step by step.
+// leaf or at least half of the lanes still descending hold one; then the
+// wave tests triangles together, so inner-node and leaf work of different
+// lanes are not interleaved step by step.  (Waiting for every lane, as the
+// original form does, leaves about half of the lanes idle in the node loop;
+// leaving at half tests leaves sooner, whose hits then cut node visits:
+// the 2048^2 sun pass on the 1.3M-triangle body 8280-8408 -> 8662-8684
+// Mrays/s, against 8515 when leaving after every step and 8591-8661 at
+// three quarters.)
 template <bool ANY_HIT, bool STATS = false, bool FIXED_PERM = true>
 __device__ inline HitRec trace_lane(const SceneView& s, const RayPre& r, float tmin, float tmax,
                                     const LaneStack& stk, TraversalStats* ts = nullptr) {
@@ -659,7 +664,8 @@ __device__ inline HitRec trace_lane(const SceneView& s, const RayPre& r, float t
                 }
             }
             cur = next;
-            if (__ballot(leaf == 0u) == 0ull) break;   // every lane still here holds a leaf
+            // at least half of the lanes still here hold a parked leaf
+            if (2u * (uint32_t)__popcll(__ballot(leaf == 0u)) <= (uint32_t)__popcll(__ballot(true))) break;
         }
         // ---- leaves: the parked one, then cur if it is a leaf ----
         while (leaf != 0u || (cur & REF_LEAF)) {
