@@ -39,8 +39,11 @@ def test_batch_equals_plain_renders(body):
 
 
 @pytest.mark.parametrize("variant", [dict(use_graph=True), dict(persistent=True, grab=4, blocks_per_cu=4),
-                                     dict(use_graph=True, persistent=True, grab=2, samples_per_pass=1)])
+                                     dict(use_graph=True, persistent=True, grab=2, samples_per_pass=1),
+                                     dict(samples_per_pass=1), dict(samples_per_pass=1, use_graph=True)])
 def test_batch_variants_identical(body, variant):
+    # samples_per_pass=1: every path folds its own sample where it ends (no
+    # k_fold); the same bits as the k_fold of passes of several samples
     m, s = body
     cams = _cams(m, 5, seed=9)
     sun = {"type": "directional", "intensity": 1.0, "direction": scenes.sun_direction(2)}
@@ -82,6 +85,15 @@ def test_batch_per_camera_spp_and_adaptive(body):
         single, _, sp1, _ = s.render(cams[k], [sun], 2, 6, 0.02, 2, seed=k + 1)
         assert np.array_equal(out2[k][1], sp1)
         assert np.array_equal(out2[k][0], single)
+    # and with one sample per pass, where each path folds its own sample and
+    # sets its pixel's stop flag in the kernel where it ends
+    out3, _ = s.render_batch(cams, [sun], 2, 6, 0.02, 2, seeds=[1, 2, 3, 4], samples_per_pass=1)
+    stopped = 0
+    for k in range(4):
+        assert np.array_equal(out3[k][1], out2[k][1])
+        assert np.array_equal(out3[k][0], out2[k][0])
+        stopped += int((out3[k][1] < 6).sum())
+    assert stopped > 0   # the threshold stops some pixels early
 
 
 @pytest.mark.parametrize("extend_mode,fused", [(1, 0), (2, 0), (3, 0), (1, 2), (3, 2), (0, 0)])
