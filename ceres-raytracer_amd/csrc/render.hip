@@ -572,7 +572,7 @@ CRT_LANE_KERNEL void k_extend_lane(Ctx C, int qsel) {
     if (!queue_entry(C, qsel, 0, LANE_BLOCK, threadIdx.x, k)) return;
     const float4 o = C.st.o[k], d = C.st.d[k];
     const RayPre r = make_ray_pre(o.x, o.y, o.z, d.x, d.y, d.z);
-    const HitRec h = trace_lane<false, false, false>(C.s, r, 0.0f, __int_as_float(0x7f800000), stk);
+    const HitRec h = trace_lane<false, false, true>(C.s, r, 0.0f, __int_as_float(0x7f800000), stk);
     C.st.m[k].w = (uint32_t)h.slot;
     C.st.h[k] = make_float2(h.u, h.v);
 }
@@ -911,7 +911,7 @@ CRT_LANE_KERNEL void k_extend_shade_lane(Ctx C, int bounce, int qsel, int gen, i
     }
     if (!alive) return;
     const RayPre r = make_ray_pre(o.x, o.y, o.z, d.x, d.y, d.z);
-    const HitRec h = trace_lane<false, false, false>(C.s, r, 0.0f, __int_as_float(0x7f800000), stk);
+    const HitRec h = trace_lane<false, false, true>(C.s, r, 0.0f, __int_as_float(0x7f800000), stk);
     const uint32_t ls = shard_append(C, h.slot >= 0, C.lcnt, block_shard());   // live position
     finish_extend<MATS>(C, bounce, k, h, ls, make_float3(d.x, d.y, d.z));
 }
