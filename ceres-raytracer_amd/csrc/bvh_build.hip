@@ -616,12 +616,21 @@ __global__ __launch_bounds__(BLK) void k_collapse(int n, const Item* __restrict_
 // C_NODE: a wide-node visit; C_LEAF: the fixed cost of entering a leaf;
 // C_TRI: one triangle test.  Chosen by measurement on the 1.31M-triangle
 // body (camera rays, shadow rays, flash LiDAR, flux, path tracing): with a
-// per-triangle cost only (C_LEAF 0, C_TRI 0.3) the DP makes leaves of 1-2
-// triangles and camera rays get 8-11 % slower than with the greedy collapse;
-// C_LEAF 2-4 with C_TRI 0.05-0.1 keeps camera rays level and makes shadow
-// rays ~6 % and path tracing ~2 % faster.
+// per-triangle cost of 0.3 the DP makes leaves of 1-2 triangles and camera
+// rays get 8-11 % slower than with the greedy collapse; C_TRI 0.05-0.1 with
+// C_LEAF 2-4 keeps camera rays level and makes shadow rays ~6 % and path
+// tracing ~2 % faster.  Since the lane traversal tests leaves once half of
+// its wave holds one, a leaf entry costs less: C_LEAF 0 / 1 / 2 / 3 / 4 with
+// C_TRI 0.1 gave the 2048^2 sun pass 8826-8880 / 8770-8778 / 8749-8783 /
+// 8681-8719 / 8643-8737 Mrays/s and the 256-camera batch 156.3-156.7 /
+// 156.1-156.3 / 155.9-156.7 / 156.7-157.0 / 156.8-157.5 ms (C_TRI 0.15 and
+// 0.2 at C_LEAF 0: sun pass 8929-8966 and 8856-8928, batch 157.6-157.7 and
+// 158.4-158.6 ms).  C_LEAF 0 merges the 36 large triangles of the Cornell
+// box into a few big leaves (its render 0.74 -> 0.84 ms); C_LEAF 1 keeps the
+// Cornell render level (0.740-0.761 ms), the sun pass at 8790-8827, the batch
+// at 156.1-156.2 ms and renders a 3.2M-triangle geodesic sphere 4 % faster.
 constexpr float DP_C_NODE = 1.0f;
-constexpr float DP_C_LEAF = 4.0f;
+constexpr float DP_C_LEAF = 1.0f;
 constexpr float DP_C_TRI = 0.1f;
 
 // Internal binary nodes of one level -> their internal children (next level).
