@@ -198,7 +198,8 @@ def load(path: str = LIB_PATH):
     return _lib
 
 
-BUILD_PLOC = 1   # crt_build_opts.flags: PLOC clustering instead of the Morton-order LBVH
+BUILD_PLOC = 1   # crt_build_opts.flags: PLOC clustering whatever the size
+BUILD_LBVH = 4   # crt_build_opts.flags: Morton-order LBVH whatever the size (default: PLOC up to 131072 triangles)
 BUILD_GREEDY_WIDE = 2   # crt_build_opts.flags: greedy collapse into 8-wide nodes instead of the SAH-optimal DP
 
 
@@ -381,9 +382,10 @@ class Scene:
     """A mesh + GPU-built BVH resident in HBM (crt_scene_create)."""
 
     def __init__(self, vertices, indices, tri_instance=None, vertex_normals=None, colors=None, smooth=None,
-                 max_leaf_size=8, ploc=False, greedy_wide=False, f64=False):
+                 max_leaf_size=8, ploc=None, greedy_wide=False, f64=False):
         """f64=True hands the vertices over in FP64 (crt_scene_create_f64):
-        the frame origin is subtracted before the rounding to FP32."""
+        the frame origin is subtracted before the rounding to FP32.  ploc:
+        None = the library's choice by size, True = PLOC, False = LBVH."""
         init()
         v = np.ascontiguousarray(vertices, dtype=np.float64 if f64 else np.float32).reshape(-1, 3)
         i = np.ascontiguousarray(indices, dtype=np.uint32).reshape(-1, 3)
@@ -394,7 +396,8 @@ class Scene:
         if col is not None:
             ni = max(ni, col.shape[0])
         sm = None if smooth is None else np.ascontiguousarray(smooth, dtype=np.uint8)
-        opts = BuildOpts(max_leaf_size, (BUILD_PLOC if ploc else 0) | (BUILD_GREEDY_WIDE if greedy_wide else 0))
+        tree = 0 if ploc is None else (BUILD_PLOC if ploc else BUILD_LBVH)
+        opts = BuildOpts(max_leaf_size, tree | (BUILD_GREEDY_WIDE if greedy_wide else 0))
         h = C.c_void_p()
         create = load().crt_scene_create_f64 if f64 else load().crt_scene_create
         _check(create(ptr(v), v.shape[0], ptr(i), i.shape[0], ptr(inst), ptr(vn), ni, ptr(col), ptr(sm),

@@ -1101,7 +1101,14 @@ void build_ploc(uint32_t n, const uint32_t* sorted_vals, DevBuf& tlo, DevBuf& th
 
 void build_bvh(const float* d_vertices, const uint32_t* d_indices, uint32_t n_tris, uint32_t max_leaf,
                uint32_t build_flags, hipStream_t st, BuildResult& out) {
-    const bool use_ploc = (build_flags & CRT_BUILD_PLOC) != 0;
+    // PLOC clusters by surface area where the LBVH only sorts centroids; on
+    // small scenes its tree is as fast or faster for every ray kind (the
+    // 36-triangle Cornell box renders 25 % faster, meshes of 1k-82k
+    // triangles trace sun passes 0-8 % faster), from ~200k triangles the
+    // LBVH tree traces faster (1.3M-triangle body: sun pass 3.6 % faster) and
+    // builds ~3x faster
+    const bool use_ploc = (build_flags & CRT_BUILD_PLOC) != 0 ||
+                          ((build_flags & CRT_BUILD_LBVH) == 0 && n_tris <= CRT_BUILD_PLOC_AUTO_MAX);
     const bool sah_wide = (build_flags & CRT_BUILD_GREEDY_WIDE) == 0;
     CRT_REQUIRE(n_tris > 0, CRT_ERR_INVALID, "scene has no triangles");
     CRT_REQUIRE(n_tris < (1u << 27), CRT_ERR_INVALID, "scene exceeds 2^27 triangles");

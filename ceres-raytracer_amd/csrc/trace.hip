@@ -584,7 +584,10 @@ int scene_create_impl(const VT* vertices, uint32_t n_vertices, const uint32_t* i
         const uint32_t max_leaf = opts && opts->max_leaf_size ? opts->max_leaf_size : 8u;
         CRT_REQUIRE(max_leaf <= 8u, CRT_ERR_INVALID, "max_leaf_size must be in 1..8");
         const uint32_t bflags = opts ? opts->flags : 0u;
-        CRT_REQUIRE((bflags & ~(CRT_BUILD_PLOC | CRT_BUILD_GREEDY_WIDE)) == 0u, CRT_ERR_INVALID, "unknown build flags");
+        CRT_REQUIRE((bflags & ~(CRT_BUILD_PLOC | CRT_BUILD_GREEDY_WIDE | CRT_BUILD_LBVH)) == 0u, CRT_ERR_INVALID,
+                    "unknown build flags");
+        CRT_REQUIRE((bflags & (CRT_BUILD_PLOC | CRT_BUILD_LBVH)) != (CRT_BUILD_PLOC | CRT_BUILD_LBVH), CRT_ERR_INVALID,
+                    "CRT_BUILD_PLOC and CRT_BUILD_LBVH exclude each other");
         s->max_leaf = max_leaf;
         s->build_flags = bflags;
         build_bvh(s->vertices.as<float>(), s->indices.as<uint32_t>(), n_tris, max_leaf, bflags, nullptr, s->bvh);

@@ -98,14 +98,18 @@ typedef struct {
     double direction[3];      /* directional: unit vector from the scene towards the light */
 } crt_light;
 
-#define CRT_BUILD_PLOC 1u     /* binary tree by locally-ordered clustering (PLOC) instead of LBVH:
+#define CRT_BUILD_PLOC 1u     /* binary tree by locally-ordered clustering (PLOC) whatever the size:
                                  lower SAH, ~3x the build time */
 #define CRT_BUILD_GREEDY_WIDE 2u /* collapse the binary tree into 8-wide nodes greedily (largest box
                                     first) instead of by SAH-optimal dynamic programming (default) */
+#define CRT_BUILD_LBVH 4u     /* Morton-order LBVH binary tree whatever the size */
+#define CRT_BUILD_PLOC_AUTO_MAX 131072u   /* without PLOC / LBVH: PLOC up to this many triangles */
 
 typedef struct {
     uint32_t max_leaf_size;   /* triangles per leaf, 1..8 (default 8)  */
-    uint32_t flags;           /* CRT_BUILD_*; 0 = LBVH (Morton radix tree) + SAH-optimal collapse */
+    uint32_t flags;           /* CRT_BUILD_*; 0 = PLOC binary tree for scenes of at most
+                                 CRT_BUILD_PLOC_AUTO_MAX triangles, LBVH (Morton radix tree) for
+                                 larger ones, then the SAH-optimal collapse */
 } crt_build_opts;
 
 typedef struct {

@@ -258,14 +258,15 @@ def test_single_triangle_scene():
 
 def test_ploc_builder_matches(icosphere, cornell):
     # the PLOC binary tree (CRT_BUILD_PLOC) changes only the BVH: hits agree
-    # with the default LBVH scene except at exact ties / edge grazes
+    # with the LBVH scene (CRT_BUILD_LBVH) except at exact ties / edge grazes
     gpu, cpu, m = icosphere
+    lbvh = capi.Scene(m.vertices, m.indices, ploc=False)
     alt = capi.Scene(m.vertices, m.indices, ploc=True)
-    g_greedy = capi.Scene(m.vertices, m.indices, greedy_wide=True)
+    g_greedy = capi.Scene(m.vertices, m.indices, ploc=False, greedy_wide=True)
     a_greedy = capi.Scene(m.vertices, m.indices, ploc=True, greedy_wide=True)
     assert a_greedy.info()["sah_cost"] < g_greedy.info()["sah_cost"]
     cam = scenes.small_body_camera(m, resolution=512, seed=1)
-    a, b = gpu.camera_pass(cam), alt.camera_pass(cam)
+    a, b = lbvh.camera_pass(cam), alt.camera_pass(cam)
     mism = a["tri"] != b["tri"]
     if mism.any():
         o, d = oracle.camera_rays(cam)
@@ -278,7 +279,7 @@ def test_ploc_builder_matches(icosphere, cornell):
     g, c, v, i = cornell
     sc = scenes.cornell_box(0)
     _, _, inst = sc.merged()
-    alt = capi.Scene(v, i, inst, ploc=True)
+    alt = capi.Scene(v, i, inst, ploc=False)   # the Cornell box's default tree is PLOC (36 triangles)
     cam = scenes.cornell_camera()
     pa, pc = alt.camera_pass(cam), c.camera_pass(cam)
     mism = pa["tri"] != pc["tri"]
@@ -287,6 +288,27 @@ def test_ploc_builder_matches(icosphere, cornell):
         sel = mism.ravel()
         ok = c.tie_flags(o[sel], d[sel]) | c.edge_flags(o[sel], d[sel], pa["tri"].ravel()[sel])
         assert ok.all()
+
+
+def test_default_tree_by_size():
+    # without a tree flag the library builds PLOC up to 131072 triangles and
+    # LBVH above: the default scene's SAH is that of the explicit choice (the
+    # SAH is a parallel sum, equal to ~1e-15) and not that of the other tree
+    for level, ploc in ((6, True), (7, False)):
+        m = scenes.displaced_icosphere(level)
+        d = capi.Scene(m.vertices, m.indices).info()["sah_cost"]
+        same = capi.Scene(m.vertices, m.indices, ploc=ploc).info()["sah_cost"]
+        other = capi.Scene(m.vertices, m.indices, ploc=not ploc).info()["sah_cost"]
+        assert abs(d - same) <= 1e-9 * same, level
+        assert abs(d - other) > 1e-4 * other, level
+    # the two tree flags exclude each other
+    with pytest.raises(ValueError):
+        opts = capi.BuildOpts(8, capi.BUILD_PLOC | capi.BUILD_LBVH)
+        h = capi.C.c_void_p()
+        v = np.ascontiguousarray(m.vertices, dtype=np.float32)
+        i = np.ascontiguousarray(m.indices, dtype=np.uint32)
+        capi._check(capi.load().crt_scene_create(capi.ptr(v), v.shape[0], capi.ptr(i), i.shape[0], None, None, 1,
+                                                 None, None, capi.C.byref(opts), capi.C.byref(h)))
 
 
 @pytest.mark.parametrize("ploc", [False, True])
