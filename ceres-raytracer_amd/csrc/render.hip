@@ -916,34 +916,55 @@ CRT_LANE_KERNEL void k_extend_shade_lane(Ctx C, int bounce, int qsel, int gen, i
     finish_extend<MATS>(C, bounce, k, h, ls, make_float3(d.x, d.y, d.z));
 }
 
-// Octet form for the bounces after the first.  Each octet traces R paths
-// one after another (R = 1..8, by launch size) and lane p of the octet then
-// shades path p, so a wave carries 8R paths through the shading and one
-// atomic per wave appends them.  Those rays leave the body more often than
-// camera rays hit it; camera rays traced by octets use k_trace + k_shade.
+// Octet form for the bounces after the first.  A block takes a chunk of
+// 8R queued paths (R = 1..8, by launch size); its octets take the chunk's
+// paths one at a time from a counter in LDS, so an octet that finishes early
+// traces more of them and the wave lasts about the chunk's mean work instead
+// of its slowest octet's share; each traced path leaves its hit (triangle,
+// u, v) in LDS, and lane p then shades path p, so a wave carries 8R paths
+// through the shading and one atomic per wave appends them.  Those rays
+// leave the body more often than camera rays hit it; camera rays traced by
+// octets use k_trace + k_shade.
+constexpr size_t OCTET_RES_BYTES = 16 + 3 * 4 * 64;   // path counter + (triangle, u, v) per chunk path
+
 template <bool MATS>
 __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8)))
 void k_extend_shade_octet(Ctx C, int bounce, int qsel, int R, int sorted) {
     OCTET_SETUP(C)
-    const uint32_t items = OCTETS_PER_BLOCK * (uint32_t)R, t0 = (uint32_t)local * (uint32_t)R;
-    uint32_t k_mine = 0;
-    HitRec h_mine;
-    h_mine.slot = -1;
-    float3 d_mine = make_float3(0.f, 0.f, 0.f);
-    bool mine = false;
+    const uint32_t items = OCTETS_PER_BLOCK * (uint32_t)R;   // <= 64
+    uint32_t* next_path = smem + (size_t)2 * OCTETS_PER_BLOCK * C.s.stack_cap;
+    uint32_t* res = next_path + 4;
+    if (threadIdx.x == 0) *next_path = 0u;
+    __syncthreads();
 #pragma unroll 1
-    for (int p = 0; p < R; ++p) {
+    while (true) {
+        uint32_t j = 0;
+        if (sub == 0) j = atomicAdd(next_path, 1u);
+        j = (uint32_t)__shfl((int)j, lane_base, 64);   // octet-uniform
         uint32_t k;
-        if (!queue_entry(C, qsel, sorted, items, t0 + (uint32_t)p, k)) break;   // octet-uniform
+        if (j >= items || !queue_entry(C, qsel, sorted, items, j, k)) break;
         const float4 o = C.st.o[k], d = C.st.d[k];
         const RayPre r = make_ray_pre(o.x, o.y, o.z, d.x, d.y, d.z);
         const HitRec h = trace_octet(C.s, r, 0.0f, __int_as_float(0x7f800000), sref, sd, sub, lane_base);
-        if (sub == p) {   // the hit is octet-uniform
-            k_mine = k;
-            h_mine = h;
-            d_mine = make_float3(d.x, d.y, d.z);
-            mine = true;
+        if (sub == 0) {
+            res[3 * j] = (uint32_t)h.slot;
+            res[3 * j + 1] = __float_as_uint(h.u);
+            res[3 * j + 2] = __float_as_uint(h.v);
         }
+    }
+    __syncthreads();
+    const uint32_t p = threadIdx.x;
+    uint32_t k_mine = 0;
+    bool mine = p < items && queue_entry(C, qsel, sorted, items, p, k_mine);
+    HitRec h_mine;
+    h_mine.t = 0.f; h_mine.u = 0.f; h_mine.v = 0.f; h_mine.slot = -1;
+    float3 d_mine = make_float3(0.f, 0.f, 0.f);
+    if (mine) {
+        h_mine.slot = (int32_t)res[3 * p];
+        h_mine.u = __uint_as_float(res[3 * p + 1]);
+        h_mine.v = __uint_as_float(res[3 * p + 2]);
+        const float4 d = C.st.d[k_mine];
+        d_mine = make_float3(d.x, d.y, d.z);
     }
     const uint32_t ls = shard_append(C, mine && h_mine.slot >= 0, C.lcnt, block_shard());
     if (mine) finish_extend<MATS>(C, bounce, k_mine, h_mine, ls, d_mine);
@@ -1113,11 +1134,11 @@ void launch_pass(const Ctx& C, uint64_t cap_paths, uint32_t cap_fold, const crt_
                                        b, cur, (int)(gen && b == 0), (int)sorted);
             } else if (b > 0 && B.separate_kernels != 2) {
                 if (mats)
-                    hipLaunchKernelGGL(k_extend_shade_octet<true>, goct, dim3(TRACE_BLOCK), lds, st, C, b, cur, ro,
-                                       (int)sorted);
+                    hipLaunchKernelGGL(k_extend_shade_octet<true>, goct, dim3(TRACE_BLOCK), lds + OCTET_RES_BYTES, st, C,
+                                       b, cur, ro, (int)sorted);
                 else
-                    hipLaunchKernelGGL(k_extend_shade_octet<false>, goct, dim3(TRACE_BLOCK), lds, st, C, b, cur, ro,
-                                       (int)sorted);
+                    hipLaunchKernelGGL(k_extend_shade_octet<false>, goct, dim3(TRACE_BLOCK), lds + OCTET_RES_BYTES, st, C,
+                                       b, cur, ro, (int)sorted);
             } else {
                 hipLaunchKernelGGL(k_trace, dim3(gsh(OCTETS_PER_BLOCK)), dim3(TRACE_BLOCK), lds, st, C, cur);
                 launch_shade(C, mats, gsh(TB), b, cur, st);
