@@ -927,6 +927,22 @@ CRT_LANE_KERNEL void k_extend_shade_lane(Ctx C, int bounce, int qsel, int gen, i
 // octets use k_trace + k_shade.
 constexpr size_t OCTET_RES_BYTES = 16 + 3 * 4 * 64;   // path counter + (triangle, u, v) per chunk path
 
+// Next slot of a wave-shared LDS counter for each lane group (octet or quad,
+// whole groups active together) still taking work: one ballot of the groups'
+// first lanes, one atomic by the wave's first active lane (always a group's
+// first lane) for all of them, and every lane derives its group's slot from
+// the ballot, so no lane reads another lane's register after a branch on the
+// lane index (in a loop with group-dependent exits, such a read let the
+// compiler move the other lanes into a loop of their own that waited for the
+// first lane forever).
+__device__ inline uint32_t group_grab(uint32_t* counter, bool group_first, int group_base) {
+    const uint64_t lead = __ballot(group_first);
+    uint32_t base = 0;
+    if ((int)(threadIdx.x & 63) == __ffsll((long long)lead) - 1) base = atomicAdd(counter, (uint32_t)__popcll(lead));
+    base = __builtin_amdgcn_readfirstlane(base);
+    return base + (uint32_t)__popcll(lead & ((1ull << group_base) - 1ull));
+}
+
 template <bool MATS>
 __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8)))
 void k_extend_shade_octet(Ctx C, int bounce, int qsel, int R, int sorted) {
@@ -938,9 +954,7 @@ void k_extend_shade_octet(Ctx C, int bounce, int qsel, int R, int sorted) {
     __syncthreads();
 #pragma unroll 1
     while (true) {
-        uint32_t j = 0;
-        if (sub == 0) j = atomicAdd(next_path, 1u);
-        j = (uint32_t)__shfl((int)j, lane_base, 64);   // octet-uniform
+        const uint32_t j = group_grab(next_path, sub == 0, lane_base);   // octet-uniform
         uint32_t k;
         if (j >= items || !queue_entry(C, qsel, sorted, items, j, k)) break;
         const float4 o = C.st.o[k], d = C.st.d[k];
@@ -970,20 +984,27 @@ void k_extend_shade_octet(Ctx C, int bounce, int qsel, int R, int sorted) {
     if (mine) finish_extend<MATS>(C, bounce, k_mine, h_mine, ls, d_mine);
 }
 
-// One quad traces the shadow rays of R live paths (R = 1..4, by launch
-// size) one after another; lane p of the quad then accumulates path p.
+// A block takes a chunk of 16R live paths (R = 1..4, by launch size); its
+// quads take the chunk's paths one at a time from a counter in LDS (as the
+// octet bounce kernel does) and test each path's shadow rays (origin triangle
+// first), leaving the path's unoccluded light in LDS; lane p then adds path
+// p's clamped direct light and queues it on.
+constexpr size_t QUAD_RES_BYTES = 16 + 3 * 4 * 64;   // path counter + light rgb per chunk path
+
 __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8)))
 void k_shadow_accum(Ctx C, int bounce, int qsel, int next, int R) {
     QUAD_SETUP(C)
-    const uint32_t items = (TRACE_BLOCK / QUAD) * (uint32_t)R, t0 = (uint32_t)qlocal * (uint32_t)R;
+    const uint32_t items = (TRACE_BLOCK / QUAD) * (uint32_t)R;   // <= 64
+    uint32_t* next_path = quad_stacks + (size_t)(TRACE_BLOCK / QUAD) * C.s.stack_cap;
+    float* res = reinterpret_cast<float*>(next_path + 4);
     if (blockIdx.x == 0) tally_bounce(C, qsel);   // block 0 is one whole wave
-    float lr_mine[3] = {0.f, 0.f, 0.f};
-    uint32_t k_mine = 0;
-    bool mine = false;
+    if (threadIdx.x == 0) *next_path = 0u;
+    __syncthreads();
 #pragma unroll 1
-    for (int p = 0; p < R; ++p) {
+    while (true) {
+        const uint32_t j = group_grab(next_path, qj == 0, qbase);   // quad-uniform
         uint32_t q;
-        if (!shard_entry(C, C.lcnt, items, t0 + (uint32_t)p, q)) break;   // quad-uniform
+        if (j >= items || !shard_entry(C, C.lcnt, items, j, q)) break;
         float lr[3] = {0.f, 0.f, 0.f};
         for (int l = 0; l < C.nl; ++l) {
             const uint32_t i = q * (uint32_t)C.nl + (uint32_t)l;
@@ -996,11 +1017,20 @@ void k_shadow_accum(Ctx C, int bounce, int qsel, int next, int R) {
                 lr[0] += c.x; lr[1] += c.y; lr[2] += c.z;
             }
         }
-        if (qj == p) {
-            k_mine = C.live[q];
-            lr_mine[0] = lr[0]; lr_mine[1] = lr[1]; lr_mine[2] = lr[2];
-            mine = true;
+        if (qj == 0) {
+            res[3 * j] = lr[0];
+            res[3 * j + 1] = lr[1];
+            res[3 * j + 2] = lr[2];
         }
+    }
+    __syncthreads();
+    const uint32_t p = threadIdx.x;
+    uint32_t q = 0, k_mine = 0;
+    const bool mine = p < items && shard_entry(C, C.lcnt, items, p, q);
+    float lr_mine[3] = {0.f, 0.f, 0.f};
+    if (mine) {
+        k_mine = C.live[q];
+        lr_mine[0] = res[3 * p]; lr_mine[1] = res[3 * p + 1]; lr_mine[2] = res[3 * p + 2];
     }
     const bool cont = mine && accumulate_path(C, bounce, k_mine, lr_mine);
     const uint32_t pos = shard_append(C, cont, C.qcnt[next], block_shard());
@@ -1149,7 +1179,8 @@ void launch_pass(const Ctx& C, uint64_t cap_paths, uint32_t cap_fold, const crt_
             // pass; the later bounces are slower with fewer paths per quad, their
             // grids being sized for the whole pass)
             const int rqb = (b == 0 && B.rays_per_quad == 0) ? std::max(1, rq / 2) : rq;
-            hipLaunchKernelGGL(k_shadow_accum, dim3(gsh(QPB * rqb)), dim3(TRACE_BLOCK), qlds, st, C, b, cur, nxt, rqb);
+            hipLaunchKernelGGL(k_shadow_accum, dim3(gsh(QPB * rqb)), dim3(TRACE_BLOCK), qlds + QUAD_RES_BYTES, st, C, b,
+                               cur, nxt, rqb);
         }
         if (C.nb == 0) hipLaunchKernelGGL(k_zero_samples, dim3(gp), dim3(TB), 0, st, C);
         if (!C.direct_fold) hipLaunchKernelGGL(k_fold, dim3(std::max(1u, cdiv(cap_fold, TB))), dim3(TB), 0, st, C);
