@@ -214,6 +214,24 @@ __device__ inline uint32_t wave_append(bool want, uint32_t* counter) {
     return base + (uint32_t)__popcll(m & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
 }
 
+// Work claim of each lane group (octet or quad; a group's lanes are active
+// together) still taking work from a shared counter: `amount` entries per
+// group.  One ballot of the groups' first lanes, one atomic by the wave's
+// first active lane (always a group's first lane) for all of them, and every
+// lane derives its group's first entry from the ballot, so no lane reads
+// another lane's register after a branch on the lane index: in a loop with
+// group-dependent exits, such a read (a first lane's atomic, then a shuffle
+// from it) let the compiler move the other lanes into an inner loop that
+// went on reading the exited first lane's register, and never left it.
+__device__ inline uint32_t group_grab(uint32_t* counter, bool group_first, int group_base, uint32_t amount = 1u) {
+    const uint64_t lead = __ballot(group_first);
+    uint32_t base = 0;
+    if ((int)(threadIdx.x & 63) == __ffsll((long long)lead) - 1)
+        base = atomicAdd(counter, (uint32_t)__popcll(lead) * amount);
+    base = __builtin_amdgcn_readfirstlane(base);
+    return base + (uint32_t)__popcll(lead & ((1ull << group_base) - 1ull)) * amount;
+}
+
 __device__ inline uint32_t block_shard() { return blockIdx.x & (SHARDS - 1); }
 
 // Position in a sharded buffer of this lane's entry, appended to shard s of
@@ -553,9 +571,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow_quad_persistent(Ctx C, u
     uint32_t* work = C.cnt + CNT_WORK_SHADOW;
     const uint32_t nl = (uint32_t)C.nl, per = span * nl, n = SHARDS * per;
     while (true) {
-        uint32_t base = 0;
-        if (qj == 0) base = atomicAdd(work, C.grab);
-        base = (uint32_t)__shfl((int)base, qbase, 64);
+        const uint32_t base = group_grab(work, qj == 0, qbase, C.grab);   // quad-uniform
         if (base >= n) break;
         const uint32_t end = min(base + C.grab, n);
         for (uint32_t q = base; q < end; ++q) {
@@ -585,9 +601,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_persistent(Ctx C, int qse
     uint32_t* work = C.cnt + CNT_WORK_EXT;
     const uint32_t n = SHARDS * span;
     while (true) {
-        uint32_t base = 0;
-        if (sub == 0) base = atomicAdd(work, C.grab);
-        base = (uint32_t)__shfl((int)base, lane_base, 64);
+        const uint32_t base = group_grab(work, sub == 0, lane_base, C.grab);   // octet-uniform
         if (base >= n) break;
         const uint32_t end = min(base + C.grab, n);
         for (uint32_t q = base; q < end; ++q) {
@@ -926,22 +940,6 @@ CRT_LANE_KERNEL void k_extend_shade_lane(Ctx C, int bounce, int qsel, int gen, i
 // leave the body more often than camera rays hit it; camera rays traced by
 // octets use k_trace + k_shade.
 constexpr size_t OCTET_RES_BYTES = 16 + 3 * 4 * 64;   // path counter + (triangle, u, v) per chunk path
-
-// Next slot of a wave-shared LDS counter for each lane group (octet or quad,
-// whole groups active together) still taking work: one ballot of the groups'
-// first lanes, one atomic by the wave's first active lane (always a group's
-// first lane) for all of them, and every lane derives its group's slot from
-// the ballot, so no lane reads another lane's register after a branch on the
-// lane index (in a loop with group-dependent exits, such a read let the
-// compiler move the other lanes into a loop of their own that waited for the
-// first lane forever).
-__device__ inline uint32_t group_grab(uint32_t* counter, bool group_first, int group_base) {
-    const uint64_t lead = __ballot(group_first);
-    uint32_t base = 0;
-    if ((int)(threadIdx.x & 63) == __ffsll((long long)lead) - 1) base = atomicAdd(counter, (uint32_t)__popcll(lead));
-    base = __builtin_amdgcn_readfirstlane(base);
-    return base + (uint32_t)__popcll(lead & ((1ull << group_base) - 1ull));
-}
 
 template <bool MATS>
 __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8)))
