@@ -1171,14 +1171,8 @@ void launch_pass(const Ctx& C, uint64_t cap_paths, uint32_t cap_fold, const crt_
                 hipLaunchKernelGGL(k_trace, dim3(gsh(OCTETS_PER_BLOCK)), dim3(TRACE_BLOCK), lds, st, C, cur);
                 launch_shade(C, mats, gsh(TB), b, cur, st);
             }
-            // the bounce-0 queue holds every camera-ray hit, the later ones only
-            // the paths that hit again: by default bounce 0 runs half the paths
-            // per quad, twice the waves (C4: its shadow kernel 5.0 -> 4.55 ms per
-            // pass; the later bounces are slower with fewer paths per quad, their
-            // grids being sized for the whole pass)
-            const int rqb = (b == 0 && B.rays_per_quad == 0) ? std::max(1, rq / 2) : rq;
-            hipLaunchKernelGGL(k_shadow_accum, dim3(gsh(QPB * rqb)), dim3(TRACE_BLOCK), qlds + QUAD_RES_BYTES, st, C, b,
-                               cur, nxt, rqb);
+            hipLaunchKernelGGL(k_shadow_accum, dim3(gsh(QPB * rq)), dim3(TRACE_BLOCK), qlds + QUAD_RES_BYTES, st, C, b,
+                               cur, nxt, rq);
         }
         if (C.nb == 0) hipLaunchKernelGGL(k_zero_samples, dim3(gp), dim3(TB), 0, st, C);
         if (!C.direct_fold) hipLaunchKernelGGL(k_fold, dim3(std::max(1u, cdiv(cap_fold, TB))), dim3(TB), 0, st, C);
