@@ -81,7 +81,7 @@ class BatchOpts(C.Structure):
     _fields_ = [("samples_per_pass", C.c_uint32), ("use_graph", C.c_uint32), ("persistent", C.c_uint32),
                 ("blocks_per_cu", C.c_uint32), ("grab", C.c_uint32), ("extend_mode", C.c_uint32),
                 ("separate_kernels", C.c_uint32), ("coherence_sort", C.c_uint32), ("rays_per_octet", C.c_uint32),
-                ("rays_per_quad", C.c_uint32), ("tile_width", C.c_uint32)]
+                ("rays_per_quad", C.c_uint32), ("tile_width", C.c_uint32), ("fixed_share", C.c_uint32)]
 
 
 class FluxOpts(C.Structure):
@@ -616,7 +616,7 @@ class Scene:
     def render_batch(self, cams, lights, min_samples=1, max_samples=1, noise_threshold=0.0, num_bounces=1,
                      seeds=None, spp=None, samples_per_pass=0, use_graph=False, persistent=False, blocks_per_cu=0,
                      grab=0, extend_mode=0, separate_kernels=0, coherence_sort=0, rays_per_octet=0, rays_per_quad=0,
-                     tile_width=0, outputs=True, stream=None):
+                     tile_width=0, fixed_share=0, outputs=True, stream=None):
         """Render many cameras in one set of launches.  Returns a list of
         per-camera (rgba f32, spp_used) and the timing dict."""
         n = len(cams)
@@ -625,7 +625,7 @@ class Scene:
         opts = RenderOpts(int(min_samples), int(max_samples), float(noise_threshold), int(num_bounces), 0, 0)
         bo = BatchOpts(int(samples_per_pass), int(bool(use_graph)), int(bool(persistent)), int(blocks_per_cu),
                        int(grab), int(extend_mode), int(separate_kernels), int(coherence_sort), int(rays_per_octet),
-                       int(rays_per_quad), int(tile_width))
+                       int(rays_per_quad), int(tile_width), int(fixed_share))
         sd = None if seeds is None else np.ascontiguousarray(seeds, dtype=np.uint64)
         ms = None if spp is None else np.ascontiguousarray(spp, dtype=np.int32)
         sizes = [(int(np.floor(c["resolution"][1])), int(np.floor(c["resolution"][0]))) for c in cams]

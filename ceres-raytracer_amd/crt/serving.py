@@ -27,7 +27,7 @@ class Server:
                             int(ro.get("grab", 0)), int(ro.get("extend_mode", 0)),
                             int(ro.get("separate_kernels", 0)), int(ro.get("coherence_sort", 0)),
                             int(ro.get("rays_per_octet", 0)), int(ro.get("rays_per_quad", 0)),
-                            int(ro.get("tile_width", 0)))
+                            int(ro.get("tile_width", 0)), int(ro.get("fixed_share", 0)))
         o = capi.ServerOpts(int(memory_budget), int(n_workers), int(max_batch), int(max_batch_paths),
                             int(bool(batching)), 0, bo)
         h = C.c_void_p()

@@ -931,104 +931,159 @@ CRT_LANE_KERNEL void k_extend_shade_lane(Ctx C, int bounce, int qsel, int gen, i
 }
 
 // Octet form for the bounces after the first.  A block takes a chunk of
-// 8R queued paths (R = 1..8, by launch size); its octets take the chunk's
-// paths one at a time from a counter in LDS, so an octet that finishes early
-// traces more of them and the wave lasts about the chunk's mean work instead
-// of its slowest octet's share; each traced path leaves its hit (triangle,
-// u, v) in LDS, and lane p then shades path p, so a wave carries 8R paths
-// through the shading and one atomic per wave appends them.  Those rays
-// leave the body more often than camera rays hit it; camera rays traced by
-// octets use k_trace + k_shade.
+// 8R queued paths (R = 1..8, by launch size) and lane p shades path p, so a
+// wave carries 8R paths through the shading and one atomic per wave appends
+// them.  With STEAL its octets take the chunk's paths one at a time from a
+// counter in LDS, so an octet that finishes early traces more of them and
+// the wave lasts about the chunk's mean work instead of its slowest octet's
+// share, and each traced path leaves its hit (triangle, u, v) in LDS for
+// its shading lane; without, octet o traces paths oR .. oR + R - 1 in turn
+// and keeps each hit in the lane that shades it.  STEAL costs OCTET_RES_BYTES
+// of LDS per wave, so it runs only where that still lets a CU hold its 32
+// waves (steal_fits).  Those rays leave the body more often than camera rays
+// hit it; camera rays traced by octets use k_trace + k_shade.
 constexpr size_t OCTET_RES_BYTES = 16 + 3 * 4 * 64;   // path counter + (triangle, u, v) per chunk path
 
-template <bool MATS>
+// gfx950: 160 KB of LDS per CU; the bounce kernels ask for 8 waves per SIMD
+constexpr size_t LDS_PER_CU = 160 * 1024;
+constexpr size_t WAVES_PER_CU = 32;
+inline bool steal_fits(size_t stack_bytes, size_t res_bytes) {
+    return (stack_bytes + res_bytes) * WAVES_PER_CU <= LDS_PER_CU;
+}
+
+template <bool MATS, bool STEAL>
 __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8)))
 void k_extend_shade_octet(Ctx C, int bounce, int qsel, int R, int sorted) {
     OCTET_SETUP(C)
     const uint32_t items = OCTETS_PER_BLOCK * (uint32_t)R;   // <= 64
-    uint32_t* next_path = smem + (size_t)2 * OCTETS_PER_BLOCK * C.s.stack_cap;
-    uint32_t* res = next_path + 4;
-    if (threadIdx.x == 0) *next_path = 0u;
-    __syncthreads();
-#pragma unroll 1
-    while (true) {
-        const uint32_t j = group_grab(next_path, sub == 0, lane_base);   // octet-uniform
-        uint32_t k;
-        if (j >= items || !queue_entry(C, qsel, sorted, items, j, k)) break;
-        const float4 o = C.st.o[k], d = C.st.d[k];
-        const RayPre r = make_ray_pre(o.x, o.y, o.z, d.x, d.y, d.z);
-        const HitRec h = trace_octet(C.s, r, 0.0f, __int_as_float(0x7f800000), sref, sd, sub, lane_base);
-        if (sub == 0) {
-            res[3 * j] = (uint32_t)h.slot;
-            res[3 * j + 1] = __float_as_uint(h.u);
-            res[3 * j + 2] = __float_as_uint(h.v);
-        }
-    }
-    __syncthreads();
-    const uint32_t p = threadIdx.x;
     uint32_t k_mine = 0;
-    bool mine = p < items && queue_entry(C, qsel, sorted, items, p, k_mine);
     HitRec h_mine;
     h_mine.t = 0.f; h_mine.u = 0.f; h_mine.v = 0.f; h_mine.slot = -1;
     float3 d_mine = make_float3(0.f, 0.f, 0.f);
-    if (mine) {
-        h_mine.slot = (int32_t)res[3 * p];
-        h_mine.u = __uint_as_float(res[3 * p + 1]);
-        h_mine.v = __uint_as_float(res[3 * p + 2]);
-        const float4 d = C.st.d[k_mine];
-        d_mine = make_float3(d.x, d.y, d.z);
+    bool mine = false;
+    if constexpr (STEAL) {
+        uint32_t* next_path = smem + (size_t)2 * OCTETS_PER_BLOCK * C.s.stack_cap;
+        uint32_t* res = next_path + 4;
+        if (threadIdx.x == 0) *next_path = 0u;
+        __syncthreads();
+#pragma unroll 1
+        while (true) {
+            const uint32_t j = group_grab(next_path, sub == 0, lane_base);   // octet-uniform
+            uint32_t k;
+            if (j >= items || !queue_entry(C, qsel, sorted, items, j, k)) break;
+            const float4 o = C.st.o[k], d = C.st.d[k];
+            const RayPre r = make_ray_pre(o.x, o.y, o.z, d.x, d.y, d.z);
+            const HitRec h = trace_octet(C.s, r, 0.0f, __int_as_float(0x7f800000), sref, sd, sub, lane_base);
+            if (sub == 0) {
+                res[3 * j] = (uint32_t)h.slot;
+                res[3 * j + 1] = __float_as_uint(h.u);
+                res[3 * j + 2] = __float_as_uint(h.v);
+            }
+        }
+        __syncthreads();
+        const uint32_t p = threadIdx.x;
+        mine = p < items && queue_entry(C, qsel, sorted, items, p, k_mine);
+        if (mine) {
+            h_mine.slot = (int32_t)res[3 * p];
+            h_mine.u = __uint_as_float(res[3 * p + 1]);
+            h_mine.v = __uint_as_float(res[3 * p + 2]);
+            const float4 d = C.st.d[k_mine];
+            d_mine = make_float3(d.x, d.y, d.z);
+        }
+    } else {
+        const uint32_t t0 = (uint32_t)local * (uint32_t)R;
+#pragma unroll 1
+        for (int p = 0; p < R; ++p) {
+            uint32_t k;
+            if (!queue_entry(C, qsel, sorted, items, t0 + (uint32_t)p, k)) break;   // octet-uniform
+            const float4 o = C.st.o[k], d = C.st.d[k];
+            const RayPre r = make_ray_pre(o.x, o.y, o.z, d.x, d.y, d.z);
+            const HitRec h = trace_octet(C.s, r, 0.0f, __int_as_float(0x7f800000), sref, sd, sub, lane_base);
+            if (sub == p) {   // the hit is octet-uniform
+                k_mine = k;
+                h_mine = h;
+                d_mine = make_float3(d.x, d.y, d.z);
+                mine = true;
+            }
+        }
     }
     const uint32_t ls = shard_append(C, mine && h_mine.slot >= 0, C.lcnt, block_shard());
     if (mine) finish_extend<MATS>(C, bounce, k_mine, h_mine, ls, d_mine);
 }
 
-// A block takes a chunk of 16R live paths (R = 1..4, by launch size); its
-// quads take the chunk's paths one at a time from a counter in LDS (as the
-// octet bounce kernel does) and test each path's shadow rays (origin triangle
-// first), leaving the path's unoccluded light in LDS; lane p then adds path
-// p's clamped direct light and queues it on.
+// A block takes a chunk of 16R live paths (R = 1..4, by launch size), tests
+// their shadow rays (origin triangle first) by quads, and lane p then adds
+// path p's clamped direct light and queues it on.  With STEAL the quads take
+// the chunk's paths one at a time from a counter in LDS (as the octet
+// bounce kernel does) and leave each path's unoccluded light in LDS; without,
+// quad q tests paths qR .. qR + R - 1 in turn.
 constexpr size_t QUAD_RES_BYTES = 16 + 3 * 4 * 64;   // path counter + light rgb per chunk path
 
+// The nl shadow rays of live path q: summed colour of the unoccluded ones.
+__device__ inline void path_shadows(const Ctx& C, uint32_t q, uint32_t* qstk, int qj, int qbase, float lr[3]) {
+    lr[0] = lr[1] = lr[2] = 0.f;
+    for (int l = 0; l < C.nl; ++l) {
+        const uint32_t i = q * (uint32_t)C.nl + (uint32_t)l;
+        const float4 o = C.shadow[i].o, d = C.shadow[i].d;
+        const RayPre r = make_ray_pre(o.x, o.y, o.z, d.x, d.y, d.z);
+        const bool occ = hint_blocks(C.s, r, __float_as_uint(d.w), 0.0f, o.w) ||
+                         trace_quad_any<false>(C.s, r, 0.0f, o.w, qstk, qj, qbase);
+        if (!occ) {
+            const float4 c = C.shadow[i].c;
+            lr[0] += c.x; lr[1] += c.y; lr[2] += c.z;
+        }
+    }
+}
+
+template <bool STEAL>
 __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8)))
 void k_shadow_accum(Ctx C, int bounce, int qsel, int next, int R) {
     QUAD_SETUP(C)
     const uint32_t items = (TRACE_BLOCK / QUAD) * (uint32_t)R;   // <= 64
-    uint32_t* next_path = quad_stacks + (size_t)(TRACE_BLOCK / QUAD) * C.s.stack_cap;
-    float* res = reinterpret_cast<float*>(next_path + 4);
     if (blockIdx.x == 0) tally_bounce(C, qsel);   // block 0 is one whole wave
-    if (threadIdx.x == 0) *next_path = 0u;
-    __syncthreads();
+    float lr_mine[3] = {0.f, 0.f, 0.f};
+    uint32_t k_mine = 0;
+    bool mine = false;
+    if constexpr (STEAL) {
+        uint32_t* next_path = quad_stacks + (size_t)(TRACE_BLOCK / QUAD) * C.s.stack_cap;
+        float* res = reinterpret_cast<float*>(next_path + 4);
+        if (threadIdx.x == 0) *next_path = 0u;
+        __syncthreads();
 #pragma unroll 1
-    while (true) {
-        const uint32_t j = group_grab(next_path, qj == 0, qbase);   // quad-uniform
-        uint32_t q;
-        if (j >= items || !shard_entry(C, C.lcnt, items, j, q)) break;
-        float lr[3] = {0.f, 0.f, 0.f};
-        for (int l = 0; l < C.nl; ++l) {
-            const uint32_t i = q * (uint32_t)C.nl + (uint32_t)l;
-            const float4 o = C.shadow[i].o, d = C.shadow[i].d;
-            const RayPre r = make_ray_pre(o.x, o.y, o.z, d.x, d.y, d.z);
-            const bool occ = hint_blocks(C.s, r, __float_as_uint(d.w), 0.0f, o.w) ||
-                             trace_quad_any<false>(C.s, r, 0.0f, o.w, qstk, qj, qbase);
-            if (!occ) {
-                const float4 c = C.shadow[i].c;
-                lr[0] += c.x; lr[1] += c.y; lr[2] += c.z;
+        while (true) {
+            const uint32_t j = group_grab(next_path, qj == 0, qbase);   // quad-uniform
+            uint32_t q;
+            if (j >= items || !shard_entry(C, C.lcnt, items, j, q)) break;
+            float lr[3];
+            path_shadows(C, q, qstk, qj, qbase, lr);
+            if (qj == 0) {
+                res[3 * j] = lr[0];
+                res[3 * j + 1] = lr[1];
+                res[3 * j + 2] = lr[2];
             }
         }
-        if (qj == 0) {
-            res[3 * j] = lr[0];
-            res[3 * j + 1] = lr[1];
-            res[3 * j + 2] = lr[2];
+        __syncthreads();
+        const uint32_t p = threadIdx.x;
+        uint32_t q = 0;
+        mine = p < items && shard_entry(C, C.lcnt, items, p, q);
+        if (mine) {
+            k_mine = C.live[q];
+            lr_mine[0] = res[3 * p]; lr_mine[1] = res[3 * p + 1]; lr_mine[2] = res[3 * p + 2];
         }
-    }
-    __syncthreads();
-    const uint32_t p = threadIdx.x;
-    uint32_t q = 0, k_mine = 0;
-    const bool mine = p < items && shard_entry(C, C.lcnt, items, p, q);
-    float lr_mine[3] = {0.f, 0.f, 0.f};
-    if (mine) {
-        k_mine = C.live[q];
-        lr_mine[0] = res[3 * p]; lr_mine[1] = res[3 * p + 1]; lr_mine[2] = res[3 * p + 2];
+    } else {
+        const uint32_t t0 = (uint32_t)qlocal * (uint32_t)R;
+#pragma unroll 1
+        for (int p = 0; p < R; ++p) {
+            uint32_t q;
+            if (!shard_entry(C, C.lcnt, items, t0 + (uint32_t)p, q)) break;   // quad-uniform
+            float lr[3];
+            path_shadows(C, q, qstk, qj, qbase, lr);
+            if (qj == p) {
+                k_mine = C.live[q];
+                lr_mine[0] = lr[0]; lr_mine[1] = lr[1]; lr_mine[2] = lr[2];
+                mine = true;
+            }
+        }
     }
     const bool cont = mine && accumulate_path(C, bounce, k_mine, lr_mine);
     const uint32_t pos = shard_append(C, cont, C.qcnt[next], block_shard());
@@ -1161,18 +1216,37 @@ void launch_pass(const Ctx& C, uint64_t cap_paths, uint32_t cap_fold, const crt_
                     hipLaunchKernelGGL(k_extend_shade_lane<false>, gl, dim3(LANE_BLOCK), lane_lds_bytes(C.s), st, C,
                                        b, cur, (int)(gen && b == 0), (int)sorted);
             } else if (b > 0 && B.separate_kernels != 2) {
-                if (mats)
-                    hipLaunchKernelGGL(k_extend_shade_octet<true>, goct, dim3(TRACE_BLOCK), lds + OCTET_RES_BYTES, st, C,
-                                       b, cur, ro, (int)sorted);
+                const bool steal = B.fixed_share == 0 && steal_fits(lds, OCTET_RES_BYTES);
+                const size_t olds = steal ? lds + OCTET_RES_BYTES : lds;
+                if (mats && steal)
+                    hipLaunchKernelGGL((k_extend_shade_octet<true, true>), goct, dim3(TRACE_BLOCK), olds, st, C, b, cur,
+                                       ro, (int)sorted);
+                else if (mats)
+                    hipLaunchKernelGGL((k_extend_shade_octet<true, false>), goct, dim3(TRACE_BLOCK), olds, st, C, b, cur,
+                                       ro, (int)sorted);
+                else if (steal)
+                    hipLaunchKernelGGL((k_extend_shade_octet<false, true>), goct, dim3(TRACE_BLOCK), olds, st, C, b,
+                                       cur, ro, (int)sorted);
                 else
-                    hipLaunchKernelGGL(k_extend_shade_octet<false>, goct, dim3(TRACE_BLOCK), lds + OCTET_RES_BYTES, st, C,
-                                       b, cur, ro, (int)sorted);
+                    hipLaunchKernelGGL((k_extend_shade_octet<false, false>), goct, dim3(TRACE_BLOCK), olds, st, C, b,
+                                       cur, ro, (int)sorted);
             } else {
                 hipLaunchKernelGGL(k_trace, dim3(gsh(OCTETS_PER_BLOCK)), dim3(TRACE_BLOCK), lds, st, C, cur);
                 launch_shade(C, mats, gsh(TB), b, cur, st);
             }
-            hipLaunchKernelGGL(k_shadow_accum, dim3(gsh(QPB * rq)), dim3(TRACE_BLOCK), qlds + QUAD_RES_BYTES, st, C, b,
-                               cur, nxt, rq);
+            if (B.fixed_share == 0 && steal_fits(qlds, QUAD_RES_BYTES)) {
+                hipLaunchKernelGGL(k_shadow_accum<true>, dim3(gsh(QPB * rq)), dim3(TRACE_BLOCK), qlds + QUAD_RES_BYTES, st,
+                                   C, b, cur, nxt, rq);
+            } else {
+                // with fixed shares the bounce-0 queue, which holds every
+                // camera-ray hit, runs best at half the paths per quad, twice
+                // the waves (C4: 5.0 -> 4.55 ms per pass; a 6.3M-triangle
+                // geodesic render 9.9 -> 9.5 ms); the later, sparse bounces
+                // keep rq
+                const int rqb = (b == 0 && B.rays_per_quad == 0) ? std::max(1, rq / 2) : rq;
+                hipLaunchKernelGGL(k_shadow_accum<false>, dim3(gsh(QPB * rqb)), dim3(TRACE_BLOCK), qlds, st, C, b, cur,
+                                   nxt, rqb);
+            }
         }
         if (C.nb == 0) hipLaunchKernelGGL(k_zero_samples, dim3(gp), dim3(TB), 0, st, C);
         if (!C.direct_fold) hipLaunchKernelGGL(k_fold, dim3(std::max(1u, cdiv(cap_fold, TB))), dim3(TB), 0, st, C);
@@ -1233,6 +1307,7 @@ void check_batch_opts(const crt_batch_opts& B) {
     CRT_REQUIRE(B.coherence_sort <= 1, CRT_ERR_INVALID, "coherence_sort must be 0 or 1");
     CRT_REQUIRE(B.rays_per_octet <= 8, CRT_ERR_INVALID, "rays_per_octet must be 0 (by pass size) or 1..8");
     CRT_REQUIRE(B.rays_per_quad <= 4, CRT_ERR_INVALID, "rays_per_quad must be 0 (by pass size) or 1..4");
+    CRT_REQUIRE(B.fixed_share <= 1, CRT_ERR_INVALID, "fixed_share must be 0 or 1");
     CRT_REQUIRE(B.tile_width == 0 || B.tile_width == 4 || B.tile_width == 8 || B.tile_width == 16 ||
                     B.tile_width == 32,
                 CRT_ERR_INVALID, "tile_width must be 0 (8), 4, 8, 16 or 32");

@@ -347,6 +347,11 @@ typedef struct {
                                  size (4 from 2^21 paths, 2 from 2^20, else 1) */
     uint32_t tile_width;      /* camera rays of a wave: a tile of 64 pixels this wide (4, 8, 16 or
                                  32; 0 = 8, i.e. 8x8) */
+    uint32_t fixed_share;     /* fused bounce kernels: 0 (default) = a wave's octets (quads) take
+                                 its paths one at a time from an LDS counter wherever the extra
+                                 LDS still lets a CU hold 32 waves (BVH stacks of at most 64
+                                 entries), else each takes a fixed share of R paths; 1 = always a
+                                 fixed share.  The image is the same either way. */
 } crt_batch_opts;
 int crt_render_batch(crt_scene* scene, uint32_t n_cams, const crt_camera* cams, const int32_t* max_samples,
                      const uint64_t* seeds, const crt_light* lights, uint32_t n_lights,

@@ -40,7 +40,8 @@ def test_batch_equals_plain_renders(body):
 
 @pytest.mark.parametrize("variant", [dict(use_graph=True), dict(persistent=True, grab=4, blocks_per_cu=4),
                                      dict(use_graph=True, persistent=True, grab=2, samples_per_pass=1),
-                                     dict(samples_per_pass=1), dict(samples_per_pass=1, use_graph=True)])
+                                     dict(samples_per_pass=1), dict(samples_per_pass=1, use_graph=True),
+                                     dict(fixed_share=1), dict(fixed_share=1, rays_per_octet=3, rays_per_quad=3)])
 def test_batch_variants_identical(body, variant):
     # samples_per_pass=1: every path folds its own sample where it ends (no
     # k_fold); the same bits as the k_fold of passes of several samples
