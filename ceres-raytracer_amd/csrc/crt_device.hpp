@@ -548,11 +548,19 @@ __device__ inline float cvt_u8(uint32_t w, uint32_t byte) {
     return (float)((w >> (8u * byte)) & 0xFFu);
 }
 
-// Test the children of compressed node `cn` in `mask` against [tmin, best].
-// Returns the hit mask; k0/ref0/tn0 = nearest hit child, tn1 = nearest of
-// the other hit children.
+// Median of three unsigned words (one VALU instruction).
+__device__ inline uint32_t med3_u32(uint32_t a, uint32_t b, uint32_t c) {
+    uint32_t d;
+    asm("v_med3_u32 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+    return d;
+}
+
+// Test the children of compressed node `cn` in `mask` against [tmin, best]
+// (tmin >= 0).  Returns the hit mask; k0/ref0 = nearest hit child, tn1 <=
+// the entry distance of every other hit child (their nearest, rounded down
+// by at most 7 ulp).
 __device__ inline uint32_t cnode_test(const float4* __restrict__ cn, const RayPre& r, float tmin, float best,
-                                      uint32_t mask, uint32_t& k0, uint32_t& ref0, float& tn0, float& tn1) {
+                                      uint32_t mask, uint32_t& k0, uint32_t& ref0, float& tn1) {
     const float4 h = cn[0];
     const uint4 a = __builtin_bit_cast(uint4, cn[1]);
     const uint4 b = __builtin_bit_cast(uint4, cn[2]);
@@ -571,11 +579,12 @@ __device__ inline uint32_t cnode_test(const float4* __restrict__ cn, const RayPr
     const uint32_t nx0 = nx ? b.z : a.x, nx1 = nx ? b.w : a.y, fx0 = nx ? a.x : b.z, fx1 = nx ? a.y : b.w;
     const uint32_t ny0 = ny ? c.x : a.z, ny1 = ny ? c.y : a.w, fy0 = ny ? a.z : c.x, fy1 = ny ? a.w : c.y;
     const uint32_t nz0 = nz ? c.z : b.x, nz1 = nz ? c.w : b.y, fz0 = nz ? b.x : c.z, fz1 = nz ? b.y : c.w;
-    const uint32_t refs[8] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
-    tn0 = __int_as_float(0x7f800000);
-    tn1 = tn0;
-    k0 = 8u;
-    ref0 = 0u;
+    // Nearest and second nearest as sort keys: tnear's bits (tnear >= +-0,
+    // so the unsigned order of the bits is the order of the distances) with
+    // the slot in the low 3 bits; an untested or missed child is ~0.  Per
+    // child one min and one median keep the two smallest keys (ties go to
+    // the lower slot); the nearest child's ref is selected once at the end.
+    uint32_t n0 = 0xFFFFFFFFu, n1 = 0xFFFFFFFFu;
     uint32_t hits = 0u;
 #pragma unroll
     for (uint32_t k = 0; k < 8u; ++k) {
@@ -592,14 +601,15 @@ __device__ inline uint32_t cnode_test(const float4* __restrict__ cn, const RayPr
         // empty slots are out of `mask` (and stored inverted, qlo 255, qhi 0)
         const bool hk = ((mask >> k) & 1u) && (tnear <= tfar * 1.00000024f);
         hits |= hk ? (1u << k) : 0u;
-        // branch-free nearest / second nearest
-        const float tk = hk ? tnear : __int_as_float(0x7f800000);
-        const bool lt = tk < tn0;
-        tn1 = lt ? tn0 : (tk < tn1 ? tk : tn1);
-        tn0 = lt ? tk : tn0;
-        k0 = lt ? k : k0;
-        ref0 = lt ? refs[k] : ref0;
+        const uint32_t key = hk ? ((__float_as_uint(tnear) & 0x7FFFFFF8u) | k) : 0xFFFFFFFFu;
+        n1 = med3_u32(n0, n1, key);   // n0 <= n1: the second smallest of the three
+        n0 = min(n0, key);
     }
+    k0 = n0 & 7u;
+    const uint4 rq = (k0 & 4u) ? r1 : r0;
+    const uint32_t rp0 = (k0 & 2u) ? rq.z : rq.x, rp1 = (k0 & 2u) ? rq.w : rq.y;
+    ref0 = (k0 & 1u) ? rp1 : rp0;
+    tn1 = __uint_as_float(n1 & ~7u);
     return hits;
 }
 
@@ -638,13 +648,13 @@ __device__ inline HitRec trace_lane(const SceneView& s, const RayPre& r, float t
         // ---- inner nodes ----
         while (cur != 0u && !(cur & REF_LEAF)) {
             uint32_t k0, ref0;
-            float tn0, tn1;
+            float tn1;
             if (STATS) {
                 ++ts->nodes;
                 ts->boxes += (uint32_t)__popc((cur & 0xFFu) & (__float_as_uint(s.cnodes[(size_t)(cur >> 8) * 6u].w) >> 24));
             }
             const uint32_t hits = cnode_test(s.cnodes + (size_t)(cur >> 8) * 6u, r, tmin, best, cur & 0xFFu, k0,
-                                             ref0, tn0, tn1);
+                                             ref0, tn1);
             uint32_t next = 0u;
             if (hits != 0u) {
                 const uint32_t rest = hits & ~(1u << k0);
