@@ -513,7 +513,7 @@ def run_served(args, capi, scenes):
                [m.n_tris for m in meshes], resident / 1e9), "requests": args.requests}}
     # plain = the drop-in API called once per request, one request at a time
     # (the server's batching = 0 mode: one worker, no batching)
-    modes = tuple(m for m in (("served", True, 3), ("plain", False, 1)) if m[0] in args.modes.split(","))
+    modes = tuple(m for m in (("served", True, args.workers), ("plain", False, 1)) if m[0] in args.modes.split(","))
     # the same render launch parameters for every mode: lane traversal of
     # camera rays; no hipGraph (a request batch renders a few passes, and a
     # capture + instantiate per batch costs more than it saves)
@@ -532,7 +532,7 @@ def run_served(args, capi, scenes):
         reqs = serving.mixed_workload(meshes, args.requests, 1e6, seed=7)
         log("%s: saturated stream of %d requests" % (mode, len(reqs)))
         _, lat, st = serving.run_stream(srv, reqs, realtime=False)
-        out[mode] = {"mrays_s": st["mrays_per_s"], "wall_s": st["wall_s"], "batches": st["batches"],
+        out[mode] = {"mrays_s": st["mrays_per_s"], "wall_s": st["wall_s"], "workers": nw, "batches": st["batches"],
                      "rays": st["rays"], "p50_ms_saturated": st["p50_ms"], "p99_ms_saturated": st["p99_ms"]}
         srv.close()
     # Batching without concurrency: the same saturated request set with each
@@ -660,6 +660,7 @@ def main():
     ap.add_argument("--requests", type=int, default=300)
     ap.add_argument("--rate", type=float, default=100.0)
     ap.add_argument("--small", action="store_true", help="served: small meshes (quick check)")
+    ap.add_argument("--workers", type=int, default=3, help="served: worker threads (one HIP stream each)")
     ap.add_argument("--modes", default="served,plain", help="served: comma list of served, plain")
     ap.add_argument("--no-rate", action="store_true", help="served: skip the fixed-rate latency runs")
     ap.add_argument("--render-once", action="store_true",
