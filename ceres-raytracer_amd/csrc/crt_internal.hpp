@@ -1,6 +1,7 @@
 // crt_internal.hpp -- host-side internals shared by the .hip translation units.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <stdint.h>
 #include <memory>
 #include <mutex>
@@ -82,9 +83,13 @@ struct DevBuf {
         CRT_HIP_CHECK(hipMalloc(&p, b));
         bytes = b;
     }
-    // grow-only reallocation (contents not preserved)
+    // grow-only reallocation (contents not preserved).  A buffer that has to
+    // grow takes at least 1.5 times its size: workspaces reused by calls of
+    // varying sizes (request batches, repeated API calls) then reallocate a
+    // few times only, and each reallocation's hipFree waits for the whole
+    // device, stalling every other stream's work.
     void reserve(size_t b) {
-        if (b > bytes) alloc(b);
+        if (b > bytes) alloc(bytes ? std::max(b, bytes + bytes / 2) : b);
     }
     template <typename T> T* as() const { return static_cast<T*>(p); }
 };
