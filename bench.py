@@ -520,19 +520,21 @@ def run_served(args, capi, scenes):
     ropts = dict(extend_mode=3)
     out["render_launch_params"] = ropts
     for mode, batching, nw in modes:
+        # steady state, as a long-running server: the server first serves a
+        # warm-up stream of the same mix (its workspaces grow to that mix's
+        # sizes), then its statistics restart for the timed stream
         srv = serving.Server(batching=batching, n_workers=nw, render_opts=ropts)
         for s in sc:
             srv.add_scene(s)
-        warm = serving.mixed_workload(meshes, 8, 1e6, seed=99)
+        warm = serving.mixed_workload(meshes, args.served_warmup, 1e6, seed=99)
         serving.run_stream(srv, warm, realtime=False)
-        srv.close()
-        srv = serving.Server(batching=batching, n_workers=nw, render_opts=ropts)
-        for s in sc:
-            srv.add_scene(s)
+        srv.drain()
+        srv.reset_stats()
         reqs = serving.mixed_workload(meshes, args.requests, 1e6, seed=7)
         log("%s: saturated stream of %d requests" % (mode, len(reqs)))
         _, lat, st = serving.run_stream(srv, reqs, realtime=False)
-        out[mode] = {"mrays_s": st["mrays_per_s"], "wall_s": st["wall_s"], "workers": nw, "batches": st["batches"],
+        out[mode] = {"mrays_s": st["mrays_per_s"], "wall_s": st["wall_s"], "workers": nw,
+                     "warmup_requests": args.served_warmup, "batches": st["batches"],
                      "rays": st["rays"], "p50_ms_saturated": st["p50_ms"], "p99_ms_saturated": st["p99_ms"]}
         srv.close()
     # Batching without concurrency: the same saturated request set with each
@@ -661,6 +663,8 @@ def main():
     ap.add_argument("--rate", type=float, default=100.0)
     ap.add_argument("--small", action="store_true", help="served: small meshes (quick check)")
     ap.add_argument("--workers", type=int, default=3, help="served: worker threads (one HIP stream each)")
+    ap.add_argument("--served-warmup", type=int, default=64,
+                    help="served: requests of the same mix served before the timed stream")
     ap.add_argument("--modes", default="served,plain", help="served: comma list of served, plain")
     ap.add_argument("--no-rate", action="store_true", help="served: skip the fixed-rate latency runs")
     ap.add_argument("--render-once", action="store_true",

@@ -175,6 +175,7 @@ SIGNATURES = {
     "crt_server_cancel": (C.c_int, [_p, C.c_uint64]),
     "crt_server_drain": (C.c_int, [_p]),
     "crt_server_stats_get": (C.c_int, [_p, _p]),
+    "crt_server_stats_reset": (C.c_int, [_p]),
     "crt_server_destroy": (None, [_p]),
     "crt_lidar_batch": (C.c_int, [_p, C.c_uint64, _p, _p, C.c_int32, _p, C.c_uint32, _p, _p]),
     "crt_flash_lidar": (C.c_int, [_p, _p, _p, _p, _p, _p, C.c_uint32, _p, _p]),

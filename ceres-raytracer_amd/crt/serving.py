@@ -112,6 +112,11 @@ class Server:
     def drain(self):
         capi._check(capi.load().crt_server_drain(self.h))
 
+    def reset_stats(self):
+        """Start the statistics afresh (e.g. after a warm-up stream); only
+        while nothing is queued or running."""
+        capi._check(capi.load().crt_server_stats_reset(self.h))
+
     def stats(self) -> dict:
         s = capi.ServerStats()
         capi._check(capi.load().crt_server_stats_get(self.h, C.byref(s)))

@@ -622,6 +622,18 @@ int crt_server_stats_get(crt_server* S, crt_server_stats* out) {
     });
 }
 
+int crt_server_stats_reset(crt_server* S) {
+    return guarded([&] {
+        CRT_REQUIRE(S, CRT_ERR_INVALID, "NULL argument");
+        std::lock_guard<std::mutex> lk(S->mu);
+        CRT_REQUIRE(S->queue.empty() && S->inflight == 0, CRT_ERR_STATE,
+                    "statistics are reset only while nothing is queued or running");
+        S->completed = S->rejected = S->cancelled = S->rays = S->batches = 0;
+        S->latencies_ms.clear();
+        S->have_first = false;
+    });
+}
+
 void crt_server_destroy(crt_server* S) {
     if (!S) return;
     {

@@ -490,6 +490,10 @@ int  crt_server_wait_for(crt_server* server, uint64_t ticket, double timeout_ms,
 int  crt_server_cancel(crt_server* server, uint64_t ticket);
 int  crt_server_drain(crt_server* server);
 int  crt_server_stats_get(crt_server* server, crt_server_stats* stats);
+/* Start the statistics afresh (counts, latencies, the wall-clock window),
+ * for example after a warm-up stream; the server's workspaces and scenes stay.
+ * Only while nothing is queued or running (CRT_ERR_STATE otherwise). */
+int  crt_server_stats_reset(crt_server* server);
 void crt_server_destroy(crt_server* server);
 
 #ifdef __cplusplus

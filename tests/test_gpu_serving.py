@@ -176,6 +176,31 @@ def test_tight_budget_counts_retained_workspaces(meshes):
     srv.close()
 
 
+def test_stats_reset_restarts_the_window(meshes):
+    # after a warm-up stream the statistics restart; the server keeps its
+    # scenes and workspaces and serves on with the same results
+    ms, sc = meshes
+    cam = scenes.small_body_camera(ms[0], resolution=64, seed=3)
+    sun = {"type": "directional", "intensity": 1.0, "direction": scenes.sun_direction(4)}
+    req = {"kind": "render", "scene": 0, "camera": cam, "spp": 2, "bounces": 2, "seed": 5, "lights": [sun]}
+    srv = serving.Server(batching=True, n_workers=2)
+    srv.add_scene(sc[0])
+    first, _ = srv.result(srv.submit(req))
+    srv.result(srv.submit(dict(req, seed=6)))
+    st = srv.stats()
+    assert st["completed"] == 2 and st["rays"] > 0 and st["wall_s"] > 0
+    srv.drain()
+    srv.reset_stats()
+    st = srv.stats()
+    assert st["completed"] == 0 and st["rays"] == 0 and st["batches"] == 0
+    assert st["wall_s"] == 0 and st["p50_ms"] == 0
+    again, _ = srv.result(srv.submit(req))
+    assert np.array_equal(again["rgba"], first["rgba"])
+    st = srv.stats()
+    assert st["completed"] == 1 and st["batches"] == 1 and st["wall_s"] > 0
+    srv.close()
+
+
 def test_cancel_queued_requests_and_wait_timeout(meshes):
     # one worker, one request at a time: everything behind a large render is
     # still queued right after submission
