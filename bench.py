@@ -663,7 +663,7 @@ def main():
     ap.add_argument("--rate", type=float, default=100.0)
     ap.add_argument("--small", action="store_true", help="served: small meshes (quick check)")
     ap.add_argument("--workers", type=int, default=3, help="served: worker threads (one HIP stream each)")
-    ap.add_argument("--served-warmup", type=int, default=64,
+    ap.add_argument("--served-warmup", type=int, default=300,
                     help="served: requests of the same mix served before the timed stream")
     ap.add_argument("--modes", default="served,plain", help="served: comma list of served, plain")
     ap.add_argument("--no-rate", action="store_true", help="served: skip the fixed-rate latency runs")
