@@ -60,6 +60,27 @@ int guarded(F&& f) {
 }
 
 // Owning device buffer.
+// Device buffers released on a thread that must not wait for the device
+// (a server worker inside a batch: hipFree waits for the whole device and
+// so stalls the other workers' streams).  While `defer` is set, released
+// buffers are kept here and freed by free_retired() at a point where the
+// wait costs nothing.
+struct RetiredBufs {
+    std::vector<void*> p;
+    size_t bytes = 0;
+    bool defer = false;
+};
+inline RetiredBufs& retired_bufs() {
+    static thread_local RetiredBufs r;
+    return r;
+}
+inline void free_retired() {
+    RetiredBufs& r = retired_bufs();
+    for (void* q : r.p) (void)hipFree(q);
+    r.p.clear();
+    r.bytes = 0;
+}
+
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
@@ -73,7 +94,15 @@ struct DevBuf {
     }
     ~DevBuf() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            RetiredBufs& r = retired_bufs();
+            if (r.defer) {
+                r.p.push_back(p);
+                r.bytes += bytes;
+            } else {
+                (void)hipFree(p);
+            }
+        }
         p = nullptr;
         bytes = 0;
     }

@@ -342,6 +342,21 @@ void worker_main(crt_server* S, int index) {
         std::vector<Workspace> drop_ws;
         std::vector<RenderWorkspace*> drop_rws;
         size_t mem = 0;
+        if (!retired_bufs().p.empty()) {
+            // buffers this worker's workspaces outgrew during its batches:
+            // freed once it has no work, where waiting for the device costs
+            // nothing
+            bool idle;
+            {
+                std::lock_guard<std::mutex> lk(S->mu);
+                idle = pick_head(S, index) == S->queue.end();
+            }
+            if (idle) {
+                free_retired();
+                std::lock_guard<std::mutex> lk(S->mu);
+                S->held[index] = workspace_bytes(me.ws, me.rws);
+            }
+        }
         {
             std::unique_lock<std::mutex> lk(S->mu);
             // on stop, queued requests still run (close drains the queue)
@@ -401,6 +416,7 @@ void worker_main(crt_server* S, int index) {
         }
         drop_ws.clear();
         for (RenderWorkspace* r : drop_rws) delete_render_workspace(r);
+        if (!drop_rws.empty()) free_retired();   // the room is needed now
         // run the batch; if it fails, rerun its requests one by one so an
         // error reaches only its own request
         std::vector<int> status(batch.size(), CRT_OK);
@@ -418,6 +434,7 @@ void worker_main(crt_server* S, int index) {
             (void)hipStreamSynchronize(me.st);
             return false;
         };
+        retired_bufs().defer = true;
         if (!run(batch, 0) && batch.size() > 1)
             for (size_t i = 0; i < batch.size(); ++i) {
                 std::vector<std::shared_ptr<Job>> one{batch[i]};
@@ -425,7 +442,8 @@ void worker_main(crt_server* S, int index) {
                 err[i].clear();
                 run(one, i);
             }
-        const size_t keep = workspace_bytes(me.ws, me.rws);
+        retired_bufs().defer = false;
+        const size_t keep = workspace_bytes(me.ws, me.rws) + retired_bufs().bytes;
         const auto now = Clock::now();
         {
             std::lock_guard<std::mutex> lk(S->mu);
@@ -449,6 +467,7 @@ void worker_main(crt_server* S, int index) {
         S->cv_done.notify_all();
         S->cv_work.notify_all();
     }
+    free_retired();
 }
 
 }  // namespace
